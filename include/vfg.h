@@ -1,0 +1,198 @@
+/*
+ * vfg.h -- C-ABI of the MI355X-native VPFHF u-/V-subproblem operators.
+ *
+ * Plain pointers and sizes only.  Every entry point returns 0 (VFG_OK) or a
+ * negative VFG_E_* code, never aborts and never prints; vfg_last_error()
+ * gives the message of the last failure on a handle (or of a failed
+ * vfg_create when called with NULL).
+ *
+ * Layout is the DMDA natural order of the reference (VFCommon.c:321-376):
+ *   node(i,j,k) = i + nx*(j + ny*k); displacement dof 3*node + c (interleaved);
+ *   scalar fields (V, VIrrev, theta, thetaRef, pressure) one value per node;
+ *   cell(ei,ej,ek) = ei + (nx-1)*(ej + (ny-1)*ek) for per-element arrays.
+ * A PETSc Vec array of daVect/daScal therefore passes through unchanged.
+ *
+ * `loc` says where the array pointers of a call live: VFG_HOST (copied
+ * through the handle's staging buffers) or VFG_DEVICE (HIP device pointers,
+ * used in place on the handle's stream).  No pointer is retained after a call
+ * returns; vfg_set_fields copies its inputs.
+ *
+ * Threading: one handle = one HIP stream; a handle is not thread-safe;
+ * distinct handles are independent.
+ */
+#ifndef VFG_H
+#define VFG_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VFG_HOST 0
+#define VFG_DEVICE 1
+
+enum {
+  VFG_OK = 0,
+  VFG_E_ARG = -1,      /* invalid argument / shape */
+  VFG_E_HIP = -2,      /* HIP runtime error */
+  VFG_E_NOMEM = -3,    /* device allocation failed */
+  VFG_E_BREAKDOWN = -4,/* CG breakdown (non-positive curvature) */
+  VFG_E_MAXIT = -5,    /* iteration limit reached before tolerance */
+  VFG_E_NAN = -6,      /* NaN/Inf detected in a solve */
+  VFG_E_BUDGET = -7,   /* request exceeds the admission budget */
+  VFG_E_NODEV = -8     /* no GPU / library not usable */
+};
+
+/* BC types, face and vertex order of VFCartFE.h:93-127 */
+enum { VFG_NONE = 0, VFG_ZERO = 1, VFG_ONE = 2, VFG_FIXED = 3 };
+enum { VFG_X0 = 0, VFG_X1, VFG_Y0, VFG_Y1, VFG_Z0, VFG_Z1 };
+enum { VFG_X0Y0Z0 = 0, VFG_X1Y0Z0, VFG_X0Y1Z0, VFG_X1Y1Z0,
+       VFG_X0Y0Z1, VFG_X1Y0Z1, VFG_X0Y1Z1, VFG_X1Y1Z1 };
+
+/* grid of nodes, as "-n nx,ny,nz -l lx,ly,lz" (VFCommon.c:277-315) */
+typedef struct {
+  int    nx, ny, nz;
+  double lx, ly, lz;
+} vfg_grid;
+
+/* VFMatProp + VFProp subset on the u/V path (VFCommon.h:69-90).  E is the
+ * uniform Young's modulus, overridden per element by vfg_set_fields(E_elem). */
+typedef struct {
+  double E, nu, alpha, beta, Gc;
+  double eta, epsilon, irrevtol;
+  int    atnum;            /* 1: AT1, 2: AT2 */
+  int    hasCrackPressure; /* -pressurize */
+} vfg_props;
+
+/* VFBC (VFCartFE.h:129-136); edges are not parsed by the reference and are
+ * absent here.  vertexValue is read for FIXED vertices. */
+typedef struct {
+  int    face[6];
+  double faceValue[6];
+  int    vertex[8];
+  double vertexValue[8];
+} vfg_bc;
+
+typedef struct vfg_handle vfg_handle;
+
+/* ------------------------------------------------------------ lifecycle */
+/* Replaces VFGeometryInitialize/VFPropGet/VFMatPropGet for one problem
+ * (VFCommon.c:277-588).  Fields start as V = VIrrev = 1, theta = thetaRef =
+ * 0, pressure = 0, uniform E (VFFieldsInitialize, VFCommon.c:629-671, with the
+ * thermal/pressure inputs supplied explicitly by the caller). */
+int vfg_create(const vfg_grid *grid, const vfg_props *props, vfg_handle **out);
+int vfg_destroy(vfg_handle *h);
+const char *vfg_last_error(const vfg_handle *h);
+int vfg_set_props(vfg_handle *h, const vfg_props *props);
+/* VFBCSetFromOptions result for "U" (3 components) and "V" (1):
+ * VFCartFE.c:1506-1574 */
+int vfg_set_bc(vfg_handle *h, const vfg_bc u_bc[3], const vfg_bc *v_bc);
+/* Nodal fields (N_n) and per-element E (N_e).  NULL keeps the current value. */
+int vfg_set_fields(vfg_handle *h, const double *V, const double *VIrrev, const double *theta,
+                   const double *thetaRef, const double *pressure, const double *E_elem,
+                   int loc);
+int vfg_get_fields(vfg_handle *h, double *V, double *VIrrev, int loc);
+int64_t vfg_num_nodes(const vfg_handle *h);
+int64_t vfg_num_cells(const vfg_handle *h);
+int vfg_synchronize(vfg_handle *h);
+
+/* ------------------------------------------------------------- u operator */
+/* MatShell MATOP_MULT of the assembled u-Jacobian: VF_UIJacobian
+ * (VFMech.c:1897-2029) then MatApplyDirichletBC (VFCartFE.c:1147-1296):
+ * y = K x on free rows, y_i = x_i on Dirichlet rows (columns kept). */
+int vfg_u_apply(vfg_handle *h, const double *x, double *y, int loc);
+int vfg_u_apply_f32(vfg_handle *h, const float *x, float *y, int loc);
+/* MATOP_GET_DIAGONAL of the same matrix (1 on Dirichlet rows) */
+int vfg_u_diag(vfg_handle *h, double *d, int loc);
+/* element load vector sum_e b_e of VF_GradientUThermoPoro3D_local
+ * (VFMech.c:386-447) + VF_GradientUCrackPressure3D_local (VFMech.c:533-589);
+ * no boundary treatment */
+int vfg_u_rhs(vfg_handle *h, double *b, int loc);
+/* SNES function callback VF_UResidual (VFMech.c:1456-1887):
+ * F = K U - b, then ResidualApplyDirichletBC (VFCartFE.c:747-999) */
+int vfg_u_residual(vfg_handle *h, const double *U, const double *BCU, double *F, int loc);
+
+/* ------------------------------------------------------------- V operator */
+/* The V-Jacobian callback VF_VIJacobian (VFMech.c:2883-3029) depends on U
+ * only through the coupling term; vfg_v_set_u refreshes it (and the V load
+ * vector) from U.  Must be called before the V calls below. */
+int vfg_v_set_u(vfg_handle *h, const double *U, int loc);
+/* MATOP_MULT of J_V after MatApplyDirichletBCRowCol (VFCartFE.c:1305-1454)
+ * and, for AT2, VF_IrrevApplyEQMat (VFMech.c:2637-2682): rows and columns
+ * of fixed nodes replaced by identity. */
+int vfg_v_apply(vfg_handle *h, const double *x, double *y, int loc);
+int vfg_v_diag(vfg_handle *h, double *d, int loc);
+/* element load vector: VF_ResidualVAT*3D_local (VFMech.c:2350-2384) +
+ * VF_ResidualVCrackPressure3D_local (VFMech.c:2294-2341); no BC */
+int vfg_v_rhs(vfg_handle *h, double *b, int loc);
+/* SNES function callback VF_VResidual (VFMech.c:2687-2879) at the U of the
+ * last vfg_v_set_u */
+int vfg_v_residual(vfg_handle *h, const double *V, double *F, int loc);
+
+/* ----------------------------------------------------------------- BC kit */
+/* which: 0 = U (3 dof/node), 1 = V (1 dof/node) */
+int vfg_bc_mask(vfg_handle *h, int which, uint8_t *mask, int loc);            /* VFCartFE.c:1147-1296 */
+int vfg_vec_set_from_bc(vfg_handle *h, int which, double *B, int loc);        /* VFCartFE.c:279-483 */
+int vfg_vec_apply_dirichlet(vfg_handle *h, int which, double *vec, const double *B,
+                            int loc);                                         /* VFCartFE.c:493-738 */
+
+/* ------------------------------------------------------------ diagnostics */
+/* out[0] elastic energy, out[1] in-situ work (always 0: out of scope),
+ * out[2] pressure work (VF_UEnergy3D, VFMech.c:1111-1427), out[3] surface
+ * energy (VF_VEnergy3D, VFMech.c:2474-2526). */
+int vfg_energies(vfg_handle *h, const double *U, const double *V, double out[4], int loc);
+/* VolumetricCrackOpening (VFPermfield.c:75-150); cellfield (N_e) optional */
+int vfg_crack_volume(vfg_handle *h, const double *U, const double *V, double *vol,
+                     double *cellfield, int loc);
+
+/* ---------------------------------------------------------- crack geometry */
+/* VFPennyCrackBuildVAT1/AT2 (VFCracks.c:136-227) for one crack into V */
+int vfg_penny_build_v(vfg_handle *h, double *V, const double center[3], double r,
+                      double theta_deg, double phi_deg, double thickness, int loc);
+
+/* ----------------------------------------------------------------- solvers */
+enum { VFG_PC_JACOBI = 0, VFG_PC_CHEBYSHEV = 1 };
+enum { VFG_PREC_FP64 = 0, VFG_PREC_MIXED = 1 };
+typedef struct {
+  double rtol;      /* relative residual tolerance (default 1e-10) */
+  double atol;      /* absolute residual tolerance (default 0) */
+  int    maxit;     /* iteration cap (default 100000) */
+  int    pc;        /* VFG_PC_* */
+  int    precision; /* VFG_PREC_* : mixed = FP32 inner solves inside FP64
+                       iterative refinement */
+  int    cheb_degree;
+} vfg_solve_opts;
+typedef struct {
+  int    iterations;  /* inner iterations (CG steps) */
+  int    outer;       /* refinement / active-set iterations */
+  double relres;      /* final FP64 relative residual */
+  int    status;      /* 0 or VFG_E_* */
+} vfg_solve_info;
+void vfg_solve_opts_default(vfg_solve_opts *o);
+/* VF_StepU (VFMech.c:1436-1451): one Newton step of SNESNEWTONLS with KSPCG
+ * on the linear u-problem.  U must carry its Dirichlet values on entry
+ * (VFTimeStepPrepare, VFCommon.c:1132-1157). */
+int vfg_u_solve(vfg_handle *h, double *U, const double *BCU, const vfg_solve_opts *o,
+                vfg_solve_info *info, int loc);
+/* VF_StepV (VFMech.c:3057-3082) with SNESVINEWTONRSLS semantics: bounds
+ * 0 <= V <= ub (ub = NULL -> 1), Dirichlet and AT2-irreversible nodes held. */
+int vfg_v_solve(vfg_handle *h, const double *U, double *V, const double *ub,
+                const vfg_solve_opts *o, vfg_solve_info *info, int loc);
+
+/* ------------------------------------------- device memory for callers */
+int vfg_device_alloc(size_t bytes, void **ptr);
+int vfg_device_free(void *ptr);
+/* kind: 0 host->device, 1 device->host, 2 device->device (synchronous) */
+int vfg_memcpy(void *dst, const void *src, size_t bytes, int kind);
+
+/* --------------------------------------------------------------- timing */
+/* Runs `nrep` u-applies (device pointers; pairs (x[i % nbuf], y[i % nbuf]))
+ * on the handle's stream between two HIP events; returns total ms. */
+int vfg_time_u_apply(vfg_handle *h, const double *const *x, double *const *y, int nbuf,
+                     int nrep, int fp32, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -559,7 +559,10 @@ int vfo_bc_parse_options(const char *text, const char *prefix, int dof, vfo_bc *
     key[n] = 0;
     while (*p && (*p == ' ' || *p == '\t')) p++;
     n = 0;
-    if (*p && *p != '\n' && *p != '-') {
+    /* a value follows unless the next token is another option ("-" + letter);
+       negative numbers ("-1.", "-.5") are values */
+    int is_key = *p == '-' && (isalpha((unsigned char)p[1]) || p[1] == '_');
+    if (*p && *p != '\n' && !is_key) {
       while (*p && !isspace((unsigned char)*p) && n < 255) val[n++] = *p++;
     }
     val[n] = 0;

@@ -1,0 +1,139 @@
+"""GPU parity of the u-/V-operators against the CPU oracle (FP64).
+
+Bar: Dirichlet masks and BC vectors bit-exact; operator products, diagonals,
+load vectors, residuals, energies and crack volume within 1e-12 relative
+(2-norm) of the oracle's assembled FP64 operators; FP32 apply within 2e-6.
+"""
+import numpy as np
+import pytest
+
+from tests import scenarios as S
+from tests.scenarios import relerr
+
+pytestmark = pytest.mark.gpu
+
+BC_SETS = [
+    ("clamped", S.CLAMPED, S.NONE_V, 1),
+    ("test1", S.TEST1_U, S.TEST1_V, 1),
+    ("test2", S.TEST2_U, S.NONE_V, 2),
+    ("mixed", S.MIXED_U, S.MIXED_V, 2),
+]
+
+
+@pytest.fixture(params=BC_SETS, ids=[b[0] for b in BC_SETS])
+def pair(request, oracle):
+    name, bu, bv, at = request.param
+    s = S.make(n=(9, 8, 7), seed=sum(map(ord, name)), bcU=bu, bcV=bv, atnum=at)
+    op, F = S.to_oracle(oracle, s)
+    P = S.to_gpu(s)
+    yield s, op, F, P
+    P.close()
+
+
+def test_masks_bit_exact(pair, oracle):
+    s, op, F, P = pair
+    assert np.array_equal(P.bc_mask("U"), oracle.bc_mask(op, "U"))
+    assert np.array_equal(P.bc_mask("V"), oracle.bc_mask(op, "V"))
+
+
+def test_bc_vectors_exact(pair, oracle):
+    s, op, F, P = pair
+    base = np.arange(3 * op.nn, dtype=np.float64) * 0.001
+    assert np.array_equal(P.VecSetFromBC("U", base.copy()), oracle.vec_set_from_bc(op, "U", base.copy()))
+    B = oracle.vec_set_from_bc(op, "U") * 1.7
+    got = P.VecApplyDirichletBC(s["U"].copy(), B, "U")
+    ref = oracle.vec_apply_dirichlet(op, s["U"].copy(), B, "U")
+    assert np.array_equal(got, ref)
+    gv = P.VecApplyDirichletBC(s["V"].copy(), s["V"], "V")
+    rv = oracle.vec_apply_dirichlet(op, s["V"].copy(), s["V"].copy(), "V")
+    assert np.array_equal(gv, rv)
+
+
+def test_u_apply_matches_assembled(pair, oracle):
+    s, op, F, P = pair
+    A = oracle.u_jacobian(op, F)
+    y_ref = A @ s["x"]
+    y = P.UIJacobian_mult(s["x"])
+    assert relerr(y, y_ref) < 1e-12
+    # Dirichlet rows are identity, columns are kept (VFCartFE.c:1293)
+    m = oracle.bc_mask(op, "U").astype(bool)
+    assert np.array_equal(y[m], s["x"][m])
+
+
+def test_u_apply_f32(pair, oracle):
+    s, op, F, P = pair
+    A = oracle.u_jacobian(op, F)
+    y = P.UIJacobian_mult(s["x"].astype(np.float32), fp32=True)
+    assert relerr(y, A @ s["x"].astype(np.float32).astype(np.float64)) < 2e-6
+
+
+def test_u_apply_deterministic(pair):
+    s, op, F, P = pair
+    y1 = P.UIJacobian_mult(s["x"])
+    y2 = P.UIJacobian_mult(s["x"])
+    assert np.array_equal(y1, y2)
+
+
+def test_u_diag(pair, oracle):
+    s, op, F, P = pair
+    A = oracle.u_jacobian(op, F)
+    assert relerr(P.u_diag(), A.diagonal()) < 1e-12
+
+
+def test_u_rhs_and_residual(pair, oracle):
+    s, op, F, P = pair
+    assert relerr(P.u_rhs(), oracle.u_rhs(op, F)) < 1e-12
+    BCU = oracle.vec_set_from_bc(op, "U") * 0.3
+    r_ref = oracle.u_residual(op, F, s["U"], BCU)
+    assert relerr(P.VF_UResidual(s["U"], BCU), r_ref) < 1e-12
+
+
+def test_v_operator(pair, oracle):
+    s, op, F, P = pair
+    P.VIJacobian_setup(s["U"])
+    J = oracle.v_jacobian(op, F, s["U"])
+    assert relerr(P.VIJacobian_mult(s["xv"]), J @ s["xv"]) < 1e-12
+    assert relerr(P.v_diag(), J.diagonal()) < 1e-12
+    assert relerr(P.v_rhs(), oracle.v_rhs(op, F, s["U"])) < 1e-12
+    assert relerr(P.VF_VResidual(s["V"]), oracle.v_residual(op, F, s["U"], s["V"])) < 1e-12
+
+
+def test_energies_and_crack_volume(pair, oracle):
+    s, op, F, P = pair
+    el, ins, pw, se = P.energies(s["U"], s["V"])
+    rel, rpw = oracle.u_energy(op, F, s["U"])
+    assert ins == 0.0
+    assert abs(el - rel) <= 1e-12 * abs(rel)
+    assert abs(pw - rpw) <= 1e-12 * max(abs(rpw), 1e-300)
+    assert abs(se - oracle.v_energy(op, s["V"])) <= 1e-12 * abs(oracle.v_energy(op, s["V"]))
+    vol, cf = P.VolumetricCrackOpening(s["U"], s["V"], cellfield=True)
+    rvol, rcf = oracle.crack_volume(op, s["U"], s["V"], cellfield=True)
+    hvol = np.prod([s["l"][d] / (s["n"][d] - 1) for d in range(3)])
+    assert abs(vol - rvol) <= 1e-12 * np.abs(rcf).sum() * hvol
+    assert relerr(cf, rcf) < 1e-12
+
+
+@pytest.mark.parametrize("atnum", [1, 2])
+def test_penny_v(oracle, atnum):
+    s = S.make(n=(17, 15, 13), atnum=atnum)
+    op, F = S.to_oracle(oracle, s)
+    P = S.to_gpu(s)
+    for c in [dict(center=(.5, .5, .5), r=.2, theta=0., phi=0., thickness=.09375),
+              dict(center=(.41, .55, .47), r=.23, theta=37., phi=71., thickness=.1)]:
+        ref = oracle.penny_v(op, c["center"], c["r"], c["theta"], c["phi"], c["thickness"])
+        got = P.penny_v(c["center"], c["r"], c["theta"], c["phi"], c["thickness"])
+        assert np.max(np.abs(got - ref)) < 1e-13
+    P.close()
+
+
+def test_larger_grid_apply(oracle):
+    """A grid spanning several workgroup tiles and z-chunks in every direction."""
+    s = S.make(n=(37, 33, 41), seed=7, bcU=S.MIXED_U)
+    op, F = S.to_oracle(oracle, s)
+    P = S.to_gpu(s)
+    A = oracle.u_jacobian(op, F)
+    assert relerr(P.UIJacobian_mult(s["x"]), A @ s["x"]) < 1e-12
+    P.VIJacobian_setup(s["U"])
+    J = oracle.v_jacobian(op, F, s["U"])
+    assert relerr(P.VIJacobian_mult(s["xv"]), J @ s["xv"]) < 1e-12
+    P.close()

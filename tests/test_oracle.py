@@ -1,0 +1,419 @@
+"""Pins the CPU oracle (oracle/vforacle.c) before it is trusted as the checker.
+
+Each check is chosen so that a plausible slip (dropped term, wrong sign or
+index, transposed operand) fails: closed forms, an independent numpy
+implementation with a different quadrature rule, brute-force assembly on tiny
+grids, KKT conditions, and the reference's own regression outputs
+(ValidationTests/results/runtest1.out, runtest2.out; tests/golden/).
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+from tests import scenarios as S
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "runtest_goldens.json")))
+
+
+# ----------------------------------------------------------- independent FE
+def gauss(npt):
+    x, w = np.polynomial.legendre.leggauss(npt)
+    return (x + 1) / 2, w / 2
+
+
+def ref_elem(h, npt):
+    """phi (8,G), dphi (8,3,G), W (G) with an npt-point Gauss rule, g = gi + npt*(gj + npt*gk)."""
+    xi, wi = gauss(npt)
+    G = npt ** 3
+    phi = np.zeros((8, G)); dphi = np.zeros((8, 3, G)); W = np.zeros(G)
+    g = 0
+    for gk in range(npt):
+        for gj in range(npt):
+            for gi in range(npt):
+                W[g] = wi[gi] * wi[gj] * wi[gk] * h[0] * h[1] * h[2]
+                p = [xi[gi], xi[gj], xi[gk]]
+                for a in range(8):
+                    ia, ja, ka = a & 1, (a >> 1) & 1, a >> 2
+                    f = [p[0] if ia else 1 - p[0], p[1] if ja else 1 - p[1], p[2] if ka else 1 - p[2]]
+                    d = [(1 if ia else -1) / h[0], (1 if ja else -1) / h[1], (1 if ka else -1) / h[2]]
+                    phi[a, g] = f[0] * f[1] * f[2]
+                    dphi[a, 0, g] = d[0] * f[1] * f[2]
+                    dphi[a, 1, g] = f[0] * d[1] * f[2]
+                    dphi[a, 2, g] = f[0] * f[1] * d[2]
+                g += 1
+    return phi, dphi, W
+
+
+def ref_Ke(v8, lam, mu, eta, h, npt):
+    """B-matrix stiffness: sum_g W s B^T C B (Voigt), independent of VFMech.c's loop form."""
+    phi, dphi, W = ref_elem(h, npt)
+    C = np.array([[lam + 2 * mu, lam, lam, 0, 0, 0], [lam, lam + 2 * mu, lam, 0, 0, 0],
+                  [lam, lam, lam + 2 * mu, 0, 0, 0], [0, 0, 0, mu, 0, 0], [0, 0, 0, 0, mu, 0],
+                  [0, 0, 0, 0, 0, mu]])
+    K = np.zeros((24, 24))
+    for g in range(len(W)):
+        s = (v8 @ phi[:, g]) ** 2 + eta
+        B = np.zeros((6, 24))
+        for a in range(8):
+            dx, dy, dz = dphi[a, :, g]
+            B[0, 3 * a] = dx; B[1, 3 * a + 1] = dy; B[2, 3 * a + 2] = dz
+            B[3, 3 * a] = dy; B[3, 3 * a + 1] = dx
+            B[4, 3 * a + 1] = dz; B[4, 3 * a + 2] = dy
+            B[5, 3 * a] = dz; B[5, 3 * a + 2] = dx
+        K += W[g] * s * B.T @ C @ B
+    return K
+
+
+def oracle_Ke(O, v8, lam, mu, eta, h):
+    e = O.elem(*h)
+    K = np.zeros(576)
+    v = np.ascontiguousarray(v8, np.float64)
+    O.lib().vfo_bilinear_u(O._dp(K), O._dp(v), lam, mu, eta, e)
+    return K.reshape(24, 24)
+
+
+# ------------------------------------------------------------------ basis
+def test_basis_tables(oracle):
+    h = (0.3, 0.7, 1.9)
+    e = oracle.elem(*h)
+    W = np.array(e.weight)
+    phi = np.array(e.phi).reshape(8, 27)          # [k][j][i] flattened = a
+    dphi = np.array(e.dphi).reshape(8, 3, 27)
+    assert abs(W.sum() - h[0] * h[1] * h[2]) < 1e-15
+    assert np.allclose(phi.sum(0), 1.0, atol=1e-15)
+    assert np.allclose(dphi.sum(0), 0.0, atol=1e-14)
+    rphi, rdphi, rW = ref_elem(h, 3)              # Gauss-Legendre 3-pt, closed form
+    assert np.allclose(W, rW, rtol=1e-15, atol=0)
+    assert np.allclose(phi, rphi, atol=1e-15)
+    assert np.allclose(dphi, rdphi, atol=1e-14)
+
+
+# ---------------------------------------------------------- u-stiffness
+def test_Ke_against_independent_quadrature(oracle):
+    rng = np.random.default_rng(1)
+    h = (0.25, 0.4, 0.3)
+    v8 = rng.uniform(0, 1, 8)
+    lam, mu, eta = 0.35, 0.8, 1e-5
+    K = oracle_Ke(oracle, v8, lam, mu, eta, h)
+    assert np.allclose(K, K.T, rtol=0, atol=1e-14 * np.abs(K).max())
+    # integrand is degree <= 4 per direction: a 5-point rule gives the same matrix
+    K5 = ref_Ke(v8, lam, mu, eta, h, 5)
+    assert np.abs(K - K5).max() < 1e-13 * np.abs(K).max()
+    # a 2-point rule is NOT exact for s(v) = v^2 + eta (why the element uses 27 points)
+    K2 = ref_Ke(v8, lam, mu, eta, h, 2)
+    assert np.abs(K - K2).max() > 1e-6 * np.abs(K).max()
+
+
+def test_Ke_constant_s_and_rigid_modes(oracle):
+    h = (0.5, 0.5, 0.5)
+    lam, mu, eta = 0.2, 0.5, 1e-5
+    K1 = oracle_Ke(oracle, np.ones(8), lam, mu, eta, h)
+    Kc = oracle_Ke(oracle, np.full(8, 0.6), lam, mu, eta, h)
+    assert np.allclose(Kc, K1 * (0.36 + eta) / (1 + eta), rtol=1e-13, atol=1e-16)
+    # textbook Q1 stiffness is exact with 2x2x2 Gauss when s is constant
+    assert np.abs(K1 - ref_Ke(np.ones(8), lam, mu, eta, h, 2)).max() < 1e-13 * np.abs(K1).max()
+    X = np.array([[a & 1, (a >> 1) & 1, a >> 2] for a in range(8)], float) * 0.5
+    modes = []
+    for c in range(3):
+        m = np.zeros(24); m[c::3] = 1; modes.append(m)
+    for (p, q) in ((0, 1), (1, 2), (0, 2)):   # infinitesimal rotations
+        m = np.zeros(24); m[p::3] = -X[:, q]; m[q::3] = X[:, p]; modes.append(m)
+    for m in modes:
+        assert np.abs(K1 @ m).max() < 1e-13
+    assert np.linalg.matrix_rank(K1, tol=1e-10) == 18
+
+
+# -------------------------------------------------------- load vectors
+def test_rhs_linearity_and_pressure_work_identity(oracle):
+    s = S.make(n=(6, 5, 7), seed=2, bcU=S.CLAMPED)
+    op, F = S.to_oracle(oracle, s)
+    b = oracle.u_rhs(op, F)
+    s2 = dict(s); s2["pressure"] = 2 * s["pressure"]; s2["theta"] = 2 * s["theta"] - s["thetaRef"]
+    op2, F2 = S.to_oracle(oracle, s2)
+    assert np.allclose(oracle.u_rhs(op2, F2), 2 * b, rtol=1e-12, atol=1e-18)
+    # uniform p: pressure work = p * crack volume (VFMech.c:963-966 vs VFPermfield.c:60)
+    s3 = dict(s); s3["pressure"] = np.full(op.nn, 1.7e-3)
+    op3, F3 = S.to_oracle(oracle, s3)
+    _, pw = oracle.u_energy(op3, F3, s["U"])
+    cv = oracle.crack_volume(op3, s["U"], s["V"])
+    assert abs(pw - 1.7e-3 * cv) < 1e-12 * abs(pw) + 1e-20
+
+
+def test_energy_density_uniaxial(oracle):
+    """W = 1/2 E eps^2 for uniaxial strain at nu = 0 (VFMech.c:32-106)."""
+    h = (0.5, 0.5, 0.5)
+    eps0 = 0.01
+    u = np.zeros(24)
+    for a in range(8):
+        u[3 * a] = eps0 * (a & 1) * 0.5
+    e = oracle.elem(*h)
+    W = np.zeros(27)
+    oracle.lib().vfo_elastic_energy_density(oracle._dp(W), oracle._dp(u), oracle._dp(np.zeros(8)),
+                                           0.0, 1.5, 0.0, e)          # E = 3 -> mu = 1.5
+    assert np.allclose(W, 0.5 * 3.0 * eps0 ** 2, rtol=1e-13)
+    # thermal strain enters the diagonal: uniform expansion alpha*dth cancels it
+    dth = np.full(8, eps0 / 1e-2)
+    oracle.lib().vfo_elastic_energy_density(oracle._dp(W), oracle._dp(u), oracle._dp(dth),
+                                           0.0, 1.5, 1e-2, e)
+    assert np.allclose(W, 1.5 * 2 * eps0 ** 2, rtol=1e-12)  # mu * (2 lateral strains of -eps0)^2
+
+
+def test_elastic_energy_uniaxial_bar(oracle):
+    """0.2 t^2 for the runtest1 bar before cracking (closed form of runtest1 steps 1-3)."""
+    g = GOLD["runtest1"]["steps"]
+    for st in g[1:4]:
+        assert abs(st["elastic"] - 0.2 * st["t"] ** 2) < 1e-9
+
+
+def test_AT_matrix_closed_form(oracle):
+    h = (0.3, 0.2, 0.5)
+    e = oracle.elem(*h)
+    Gc, eps = 1.3, 0.07
+    M1 = lambda l: l / 6 * np.array([[2., 1.], [1., 2.]])
+    L1 = lambda l: 1 / l * np.array([[1., -1.], [-1., 1.]])
+    Mx, My, Mz = M1(h[0]), M1(h[1]), M1(h[2])
+    Lx, Ly, Lz = L1(h[0]), L1(h[1]), L1(h[2])
+    M = np.kron(Mz, np.kron(My, Mx))
+    L = np.kron(Mz, np.kron(My, Lx)) + np.kron(Mz, np.kron(Ly, Mx)) + np.kron(Lz, np.kron(My, Mx))
+    for atnum, atcv in ((1, 2 / 3), (2, .5)):
+        K = np.zeros(64)
+        oracle.lib().vfo_bilinear_v_at(oracle._dp(K), atnum, Gc, atcv, eps, e)
+        c = Gc / atcv * .5
+        ref = c * eps * L + (c / eps * M if atnum == 2 else 0)
+        assert np.abs(K.reshape(8, 8) - ref).max() < 1e-13 * np.abs(ref).max()
+        b = np.zeros(8)
+        oracle.lib().vfo_residual_v_at(oracle._dp(b), atnum, Gc, atcv, eps, e)
+        cr = Gc / atcv / eps * (.5 if atnum == 2 else .25)
+        assert np.allclose(b, cr * h[0] * h[1] * h[2] / 8, rtol=1e-13)
+
+
+# --------------------------------------------------- assembly and masks
+def test_assembly_brute_force(oracle):
+    s = S.make(n=(3, 3, 3), seed=4, bcU=S.MIXED_U, bcV=S.MIXED_V, atnum=2)
+    op, F = S.to_oracle(oracle, s)
+    A = oracle.u_jacobian(op, F).toarray()
+    nn = op.nn
+    ref = np.zeros((3 * nn, 3 * nn))
+    lam1 = s["nu"] / (1 + s["nu"]) / (1 - 2 * s["nu"])
+    mu1 = 1 / (1 + s["nu"]) * .5
+    h = [s["l"][d] / (s["n"][d] - 1) for d in range(3)]
+    for ek in range(2):
+        for ej in range(2):
+            for ei in range(2):
+                cell = ei + 2 * (ej + 2 * ek)
+                nodes = [(ei + (a & 1)) + 3 * ((ej + ((a >> 1) & 1)) + 3 * (ek + (a >> 2))) for a in range(8)]
+                E = s["E_elem"][cell]
+                K = ref_Ke(s["V"][nodes], E * lam1, E * mu1, s["eta"], h, 5)
+                idx = np.array([3 * n + c for n in nodes for c in range(3)])
+                ref[np.ix_(idx, idx)] += K
+    # mask by formula: a dof is Dirichlet iff it lies on a non-NONE face or vertex
+    mask = np.zeros(3 * nn, bool)
+    for n in range(nn):
+        i, j, k = n % 3, (n // 3) % 3, n // 9
+        for c in range(3):
+            faces, _, verts = s["bcU"][c]
+            on = {0: i == 0, 1: i == 2, 2: j == 0, 3: j == 2, 4: k == 0, 5: k == 2}
+            m = any(on[f] for f, t in faces.items() if t)
+            vid = (i == 2) + 2 * (j == 2) + 4 * (k == 2)
+            if i in (0, 2) and j in (0, 2) and k in (0, 2) and verts.get(vid, 0):
+                m = True
+            mask[3 * n + c] = m
+    assert np.array_equal(oracle.bc_mask(op, "U").astype(bool), mask)
+    ref[mask, :] = 0
+    ref[mask, mask] = 1
+    assert np.abs(A - ref).max() < 1e-13 * np.abs(ref).max()
+    # V-Jacobian: rows and columns of fixed nodes zeroed (Dirichlet + AT2 irreversibility)
+    J = oracle.v_jacobian(op, F, s["U"]).toarray()
+    fixed = oracle.bc_mask(op, "V").astype(bool) | (s["VIrrev"] <= s["irrevtol"])
+    assert np.allclose(J, J.T, atol=1e-14 * np.abs(J).max())
+    assert np.all(J[fixed][:, ~fixed] == 0) and np.all(J[~fixed][:, fixed] == 0)
+    assert np.all(np.diag(J)[fixed] == 1.0)
+
+
+def test_residual_identities(oracle):
+    """F_u = A U - b on free rows, U - BC on Dirichlet rows; F_V = K V - b_V (quadratic)."""
+    s = S.make(n=(5, 6, 4), seed=8, bcU=S.MIXED_U, bcV=S.MIXED_V, atnum=2)
+    op, F = S.to_oracle(oracle, s)
+    A = oracle.u_jacobian(op, F)
+    BCU = oracle.vec_set_from_bc(op, "U") * 0.4
+    r = oracle.u_residual(op, F, s["U"], BCU)
+    m = oracle.bc_mask(op, "U").astype(bool)
+    ref = A @ s["U"] - oracle.u_rhs(op, F)
+    assert np.abs(r[~m] - ref[~m]).max() < 1e-12 * np.abs(ref).max()
+    Bv = oracle.vec_apply_dirichlet(op, s["U"].copy(), BCU, "U")
+    assert np.allclose(r[m], (s["U"] - Bv)[m], atol=1e-15)
+    rv = oracle.v_residual(op, F, s["U"], s["V"])
+    q = dict(s); q["bcV"] = S.NONE_V; q["irrevtol"] = -1.0
+    opq, Fq = S.to_oracle(oracle, q)
+    K = oracle.v_jacobian(opq, Fq, s["U"])
+    fixed = oracle.bc_mask(op, "V").astype(bool) | (s["VIrrev"] <= s["irrevtol"])
+    refv = K @ s["V"] - oracle.v_rhs(op, F, s["U"])
+    assert np.abs(rv[~fixed] - refv[~fixed]).max() < 1e-12 * np.abs(refv).max()
+
+
+def test_bcfile_parser(oracle):
+    txt = open(os.path.join(os.path.dirname(__file__), "golden", "bcfiles", "3DClamped.txt")).read()
+    bc = oracle.parse_bc_options(txt, "U", 3)
+    for c in range(3):
+        assert list(bc[c].face) == [oracle.ZERO] * 6
+        assert list(bc[c].vertex) == [0] * 8
+    txt = open(os.path.join(os.path.dirname(__file__), "golden", "bcfiles", "3DRigidMotion.txt")).read()
+    bc = oracle.parse_bc_options(txt, "U", 3)
+    assert list(bc[0].face) == [0] * 6
+    assert bc[0].vertex[0] == oracle.ZERO and bc[2].vertex[1] == oracle.ZERO
+    assert bc[0].vertex[2] == oracle.ZERO and bc[1].vertex[4] == oracle.ZERO
+    v = oracle.parse_bc_options("-V_X0_BC ONE\n-V_X1_BC ONE\n-U_X1_0 0.5\n", "V", 1)
+    assert v[0].face[0] == oracle.ONE and v[0].face[1] == oracle.ONE
+    u = oracle.parse_bc_options("-U_X0_BC_0 FIXED -U_X0_0 -1.", "U", 3)
+    assert u[0].face[0] == oracle.FIXED and u[0].faceValue[0] == -1.0
+
+
+# ------------------------------------------------------------- cracks
+def test_penny_distance_closed_forms(oracle):
+    L = oracle.lib()
+    c = np.array([.5, .5, .5])
+    d = lambda x, th, ph, r=.2: L.vfo_distance_to_penny(oracle._dp(np.array(x, float)), oracle._dp(c), r, th, ph)
+    assert abs(d([.5, .5, .73], 0., 0.) - .23) < 1e-15          # on axis
+    assert abs(d([.9, .5, .5], 0., 0.) - .2) < 1e-15            # in plane, outside rim
+    assert abs(d([.6, .5, .5], 0., 0.)) < 1e-15                 # on the disc
+    assert abs(d([.9, .5, .8], 0., 0.) - math.hypot(.3, .2)) < 1e-15
+    assert abs(d([.73, .5, .5], 0., 90.) - .23) < 1e-15         # normal along x
+
+
+def test_penny_profiles(oracle):
+    op = oracle.OracleProblem((33, 33, 33), epsilon=0.0625, atnum=1)
+    V = oracle.penny_v(op, (.5, .5, .5), .2, 0., 0., .09375)
+    z = np.arange(33) / 32.
+    col = V.reshape(33, 33, 33)[:, 16, 16]
+    dist = np.abs(z - .5) - .09375 / 2
+    ref = np.where(dist <= 0, 0., np.where(dist < .125, dist / .0625 * (1 - .25 * dist / .0625), 1.))
+    assert np.allclose(col, ref, atol=1e-15)
+    op2 = oracle.OracleProblem((33, 33, 33), epsilon=0.0625, atnum=2)
+    V2 = oracle.penny_v(op2, (.5, .5, .5), .2, 0., 0., .09375)
+    ref2 = np.where(dist <= 0, 0., 1 - np.exp(-dist / 2 / .0625))
+    assert np.allclose(V2.reshape(33, 33, 33)[:, 16, 16], ref2, atol=1e-15)
+
+
+# ------------------------------------------------------------- solvers
+def test_u_solve_equals_direct(oracle):
+    s = S.make(n=(7, 6, 5), seed=9, bcU=S.MIXED_U)
+    op, F = S.to_oracle(oracle, s)
+    BCU = oracle.vec_set_from_bc(op, "U") * 0.01
+    U = oracle.vec_apply_dirichlet(op, np.zeros(3 * op.nn), BCU, "U")
+    oracle.u_solve(op, F, U, BCU, rtol=1e-14)
+    A = oracle.u_jacobian(op, F)
+    b = oracle.u_rhs(op, F)
+    m = oracle.bc_mask(op, "U").astype(bool)
+    rhs = np.where(m, oracle.vec_apply_dirichlet(op, np.zeros(3 * op.nn), BCU, "U"), b)
+    Ud = spla.spsolve(A.tocsc(), rhs)
+    assert np.abs(U - Ud).max() < 1e-10 * np.abs(Ud).max()
+
+
+@pytest.mark.parametrize("atnum", [1, 2])
+def test_v_solve_kkt(oracle, atnum):
+    s = S.make(n=(8, 7, 6), seed=11, atnum=atnum, bcV=S.TEST1_V if atnum == 1 else S.MIXED_V)
+    s["U"] *= 40.0
+    op, F = S.to_oracle(oracle, s)
+    ub = s["VIrrev"] if atnum == 1 else np.ones(op.nn)
+    V = s["V"].copy()
+    it = oracle.v_solve(op, F, s["U"], V, ub=ub if atnum == 1 else None, tol=1e-13)
+    assert it >= 0
+    fixed = oracle.bc_mask(op, "V").astype(bool)
+    if atnum == 2:
+        fixed |= s["VIrrev"] <= s["irrevtol"]
+    free = ~fixed
+    q = dict(s); q["bcV"] = S.NONE_V; q["irrevtol"] = -1.0
+    opq, Fq = S.to_oracle(oracle, q)
+    g = oracle.v_jacobian(opq, Fq, s["U"]) @ V - oracle.v_rhs(op, F, s["U"])
+    assert np.all(V[free] >= -1e-15) and np.all(V[free] <= ub[free] + 1e-15)
+    pinned = free & (ub <= 1e-12)                    # lb = ub: any multiplier sign
+    lo = free & ~pinned & (V <= 1e-12)
+    hi = free & ~pinned & (V >= ub - 1e-12)
+    inn = free & ~lo & ~hi & ~pinned
+    scale = np.abs(g).max()
+    assert np.abs(g[inn]).max() < 1e-9 * scale       # stationarity on the inactive set
+    assert np.all(g[lo] >= -1e-9 * scale)            # multipliers have the right sign
+    assert np.all(g[hi] <= 1e-9 * scale)
+    if atnum == 1:
+        assert lo.sum() + hi.sum() > 0               # the AT1 bounds were active
+
+
+# ------------------------------------------------------ reference goldens
+def test_runtest2_goldens(oracle):
+    g = GOLD["runtest2"]
+    nx, ny, nz = 41, 81, 6
+    bcU = [oracle.make_bc({oracle.X0: oracle.ZERO}), oracle.make_bc({oracle.X1: oracle.ZERO}),
+           oracle.make_bc({oracle.Z0: oracle.ZERO, oracle.Z1: oracle.ZERO})]
+    op = oracle.OracleProblem((nx, ny, nz), (1., 2., .25), E=1., nu=0., alpha=0., beta=0., Gc=1.,
+                              eta=1e-8, epsilon=.05, atnum=2, bcU=bcU, bcV=oracle.make_bc())
+    nn = op.nn
+    X = np.array([0.] + [i * 1. / (nx - 1.) for i in range(1, nx)])
+    i = np.arange(nn) % nx
+    j = (np.arange(nn) // nx) % ny
+    VIrrev = np.ones(nn)
+    VIrrev[((j == ny // 2) | (j == ny // 2 - 1)) & (X[i] < .4)] = 0.
+    V = VIrrev.copy()
+    U = np.zeros(3 * nn)
+    zero = np.zeros(nn)
+    p = np.full(nn, 1e-3)
+    F = op.fields(V=V, VIrrev=VIrrev, theta=zero, thetaRef=zero, pressure=p)
+    oracle.v_solve(op, F, U, V, tol=1e-12)
+    op.P.hasCrackPressure = 1
+    F = op.fields(V=V, VIrrev=VIrrev, theta=zero, thetaRef=zero, pressure=p)
+    oracle.u_solve(op, F, U, np.zeros(3 * nn), rtol=1e-12, maxit=200000)
+    el, pw = oracle.u_energy(op, F, U)
+    se = oracle.v_energy(op, V)
+    assert abs(se - g["surface"]) < 5e-7 * g["surface"]
+    assert abs(el - g["elastic"]) < 5e-7 * g["elastic"]
+    assert abs(pw - g["pressure_work"]) < 5e-7 * g["pressure_work"]
+
+
+def run_test1(O, steps=11):
+    """ValidationTests/test1.c with the runtest1 options (ValidationTests/makefile:198)."""
+    bcU = [O.make_bc({O.X0: O.FIXED, O.X1: O.FIXED}, {O.X0: -1., O.X1: 1.}),
+           O.make_bc({O.X0: O.ZERO, O.X1: O.ZERO}), O.make_bc({O.X0: O.ZERO, O.X1: O.ZERO})]
+    op = O.OracleProblem((101, 5, 5), (10., 1., 1.), E=1., nu=0., alpha=0., beta=0., Gc=.1, eta=1e-8,
+                         epsilon=.5, atnum=1, bcU=bcU, bcV=O.make_bc({O.X0: O.ONE, O.X1: O.ONE}))
+    nn = op.nn
+    V = np.ones(nn); U = np.zeros(3 * nn); zero = np.zeros(nn)
+    out = []
+    for ts in range(steps):
+        t = 4.0 / 10 * ts
+        BCU = O.vec_set_from_bc(op, "U") * t
+        VIrrev = V.copy()
+        O.vec_apply_dirichlet(op, U, BCU, "U")
+        O.vec_apply_dirichlet(op, V, V, "V")
+        it = 0
+        while True:
+            Vold = V.copy()
+            F = op.fields(V=V, VIrrev=VIrrev, theta=zero, thetaRef=zero, pressure=zero)
+            O.u_solve(op, F, U, BCU, rtol=1e-12)
+            F = op.fields(V=V, VIrrev=VIrrev, theta=zero, thetaRef=zero, pressure=zero)
+            O.v_solve(op, F, U, V, ub=VIrrev.copy(), tol=1e-11)
+            it += 1
+            if not (np.abs(V - Vold).max() >= 1e-3 and it <= 10000):
+                break
+        cv = O.crack_volume(op, U, V)
+        F = op.fields(V=V, VIrrev=V.copy(), theta=zero, thetaRef=zero, pressure=zero)
+        el, _ = O.u_energy(op, F, U)
+        out.append(dict(t=t, altmin=it, crack_opening=cv, elastic=el, surface=O.v_energy(op, V)))
+    return out
+
+
+def test_runtest1_goldens(oracle):
+    """Steps 0-5 to all 7 printed digits; later steps follow the same crack
+    within the reference's loose alternate-minimisation tolerance (1e-3)."""
+    gold = GOLD["runtest1"]["steps"]
+    res = run_test1(oracle)
+    for r, g in zip(res, gold):
+        assert r["altmin"] == g["altmin"] or g["step"] >= 6
+        if g["step"] <= 5:
+            for k in ("elastic", "surface", "crack_opening"):
+                assert abs(r[k] - g[k]) <= 6e-7 * abs(g[k]) + 1e-6 * (k == "crack_opening"), (g["step"], k)
+        else:
+            assert abs(r["surface"] - g["surface"]) < 1e-4 * g["surface"]
+            assert abs(r["crack_opening"] - g["crack_opening"]) < 2e-4 * g["crack_opening"]

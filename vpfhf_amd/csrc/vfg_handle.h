@@ -1,0 +1,30 @@
+// vfg_handle.h -- the opaque vfg_handle (internal).
+#pragma once
+#include <string>
+#include <vector>
+#include "vfg_kernels.h"
+
+struct vfg_handle {
+  vfg::DevProblem P;
+  vfg_grid grid;
+  vfg_props props;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::vector<void *> allocs;
+  size_t bytes = 0;
+  double *stage = nullptr;  // staging for host-pointer calls
+  int64_t stage_n = 0;
+  double *part = nullptr;      // reduction partials (4 x 65536)
+  double *part_big = nullptr;  // partials for very large grids
+  double *scal = nullptr;      // device scalars (64)
+  bool v_ready = false;
+  // solver work space (allocated on first use)
+  double *w[12] = {nullptr};
+  float *wf[8] = {nullptr};
+  uint8_t *act = nullptr;
+};
+
+int vfg_fail(vfg_handle *h, int code, const char *fmt, ...);
+int vfg_alloc(vfg_handle *h, void **p, size_t bytes);
+int vfg_scratch(vfg_handle *h, int64_t ndoubles);
+int vfg_in(vfg_handle *h, void *dst, const void *src, size_t bytes, int loc);

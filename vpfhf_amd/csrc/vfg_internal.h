@@ -1,0 +1,181 @@
+// vfg_internal.h -- device-side building blocks shared by the vfg kernels.
+//
+// Element basis: trilinear Q1 on an axis-aligned box, 3-point Gauss-Legendre
+// per direction (27 points), exactly the rule of VFCartFEElement3DInit
+// (VFCartFE.c:90-116, 216-269).  The 27-point tables are never stored: every
+// quantity is evaluated by sum factorisation with the 1-D tables below.
+//
+// Node accumulation ("gather") without atomics: a workgroup owns a tile of
+// (TX-1) x (TY-1) node columns and marches upward through a z-chunk of node
+// planes.  One thread computes one element of the (TX x TY) element columns
+// that touch the owned nodes (a one-element halo is recomputed), element
+// contributions are combined across the x neighbour with a lane shuffle and
+// across the y neighbour through LDS, and the plane below the current element
+// layer is complete after each step.  Every node sums its 8 element
+// contributions in a fixed order, so results are bitwise reproducible.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vfg {
+
+// 1-D Gauss tables on [0,1]: P0[g] = phi_0(xi_g), P1[g] = phi_1(xi_g) and the
+// weights w[g] (times the element length), VFCartFE.c:100-113.
+#define VFG_SQ35 0.77459666924148337704  // sqrt(3/5)
+__host__ __device__ constexpr double gP0(int g) { return g == 0 ? (1. + VFG_SQ35) * .5 : g == 1 ? .5 : (1. - VFG_SQ35) * .5; }
+__host__ __device__ constexpr double gP1(int g) { return g == 0 ? (1. - VFG_SQ35) * .5 : g == 1 ? .5 : (1. + VFG_SQ35) * .5; }
+__host__ __device__ constexpr double gW(int g) { return g == 1 ? 8. / 18. : 5. / 18.; }
+
+struct Grid {
+  int nx, ny, nz;     // nodes
+  int nex, ney, nez;  // elements
+  double hx, hy, hz;  // uniform element size
+};
+
+__device__ __forceinline__ int64_t node_id(const Grid &G, int i, int j, int k) {
+  return (int64_t)i + (int64_t)G.nx * ((int64_t)j + (int64_t)G.ny * k);
+}
+__device__ __forceinline__ int64_t cell_id(const Grid &G, int i, int j, int k) {
+  return (int64_t)i + (int64_t)G.nex * ((int64_t)j + (int64_t)G.ney * k);
+}
+
+// Dirichlet row set of one component, computed from node indices
+// (MatApplyDirichletBC, VFCartFE.c:1147-1296): bit f of `face` set iff face f
+// is not NONE; bit v of `vert` set iff vertex v is not NONE.
+struct BcMask {
+  uint8_t face[3];
+  uint8_t vert[3];
+};
+
+__device__ __forceinline__ bool on_dirichlet(const Grid &G, uint8_t face, uint8_t vert, int i, int j,
+                                             int k) {
+  bool m = ((face & 1) && i == 0) || ((face & 2) && i == G.nx - 1) || ((face & 4) && j == 0) ||
+           ((face & 8) && j == G.ny - 1) || ((face & 16) && k == 0) || ((face & 32) && k == G.nz - 1);
+  if (vert) {
+    int ix = (i == 0) ? 0 : (i == G.nx - 1) ? 1 : -1;
+    int iy = (j == 0) ? 0 : (j == G.ny - 1) ? 1 : -1;
+    int iz = (k == 0) ? 0 : (k == G.nz - 1) ? 1 : -1;
+    if (ix >= 0 && iy >= 0 && iz >= 0 && ((vert >> (ix + 2 * iy + 4 * iz)) & 1)) m = true;
+  }
+  return m;
+}
+
+// ---------------------------------------------------------------- marching
+// Op interface:
+//   static constexpr int NF  -- staged nodal scalar fields
+//   static constexpr int NC  -- accumulated components per node
+//   __device__ T load(int f, int64_t node)  -- value of staged field f at a node
+//   __device__ void element(int ex,int ey,int ek, const T (&xin)[2][4][NF], T (&r)[8][NC])
+//   __device__ void store(int i,int j,int k, const T (&acc)[NC], const T (&own)[NF])
+// Corners of an element are a = (k*2+j)*2+i; xin[k][j*2+i][f].
+template <typename T, int TX, int TY, class Op>
+__device__ __forceinline__ void march(const Grid &G, const Op &op, int OZ) {
+  constexpr int NF = Op::NF, NC = Op::NC;
+  constexpr int NT = TX * TY, OX = TX - 1, OY = TY - 1, SX = TX + 1, SY = TY + 1;
+  __shared__ T s_in[NF][SY * SX];
+  __shared__ T s_red[2 * NC][NT];
+  const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
+  const int ox0 = blockIdx.x * OX, oy0 = blockIdx.y * OY, oz0 = blockIdx.z * OZ;
+  const int ex = ox0 - 1 + tx, ey = oy0 - 1 + ty;
+  const bool exy_ok = ex >= 0 && ex < G.nex && ey >= 0 && ey < G.ney;
+  const bool owner = tx < OX && ty < OY && (ox0 + tx) < G.nx && (oy0 + ty) < G.ny;
+
+  auto stage = [&](int iz) {
+    for (int n = tid; n < SX * SY; n += NT) {
+      int lx = n % SX, ly = n / SX;
+      int gx = ox0 - 1 + lx, gy = oy0 - 1 + ly;
+      bool ok = gx >= 0 && gx < G.nx && gy >= 0 && gy < G.ny && iz >= 0 && iz < G.nz;
+      int64_t nd = ok ? node_id(G, gx, gy, iz) : 0;
+#pragma unroll
+      for (int f = 0; f < NF; f++) s_in[f][n] = ok ? op.load(f, nd) : T(0);
+    }
+  };
+  T xin[2][4][NF];
+  T carry[NC];
+#pragma unroll
+  for (int c = 0; c < NC; c++) carry[c] = T(0);
+
+  stage(oz0 - 1);
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+#pragma unroll
+    for (int f = 0; f < NF; f++) xin[1][q][f] = s_in[f][(ty + (q >> 1)) * SX + tx + (q & 1)];
+
+  const int ek_end = min(oz0 + OZ - 1, G.nz - 1);
+  for (int ek = oz0 - 1; ek <= ek_end; ek++) {
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int f = 0; f < NF; f++) xin[0][q][f] = xin[1][q][f];
+    __syncthreads();
+    stage(ek + 1);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int f = 0; f < NF; f++) xin[1][q][f] = s_in[f][(ty + (q >> 1)) * SX + tx + (q & 1)];
+
+    T r[8][NC];
+#pragma unroll
+    for (int a = 0; a < 8; a++)
+#pragma unroll
+      for (int c = 0; c < NC; c++) r[a][c] = T(0);
+    if (exy_ok && ek >= 0 && ek < G.nez) op.element(ex, ey, ek, xin, r);
+
+    // x-combine: node (ex+1) gets corner i=1 of this element and corner i=0
+    // of the element to the right (next lane inside the 16-lane row).
+    T cx[2][2][NC];
+#pragma unroll
+    for (int kz = 0; kz < 2; kz++)
+#pragma unroll
+      for (int jy = 0; jy < 2; jy++)
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+          T right = __shfl_down(r[(kz * 2 + jy) * 2 + 0][c], 1, TX);
+          cx[kz][jy][c] = r[(kz * 2 + jy) * 2 + 1][c] + right;
+        }
+    // y-combine through LDS: node (ey+1) gets row jy=1 of this element row and
+    // row jy=0 of the element row above.
+#pragma unroll
+    for (int kz = 0; kz < 2; kz++)
+#pragma unroll
+      for (int c = 0; c < NC; c++) s_red[kz * NC + c][tid] = cx[kz][0][c];
+    __syncthreads();
+    if (owner) {
+      T bot[NC];
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        bot[c] = carry[c] + (cx[0][1][c] + s_red[c][tid + TX]);
+        carry[c] = cx[1][1][c] + s_red[NC + c][tid + TX];
+      }
+      if (ek >= oz0 && ek >= 0) op.store(ox0 + tx, oy0 + ty, ek, bot, xin[0][3]);
+    }
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T interp(int g, T f0, T f1) {
+  return T(gP0(g)) * f0 + T(gP1(g)) * f1;
+}
+
+// wave64 / block reductions in FP64 with a fixed order (deterministic)
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double *sh /* NT/64 */) {
+  v = wave_sum(v);
+  const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  double s = 0.;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < NT / 64; i++) s += sh[i];
+  __syncthreads();
+  return s;  // valid in thread 0
+}
+
+}  // namespace vfg

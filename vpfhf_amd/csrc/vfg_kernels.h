@@ -1,0 +1,68 @@
+// vfg_kernels.h -- device problem description and kernel launchers (internal).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "vfg_internal.h"
+#include "../../include/vfg.h"
+
+namespace vfg {
+
+// Everything a kernel needs about one problem.  Device pointers are owned by
+// the handle (vfg_api.hip); host copies of the BC tables travel by value.
+struct DevProblem {
+  Grid G;
+  int64_t nn, ne;
+  // properties (VFMatProp/VFProp); lam1/mu1 are the Lame constants for E = 1
+  double E, nu, lam1, mu1, alpha, beta, Gc, eta, eps, irrevtol, atcv;
+  int atnum, hasCrackPressure;
+  BcMask bcU, bcV;
+  vfg_bc hbcU[3], hbcV;
+  // fields
+  double *V, *VIrrev, *dth, *p, *E_elem;  // E_elem may be null (uniform E)
+  float *V32;                             // FP32 copy of V for the FP32 apply
+  double *zero3;                          // 3*nn zeros
+  // V-subproblem state, refreshed by vfg_v_set_u
+  double *Vq;      // [27][ne] W_g * (c_AT/eps [AT2] + 2 W^el_g)
+  double *bV;      // V load vector (nn)
+  uint8_t *vfix;   // fixed V rows/cols (Dirichlet + AT2 irreversibility)
+};
+
+// u operator (vfg_u.hip)
+void launch_u_apply(const DevProblem &P, const double *x, double *y, hipStream_t s);
+void launch_u_apply_f32(const DevProblem &P, const float *x, float *y, hipStream_t s);
+void launch_u_diag(const DevProblem &P, double *d, hipStream_t s);
+// mode 0: out = element load vector b; mode 1: out = VF_UResidual(U)
+void launch_u_rhs(const DevProblem &P, const double *U, const double *BCU, double *out, int mode,
+                  hipStream_t s);
+
+// V operator (vfg_v.hip)
+void launch_v_setup(const DevProblem &P, const double *U, hipStream_t s);  // Vq, bV
+void launch_v_fixmask(const DevProblem &P, hipStream_t s);                 // vfix
+// mode 0: masked Jacobian apply; mode 1: residual K x - bV with BC rows
+// mode 2: plain K x (no masks, no BC).  `mask` overrides vfix (reduced systems).
+void launch_v_apply(const DevProblem &P, const double *x, double *y, int mode, hipStream_t s,
+                    const uint8_t *mask = nullptr);
+void launch_v_diag(const DevProblem &P, double *d, hipStream_t s, const uint8_t *mask = nullptr);
+
+// misc (vfg_misc.hip)
+void launch_bc_mask(const DevProblem &P, int which, uint8_t *mask, hipStream_t s);
+void launch_vec_set_from_bc(const DevProblem &P, int which, double *B, hipStream_t s);
+void launch_vec_apply_dirichlet(const DevProblem &P, int which, double *vec, const double *B,
+                                hipStream_t s);
+// partial sums per block: out[b*4 + {elastic, pressure work, surface, crack volume}]
+int energy_blocks(const DevProblem &P);
+void launch_energies(const DevProblem &P, const double *U, const double *V, double *partials,
+                     double *cellfield, double *result4, hipStream_t s);
+void launch_penny(const DevProblem &P, double *V, const double c[3], double r, double theta,
+                  double phi, double thick, hipStream_t s);
+
+// vector kernels for the solvers (vfg_misc.hip); all reductions deterministic
+int red_blocks(int64_t n);
+void launch_dot(const double *a, const double *b, int64_t n, double *partials, double *out,
+                hipStream_t s);
+void launch_fill(double *x, double v, int64_t n, hipStream_t s);
+void launch_sub(const double *a, const double *b, double *out, int64_t n, hipStream_t s);
+void launch_copy_d2f(const double *x, float *y, int64_t n, hipStream_t s);
+void launch_copy_f2d(const float *x, double *y, int64_t n, hipStream_t s);
+
+}  // namespace vfg

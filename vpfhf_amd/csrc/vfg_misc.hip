@@ -1,0 +1,261 @@
+// vfg_misc.hip -- BC kit, energies / crack volume, penny cracks and the
+// vector kernels of the solvers.  All reductions are two-stage with a fixed
+// summation order, so results are bitwise reproducible run to run.
+#include "vfg_internal.h"
+#include "vfg_kernels.h"
+
+namespace vfg {
+
+// ----------------------------------------------------------------- BC kit
+struct BcTable {
+  uint8_t face[3][6], vert[3][8];
+  double faceValue[3][6], vertValue[3][8];
+  int dof;
+};
+static BcTable bc_table(const DevProblem &P, int which) {
+  BcTable t;
+  t.dof = which == 0 ? 3 : 1;
+  for (int c = 0; c < t.dof; c++) {
+    const vfg_bc &b = which == 0 ? P.hbcU[c] : P.hbcV;
+    for (int f = 0; f < 6; f++) { t.face[c][f] = (uint8_t)b.face[f]; t.faceValue[c][f] = b.faceValue[f]; }
+    for (int v = 0; v < 8; v++) { t.vert[c][v] = (uint8_t)b.vertex[v]; t.vertValue[c][v] = b.vertex[v] ? b.vertexValue[v] : 0.; }
+  }
+  return t;
+}
+
+// Last writer over faces X0,X1,Y0,Y1,Z0,Z1 then the vertex (VFCartFE.c:279-483,
+// 493-738).  Returns the BC type (0 if the dof is free) and the value it
+// writes: FIXED takes faceValue/vertexValue (mode 0, VecSetFromBC) or B[d]
+// (mode 1, VecApplyDirichletBC).
+__device__ int bc_last(const Grid &G, const BcTable &t, int c, int i, int j, int k, double Bd, int mode,
+                       double *val) {
+  int ty = 0;
+  double v = 0.;
+  const bool onf[6] = {i == 0, i == G.nx - 1, j == 0, j == G.ny - 1, k == 0, k == G.nz - 1};
+  for (int f = 0; f < 6; f++)
+    if (t.face[c][f] && onf[f]) {
+      ty = t.face[c][f];
+      v = ty == VFG_ZERO ? 0. : ty == VFG_ONE ? 1. : (mode == 0 ? t.faceValue[c][f] : Bd);
+    }
+  int ix = (i == 0) ? 0 : (i == G.nx - 1) ? 1 : -1;
+  int iy = (j == 0) ? 0 : (j == G.ny - 1) ? 1 : -1;
+  int iz = (k == 0) ? 0 : (k == G.nz - 1) ? 1 : -1;
+  if (ix >= 0 && iy >= 0 && iz >= 0) {
+    int vv = ix + 2 * iy + 4 * iz;
+    if (t.vert[c][vv]) {
+      ty = t.vert[c][vv];
+      v = ty == VFG_ZERO ? 0. : ty == VFG_ONE ? 1. : (mode == 0 ? t.vertValue[c][vv] : Bd);
+    }
+  }
+  *val = v;
+  return ty;
+}
+
+__global__ void k_bc_vec(Grid G, BcTable t, uint8_t *mask, double *vec, const double *B, int mode) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= (int64_t)G.nx * G.ny * G.nz) return;
+  int i = (int)(n % G.nx), j = (int)((n / G.nx) % G.ny), k = (int)(n / ((int64_t)G.nx * G.ny));
+  for (int c = 0; c < t.dof; c++) {
+    int64_t d = t.dof * n + c;
+    double val;
+    int ty = bc_last(G, t, c, i, j, k, B ? B[d] : 0., mode, &val);
+    if (mask) mask[d] = ty ? 1 : 0;
+    if (vec && ty) vec[d] = val;
+  }
+}
+
+static unsigned nblk(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+void launch_bc_mask(const DevProblem &P, int which, uint8_t *mask, hipStream_t s) {
+  hipLaunchKernelGGL(k_bc_vec, dim3(nblk(P.nn)), dim3(256), 0, s, P.G, bc_table(P, which), mask,
+                     (double *)nullptr, (const double *)nullptr, 0);
+}
+void launch_vec_set_from_bc(const DevProblem &P, int which, double *B, hipStream_t s) {
+  hipLaunchKernelGGL(k_bc_vec, dim3(nblk(P.nn)), dim3(256), 0, s, P.G, bc_table(P, which),
+                     (uint8_t *)nullptr, B, (const double *)nullptr, 0);
+}
+void launch_vec_apply_dirichlet(const DevProblem &P, int which, double *vec, const double *B, hipStream_t s) {
+  hipLaunchKernelGGL(k_bc_vec, dim3(nblk(P.nn)), dim3(256), 0, s, P.G, bc_table(P, which),
+                     (uint8_t *)nullptr, vec, B, 1);
+}
+
+// ------------------------------------------------------- energies / volume
+// Per element (VFMech.c:725-809, 920-974, 2394-2465; VFPermfield.c:15-71):
+//   elastic = sum_g W [mu(1+eta) v^2 |e^D|^2 + 3(3lam+2mu)/2 ((tr/3 - alpha dth) v
+//             - beta p/(3lam+2mu))^2 + (3lam+2mu)/6 tr^2 eta]
+//   pressure work = sum_g W p (u . grad v)
+//   surface = Gc/(4 atCv) sum_g W [(1-v)^k/eps + eps |grad v|^2], k = 1 (AT1), 2 (AT2)
+//   crack volume = sum_g W u . grad v
+constexpr int EB = 256;
+int energy_blocks(const DevProblem &P) { return (int)((P.ne + EB - 1) / EB); }
+
+__global__ __launch_bounds__(EB) void k_energies(DevProblem P, const double *__restrict__ U,
+                                                 const double *__restrict__ V, double *partials,
+                                                 double *cellfield) {
+  __shared__ double sh[EB / 64];
+  const int64_t e = (int64_t)blockIdx.x * EB + threadIdx.x;
+  double el = 0., pw = 0., se = 0., cv = 0.;
+  if (e < P.ne) {
+    const Grid &G = P.G;
+    const int ei = (int)(e % G.nex), ej = (int)((e / G.nex) % G.ney), ek = (int)(e / ((int64_t)G.nex * G.ney));
+    double u[8][3], v[8], th[8], pp[8];
+    for (int a = 0; a < 8; a++) {
+      int64_t n = node_id(G, ei + (a & 1), ej + ((a >> 1) & 1), ek + (a >> 2));
+      for (int c = 0; c < 3; c++) u[a][c] = U[3 * n + c];
+      v[a] = V[n]; th[a] = P.dth[n]; pp[a] = P.p[n];
+    }
+    const double Ee = P.E_elem ? P.E_elem[e] : P.E;
+    const double lam = P.lam1 * Ee, mu = P.mu1 * Ee;
+    const double tk = 3. * lam + 2. * mu, cb = P.beta / tk;
+    const double ih[3] = {1. / G.hx, 1. / G.hy, 1. / G.hz};
+    const double vol = G.hx * G.hy * G.hz;
+    const double csurf = P.Gc / P.atcv * .25;
+    double cell = 0.;
+    for (int g = 0; g < 27; g++) {
+      int gx = g % 3, gy = (g / 3) % 3, gz = g / 9;
+      double H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, ug[3] = {0, 0, 0}, gv[3] = {0, 0, 0};
+      double vg = 0., pg = 0., tg = 0.;
+      for (int a = 0; a < 8; a++) {
+        int i = a & 1, j = (a >> 1) & 1, k = a >> 2;
+        double px = i ? gP1(gx) : gP0(gx), py = j ? gP1(gy) : gP0(gy), pz = k ? gP1(gz) : gP0(gz);
+        double ph = px * py * pz;
+        double dg[3] = {(i ? ih[0] : -ih[0]) * py * pz, (j ? ih[1] : -ih[1]) * px * pz,
+                        (k ? ih[2] : -ih[2]) * px * py};
+        for (int c = 0; c < 3; c++) {
+          ug[c] += ph * u[a][c];
+          gv[c] += dg[c] * v[a];
+          for (int d = 0; d < 3; d++) H[c][d] += dg[d] * u[a][c];
+        }
+        vg += ph * v[a]; pg += ph * pp[a]; tg += ph * th[a];
+      }
+      const double W = vol * gW(gx) * gW(gy) * gW(gz);
+      double tr = H[0][0] + H[1][1] + H[2][2];
+      double d11 = H[0][0] - tr / 3., d22 = H[1][1] - tr / 3., d33 = H[2][2] - tr / 3.;
+      double d12 = (H[0][1] + H[1][0]) * .5, d23 = (H[1][2] + H[2][1]) * .5, d13 = (H[0][2] + H[2][0]) * .5;
+      el += (d11 * d11 + d22 * d22 + d33 * d33 + (d12 * d12 + d23 * d23 + d13 * d13) * 2.) * vg * vg *
+            (1. + P.eta) * mu * W;
+      double WD = (tr / 3. - P.alpha * tg) * vg - cb * pg;
+      el += 3. * tk / 2. * WD * WD * W;
+      el += tk / 6. * tr * tr * P.eta * W;
+      double ugv = ug[0] * gv[0] + ug[1] * gv[1] + ug[2] * gv[2];
+      if (P.hasCrackPressure) pw += W * pg * ugv;
+      double g2 = gv[0] * gv[0] + gv[1] * gv[1] + gv[2] * gv[2];
+      double omv = 1. - vg;
+      se += W * ((P.atnum == 2 ? omv * omv : omv) / P.eps + g2 * P.eps) * csurf;
+      cv += ugv * W;
+      cell += ugv * W;
+    }
+    if (cellfield) cellfield[e] = cell / vol;
+  }
+  double r0 = block_sum<EB>(el, sh), r1 = block_sum<EB>(pw, sh), r2 = block_sum<EB>(se, sh),
+         r3 = block_sum<EB>(cv, sh);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x * 4 + 0] = r0; partials[blockIdx.x * 4 + 1] = r1;
+    partials[blockIdx.x * 4 + 2] = r2; partials[blockIdx.x * 4 + 3] = r3;
+  }
+}
+
+// sums `nb` groups of `width` partials column-wise with one block
+__global__ __launch_bounds__(256) void k_reduce_partials(const double *partials, int nb, int width, double *out) {
+  __shared__ double sh[4];
+  for (int c = 0; c < width; c++) {
+    double s = 0.;
+    for (int b = threadIdx.x; b < nb; b += 256) s += partials[(int64_t)b * width + c];
+    s = block_sum<256>(s, sh);
+    if (threadIdx.x == 0) out[c] = s;
+  }
+}
+
+void launch_energies(const DevProblem &P, const double *U, const double *V, double *partials,
+                     double *cellfield, double *result4, hipStream_t s) {
+  int nb = energy_blocks(P);
+  hipLaunchKernelGGL(k_energies, dim3(nb), dim3(EB), 0, s, P, U, V, partials, cellfield);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, s, (const double *)partials, nb, 4, result4);
+}
+
+// ------------------------------------------------------------ penny crack
+// VFDistanceToPennyCrack + VFPennyCrackBuildVAT1/AT2 (VFCracks.c:101-227);
+// node coordinates X[i] = i*lx/(nx-1) (VFCommon.c:395-397).
+__global__ void k_penny(Grid G, double lx, double ly, double lz, int atnum, double eps, double *V,
+                        double c0, double c1, double c2, double r, double th, double ph, double thick) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= (int64_t)G.nx * G.ny * G.nz) return;
+  int i = (int)(n % G.nx), j = (int)((n / G.nx) % G.ny), k = (int)(n / ((int64_t)G.nx * G.ny));
+  double x0 = i == 0 ? 0. : i * lx / (G.nx - 1.);
+  double x1 = j == 0 ? 0. : j * ly / (G.ny - 1.);
+  double x2 = k == 0 ? 0. : k * lz / (G.nz - 1.);
+  if (r == 0) { V[n] = 1.; return; }
+  const double PI = 3.14159265358979323846;
+  double nv0 = cos(th * PI / 180.) * sin(ph * PI / 180.);
+  double nv1 = sin(th * PI / 180.) * sin(ph * PI / 180.);
+  double nv2 = cos(ph * PI / 180.);
+  double xdn = (x0 - c0) * nv0 + (x1 - c1) * nv1 + (x2 - c2) * nv2;
+  double t0 = (x0 - c0) - xdn * nv0, t1 = (x1 - c1) - xdn * nv1, t2 = (x2 - c2) - xdn * nv2;
+  double l = sqrt(t0 * t0 + t1 * t1 + t2 * t2);
+  double dist = (l <= r) ? sqrt(xdn * xdn) : sqrt(xdn * xdn + (l - r) * (l - r));
+  dist -= thick / 2.;
+  double val;
+  if (atnum == 1) val = dist <= 0 ? 0. : dist < 2. * eps ? dist / eps * (1. - .25 * dist / eps) : 1.;
+  else val = dist <= 0 ? 0. : 1. - exp(-dist / 2 / eps);
+  V[n] = val;
+}
+
+void launch_penny(const DevProblem &P, double *V, const double c[3], double r, double theta, double phi,
+                  double thick, hipStream_t s) {
+  double lx = P.G.hx * P.G.nex, ly = P.G.hy * P.G.ney, lz = P.G.hz * P.G.nez;
+  hipLaunchKernelGGL(k_penny, dim3(nblk(P.nn)), dim3(256), 0, s, P.G, lx, ly, lz, P.atnum, P.eps, V,
+                     c[0], c[1], c[2], r, theta, phi, thick);
+}
+
+// ------------------------------------------------------------ vector ops
+constexpr int RB = 256;
+int red_blocks(int64_t n) {
+  int64_t b = (n + RB * 4 - 1) / (RB * 4);
+  return (int)(b < 1 ? 1 : b > 2048 ? 2048 : b);
+}
+
+__global__ __launch_bounds__(RB) void k_dot(const double *__restrict__ a, const double *__restrict__ b,
+                                            int64_t n, double *partials) {
+  __shared__ double sh[RB / 64];
+  double s = 0.;
+  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB) s += a[i] * b[i];
+  s = block_sum<RB>(s, sh);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+void launch_dot(const double *a, const double *b, int64_t n, double *partials, double *out, hipStream_t s) {
+  int nb = red_blocks(n);
+  hipLaunchKernelGGL(k_dot, dim3(nb), dim3(RB), 0, s, a, b, n, partials);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, s, (const double *)partials, nb, 1, out);
+}
+
+__global__ void k_fill(double *x, double v, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = v;
+}
+void launch_fill(double *x, double v, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill, dim3(nblk(n)), dim3(256), 0, s, x, v, n);
+}
+__global__ void k_sub(const double *a, const double *b, double *o, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) o[i] = a[i] - b[i];
+}
+void launch_sub(const double *a, const double *b, double *out, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_sub, dim3(nblk(n)), dim3(256), 0, s, a, b, out, n);
+}
+__global__ void k_d2f(const double *x, float *y, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (float)x[i];
+}
+__global__ void k_f2d(const float *x, double *y, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (double)x[i];
+}
+void launch_copy_d2f(const double *x, float *y, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_d2f, dim3(nblk(n)), dim3(256), 0, s, x, y, n);
+}
+void launch_copy_f2d(const float *x, double *y, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_f2d, dim3(nblk(n)), dim3(256), 0, s, x, y, n);
+}
+
+}  // namespace vfg

@@ -1,0 +1,359 @@
+// vfg_solve.hip -- GPU solvers behind vfg_u_solve / vfg_v_solve.
+//
+// u: VF_StepU (VFMech.c:1436-1451) runs one Newton step of SNESNEWTONLS on a
+//    problem that is linear in U, with KSPCG on the Jacobian whose Dirichlet
+//    rows are identity (VFCommon.c:1018-1048).  With U carrying its Dirichlet
+//    values the Newton system is K_II d_I = -F_I, solved here by
+//    preconditioned CG started from U (r = -F(U)), which is the same iterate
+//    sequence without the separate correction vector.
+// V: VF_StepV (VFMech.c:3057-3082) with SNESVINEWTONRSLS (VFCommon.c:1052-1086):
+//    minimise the quadratic V-energy over lb <= V <= ub with Dirichlet and
+//    (AT2) irreversible nodes held.  Reduced-space active-set Newton: the
+//    active set holds nodes at a bound whose gradient points outward; the
+//    Newton step solves the reduced system on the inactive set by PCG; the
+//    projected step is accepted under an Armijo test.
+// All vectors, scalars and reductions stay on the device; the host reads a
+// few scalars every `check` iterations to test convergence.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <functional>
+#include "vfg_handle.h"
+
+namespace vfg {
+
+enum { S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_RR0, S_FLAG, S_TMP0, S_TMP1, S_TMP2 };
+constexpr int RB = 256;
+
+__device__ __forceinline__ double bsum(double v) {
+  __shared__ double sh[RB / 64];
+  return block_sum<RB>(v, sh);
+}
+
+// z = dinv * r; p = z (init) ; partial sums of r.z and r.r
+__global__ __launch_bounds__(RB) void k_pcg_init(const double *__restrict__ r, double *__restrict__ z,
+                                                 double *__restrict__ p, const double *__restrict__ dinv,
+                                                 int64_t n, double *part) {
+  double rz = 0., rr = 0.;
+  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB) {
+    double ri = r[i], zi = dinv[i] * ri;
+    z[i] = zi;
+    p[i] = zi;
+    rz += ri * zi;
+    rr += ri * ri;
+  }
+  rz = bsum(rz);
+  rr = bsum(rr);
+  if (threadIdx.x == 0) { part[2 * blockIdx.x] = rz; part[2 * blockIdx.x + 1] = rr; }
+}
+
+// op 0: rz, rr, rr0 from init; op 1: alpha = rz / (p.q); op 2: beta
+__global__ __launch_bounds__(RB) void k_pcg_scalar(const double *part, int nb, int op, double *sc) {
+  double a = 0., b = 0.;
+  const int w = op == 1 ? 1 : 2;
+  for (int i = threadIdx.x; i < nb; i += RB) {
+    a += part[(int64_t)i * w];
+    if (w == 2) b += part[(int64_t)i * w + 1];
+  }
+  a = bsum(a);
+  b = bsum(b);
+  if (threadIdx.x != 0) return;
+  if (op == 0) {
+    sc[S_RZ] = a; sc[S_RR] = b; sc[S_RR0] = b; sc[S_FLAG] = 0.;
+  } else if (op == 1) {
+    sc[S_PQ] = a;
+    if (!(a > 0.) || !isfinite(a)) { sc[S_ALPHA] = 0.; sc[S_FLAG] = isfinite(a) ? 1. : 2.; }
+    else sc[S_ALPHA] = sc[S_RZ] / a;
+  } else {
+    double rz = sc[S_RZ];
+    sc[S_BETA] = rz != 0. ? a / rz : 0.;
+    sc[S_RZ] = a;
+    sc[S_RR] = b;
+    if (!isfinite(a) || !isfinite(b)) sc[S_FLAG] = 2.;
+  }
+}
+
+__global__ __launch_bounds__(RB) void k_dotp(const double *__restrict__ a, const double *__restrict__ b, int64_t n,
+                                             double *part) {
+  double s = 0.;
+  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB) s += a[i] * b[i];
+  s = bsum(s);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(RB) void k_pcg_xr(double *__restrict__ x, double *__restrict__ r,
+                                               double *__restrict__ z, const double *__restrict__ p,
+                                               const double *__restrict__ q, const double *__restrict__ dinv,
+                                               int64_t n, const double *sc, double *part) {
+  const double al = sc[S_ALPHA];
+  double rz = 0., rr = 0.;
+  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB) {
+    x[i] += al * p[i];
+    double ri = r[i] - al * q[i];
+    r[i] = ri;
+    double zi = dinv[i] * ri;
+    z[i] = zi;
+    rz += ri * zi;
+    rr += ri * ri;
+  }
+  rz = bsum(rz);
+  rr = bsum(rr);
+  if (threadIdx.x == 0) { part[2 * blockIdx.x] = rz; part[2 * blockIdx.x + 1] = rr; }
+}
+
+__global__ __launch_bounds__(RB) void k_pcg_p(double *__restrict__ p, const double *__restrict__ z, int64_t n,
+                                              const double *sc) {
+  const double be = sc[S_BETA];
+  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB)
+    p[i] = z[i] + be * p[i];
+}
+
+__global__ void k_recip(const double *d, double *dinv, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dinv[i] = d[i] != 0. ? 1. / d[i] : 1.;
+}
+__global__ void k_neg(const double *a, double *b, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = -a[i];
+}
+
+static unsigned nb_(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+typedef std::function<void(const double *, double *)> ApplyFn;
+
+// Preconditioned CG on the device.  On entry x is the initial guess and r the
+// initial residual b - A x.  Converged when ||r|| <= max(rtol*||r0||, atol).
+static int pcg(vfg_handle *h, int64_t n, const ApplyFn &A, const double *dinv, double *x, double *r, double *z,
+               double *p, double *q, double rtol, double atol, int maxit, int check, vfg_solve_info *info) {
+  hipStream_t s = h->stream;
+  double *sc = h->scal, *part = h->part;
+  const int nb = red_blocks(n);
+  hipLaunchKernelGGL(k_pcg_init, dim3(nb), dim3(RB), 0, s, r, z, p, dinv, n, part);
+  hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(RB), 0, s, (const double *)part, nb, 0, sc);
+  double hs[8];
+  if (hipMemcpyAsync(hs, sc, 64, hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+    return vfg_fail(h, VFG_E_HIP, "pcg: initial reduction failed");
+  const double rn0 = sqrt(hs[S_RR0]);
+  double rn = rn0;
+  int it = 0, status = VFG_OK;
+  const double tol = fmax(rtol * rn0, atol);
+  if (!isfinite(rn0)) status = VFG_E_NAN;
+  while (status == VFG_OK && rn > tol && it < maxit) {
+    int todo = check < maxit - it ? check : maxit - it;
+    for (int k = 0; k < todo; k++) {
+      A(p, q);
+      hipLaunchKernelGGL(k_dotp, dim3(nb), dim3(RB), 0, s, (const double *)p, (const double *)q, n, part);
+      hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(RB), 0, s, (const double *)part, nb, 1, sc);
+      hipLaunchKernelGGL(k_pcg_xr, dim3(nb), dim3(RB), 0, s, x, r, z, (const double *)p, (const double *)q, dinv, n,
+                         (const double *)sc, part);
+      hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(RB), 0, s, (const double *)part, nb, 2, sc);
+      hipLaunchKernelGGL(k_pcg_p, dim3(nb), dim3(RB), 0, s, p, (const double *)z, n, (const double *)sc);
+    }
+    it += todo;
+    if (hipMemcpyAsync(hs, sc, 64, hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+      return vfg_fail(h, VFG_E_HIP, "pcg: iteration failed");
+    rn = sqrt(hs[S_RR]);
+    if (hs[S_FLAG] == 2. || !isfinite(rn)) status = VFG_E_NAN;
+    else if (hs[S_FLAG] == 1.) status = rn <= tol ? VFG_OK : VFG_E_BREAKDOWN;
+    if (hs[S_FLAG] != 0.) break;
+  }
+  if (status == VFG_OK && rn > tol) status = VFG_E_MAXIT;
+  if (info) {
+    info->iterations += it;
+    info->relres = rn0 > 0. ? rn / rn0 : 0.;
+    info->status = status;
+  }
+  return status;
+}
+
+static int work(vfg_handle *h, int nvec, int64_t n) {
+  for (int i = 0; i < nvec; i++) {
+    if (h->w[i]) continue;
+    int rc = vfg_alloc(h, (void **)&h->w[i], 8 * (size_t)n);
+    if (rc) return rc;
+  }
+  return VFG_OK;
+}
+
+// ---------------------------------------------------------------- V helpers
+__global__ void k_v_clamp(double *V, const uint8_t *fix, const double *ub, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || fix[i]) return;
+  double hi = ub ? ub[i] : 1., v = V[i];
+  V[i] = v < 0. ? 0. : v > hi ? hi : v;
+}
+
+// active set + projected-gradient partial norm; r = -g on the inactive set
+__global__ __launch_bounds__(RB) void k_v_active(const double *V, const double *g, const uint8_t *fix,
+                                                 const double *ub, uint8_t *act, double *r, int64_t n,
+                                                 double *part) {
+  double s = 0.;
+  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB) {
+    double hi = ub ? ub[i] : 1., v = V[i], gi = g[i];
+    bool a = fix[i] || (v <= 0. && gi > 0.) || (v >= hi && gi < 0.);
+    act[i] = a ? 1 : 0;
+    r[i] = a ? 0. : -gi;
+    if (!a) s += gi * gi;
+  }
+  s = bsum(s);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// Vn = P(V + lam d) on free nodes; D = Vn - V
+__global__ void k_v_step(const double *V, const double *d, const uint8_t *act, const uint8_t *fix,
+                         const double *ub, double lam, double *Vn, double *D, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double v = V[i], x = v;
+  if (!act[i]) {
+    double hi = ub ? ub[i] : 1.;
+    x = v + lam * d[i];
+    if (!fix[i]) x = x < 0. ? 0. : x > hi ? hi : x;
+  }
+  Vn[i] = x;
+  D[i] = x - v;
+}
+
+__global__ void k_zero_fixed(double *g, const uint8_t *fix, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && fix[i]) g[i] = 0.;
+}
+
+static double dot_host(vfg_handle *h, const double *a, const double *b, int64_t n, int *rc) {
+  launch_dot(a, b, n, h->part, h->scal + 16, h->stream);
+  double v = 0.;
+  if (hipMemcpyAsync(&v, h->scal + 16, 8, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+      hipStreamSynchronize(h->stream) != hipSuccess)
+    *rc = vfg_fail(h, VFG_E_HIP, "dot failed");
+  return v;
+}
+
+}  // namespace vfg
+
+using namespace vfg;
+
+extern "C" {
+
+void vfg_solve_opts_default(vfg_solve_opts *o) {
+  if (!o) return;
+  o->rtol = 1e-10;
+  o->atol = 0.;
+  o->maxit = 100000;
+  o->pc = VFG_PC_JACOBI;
+  o->precision = VFG_PREC_FP64;
+  o->cheb_degree = 4;
+}
+
+int vfg_u_solve(vfg_handle *h, double *U, const double *BCU, const vfg_solve_opts *o, vfg_solve_info *info, int loc) {
+  if (!h) return vfg_fail(nullptr, VFG_E_ARG, "NULL handle");
+  if (!U || !BCU) return vfg_fail(h, VFG_E_ARG, "vfg_u_solve: NULL array");
+  vfg_solve_opts d;
+  vfg_solve_opts_default(&d);
+  if (!o) o = &d;
+  vfg_solve_info inf = {0, 0, 0., 0};
+  const int64_t n = 3 * h->P.nn;
+  hipStream_t s = h->stream;
+  int rc;
+  if ((rc = work(h, 7, n))) return rc;
+  double *x = h->w[0], *r = h->w[1], *z = h->w[2], *p = h->w[3], *q = h->w[4], *dinv = h->w[5], *B = h->w[6];
+  if ((rc = vfg_in(h, x, U, 8 * n, loc)) || (rc = vfg_in(h, B, BCU, 8 * n, loc))) return rc;
+  // VFTimeStepPrepare: U carries its Dirichlet values
+  launch_vec_apply_dirichlet(h->P, 0, x, B, s);
+  launch_u_rhs(h->P, x, B, r, 1, s);  // F(U)
+  hipLaunchKernelGGL(k_neg, dim3(nb_(n)), dim3(256), 0, s, (const double *)r, r, n);
+  launch_u_diag(h->P, q, s);
+  hipLaunchKernelGGL(k_recip, dim3(nb_(n)), dim3(256), 0, s, (const double *)q, dinv, n);
+  const DevProblem &P = h->P;
+  ApplyFn A = [&](const double *in, double *out) { launch_u_apply(P, in, out, s); };
+  rc = pcg(h, n, A, dinv, x, r, z, p, q, o->rtol, o->atol, o->maxit, 20, &inf);
+  inf.outer = 1;
+  if (info) *info = inf;
+  if (rc == VFG_E_HIP) return rc;
+  hipError_t e = hipMemcpyAsync(U, x, 8 * n, loc == VFG_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, s);
+  if (e != hipSuccess || hipStreamSynchronize(s) != hipSuccess) return vfg_fail(h, VFG_E_HIP, "vfg_u_solve: copy-out failed");
+  if (rc) return vfg_fail(h, rc, "vfg_u_solve: CG stopped with status %d after %d iterations (relres %.3e)", rc,
+                          inf.iterations, inf.relres);
+  return VFG_OK;
+}
+
+int vfg_v_solve(vfg_handle *h, const double *U, double *V, const double *ub, const vfg_solve_opts *o,
+                vfg_solve_info *info, int loc) {
+  if (!h) return vfg_fail(nullptr, VFG_E_ARG, "NULL handle");
+  if (!U || !V) return vfg_fail(h, VFG_E_ARG, "vfg_v_solve: NULL array");
+  vfg_solve_opts d;
+  vfg_solve_opts_default(&d);
+  if (!o) o = &d;
+  vfg_solve_info inf = {0, 0, 0., 0};
+  DevProblem &P = h->P;
+  const int64_t n = P.nn;
+  hipStream_t s = h->stream;
+  int rc;
+  if ((rc = work(h, 12, 3 * n))) return rc;
+  if ((rc = vfg_alloc(h, (void **)&h->act, n))) return rc;
+  double *x = h->w[0], *r = h->w[1], *z = h->w[2], *p = h->w[3], *q = h->w[4], *dinv = h->w[5];
+  double *Vd = h->w[6], *g = h->w[7], *Vn = h->w[8], *D = h->w[9], *KD = h->w[10], *ubd = nullptr;
+  double *Ud = h->w[11];
+  uint8_t *act = h->act;
+  if ((rc = vfg_in(h, Ud, U, 24 * n, loc)) || (rc = vfg_in(h, Vd, V, 8 * n, loc))) return rc;
+  if (ub) {
+    ubd = KD + n;  // KD is 3n long: second third holds the bound
+    if ((rc = vfg_in(h, ubd, ub, 8 * n, loc))) return rc;
+  }
+  // VF_VIJacobian refresh: coupling from U; Dirichlet values on V
+  if ((rc = vfg_alloc(h, (void **)&P.Vq, 8 * 27 * (size_t)P.ne))) return rc;
+  launch_v_setup(P, Ud, s);
+  h->v_ready = true;
+  launch_v_fixmask(P, s);
+  launch_vec_apply_dirichlet(P, 1, Vd, Vd, s);
+  hipLaunchKernelGGL(k_v_clamp, dim3(nb_(n)), dim3(256), 0, s, Vd, (const uint8_t *)P.vfix, (const double *)ubd, n);
+  const double inner_rtol = o->rtol < 1e-12 ? o->rtol : 1e-12;
+  double pg0 = -1.;
+  int it;
+  const int nb = red_blocks(n);
+  int status = VFG_OK;
+  for (it = 0; it < (o->maxit < 200 ? o->maxit : 200); it++) {
+    launch_v_apply(P, Vd, g, 1, s);  // F(V): free rows = K V - b
+    hipLaunchKernelGGL(k_v_active, dim3(nb), dim3(RB), 0, s, (const double *)Vd, (const double *)g,
+                       (const uint8_t *)P.vfix, (const double *)ubd, act, r, n, h->part);
+    launch_dot(r, r, n, h->part + 4096, h->scal + 16, s);  // ||g_I||^2 (same value, fixed order)
+    double pg2 = 0.;
+    if (hipMemcpyAsync(&pg2, h->scal + 16, 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return vfg_fail(h, VFG_E_HIP, "vfg_v_solve: reduction failed");
+    double pg = sqrt(pg2);
+    if (!isfinite(pg)) { status = VFG_E_NAN; break; }
+    if (pg0 < 0.) pg0 = pg > 0. ? pg : 1.;
+    if (pg <= o->rtol * pg0 || pg <= 1e-14) break;
+    // reduced Newton system on the inactive set
+    launch_v_diag(P, q, s, act);
+    hipLaunchKernelGGL(k_recip, dim3(nb_(n)), dim3(256), 0, s, (const double *)q, dinv, n);
+    launch_fill(x, 0., n, s);
+    ApplyFn A = [&](const double *in, double *out) { launch_v_apply(P, in, out, 0, s, act); };
+    int prc = pcg(h, n, A, dinv, x, r, z, p, q, inner_rtol, 0., 20000, 20, &inf);
+    if (prc == VFG_E_HIP || prc == VFG_E_NAN) { status = prc; break; }
+    // projected step with Armijo backtracking on f(V) = 1/2 V.KV - b.V
+    hipLaunchKernelGGL(k_zero_fixed, dim3(nb_(n)), dim3(256), 0, s, g, (const uint8_t *)P.vfix, n);
+    double lam = 1.;
+    for (int ls = 0; ls < 40; ls++) {
+      hipLaunchKernelGGL(k_v_step, dim3(nb_(n)), dim3(256), 0, s, (const double *)Vd, (const double *)x,
+                         (const uint8_t *)act, (const uint8_t *)P.vfix, (const double *)ubd, lam, Vn, D, n);
+      launch_v_apply(P, D, KD, 2, s);
+      int e2 = 0;
+      double gd = dot_host(h, g, D, n, &e2);
+      double dkd = dot_host(h, D, KD, n, &e2);
+      if (e2) return e2;
+      if (gd + .5 * dkd <= 1e-4 * gd || ls == 39) break;
+      lam *= .5;
+    }
+    hipMemcpyAsync(Vd, Vn, 8 * n, hipMemcpyDeviceToDevice, s);
+  }
+  inf.outer = it;
+  if (status == VFG_OK && it >= (o->maxit < 200 ? o->maxit : 200)) status = VFG_E_MAXIT;
+  inf.status = status;
+  if (info) *info = inf;
+  hipError_t e = hipMemcpyAsync(V, Vd, 8 * n, loc == VFG_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, s);
+  if (e != hipSuccess || hipStreamSynchronize(s) != hipSuccess) return vfg_fail(h, VFG_E_HIP, "vfg_v_solve: copy-out failed");
+  if (status) return vfg_fail(h, status, "vfg_v_solve: stopped with status %d after %d Newton steps", status, it);
+  return VFG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,374 @@
+// vfg_u.hip -- u-subproblem operators on gfx950.
+//
+// Element bilinear form (VF_BilinearFormU3D_local, VFMech.c:227-290):
+//   Ke[(a,c1),(b,c2)] = sum_g W_g s_g [lam d_c1 phi_a d_c2 phi_b
+//                                      + mu d_c2 phi_a d_c1 phi_b
+//                                      + delta_c1c2 mu grad phi_a . grad phi_b]
+//   s_g = v_g^2 + eta.
+// Applied matrix-free: (Ke x)_{a,c} = sum_g sum_d sigma_cd(g) d_d phi_a(g) with
+//   sigma = W s (lam tr(H) I + mu (H + H^T)),  H_cd = d_d u_c  (displacement
+//   gradient of x).  All 27-point quantities are obtained by sum
+// factorisation with the 1-D Gauss tables; d_x u_c is constant along x, so it
+// takes 9 distinct values per component.
+#include "vfg_internal.h"
+#include "vfg_kernels.h"
+
+namespace vfg {
+
+// Apply of one element: xin holds (x0,x1,x2,v) at the 8 corners.
+// cE = E_e * hx*hy*hz; lam1/mu1 are the Lame constants for E = 1.
+template <typename T>
+__device__ __forceinline__ void u_elem_apply(const T (&X)[2][4][4], T cE, T lam1, T mu1, T eta,
+                                             T ihx, T ihy, T ihz, T (&r)[8][3]) {
+  // differences along each direction, scaled by 1/h
+  T Dx[3][2][2], Dy[3][2][2], Dz[3][2][2];  // Dx[c][k][j], Dy[c][k][i], Dz[c][j][i]
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int p = 0; p < 2; p++)
+#pragma unroll
+      for (int q = 0; q < 2; q++) {
+        Dx[c][p][q] = (X[p][q * 2 + 1][c] - X[p][q * 2 + 0][c]) * ihx;
+        Dy[c][p][q] = (X[p][2 + q][c] - X[p][q][c]) * ihy;
+        Dz[c][p][q] = (X[1][p * 2 + q][c] - X[0][p * 2 + q][c]) * ihz;
+      }
+  // d_z u_c at (gx,gy): constant in z
+  T Gz[3][3][3];  // [c][gy][gx]
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int gx = 0; gx < 3; gx++) {
+      T a0 = interp<T>(gx, Dz[c][0][0], Dz[c][0][1]);
+      T a1 = interp<T>(gx, Dz[c][1][0], Dz[c][1][1]);
+#pragma unroll
+      for (int gy = 0; gy < 3; gy++) Gz[c][gy][gx] = interp<T>(gy, a0, a1);
+    }
+  // v interpolated in x and y on both z-faces
+  T vxy[2][3][3];  // [k][gy][gx]
+#pragma unroll
+  for (int k = 0; k < 2; k++)
+#pragma unroll
+    for (int gx = 0; gx < 3; gx++) {
+      T a0 = interp<T>(gx, X[k][0][3], X[k][1][3]);
+      T a1 = interp<T>(gx, X[k][2][3], X[k][3][3]);
+#pragma unroll
+      for (int gy = 0; gy < 3; gy++) vxy[k][gy][gx] = interp<T>(gy, a0, a1);
+    }
+
+  T Rx[3][2][2], Ry[3][2][2], Sz[3][3][3];  // Rx[c][k][j], Ry[c][k][i], Sz[c][gy][gx]
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int p = 0; p < 2; p++)
+#pragma unroll
+      for (int q = 0; q < 2; q++) Rx[c][p][q] = Ry[c][p][q] = T(0);
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int gy = 0; gy < 3; gy++)
+#pragma unroll
+      for (int gx = 0; gx < 3; gx++) Sz[c][gy][gx] = T(0);
+
+#pragma unroll
+  for (int gz = 0; gz < 3; gz++) {
+    T Gx[3][3], Gy[3][3];  // Gx[c][gy], Gy[c][gx]
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      T bx0 = interp<T>(gz, Dx[c][0][0], Dx[c][1][0]);
+      T bx1 = interp<T>(gz, Dx[c][0][1], Dx[c][1][1]);
+      T by0 = interp<T>(gz, Dy[c][0][0], Dy[c][1][0]);
+      T by1 = interp<T>(gz, Dy[c][0][1], Dy[c][1][1]);
+#pragma unroll
+      for (int g = 0; g < 3; g++) {
+        Gx[c][g] = interp<T>(g, bx0, bx1);
+        Gy[c][g] = interp<T>(g, by0, by1);
+      }
+    }
+    T Sx[3][3], Sy[3][3];  // Sx[c][gy] = sum_gx sigma_c0, Sy[c][gx] = sum_gy sigma_c1
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+      for (int g = 0; g < 3; g++) Sx[c][g] = Sy[c][g] = T(0);
+    const T wz = cE * T(gW(gz));
+#pragma unroll
+    for (int gy = 0; gy < 3; gy++) {
+      const T wzy = wz * T(gW(gy));
+#pragma unroll
+      for (int gx = 0; gx < 3; gx++) {
+        T v = interp<T>(gz, vxy[0][gy][gx], vxy[1][gy][gx]);
+        T cq = wzy * T(gW(gx)) * (v * v + eta);
+        T H00 = Gx[0][gy], H01 = Gy[0][gx], H02 = Gz[0][gy][gx];
+        T H10 = Gx[1][gy], H11 = Gy[1][gx], H12 = Gz[1][gy][gx];
+        T H20 = Gx[2][gy], H21 = Gy[2][gx], H22 = Gz[2][gy][gx];
+        T m = mu1 * cq;
+        T lt = lam1 * cq * (H00 + H11 + H22);
+        T m2 = m + m;
+        T s00 = lt + m2 * H00, s11 = lt + m2 * H11, s22 = lt + m2 * H22;
+        T s01 = m * (H01 + H10), s02 = m * (H02 + H20), s12 = m * (H12 + H21);
+        Sx[0][gy] += s00; Sx[1][gy] += s01; Sx[2][gy] += s02;
+        Sy[0][gx] += s01; Sy[1][gx] += s11; Sy[2][gx] += s12;
+        Sz[0][gy][gx] += s02; Sz[1][gy][gx] += s12; Sz[2][gy][gx] += s22;
+      }
+    }
+    const T pz0 = T(gP0(gz)), pz1 = T(gP1(gz));
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      T ax0 = T(0), ax1 = T(0), ay0 = T(0), ay1 = T(0);
+#pragma unroll
+      for (int g = 0; g < 3; g++) {
+        ax0 += T(gP0(g)) * Sx[c][g]; ax1 += T(gP1(g)) * Sx[c][g];
+        ay0 += T(gP0(g)) * Sy[c][g]; ay1 += T(gP1(g)) * Sy[c][g];
+      }
+      Rx[c][0][0] += pz0 * ax0; Rx[c][0][1] += pz0 * ax1;
+      Rx[c][1][0] += pz1 * ax0; Rx[c][1][1] += pz1 * ax1;
+      Ry[c][0][0] += pz0 * ay0; Ry[c][0][1] += pz0 * ay1;
+      Ry[c][1][0] += pz1 * ay0; Ry[c][1][1] += pz1 * ay1;
+    }
+  }
+  // project Sz onto the bilinear basis in (x,y)
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    T b[3][2];  // [gy][i]
+#pragma unroll
+    for (int gy = 0; gy < 3; gy++) {
+      T a0 = T(0), a1 = T(0);
+#pragma unroll
+      for (int gx = 0; gx < 3; gx++) { a0 += T(gP0(gx)) * Sz[c][gy][gx]; a1 += T(gP1(gx)) * Sz[c][gy][gx]; }
+      b[gy][0] = a0; b[gy][1] = a1;
+    }
+    T Rz[2][2];  // [j][i]
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      T a0 = T(0), a1 = T(0);
+#pragma unroll
+      for (int gy = 0; gy < 3; gy++) { a0 += T(gP0(gy)) * b[gy][i]; a1 += T(gP1(gy)) * b[gy][i]; }
+      Rz[0][i] = a0; Rz[1][i] = a1;
+    }
+#pragma unroll
+    for (int k = 0; k < 2; k++)
+#pragma unroll
+      for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+          T vxv = (i ? Rx[c][k][j] : -Rx[c][k][j]) * ihx;
+          T vyv = (j ? Ry[c][k][i] : -Ry[c][k][i]) * ihy;
+          T vzv = (k ? Rz[j][i] : -Rz[j][i]) * ihz;
+          r[(k * 2 + j) * 2 + i][c] = vxv + vyv + vzv;
+        }
+  }
+}
+
+template <typename T>
+struct UApplyOp {
+  static constexpr int NF = 4, NC = 3;
+  const T *x;
+  T *y;
+  const T *v;
+  const double *E_elem;
+  Grid G;
+  BcMask bc;
+  T E, lam1, mu1, eta, vol, ihx, ihy, ihz;
+  __device__ T load(int f, int64_t n) const { return f < 3 ? x[3 * n + f] : v[n]; }
+  __device__ void element(int ex, int ey, int ek, const T (&xin)[2][4][4], T (&r)[8][3]) const {
+    T Ee = E_elem ? T(E_elem[cell_id(G, ex, ey, ek)]) : E;
+    u_elem_apply<T>(xin, Ee * vol, lam1, mu1, eta, ihx, ihy, ihz, r);
+  }
+  __device__ void store(int i, int j, int k, const T (&acc)[3], const T (&own)[4]) const {
+    int64_t n = node_id(G, i, j, k);
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      y[3 * n + c] = on_dirichlet(G, bc.face[c], bc.vert[c], i, j, k) ? own[c] : acc[c];
+  }
+};
+
+template <typename T, int TX, int TY>
+__global__ __launch_bounds__(TX *TY) void k_u_apply(UApplyOp<T> op, int OZ) {
+  march<T, TX, TY>(op.G, op, OZ);
+}
+
+// ---------------------------------------------------------------- diagonal
+// Ke[(a,c),(a,c)] = sum_g W s [(lam+mu) (d_c phi_a)^2 + mu |grad phi_a|^2]
+struct UDiagOp {
+  static constexpr int NF = 1, NC = 3;
+  const double *v;
+  double *d;
+  const double *E_elem;
+  Grid G;
+  BcMask bc;
+  double E, lam1, mu1, eta, vol;
+  __device__ double load(int, int64_t n) const { return v[n]; }
+  __device__ void element(int ex, int ey, int ek, const double (&xin)[2][4][1], double (&r)[8][3]) const {
+    double Ee = E_elem ? E_elem[cell_id(G, ex, ey, ek)] : E;
+    double cE = Ee * vol;
+    const double ih[3] = {1. / G.hx, 1. / G.hy, 1. / G.hz};
+    for (int gz = 0; gz < 3; gz++)
+      for (int gy = 0; gy < 3; gy++)
+        for (int gx = 0; gx < 3; gx++) {
+          double vv = 0.;
+          for (int a = 0; a < 8; a++) {
+            int i = a & 1, j = (a >> 1) & 1, k = a >> 2;
+            vv += (i ? gP1(gx) : gP0(gx)) * (j ? gP1(gy) : gP0(gy)) * (k ? gP1(gz) : gP0(gz)) * xin[k][j * 2 + i][0];
+          }
+          double cq = cE * gW(gx) * gW(gy) * gW(gz) * (vv * vv + eta);
+          for (int a = 0; a < 8; a++) {
+            int i = a & 1, j = (a >> 1) & 1, k = a >> 2;
+            double px = i ? gP1(gx) : gP0(gx), py = j ? gP1(gy) : gP0(gy), pz = k ? gP1(gz) : gP0(gz);
+            double dg[3] = {(i ? ih[0] : -ih[0]) * py * pz, (j ? ih[1] : -ih[1]) * px * pz,
+                            (k ? ih[2] : -ih[2]) * px * py};
+            double g2 = dg[0] * dg[0] + dg[1] * dg[1] + dg[2] * dg[2];
+            for (int c = 0; c < 3; c++) r[a][c] += cq * ((lam1 + mu1) * dg[c] * dg[c] + mu1 * g2);
+          }
+        }
+  }
+  __device__ void store(int i, int j, int k, const double (&acc)[3], const double (&)[1]) const {
+    int64_t n = node_id(G, i, j, k);
+    for (int c = 0; c < 3; c++) d[3 * n + c] = on_dirichlet(G, bc.face[c], bc.vert[c], i, j, k) ? 1. : acc[c];
+  }
+};
+
+__global__ __launch_bounds__(256) void k_u_diag(UDiagOp op, int OZ) { march<double, 16, 16>(op.G, op, OZ); }
+
+// ------------------------------------------------------- load vector / residual
+// b_{a,c} = sum_g W d_c phi_a [(3lam+2mu) alpha dth s + beta p v]          (thermo-poro)
+//         + sum_g W p (d_c v) phi_a                       (crack pressure, optional)
+// mode 0: out = b (no BC); mode 1: out = K U - b with ResidualApplyDirichletBC
+struct URhsOp {
+  static constexpr int NF = 6, NC = 3;  // u0,u1,u2,v,dth,p
+  const double *U, *v, *dth, *p, *BCU;
+  double *out;
+  const double *E_elem;
+  Grid G;
+  BcMask bc;
+  uint8_t ftype[3][6];   // face types for the residual BC values
+  uint8_t vtype[3][8];
+  double E, lam1, mu1, eta, vol, alpha, beta;
+  int crack, mode;
+  __device__ double load(int f, int64_t n) const {
+    return f < 3 ? U[3 * n + f] : f == 3 ? v[n] : f == 4 ? dth[n] : p[n];
+  }
+  __device__ void element(int ex, int ey, int ek, const double (&X)[2][4][6], double (&r)[8][3]) const {
+    double Ee = E_elem ? E_elem[cell_id(G, ex, ey, ek)] : E;
+    double lam = lam1 * Ee, mu = mu1 * Ee;
+    const double ih[3] = {1. / G.hx, 1. / G.hy, 1. / G.hz};
+    if (mode == 1) {
+      double Xa[2][4][4];
+      for (int k = 0; k < 2; k++)
+        for (int q = 0; q < 4; q++)
+          for (int f = 0; f < 4; f++) Xa[k][q][f] = X[k][q][f];
+      u_elem_apply<double>(Xa, Ee * vol, lam1, mu1, eta, ih[0], ih[1], ih[2], r);
+    }
+    const double coefalpha = (3. * lam + 2. * mu) * alpha;
+    const double sgn = mode == 1 ? -1. : 1.;
+    for (int gz = 0; gz < 3; gz++)
+      for (int gy = 0; gy < 3; gy++)
+        for (int gx = 0; gx < 3; gx++) {
+          double vv = 0., th = 0., pr = 0., gv[3] = {0., 0., 0.};
+          for (int a = 0; a < 8; a++) {
+            int i = a & 1, j = (a >> 1) & 1, k = a >> 2;
+            double px = i ? gP1(gx) : gP0(gx), py = j ? gP1(gy) : gP0(gy), pz = k ? gP1(gz) : gP0(gz);
+            double ph = px * py * pz;
+            const double *xa = X[k][j * 2 + i];
+            vv += ph * xa[3]; th += ph * xa[4]; pr += ph * xa[5];
+            gv[0] += (i ? ih[0] : -ih[0]) * py * pz * xa[3];
+            gv[1] += (j ? ih[1] : -ih[1]) * px * pz * xa[3];
+            gv[2] += (k ? ih[2] : -ih[2]) * px * py * xa[3];
+          }
+          double W = vol * gW(gx) * gW(gy) * gW(gz);
+          double tp = W * (coefalpha * th * (vv * vv + eta) + beta * pr * vv);
+          double cp = crack ? W * pr : 0.;
+          for (int a = 0; a < 8; a++) {
+            int i = a & 1, j = (a >> 1) & 1, k = a >> 2;
+            double px = i ? gP1(gx) : gP0(gx), py = j ? gP1(gy) : gP0(gy), pz = k ? gP1(gz) : gP0(gz);
+            double dg[3] = {(i ? ih[0] : -ih[0]) * py * pz, (j ? ih[1] : -ih[1]) * px * pz,
+                            (k ? ih[2] : -ih[2]) * px * py};
+            double ph = px * py * pz;
+            for (int c = 0; c < 3; c++) r[a][c] += sgn * (tp * dg[c] + cp * gv[c] * ph);
+          }
+        }
+  }
+  // value written by ResidualApplyDirichletBC: traversal order X0..Z1 then
+  // vertices, later writers win (VFCartFE.c:747-999)
+  __device__ void store(int i, int j, int k, const double (&acc)[3], const double (&own)[6]) const {
+    int64_t n = node_id(G, i, j, k);
+    for (int c = 0; c < 3; c++) {
+      double val = acc[c];
+      if (mode == 1 && on_dirichlet(G, bc.face[c], bc.vert[c], i, j, k)) {
+        int t = 0;
+        const bool onf[6] = {i == 0, i == G.nx - 1, j == 0, j == G.ny - 1, k == 0, k == G.nz - 1};
+        for (int f = 0; f < 6; f++)
+          if (ftype[c][f] && onf[f]) t = ftype[c][f];
+        int ix = (i == 0) ? 0 : (i == G.nx - 1) ? 1 : -1;
+        int iy = (j == 0) ? 0 : (j == G.ny - 1) ? 1 : -1;
+        int iz = (k == 0) ? 0 : (k == G.nz - 1) ? 1 : -1;
+        if (ix >= 0 && iy >= 0 && iz >= 0 && vtype[c][ix + 2 * iy + 4 * iz]) t = vtype[c][ix + 2 * iy + 4 * iz];
+        double u = own[c];
+        val = t == 3 ? u - BCU[3 * n + c] : t == 2 ? u - 1. : u;
+      }
+      out[3 * n + c] = val;
+    }
+  }
+};
+
+__global__ __launch_bounds__(256) void k_u_rhs(URhsOp op, int OZ) { march<double, 16, 16>(op.G, op, OZ); }
+
+// ------------------------------------------------------------ host launchers
+static dim3 march_grid(const Grid &G, int TX, int TY, int OZ) {
+  return dim3((G.nx + TX - 2) / (TX - 1), (G.ny + TY - 2) / (TY - 1), (G.nz + OZ - 1) / OZ);
+}
+
+static int choose_oz(const Grid &G, int tiles_xy) {
+  // enough workgroups to cover 256 CUs several times, chunks of >= 8 planes
+  int oz = G.nz;
+  while (oz > 8 && (int64_t)tiles_xy * ((G.nz + oz - 1) / oz) < 2048) oz = (oz + 1) / 2;
+  return oz;
+}
+
+template <typename T>
+static void launch_u_apply_t(const DevProblem &P, const T *x, T *y, const T *v, hipStream_t s) {
+  UApplyOp<T> op;
+  op.x = x; op.y = y; op.v = v;
+  op.E_elem = P.E_elem;
+  op.G = P.G; op.bc = P.bcU;
+  op.E = T(P.E); op.lam1 = T(P.lam1); op.mu1 = T(P.mu1); op.eta = T(P.eta);
+  op.vol = T(P.G.hx * P.G.hy * P.G.hz);
+  op.ihx = T(1. / P.G.hx); op.ihy = T(1. / P.G.hy); op.ihz = T(1. / P.G.hz);
+  int tiles = ((P.G.nx + 14) / 15) * ((P.G.ny + 14) / 15);
+  int OZ = choose_oz(P.G, tiles);
+  hipLaunchKernelGGL((k_u_apply<T, 16, 16>), march_grid(P.G, 16, 16, OZ), dim3(256), 0, s, op, OZ);
+}
+
+void launch_u_apply(const DevProblem &P, const double *x, double *y, hipStream_t s) {
+  launch_u_apply_t<double>(P, x, y, P.V, s);
+}
+void launch_u_apply_f32(const DevProblem &P, const float *x, float *y, hipStream_t s) {
+  launch_u_apply_t<float>(P, x, y, P.V32, s);
+}
+
+void launch_u_diag(const DevProblem &P, double *d, hipStream_t s) {
+  UDiagOp op;
+  op.v = P.V; op.d = d; op.E_elem = P.E_elem; op.G = P.G; op.bc = P.bcU;
+  op.E = P.E; op.lam1 = P.lam1; op.mu1 = P.mu1; op.eta = P.eta;
+  op.vol = P.G.hx * P.G.hy * P.G.hz;
+  int tiles = ((P.G.nx + 14) / 15) * ((P.G.ny + 14) / 15);
+  int OZ = choose_oz(P.G, tiles);
+  hipLaunchKernelGGL(k_u_diag, march_grid(P.G, 16, 16, OZ), dim3(256), 0, s, op, OZ);
+}
+
+void launch_u_rhs(const DevProblem &P, const double *U, const double *BCU, double *out, int mode,
+                  hipStream_t s) {
+  URhsOp op;
+  op.U = U ? U : P.zero3; op.v = P.V; op.dth = P.dth; op.p = P.p; op.BCU = BCU;
+  op.out = out; op.E_elem = P.E_elem; op.G = P.G; op.bc = P.bcU;
+  for (int c = 0; c < 3; c++) {
+    for (int f = 0; f < 6; f++) op.ftype[c][f] = (uint8_t)P.hbcU[c].face[f];
+    for (int v = 0; v < 8; v++) op.vtype[c][v] = (uint8_t)P.hbcU[c].vertex[v];
+  }
+  op.E = P.E; op.lam1 = P.lam1; op.mu1 = P.mu1; op.eta = P.eta;
+  op.vol = P.G.hx * P.G.hy * P.G.hz; op.alpha = P.alpha; op.beta = P.beta;
+  op.crack = P.hasCrackPressure; op.mode = mode;
+  int tiles = ((P.G.nx + 14) / 15) * ((P.G.ny + 14) / 15);
+  int OZ = choose_oz(P.G, tiles);
+  hipLaunchKernelGGL(k_u_rhs, march_grid(P.G, 16, 16, OZ), dim3(256), 0, s, op, OZ);
+}
+
+}  // namespace vfg

@@ -1,0 +1,334 @@
+// vfg_v.hip -- V-subproblem (AT1/AT2 phase field) operators on gfx950.
+//
+// Element matrix (VFMech.c:2040-2115):
+//   K[a][b] = c sum_g W_g (phi_a phi_b / eps [AT2 only] + eps grad phi_a . grad phi_b)
+//           + sum_g W_g phi_a phi_b 2 W^el_g,            c = Gc / atCv / 2,
+//   W^el_g = 1/2 sigma(e~):e~,  e~ = e(u) - alpha dtheta I  (VFMech.c:32-106).
+// Split as K = ks * L + M_q with the constant Q1 stiffness L (8x8, exact for
+// the 27-point rule) and the point-weighted mass M_q, q_g = W_g (c/eps [AT2] +
+// 2 W^el_g) precomputed once per U (vfg_v_set_u) and stored [27][N_e].
+#include "vfg_internal.h"
+#include "vfg_kernels.h"
+
+namespace vfg {
+
+struct VConst {
+  double L[64];     // sum_g W_g grad phi_a . grad phi_b (element of size h)
+  double ks;        // c * eps
+  double crhs;      // Gc/atCv/eps * (.5 AT2 | .25 AT1), VFMech.c:2350-2384
+};
+
+static VConst make_vconst(const DevProblem &P) {
+  VConst C;
+  const double h[3] = {P.G.hx, P.G.hy, P.G.hz};
+  for (int a = 0; a < 8; a++)
+    for (int b = 0; b < 8; b++) {
+      double s = 0.;
+      for (int g = 0; g < 27; g++) {
+        int gx = g % 3, gy = (g / 3) % 3, gz = g / 9;
+        const int ga[3] = {gx, gy, gz};
+        double W = h[0] * h[1] * h[2] * gW(gx) * gW(gy) * gW(gz);
+        double da[3], db[3];
+        for (int d = 0; d < 3; d++) {
+          double pa = 1., pb = 1.;
+          for (int e = 0; e < 3; e++) {
+            int ia = (a >> e) & 1, ib = (b >> e) & 1;
+            if (e == d) {
+              pa *= (ia ? 1. : -1.) / h[e];
+              pb *= (ib ? 1. : -1.) / h[e];
+            } else {
+              pa *= ia ? gP1(ga[e]) : gP0(ga[e]);
+              pb *= ib ? gP1(ga[e]) : gP0(ga[e]);
+            }
+          }
+          da[d] = pa; db[d] = pb;
+        }
+        s += W * (da[0] * db[0] + da[1] * db[1] + da[2] * db[2]);
+      }
+      C.L[a * 8 + b] = s;
+    }
+  double c = P.Gc / P.atcv * .5;
+  C.ks = c * P.eps;
+  C.crhs = P.atnum == 2 ? P.Gc / P.atcv / P.eps * .5 : P.Gc / P.atcv / P.eps * .25;
+  return C;
+}
+
+// -------------------------------------------------------------- setup: Vq
+__global__ __launch_bounds__(256) void k_v_setup(DevProblem P, const double *__restrict__ U) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= P.ne) return;
+  const Grid &G = P.G;
+  const int ei = (int)(e % G.nex), ej = (int)((e / G.nex) % G.ney), ek = (int)(e / ((int64_t)G.nex * G.ney));
+  double u[8][3], th[8];
+  for (int a = 0; a < 8; a++) {
+    int64_t n = node_id(G, ei + (a & 1), ej + ((a >> 1) & 1), ek + (a >> 2));
+    for (int c = 0; c < 3; c++) u[a][c] = U[3 * n + c];
+    th[a] = P.dth[n];
+  }
+  double Ee = P.E_elem ? P.E_elem[e] : P.E;
+  double lam = P.lam1 * Ee, mu = P.mu1 * Ee;
+  const double ih[3] = {1. / G.hx, 1. / G.hy, 1. / G.hz};
+  const double vol = G.hx * G.hy * G.hz;
+  const double cm = P.atnum == 2 ? P.Gc / P.atcv * .5 / P.eps : 0.;
+  for (int g = 0; g < 27; g++) {
+    int gx = g % 3, gy = (g / 3) % 3, gz = g / 9;
+    double H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, t = 0.;
+    for (int a = 0; a < 8; a++) {
+      int i = a & 1, j = (a >> 1) & 1, k = a >> 2;
+      double px = i ? gP1(gx) : gP0(gx), py = j ? gP1(gy) : gP0(gy), pz = k ? gP1(gz) : gP0(gz);
+      double dg[3] = {(i ? ih[0] : -ih[0]) * py * pz, (j ? ih[1] : -ih[1]) * px * pz,
+                      (k ? ih[2] : -ih[2]) * px * py};
+      for (int c = 0; c < 3; c++)
+        for (int d = 0; d < 3; d++) H[c][d] += dg[d] * u[a][c];
+      t += px * py * pz * th[a];
+    }
+    double e11 = H[0][0] - P.alpha * t, e22 = H[1][1] - P.alpha * t, e33 = H[2][2] - P.alpha * t;
+    double e12 = (H[1][0] + H[0][1]) * .5, e23 = (H[2][1] + H[1][2]) * .5, e13 = (H[2][0] + H[0][2]) * .5;
+    double s11 = (lam + 2. * mu) * e11 + lam * e22 + lam * e33;
+    double s22 = lam * e11 + (lam + 2. * mu) * e22 + lam * e33;
+    double s33 = lam * e11 + lam * e22 + (lam + 2. * mu) * e33;
+    double W = (s11 * e11 + s22 * e22 + s33 * e33) * .5 + 2. * mu * (e12 * e12 + e23 * e23 + e13 * e13);
+    double Wg = vol * gW(gx) * gW(gy) * gW(gz);
+    P.Vq[(int64_t)g * P.ne + e] = Wg * (cm + 2. * W);
+  }
+}
+
+__global__ void k_v_fixmask(DevProblem P) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= P.nn) return;
+  const Grid &G = P.G;
+  int i = (int)(n % G.nx), j = (int)((n / G.nx) % G.ny), k = (int)(n / ((int64_t)G.nx * G.ny));
+  bool f = on_dirichlet(G, P.bcV.face[0], P.bcV.vert[0], i, j, k);
+  if (P.atnum == 2 && P.VIrrev[n] <= P.irrevtol) f = true;  // VF_IrrevApplyEQMat
+  P.vfix[n] = f ? 1 : 0;
+}
+
+// ------------------------------------------------------------------- apply
+// y_a = ks sum_b L[a][b] x_b + sum_g q_g phi_a(g) x(g)
+__device__ __forceinline__ void v_elem_apply(const double (&x)[8], const double *__restrict__ q,
+                                             int64_t e, int64_t ne, const VConst &C,
+                                             double (&r)[8]) {
+#pragma unroll
+  for (int a = 0; a < 8; a++) {
+    double s = 0.;
+#pragma unroll
+    for (int b = 0; b < 8; b++) s += C.L[a * 8 + b] * x[b];
+    r[a] = C.ks * s;
+  }
+  // interpolate x to the 27 points: x-direction, then y, then z
+  double ax[2][2][3];  // [k][j][gx]
+#pragma unroll
+  for (int k = 0; k < 2; k++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int gx = 0; gx < 3; gx++) ax[k][j][gx] = interp<double>(gx, x[(k * 2 + j) * 2], x[(k * 2 + j) * 2 + 1]);
+  double bz[2][2][3];  // projected back: [k][j][gx] accumulators
+#pragma unroll
+  for (int k = 0; k < 2; k++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int gx = 0; gx < 3; gx++) bz[k][j][gx] = 0.;
+#pragma unroll
+  for (int gz = 0; gz < 3; gz++)
+#pragma unroll
+    for (int gy = 0; gy < 3; gy++) {
+#pragma unroll
+      for (int gx = 0; gx < 3; gx++) {
+        double a0 = interp<double>(gy, ax[0][0][gx], ax[0][1][gx]);
+        double a1 = interp<double>(gy, ax[1][0][gx], ax[1][1][gx]);
+        double xv = interp<double>(gz, a0, a1);
+        double f = q[(int64_t)((gz * 3 + gy) * 3 + gx) * ne + e] * xv;
+        double f0 = gP0(gz) * f, f1 = gP1(gz) * f;
+        bz[0][0][gx] += gP0(gy) * f0; bz[0][1][gx] += gP1(gy) * f0;
+        bz[1][0][gx] += gP0(gy) * f1; bz[1][1][gx] += gP1(gy) * f1;
+      }
+    }
+#pragma unroll
+  for (int k = 0; k < 2; k++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      double s0 = 0., s1 = 0.;
+#pragma unroll
+      for (int gx = 0; gx < 3; gx++) { s0 += gP0(gx) * bz[k][j][gx]; s1 += gP1(gx) * bz[k][j][gx]; }
+      r[(k * 2 + j) * 2] += s0;
+      r[(k * 2 + j) * 2 + 1] += s1;
+    }
+}
+
+struct VApplyOp {
+  static constexpr int NF = 1, NC = 1;
+  const double *x;
+  double *y;
+  const double *q, *bV, *VIrrev;
+  const uint8_t *vfix;
+  Grid G;
+  int64_t ne;
+  VConst C;
+  BcMask bc;
+  uint8_t ftype[6], vtype[8];
+  int mode, atnum;
+  double irrevtol;
+  __device__ double load(int, int64_t n) const { return (mode == 0 && vfix[n]) ? 0. : x[n]; }
+  // mode 0: masked Jacobian; 1: VF_VResidual; 2: plain K x
+  __device__ void element(int ex, int ey, int ek, const double (&xin)[2][4][1], double (&r)[8][1]) const {
+    double xe[8], re[8];
+#pragma unroll
+    for (int a = 0; a < 8; a++) xe[a] = xin[a >> 2][a & 3][0];
+    v_elem_apply(xe, q, cell_id(G, ex, ey, ek), ne, C, re);
+#pragma unroll
+    for (int a = 0; a < 8; a++) r[a][0] = re[a];
+  }
+  __device__ void store(int i, int j, int k, const double (&acc)[1], const double (&)[1]) const {
+    int64_t n = node_id(G, i, j, k);
+    if (mode == 0) {
+      y[n] = vfix[n] ? x[n] : acc[0];
+      return;
+    }
+    if (mode == 2) {
+      y[n] = acc[0];
+      return;
+    }
+    // VF_VResidual: F = K V - b; AT2 irreversible rows -> 0 (VF_IrrevApplyEQVec);
+    // then ResidualApplyDirichletBC with V as its own BC vector.
+    double F = acc[0] - bV[n];
+    if (atnum == 2 && VIrrev[n] <= irrevtol) F = 0.;
+    if (on_dirichlet(G, bc.face[0], bc.vert[0], i, j, k)) {
+      int t = 0;
+      const bool onf[6] = {i == 0, i == G.nx - 1, j == 0, j == G.ny - 1, k == 0, k == G.nz - 1};
+      for (int f = 0; f < 6; f++)
+        if (ftype[f] && onf[f]) t = ftype[f];
+      int ix = (i == 0) ? 0 : (i == G.nx - 1) ? 1 : -1;
+      int iy = (j == 0) ? 0 : (j == G.ny - 1) ? 1 : -1;
+      int iz = (k == 0) ? 0 : (k == G.nz - 1) ? 1 : -1;
+      if (ix >= 0 && iy >= 0 && iz >= 0 && vtype[ix + 2 * iy + 4 * iz]) t = vtype[ix + 2 * iy + 4 * iz];
+      double v = x[n];
+      F = t == 2 ? v - 1. : t == 3 ? 0. : v;
+    }
+    y[n] = F;
+  }
+};
+
+__global__ __launch_bounds__(256) void k_v_apply(VApplyOp op, int OZ) { march<double, 16, 16>(op.G, op, OZ); }
+
+// ---------------------------------------------------------------- diagonal
+struct VDiagOp {
+  static constexpr int NF = 1, NC = 1;
+  double *d;
+  const double *q;
+  const uint8_t *vfix;
+  Grid G;
+  int64_t ne;
+  VConst C;
+  __device__ double load(int, int64_t) const { return 0.; }
+  __device__ void element(int ex, int ey, int ek, const double (&)[2][4][1], double (&r)[8][1]) const {
+    int64_t e = cell_id(G, ex, ey, ek);
+    for (int a = 0; a < 8; a++) {
+      int i = a & 1, j = (a >> 1) & 1, k = a >> 2;
+      double s = C.ks * C.L[a * 9];
+      for (int g = 0; g < 27; g++) {
+        int gx = g % 3, gy = (g / 3) % 3, gz = g / 9;
+        double ph = (i ? gP1(gx) : gP0(gx)) * (j ? gP1(gy) : gP0(gy)) * (k ? gP1(gz) : gP0(gz));
+        s += q[(int64_t)g * ne + e] * ph * ph;
+      }
+      r[a][0] = s;
+    }
+  }
+  __device__ void store(int i, int j, int k, const double (&acc)[1], const double (&)[1]) const {
+    int64_t n = node_id(G, i, j, k);
+    d[n] = vfix[n] ? 1. : acc[0];
+  }
+};
+
+__global__ __launch_bounds__(256) void k_v_diag(VDiagOp op, int OZ) { march<double, 16, 16>(op.G, op, OZ); }
+
+// ------------------------------------------------------------ load vector
+// b_a = crhs sum_g W phi_a + [pressurize] sum_g W p (u . grad phi_a)
+struct VRhsOp {
+  static constexpr int NF = 4, NC = 1;
+  const double *U, *p;
+  double *b;
+  Grid G;
+  double crhs;
+  int crack;
+  __device__ double load(int f, int64_t n) const { return f < 3 ? U[3 * n + f] : p[n]; }
+  __device__ void element(int, int, int, const double (&X)[2][4][4], double (&r)[8][1]) const {
+    const double ih[3] = {1. / G.hx, 1. / G.hy, 1. / G.hz};
+    const double vol = G.hx * G.hy * G.hz;
+    for (int g = 0; g < 27; g++) {
+      int gx = g % 3, gy = (g / 3) % 3, gz = g / 9;
+      double W = vol * gW(gx) * gW(gy) * gW(gz);
+      double ug[3] = {0., 0., 0.}, pg = 0.;
+      for (int a = 0; a < 8; a++) {
+        int i = a & 1, j = (a >> 1) & 1, k = a >> 2;
+        double ph = (i ? gP1(gx) : gP0(gx)) * (j ? gP1(gy) : gP0(gy)) * (k ? gP1(gz) : gP0(gz));
+        const double *xa = X[k][j * 2 + i];
+        for (int c = 0; c < 3; c++) ug[c] += ph * xa[c];
+        pg += ph * xa[3];
+      }
+      for (int a = 0; a < 8; a++) {
+        int i = a & 1, j = (a >> 1) & 1, k = a >> 2;
+        double px = i ? gP1(gx) : gP0(gx), py = j ? gP1(gy) : gP0(gy), pz = k ? gP1(gz) : gP0(gz);
+        double val = W * px * py * pz * crhs;
+        if (crack) {
+          double dg[3] = {(i ? ih[0] : -ih[0]) * py * pz, (j ? ih[1] : -ih[1]) * px * pz,
+                          (k ? ih[2] : -ih[2]) * px * py};
+          val += W * pg * (ug[0] * dg[0] + ug[1] * dg[1] + ug[2] * dg[2]);
+        }
+        r[a][0] += val;
+      }
+    }
+  }
+  __device__ void store(int i, int j, int k, const double (&acc)[1], const double (&)[4]) const {
+    b[node_id(G, i, j, k)] = acc[0];
+  }
+};
+
+__global__ __launch_bounds__(256) void k_v_rhs(VRhsOp op, int OZ) { march<double, 16, 16>(op.G, op, OZ); }
+
+// ------------------------------------------------------------ launchers
+static dim3 mgrid(const Grid &G, int OZ) {
+  return dim3((G.nx + 14) / 15, (G.ny + 14) / 15, (G.nz + OZ - 1) / OZ);
+}
+static int oz_for(const Grid &G) {
+  int tiles = ((G.nx + 14) / 15) * ((G.ny + 14) / 15);
+  int oz = G.nz;
+  while (oz > 8 && (int64_t)tiles * ((G.nz + oz - 1) / oz) < 2048) oz = (oz + 1) / 2;
+  return oz;
+}
+
+void launch_v_setup(const DevProblem &P, const double *U, hipStream_t s) {
+  hipLaunchKernelGGL(k_v_setup, dim3((unsigned)((P.ne + 255) / 256)), dim3(256), 0, s, P, U);
+  VRhsOp op;
+  op.U = U; op.p = P.p; op.b = P.bV; op.G = P.G;
+  op.crhs = make_vconst(P).crhs;
+  op.crack = P.hasCrackPressure;
+  int OZ = oz_for(P.G);
+  hipLaunchKernelGGL(k_v_rhs, mgrid(P.G, OZ), dim3(256), 0, s, op, OZ);
+}
+
+void launch_v_fixmask(const DevProblem &P, hipStream_t s) {
+  hipLaunchKernelGGL(k_v_fixmask, dim3((unsigned)((P.nn + 255) / 256)), dim3(256), 0, s, P);
+}
+
+void launch_v_apply(const DevProblem &P, const double *x, double *y, int mode, hipStream_t s,
+                    const uint8_t *mask) {
+  VApplyOp op;
+  op.x = x; op.y = y; op.q = P.Vq; op.bV = P.bV; op.VIrrev = P.VIrrev; op.vfix = mask ? mask : P.vfix;
+  op.G = P.G; op.ne = P.ne; op.C = make_vconst(P); op.bc = P.bcV;
+  for (int f = 0; f < 6; f++) op.ftype[f] = (uint8_t)P.hbcV.face[f];
+  for (int v = 0; v < 8; v++) op.vtype[v] = (uint8_t)P.hbcV.vertex[v];
+  op.mode = mode; op.atnum = P.atnum; op.irrevtol = P.irrevtol;
+  int OZ = oz_for(P.G);
+  hipLaunchKernelGGL(k_v_apply, mgrid(P.G, OZ), dim3(256), 0, s, op, OZ);
+}
+
+void launch_v_diag(const DevProblem &P, double *d, hipStream_t s, const uint8_t *mask) {
+  VDiagOp op;
+  op.d = d; op.q = P.Vq; op.vfix = mask ? mask : P.vfix; op.G = P.G; op.ne = P.ne; op.C = make_vconst(P);
+  int OZ = oz_for(P.G);
+  hipLaunchKernelGGL(k_v_diag, mgrid(P.G, OZ), dim3(256), 0, s, op, OZ);
+}
+
+}  // namespace vfg
