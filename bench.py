@@ -1,0 +1,169 @@
+"""Benchmark: GDoF/s of the damaged-elasticity apply (MatShell MatMult of the
+u-Jacobian) on one MI355X, plus u/V solve throughput, with the roofline of the
+apply kernel and the reference algorithm's CPU path timed on the host cores.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128] [--no-cpu] [--no-solves]
+
+Workload (N=1): 128^3 Q1 hex elements (6.44 M displacement dof), lognormal E
+(log-std 0.5, correlation 8 cells), 3 random penny cracks, smooth pore
+pressure in [0, 2e-3], uniform temperature rise, clamped faces, FP64.  A step
+is one apply y = A_u x over the whole grid; inputs are resident in HBM and
+the x/y pairs rotate over 4 buffers (>256 MB) so the last-level cache cannot
+serve them.  Prints one JSON line.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+
+
+def apply_bytes(nn, ne, fp32=False, hetero=True):
+    """Algorithmic HBM bytes of one apply: read x and V, write y (per node),
+    read E (per element)."""
+    w = 4 if fp32 else 8
+    return (3 * w + 3 * w + w) * nn + (8 if hetero else 0) * ne
+
+
+def read_traffic(name):
+    p = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(p):
+        return None
+    try:
+        d = json.load(open(p))
+        return d.get("hbm_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def cpu_baseline(n_sample=48, seconds=3.0):
+    """The reference's per-iteration kernel on the host: the assembled AIJ
+    u-Jacobian (VF_UIJacobian + MatApplyDirichletBC, 81 nnz/row) times a
+    vector, from the oracle's C restatement, OpenMP over rows."""
+    from oracle import vforacle as O
+    from vpfhf_amd import inputs
+    O.lib()
+    n = n_sample
+    E = inputs.lognormal_E((n, n, n), .5, 8., 0)
+    bc = [O.make_bc({f: O.ZERO for f in range(6)})] * 3
+    op = O.OracleProblem((n + 1,) * 3, E=1., nu=.2, beta=1., atnum=1, epsilon=2. / n, pressurize=True,
+                         E_elem=E, bcU=bc)
+    V = O.penny_v(op, (.5, .5, .5), .2, 0., 0., 3. / n)
+    F = op.fields(V=V, VIrrev=V, theta=np.zeros(op.nn), thetaRef=np.zeros(op.nn), pressure=np.full(op.nn, 1e-3))
+    t0 = time.time()
+    A = O.u_jacobian(op, F, csr=False)
+    t_asm = time.time() - t0
+    x = np.random.default_rng(0).standard_normal(3 * op.nn)
+    O.ell_matvec(A, x)
+    reps, t0 = 0, time.time()
+    while time.time() - t0 < seconds:
+        O.ell_matvec(A, x)
+        reps += 1
+    t = (time.time() - t0) / reps
+    O.lib().vfo_ell_free(A)
+    return {"value": round(3 * op.nn / t / 1e9, 4), "unit": "GDoF/s", "cores": int(O.lib().vfo_num_threads()),
+            "kind": "port",
+            "sample": "assembled AIJ MatMult (81 nnz/row, FP64) of the %d^3-element u-Jacobian, %d products "
+                      "after a %.1f s assembly; oracle C restatement of VF_UIJacobian" % (n, reps, t_asm)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-solves", action="store_true")
+    ap.add_argument("--seed", type=int, default=0)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: every box here has one MI355X and this path has no multi-GPU "
+              "decomposition to run (see DESIGN.md for how it would shard). Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import vpfhf_amd as V
+    from vpfhf_amd import _lib, workloads
+
+    spec = workloads.spec_hetero(a.n, seed=a.seed)
+    H, fields = workloads.make_host(spec, u_rtol=1e-10)
+    L = _lib.load()
+    h = H.h
+    nn, ne = H.nn, H.ne
+    nbuf = 4
+    rng = np.random.default_rng(1)
+    xs = [V.DeviceBuffer(24 * nn).upload(rng.standard_normal(3 * nn)) for _ in range(nbuf)]
+    ys = [V.DeviceBuffer(24 * nn) for _ in range(nbuf)]
+    xa = (C.c_void_p * nbuf)(*[b.ptr for b in xs])
+    ya = (C.c_void_p * nbuf)(*[b.ptr for b in ys])
+    ms = C.c_double()
+    _lib.check(L.vfg_synchronize(h), h)
+    _lib.check(L.vfg_time_u_apply(h, xa, ya, nbuf, max(a.warmup, 1), 0, C.byref(ms)), h)
+    _lib.check(L.vfg_time_u_apply(h, xa, ya, nbuf, a.steps, 0, C.byref(ms)), h)
+    t_step = ms.value * 1e-3 / a.steps
+    gdofs = 3 * nn / t_step / 1e9
+    byt = apply_bytes(nn, ne)
+    achieved = byt / t_step / 1e9
+    traffic = read_traffic("u_apply_hetero%d_traffic.json" % a.n)
+
+    # FP32 apply (precision form) on the same operator
+    xs32 = [V.DeviceBuffer(12 * nn).upload(rng.standard_normal(3 * nn).astype(np.float32)) for _ in range(nbuf)]
+    ys32 = [V.DeviceBuffer(12 * nn) for _ in range(nbuf)]
+    xa32 = (C.c_void_p * nbuf)(*[b.ptr for b in xs32])
+    ya32 = (C.c_void_p * nbuf)(*[b.ptr for b in ys32])
+    _lib.check(L.vfg_time_u_apply(h, xa32, ya32, nbuf, max(a.warmup, 1), 1, C.byref(ms)), h)
+    _lib.check(L.vfg_time_u_apply(h, xa32, ya32, nbuf, a.steps, 1, C.byref(ms)), h)
+    t32 = ms.value * 1e-3 / a.steps
+    for b in xs + ys + xs32 + ys32:
+        b.free()
+
+    extra = {}
+    if not a.no_solves:
+        # one alternate-minimisation iteration (u-solve to rtol 1e-10, then the
+        # bound-constrained V-solve) of the same problem
+        t0 = time.time()
+        info = H.altmin(maxit=1)
+        dt = time.time() - t0
+        en = H.energies()
+        extra["solves"] = {"u_cg_iterations": info.u_its, "v_newton": info.v_outer, "v_cg": info.v_its,
+                           "seconds_per_altmin_iteration": round(dt, 3), "solves_per_s": round(2.0 / dt, 3),
+                           "surface_energy": en["surface"], "crack_volume": en["crack_volume"]}
+
+    out = {
+        "metric": "GDoF/s elasticity apply; solves/s and p99 latency at 1e-6 rel err vs FP64",
+        "value": round(gdofs, 3),
+        "unit": "GDoF/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(t_step * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded lognormal E, penny cracks, pressure field)",
+        "config": {"workload": "u-Jacobian MatShell apply, %d^3 Q1 hex elements, %d displacement dof, FP64 "
+                               "plain GPU drop-in, lognormal E, 3 penny cracks, clamped" % (a.n, 3 * nn),
+                   "elements": ne, "dof": 3 * nn},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic},
+        "fp32_apply": {"ms": round(t32 * 1e3, 4), "gdofs": round(3 * nn / t32 / 1e9, 3),
+                       "achieved_GBs": round(apply_bytes(nn, ne, fp32=True) / t32 / 1e9, 1)},
+    }
+    out.update(extra)
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    H.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -180,6 +180,16 @@ int vfg_u_solve(vfg_handle *h, double *U, const double *BCU, const vfg_solve_opt
 int vfg_v_solve(vfg_handle *h, const double *U, double *V, const double *ub,
                 const vfg_solve_opts *o, vfg_solve_info *info, int loc);
 
+/* ------------------------------------------------- device vector helpers */
+/* The PETSc Vec operations the host loop of the reference performs on the
+ * fields (VecCopy, VecPointwiseMin, VecAXPY+VecNorm(NORM_INFINITY), VecScale,
+ * VecSet; VFCommon.c:827-857, 1141, 1209-1215), on device arrays of length n. */
+int vfg_vec_copy(vfg_handle *h, const double *x, double *y, int64_t n);
+int vfg_vec_pointwise_min(vfg_handle *h, const double *a, const double *b, double *out, int64_t n);
+int vfg_vec_maxabs_diff(vfg_handle *h, const double *a, const double *b, int64_t n, double *out);
+int vfg_vec_scale(vfg_handle *h, double *x, double s, int64_t n);
+int vfg_vec_set(vfg_handle *h, double *x, double v, int64_t n);
+
 /* ------------------------------------------- device memory for callers */
 int vfg_device_alloc(size_t bytes, void **ptr);
 int vfg_device_free(void *ptr);

@@ -1,0 +1,91 @@
+/*
+ * vfhost.h -- C host layer above the vfg C-ABI: the parts of the reference's
+ * context/driver code that sit on the u/V path, with every field resident on
+ * the GPU.
+ *
+ *   vfh_options_*          VFCtxGet / VFPropGet / VFMatPropGet / VFGeometryInitialize
+ *                          option subset and defaults (VFCommon.c:90-588),
+ *                          VFBCSetFromOptions (VFCartFE.c:1506-1574),
+ *                          VFPennyCrackGet (VFCracks.c:14-31)
+ *   vfh_fields_initialize  V / VIrrev from penny cracks (VFCommon.c:822-857)
+ *   vfh_timestep_prepare   VFTimeStepPrepare (VFCommon.c:1132-1157)
+ *   vfh_altmin             the alternate-minimisation loop of
+ *                          VFFractureTimeStep (VFCommon.c:1192-1231) and
+ *                          test1.c:53-69: u-solve, V-solve, ||V - Vold||_inf
+ *   vfh_energies           VF_UEnergy3D + VF_VEnergy3D + VolumetricCrackOpening
+ *
+ * Errors: negative VFG_E_* codes; vfh_last_error() gives the message.
+ */
+#ifndef VFHOST_H
+#define VFHOST_H
+#include "vfg.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VFH_MAX_PENNY 64
+
+typedef struct {
+  double center[3], r, theta, phi, thickness;
+} vfh_penny;
+
+typedef struct {
+  vfg_grid  grid;          /* -n nx,ny,nz -l lx,ly,lz */
+  vfg_props props;         /* -E -nu -alpha -beta -Gc -epsilon -eta -irrevtol -atnum -pressurize */
+  vfg_bc    bcU[3], bcV;   /* -U_<ENT>_BC_<c> TYPE, -U_<FACE>_<c> val, -V_<ENT>_BC TYPE */
+  double    altmintol;     /* -altmintol (1e-4) */
+  int       altminmaxit;   /* -altminmaxit (10000) */
+  double    u_rtol;        /* -U_ksp_rtol (1e-8, the KSP default of VFSolversInitialize) */
+  double    v_rtol;        /* -V_snes_rtol (1e-8) */
+  int       npc;           /* -npc */
+  vfh_penny pc[VFH_MAX_PENNY]; /* -pc<i>_center -pc<i>_r -pc<i>_theta -pc<i>_phi -pc<i>_thickness */
+  int       precision;     /* VFG_PREC_FP64 (default) or VFG_PREC_MIXED (-vfg_precision mixed) */
+  int       pc_type;       /* VFG_PC_JACOBI (default) or VFG_PC_CHEBYSHEV (-vfg_pc chebyshev) */
+} vfh_options;
+
+typedef struct vfh_ctx vfh_ctx;
+
+typedef struct {
+  int    iterations;   /* alternate-minimisation iterations done */
+  int    u_its;        /* CG iterations of all u-solves */
+  int    v_outer;      /* active-set Newton steps of all V-solves */
+  int    v_its;        /* CG iterations of all V-solves */
+  double errV;         /* last ||V - Vold||_inf */
+  int    status;       /* 0 or VFG_E_* of the first failing solve */
+} vfh_altmin_info;
+
+void vfh_options_default(vfh_options *o);
+/* PETSc-style option text ("-key value" tokens, '#' comments, case-insensitive
+ * keys); may be called repeatedly (options files, then command line). */
+int  vfh_options_parse(vfh_options *o, const char *text);
+const char *vfh_last_error(const vfh_ctx *ctx);
+
+int  vfh_create(const vfh_options *o, vfh_ctx **out);
+int  vfh_destroy(vfh_ctx *ctx);
+vfg_handle *vfh_handle(vfh_ctx *ctx);
+/* device arrays owned by the context: U, BCU (3 N_n), V, VIrrev (N_n) */
+int  vfh_device_fields(vfh_ctx *ctx, double **U, double **BCU, double **V, double **VIrrev);
+
+/* material / loading inputs (host arrays, NULL keeps): per-element E,
+ * temperature, reference temperature, pressure */
+int  vfh_set_inputs(vfh_ctx *ctx, const double *E_elem, const double *theta, const double *thetaRef,
+                    const double *pressure);
+/* V = VIrrev = min over the penny cracks of their V profiles (or 1) */
+int  vfh_fields_initialize(vfh_ctx *ctx);
+/* BCU = scale * VecSetFromBC(bcU)  (test1.c:49-50 scales by the time value) */
+int  vfh_set_bcu(vfh_ctx *ctx, double scale);
+int  vfh_timestep_prepare(vfh_ctx *ctx);
+/* maxit <= 0: run until ||V - Vold|| < altmintol or altminmaxit (the
+ * do-while of test1.c:53-69); maxit > 0: exactly min(maxit, altminmaxit)
+ * iterations unless converged earlier when stop_on_tol is set */
+int  vfh_altmin(vfh_ctx *ctx, int maxit, int stop_on_tol, vfh_altmin_info *info);
+/* out: elastic, in-situ work (0), pressure work, surface energy, crack volume */
+int  vfh_energies(vfh_ctx *ctx, double out[5]);
+int  vfh_get_state(vfh_ctx *ctx, double *U, double *V, double *VIrrev);
+int  vfh_set_state(vfh_ctx *ctx, const double *U, const double *V, const double *VIrrev);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1166,3 +1166,7 @@ int vfo_v_solve(const vfo_problem *P, const vfo_fields *F, const double *U, doub
   vfo_ell_free(K);
   return it < maxit ? it : -it;
 }
+
+/* threads the OpenMP loops of the oracle use (for the CPU-baseline report) */
+#include <omp.h>
+int vfo_num_threads(void) { return omp_get_max_threads(); }

@@ -186,6 +186,8 @@ int vfo_u_solve(const vfo_problem *P, const vfo_fields *F, double *U,
 int vfo_v_solve(const vfo_problem *P, const vfo_fields *F, const double *U,
                 double *V, const double *ub, double tol, int maxit);
 
+int vfo_num_threads(void);
+
 #ifdef __cplusplus
 }
 #endif

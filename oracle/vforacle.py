@@ -105,6 +105,7 @@ def lib():
         L.vfo_v_solve.restype = C.c_int
         L.vfo_pcg.argtypes = [Ep, dp, dp, C.c_double, C.c_double, C.c_int, dp]
         L.vfo_pcg.restype = C.c_int
+        L.vfo_num_threads.restype = C.c_int
         L.vfo_elastic_energy_density.argtypes = [dp, dp, dp, C.c_double, C.c_double, C.c_double, C.POINTER(Elem)]
         L.vfo_bilinear_v_at.argtypes = [dp, C.c_int, C.c_double, C.c_double, C.c_double, C.POINTER(Elem)]
         L.vfo_bilinear_v_coupling.argtypes = [dp, dp, dp, C.c_double, C.c_double, C.c_double, C.POINTER(Elem)]

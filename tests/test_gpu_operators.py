@@ -47,6 +47,10 @@ def test_bc_vectors_exact(pair, oracle):
     gv = P.VecApplyDirichletBC(s["V"].copy(), s["V"], "V")
     rv = oracle.vec_apply_dirichlet(op, s["V"].copy(), s["V"].copy(), "V")
     assert np.array_equal(gv, rv)
+    # in place (vec is its own BC vector), VFCommon.c:1146
+    vin = s["V"].copy()
+    rv2 = oracle.lib().vfo_vec_apply_dirichlet(op.ref, oracle.C.pointer(op.P.bcV), 1, oracle._dp(vin), oracle._dp(vin))
+    assert np.array_equal(P.VecApplyDirichletBC(s["V"].copy(), None, "V"), vin)
 
 
 def test_u_apply_matches_assembled(pair, oracle):

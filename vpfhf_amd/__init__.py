@@ -198,11 +198,13 @@ class Problem:
         self._c(self.L.vfg_vec_set_from_bc(self.h, w, _ptr(B), HOST))
         return B
 
-    def VecApplyDirichletBC(self, vec, B, which="U"):
+    def VecApplyDirichletBC(self, vec, B=None, which="U"):
+        """B=None applies vec's own values as the BC vector, in place
+        (VecApplyDirichletBC(V, V, bcV) of VFTimeStepPrepare)."""
         w = 0 if which == "U" else 1
         vec = np.ascontiguousarray(vec, np.float64)
-        self._c(self.L.vfg_vec_apply_dirichlet(self.h, w, _ptr(vec), _ptr(np.ascontiguousarray(B, np.float64)),
-                                               HOST))
+        Bp = _ptr(vec) if B is None else _ptr(np.ascontiguousarray(B, np.float64))
+        self._c(self.L.vfg_vec_apply_dirichlet(self.h, w, _ptr(vec), Bp, HOST))
         return vec
 
     # -- diagnostics ---------------------------------------------------------

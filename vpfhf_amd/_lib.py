@@ -86,6 +86,11 @@ PROTOS = {
     "vfg_solve_opts_default": (None, [C.POINTER(SolveOpts)]),
     "vfg_u_solve": (C.c_int, [H, vp, vp, C.POINTER(SolveOpts), C.POINTER(SolveInfo), C.c_int]),
     "vfg_v_solve": (C.c_int, [H, vp, vp, vp, C.POINTER(SolveOpts), C.POINTER(SolveInfo), C.c_int]),
+    "vfg_vec_copy": (C.c_int, [H, vp, vp, C.c_int64]),
+    "vfg_vec_pointwise_min": (C.c_int, [H, vp, vp, vp, C.c_int64]),
+    "vfg_vec_maxabs_diff": (C.c_int, [H, vp, vp, C.c_int64, dp]),
+    "vfg_vec_scale": (C.c_int, [H, vp, C.c_double, C.c_int64]),
+    "vfg_vec_set": (C.c_int, [H, vp, C.c_double, C.c_int64]),
     "vfg_device_alloc": (C.c_int, [C.c_size_t, C.POINTER(vp)]),
     "vfg_device_free": (C.c_int, [vp]),
     "vfg_memcpy": (C.c_int, [vp, vp, C.c_size_t, C.c_int]),

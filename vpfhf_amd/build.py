@@ -43,8 +43,16 @@ def build(verbose=False):
     if p.returncode != 0:
         raise RuntimeError("link failed:\n%s" % p.stderr[-4000:])
     os.replace(tmp, LIB)
+    # C host layer (plain C, gcc) over the C-ABI
+    host = os.path.join(HERE, "libvfhost.so")
+    cmd = ["gcc", "-O2", "-std=c11", "-fPIC", "-shared", "-Wall", "-o", host + ".tmp",
+           os.path.join(HERE, "host", "vfhost.c"), "-L" + HERE, "-lvfg", "-Wl,-rpath,$ORIGIN"]
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError("gcc failed for vfhost.c:\n%s" % p.stderr[-4000:])
+    os.replace(host + ".tmp", host)
     if verbose:
-        print("built", LIB)
+        print("built", LIB, host)
     return LIB
 
 

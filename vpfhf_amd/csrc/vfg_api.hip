@@ -411,8 +411,10 @@ int vfg_vec_apply_dirichlet(vfg_handle *h, int which, double *vec, const double 
   }
   int rc;
   if ((rc = vfg_scratch(h, (int64_t)(2 * nb / 8)))) return rc;
-  double *dv = h->stage, *dB = h->stage + nb / 8;
-  if ((rc = vfg_in(h, dv, vec, nb, VFG_HOST)) || (rc = vfg_in(h, dB, B, nb, VFG_HOST))) return rc;
+  // the same host array for vec and B keeps the reference's in-place semantics
+  const bool alias = (const void *)vec == (const void *)B;
+  double *dv = h->stage, *dB = alias ? h->stage : h->stage + nb / 8;
+  if ((rc = vfg_in(h, dv, vec, nb, VFG_HOST)) || (!alias && (rc = vfg_in(h, dB, B, nb, VFG_HOST)))) return rc;
   launch_vec_apply_dirichlet(h->P, which, dv, dB, h->stream);
   HIPCHK(h, hipMemcpyAsync(vec, dv, nb, hipMemcpyDeviceToHost, h->stream));
   return vfg_synchronize(h);
@@ -474,6 +476,39 @@ int vfg_penny_build_v(vfg_handle *h, double *V, const double center[3], double r
   if (io.rc) return io.rc;
   launch_penny(h->P, (double *)io.dout, center, r, theta_deg, phi_deg, thickness, h->stream);
   return io.finish();
+}
+
+// ------------------------------------------------------ device vector helpers
+int vfg_vec_copy(vfg_handle *h, const double *x, double *y, int64_t n) {
+  NEED_H(h);
+  if (!x || !y || n < 0) return vfg_fail(h, VFG_E_ARG, "vfg_vec_copy: bad arguments");
+  HIPCHK(h, hipMemcpyAsync(y, x, 8 * n, hipMemcpyDeviceToDevice, h->stream));
+  return VFG_OK;
+}
+int vfg_vec_pointwise_min(vfg_handle *h, const double *a, const double *b, double *out, int64_t n) {
+  NEED_H(h);
+  if (!a || !b || !out || n < 0) return vfg_fail(h, VFG_E_ARG, "vfg_vec_pointwise_min: bad arguments");
+  launch_vmin(a, b, out, n, h->stream);
+  return VFG_OK;
+}
+int vfg_vec_maxabs_diff(vfg_handle *h, const double *a, const double *b, int64_t n, double *out) {
+  NEED_H(h);
+  if (!a || !b || !out || n < 0) return vfg_fail(h, VFG_E_ARG, "vfg_vec_maxabs_diff: bad arguments");
+  launch_maxabs_diff(a, b, n, h->part, h->scal + 32, h->stream);
+  HIPCHK(h, hipMemcpyAsync(out, h->scal + 32, 8, hipMemcpyDeviceToHost, h->stream));
+  return vfg_synchronize(h);
+}
+int vfg_vec_scale(vfg_handle *h, double *x, double v, int64_t n) {
+  NEED_H(h);
+  if (!x || n < 0) return vfg_fail(h, VFG_E_ARG, "vfg_vec_scale: bad arguments");
+  launch_scale(x, v, n, h->stream);
+  return VFG_OK;
+}
+int vfg_vec_set(vfg_handle *h, double *x, double v, int64_t n) {
+  NEED_H(h);
+  if (!x || n < 0) return vfg_fail(h, VFG_E_ARG, "vfg_vec_set: bad arguments");
+  launch_fill(x, v, n, h->stream);
+  return VFG_OK;
 }
 
 // ------------------------------------------------------------ memory helpers

@@ -62,6 +62,9 @@ void launch_dot(const double *a, const double *b, int64_t n, double *partials, d
                 hipStream_t s);
 void launch_fill(double *x, double v, int64_t n, hipStream_t s);
 void launch_sub(const double *a, const double *b, double *out, int64_t n, hipStream_t s);
+void launch_vmin(const double *a, const double *b, double *o, int64_t n, hipStream_t s);
+void launch_scale(double *x, double v, int64_t n, hipStream_t s);
+void launch_maxabs_diff(const double *a, const double *b, int64_t n, double *part, double *out, hipStream_t s);
 void launch_copy_d2f(const double *x, float *y, int64_t n, hipStream_t s);
 void launch_copy_f2d(const float *x, double *y, int64_t n, hipStream_t s);
 

@@ -27,15 +27,18 @@ static BcTable bc_table(const DevProblem &P, int which) {
 // 493-738).  Returns the BC type (0 if the dof is free) and the value it
 // writes: FIXED takes faceValue/vertexValue (mode 0, VecSetFromBC) or B[d]
 // (mode 1, VecApplyDirichletBC).
+// With alias (vec and B are the same array, as in VecApplyDirichletBC(V,V,bcV)
+// of VFTimeStepPrepare, VFCommon.c:1146) a FIXED entity copies the value the
+// previous writer just stored, exactly as the in-place reference loops do.
 __device__ int bc_last(const Grid &G, const BcTable &t, int c, int i, int j, int k, double Bd, int mode,
-                       double *val) {
+                       int alias, double *val) {
   int ty = 0;
-  double v = 0.;
+  double v = Bd;
   const bool onf[6] = {i == 0, i == G.nx - 1, j == 0, j == G.ny - 1, k == 0, k == G.nz - 1};
   for (int f = 0; f < 6; f++)
     if (t.face[c][f] && onf[f]) {
       ty = t.face[c][f];
-      v = ty == VFG_ZERO ? 0. : ty == VFG_ONE ? 1. : (mode == 0 ? t.faceValue[c][f] : Bd);
+      v = ty == VFG_ZERO ? 0. : ty == VFG_ONE ? 1. : (mode == 0 ? t.faceValue[c][f] : alias ? v : Bd);
     }
   int ix = (i == 0) ? 0 : (i == G.nx - 1) ? 1 : -1;
   int iy = (j == 0) ? 0 : (j == G.ny - 1) ? 1 : -1;
@@ -44,21 +47,22 @@ __device__ int bc_last(const Grid &G, const BcTable &t, int c, int i, int j, int
     int vv = ix + 2 * iy + 4 * iz;
     if (t.vert[c][vv]) {
       ty = t.vert[c][vv];
-      v = ty == VFG_ZERO ? 0. : ty == VFG_ONE ? 1. : (mode == 0 ? t.vertValue[c][vv] : Bd);
+      v = ty == VFG_ZERO ? 0. : ty == VFG_ONE ? 1. : (mode == 0 ? t.vertValue[c][vv] : alias ? v : Bd);
     }
   }
-  *val = v;
+  *val = ty ? v : 0.;
   return ty;
 }
 
 __global__ void k_bc_vec(Grid G, BcTable t, uint8_t *mask, double *vec, const double *B, int mode) {
+  const int alias = (mode == 1 && (const double *)vec == B);
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= (int64_t)G.nx * G.ny * G.nz) return;
   int i = (int)(n % G.nx), j = (int)((n / G.nx) % G.ny), k = (int)(n / ((int64_t)G.nx * G.ny));
   for (int c = 0; c < t.dof; c++) {
     int64_t d = t.dof * n + c;
     double val;
-    int ty = bc_last(G, t, c, i, j, k, B ? B[d] : 0., mode, &val);
+    int ty = bc_last(G, t, c, i, j, k, B ? B[d] : 0., mode, alias, &val);
     if (mask) mask[d] = ty ? 1 : 0;
     if (vec && ty) vec[d] = val;
   }
@@ -256,6 +260,45 @@ void launch_copy_d2f(const double *x, float *y, int64_t n, hipStream_t s) {
 }
 void launch_copy_f2d(const float *x, double *y, int64_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_f2d, dim3(nblk(n)), dim3(256), 0, s, x, y, n);
+}
+
+__global__ void k_vmin(const double *a, const double *b, double *o, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) o[i] = fmin(a[i], b[i]);
+}
+__global__ void k_scale(double *x, double s, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] *= s;
+}
+__global__ __launch_bounds__(RB) void k_maxabs(const double *a, const double *b, int64_t n, double *part) {
+  __shared__ double sh[RB];
+  double m = 0.;
+  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB)
+    m = fmax(m, fabs(a[i] - b[i]));
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int o = RB / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+__global__ void k_maxreduce(const double *part, int nb, double *out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double m = 0.;
+  for (int i = 0; i < nb; i++) m = fmax(m, part[i]);
+  *out = m;
+}
+void launch_vmin(const double *a, const double *b, double *o, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_vmin, dim3(nblk(n)), dim3(256), 0, s, a, b, o, n);
+}
+void launch_scale(double *x, double v, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_scale, dim3(nblk(n)), dim3(256), 0, s, x, v, n);
+}
+void launch_maxabs_diff(const double *a, const double *b, int64_t n, double *part, double *out, hipStream_t s) {
+  int nb = red_blocks(n);
+  hipLaunchKernelGGL(k_maxabs, dim3(nb), dim3(RB), 0, s, a, b, n, part);
+  hipLaunchKernelGGL(k_maxreduce, dim3(1), dim3(64), 0, s, (const double *)part, nb, out);
 }
 
 }  // namespace vfg

@@ -1,0 +1,156 @@
+"""ctypes binding of the C host layer (``libvfhost.so``, include/vfhost.h).
+
+``Host`` wraps one ``vfh_ctx``: the option subset of the reference's
+context (same option names: ``-n``, ``-l``, ``-nu``, ``-atnum``,
+``-U_X0_BC_0 ZERO``, ``-pc0_center`` ...), device-resident U, BCU, V, VIrrev,
+VFTimeStepPrepare and the alternate-minimisation loop.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _lib
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libvfhost.so")
+MAX_PENNY = 64
+
+
+class Penny(C.Structure):
+    _fields_ = [("center", C.c_double * 3), ("r", C.c_double), ("theta", C.c_double), ("phi", C.c_double),
+                ("thickness", C.c_double)]
+
+
+class Options(C.Structure):
+    _fields_ = [("grid", _lib.Grid), ("props", _lib.Props), ("bcU", _lib.BC * 3), ("bcV", _lib.BC),
+                ("altmintol", C.c_double), ("altminmaxit", C.c_int), ("u_rtol", C.c_double),
+                ("v_rtol", C.c_double), ("npc", C.c_int), ("pc", Penny * MAX_PENNY),
+                ("precision", C.c_int), ("pc_type", C.c_int)]
+
+
+class AltminInfo(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("u_its", C.c_int), ("v_outer", C.c_int), ("v_its", C.c_int),
+                ("errV", C.c_double), ("status", C.c_int)]
+
+
+_hl = None
+
+
+def load():
+    global _hl
+    if _hl is None:
+        _lib.load()  # libvfg.so first (libvfhost links it)
+        if not os.path.exists(LIBPATH):
+            raise ImportError("libvfhost.so not built: run `python -m vpfhf_amd.build`")
+        L = C.CDLL(LIBPATH)
+        vp, dp, O = C.c_void_p, C.POINTER(C.c_double), C.POINTER(Options)
+        protos = {
+            "vfh_options_default": (None, [O]),
+            "vfh_options_parse": (C.c_int, [O, C.c_char_p]),
+            "vfh_last_error": (C.c_char_p, [vp]),
+            "vfh_create": (C.c_int, [O, C.POINTER(vp)]),
+            "vfh_destroy": (C.c_int, [vp]),
+            "vfh_handle": (vp, [vp]),
+            "vfh_device_fields": (C.c_int, [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)]),
+            "vfh_set_inputs": (C.c_int, [vp, vp, vp, vp, vp]),
+            "vfh_fields_initialize": (C.c_int, [vp]),
+            "vfh_set_bcu": (C.c_int, [vp, C.c_double]),
+            "vfh_timestep_prepare": (C.c_int, [vp]),
+            "vfh_altmin": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(AltminInfo)]),
+            "vfh_energies": (C.c_int, [vp, dp]),
+            "vfh_get_state": (C.c_int, [vp, vp, vp, vp]),
+            "vfh_set_state": (C.c_int, [vp, vp, vp, vp]),
+        }
+        for name, (res, args) in protos.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _hl = L
+    return _hl
+
+
+def options(text=""):
+    o = Options()
+    L = load()
+    L.vfh_options_default(C.byref(o))
+    if text:
+        rc = L.vfh_options_parse(C.byref(o), text.encode())
+        if rc:
+            raise _lib.VfgError(rc, L.vfh_last_error(None).decode())
+    return o
+
+
+def _p(a):
+    return None if a is None else np.ascontiguousarray(a, np.float64).ctypes.data
+
+
+class Host:
+    """One problem driven by the C host (device-resident state)."""
+
+    def __init__(self, opts):
+        self.L = load()
+        self.opts = opts
+        ctx = C.c_void_p()
+        self._c(self.L.vfh_create(C.byref(opts), C.byref(ctx)), None)
+        self.ctx = ctx.value
+        g = opts.grid
+        self.nn = g.nx * g.ny * g.nz
+        self.ne = (g.nx - 1) * (g.ny - 1) * (g.nz - 1)
+        self.h = self.L.vfh_handle(self.ctx)
+
+    def _c(self, rc, ctx="self"):
+        if rc:
+            c = self.ctx if ctx == "self" else ctx
+            raise _lib.VfgError(rc, self.L.vfh_last_error(c).decode())
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.L.vfh_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def device_fields(self):
+        ps = [C.c_void_p() for _ in range(4)]
+        self._c(self.L.vfh_device_fields(self.ctx, *[C.byref(p) for p in ps]))
+        return [p.value for p in ps]
+
+    def set_inputs(self, E_elem=None, theta=None, thetaRef=None, pressure=None):
+        keep = [None if a is None else np.ascontiguousarray(a, np.float64) for a in (E_elem, theta, thetaRef, pressure)]
+        self._c(self.L.vfh_set_inputs(self.ctx, *[None if a is None else a.ctypes.data for a in keep]))
+
+    def fields_initialize(self):
+        self._c(self.L.vfh_fields_initialize(self.ctx))
+
+    def set_bcu(self, scale=1.0):
+        self._c(self.L.vfh_set_bcu(self.ctx, scale))
+
+    def timestep_prepare(self):
+        self._c(self.L.vfh_timestep_prepare(self.ctx))
+
+    def altmin(self, maxit=0, stop_on_tol=False):
+        info = AltminInfo()
+        rc = self.L.vfh_altmin(self.ctx, maxit, int(stop_on_tol), C.byref(info))
+        self._c(rc)
+        return info
+
+    def energies(self):
+        out = (C.c_double * 5)()
+        self._c(self.L.vfh_energies(self.ctx, out))
+        return dict(elastic=out[0], insitu=out[1], pressure_work=out[2], surface=out[3], crack_volume=out[4])
+
+    def state(self):
+        U = np.empty(3 * self.nn)
+        V = np.empty(self.nn)
+        VI = np.empty(self.nn)
+        self._c(self.L.vfh_get_state(self.ctx, U.ctypes.data, V.ctypes.data, VI.ctypes.data))
+        return U, V, VI
+
+    def set_state(self, U=None, V=None, VIrrev=None):
+        keep = [None if a is None else np.ascontiguousarray(a, np.float64) for a in (U, V, VIrrev)]
+        self._c(self.L.vfh_set_state(self.ctx, *[None if a is None else a.ctypes.data for a in keep]))

@@ -1,0 +1,132 @@
+"""Problem builders for the benchmark workloads (unit cube, clamped faces).
+
+``penny32``  -- 32^3 elements, E = 1, nu = 0.2, Biot beta = 1, one centred
+               penny crack r = 0.2 normal to z (V = 0 within 1.5 cells),
+               p = 1e-3 with -pressurize, AT1, Gc = 1, eps = 2h, u-CG rtol 1e-10.
+``hetero``   -- n^3 elements (128 for the plain FP64 drop-in, 256 for the
+               precision form): lognormal E (log-std 0.5, correlation 8 cells),
+               3 random penny cracks r in [0.1, 0.25], smooth pore pressure in
+               [0, 2e-3], uniform temperature rise in [0, 10] with alpha = 1e-5.
+``ensemble`` -- per-member size and seeds for the batched form.
+
+Option mapping to the reference (VFCommon.c, VFCracks.c): "-n N+1,N+1,N+1
+-l 1,1,1 -nu 0.2 -beta 1 -atnum 1 -epsilon 2h -pressurize -npc k
+-pc<i>_center/_r/_theta/_phi/_thickness" with BCfiles/3DClamped.txt.
+"""
+import math
+
+import numpy as np
+
+from . import inputs
+
+CLAMPED = [{f: 1 for f in range(6)}] * 3
+
+
+def spec_penny32(seed=0):
+    n = 32
+    h = 1.0 / n
+    return dict(name="penny32", n=n, E=1.0, nu=0.2, beta=1.0, alpha=1e-5, Gc=1.0, eps=2 * h, eta=1e-5, atnum=1,
+                pressurize=True, cracks=[dict(center=(.5, .5, .5), r=.2, theta=0., phi=0., thickness=3 * h)],
+                E_elem=None, pressure=("const", 1e-3), dtheta=0.0, seed=seed)
+
+
+def spec_hetero(n=128, seed=0, sigma=0.5, corr=8.0, ncracks=3, p_range=(0., 2e-3), dtheta=None,
+                r_range=(0.1, 0.25)):
+    h = 1.0 / n
+    u = inputs.uniform(seed, 7, 1)[0]
+    return dict(name="hetero%d" % n, n=n, E=1.0, nu=0.2, beta=1.0, alpha=1e-5, Gc=1.0, eps=2 * h, eta=1e-5,
+                atnum=1, pressurize=True,
+                cracks=inputs.random_pennies(ncracks, h, seed, r_range=r_range),
+                E_elem=("lognormal", sigma, corr), pressure=("grf", p_range[0], p_range[1], corr),
+                dtheta=10.0 * u if dtheta is None else dtheta, seed=seed)
+
+
+def spec_ensemble_member(i, seed=0, lo=16, hi=64, uniform_p=True):
+    """Member i of the batched ensemble: side log-uniform in [lo, hi], its own
+    lognormal E (log-std in [0.2, 0.8]), 1-4 cracks, injection pressure in
+    [5e-4, 5e-3]."""
+    u = inputs.uniform(seed, 100 + i, 4)
+    n = int(round(math.exp(math.log(lo) + (math.log(hi) - math.log(lo)) * u[0])))
+    sigma = 0.2 + 0.6 * u[1]
+    nc = 1 + int(4 * u[2])
+    p = 5e-4 + (5e-3 - 5e-4) * u[3]
+    s = spec_hetero(n, seed=seed * 1000 + i, sigma=sigma, ncracks=nc, dtheta=0.0)
+    s["pressure"] = ("const", p)
+    s["name"] = "member%d" % i
+    return s
+
+
+def build_fields(spec, problem=None):
+    """Host arrays for a spec.  V is built by the GPU penny kernel when a
+    Problem is given (pointwise min over cracks, VFCommon.c:827-857)."""
+    n = spec["n"]
+    nn = (n + 1) ** 3
+    ne = n ** 3
+    seed = spec["seed"]
+    out = {}
+    if spec["E_elem"] is not None:
+        _, sig, corr = spec["E_elem"]
+        out["E_elem"] = inputs.lognormal_E((n, n, n), sig, corr, seed)
+    p = spec["pressure"]
+    if p[0] == "const":
+        out["pressure"] = np.full(nn, p[1])
+    else:
+        out["pressure"] = inputs.smooth_field((n + 1,) * 3, p[3], p[1], p[2], seed)
+    out["theta"] = np.full(nn, float(spec["dtheta"]))
+    out["thetaRef"] = np.zeros(nn)
+    if problem is not None:
+        V = np.ones(nn)
+        for c in spec["cracks"]:
+            V = np.minimum(V, problem.penny_v(c["center"], c["r"], c["theta"], c["phi"], c["thickness"]))
+        out["V"] = V
+    return out
+
+
+def make_problem(spec):
+    """A configured vpfhf_amd.Problem plus its host fields."""
+    from . import Problem, make_bc
+    n = spec["n"]
+    P = Problem((n + 1,) * 3, (1., 1., 1.), E=spec["E"], nu=spec["nu"], alpha=spec["alpha"], beta=spec["beta"],
+                Gc=spec["Gc"], eta=spec["eta"], epsilon=spec["eps"], atnum=spec["atnum"],
+                pressurize=spec["pressurize"])
+    P.set_bc([make_bc(CLAMPED[c]) for c in range(3)], make_bc())
+    f = build_fields(spec, P)
+    P.set_fields(V=f["V"], VIrrev=f["V"], theta=f["theta"], thetaRef=f["thetaRef"], pressure=f["pressure"],
+                 E_elem=f.get("E_elem"))
+    return P, f
+
+
+CLAMPED_OPTIONS = "\n".join("-U_%s_BC_%d ZERO" % (f, c) for f in ("X0", "X1", "Y0", "Y1", "Z0", "Z1")
+                            for c in range(3))  # BCfiles/3DClamped.txt
+
+
+def options_text(spec, u_rtol=1e-10, extra=""):
+    """The reference's command-line options for a spec (VFCommon.c, VFCracks.c names)."""
+    n = spec["n"]
+    lines = ["-n %d,%d,%d -l 1,1,1" % (n + 1, n + 1, n + 1),
+             "-E %r -nu %r -beta %r -alpha %r -Gc %r -epsilon %r -eta %r -atnum %d" % (
+                 spec["E"], spec["nu"], spec["beta"], spec["alpha"], spec["Gc"], spec["eps"], spec["eta"],
+                 spec["atnum"]),
+             "-U_ksp_rtol %r" % u_rtol, "-npc %d" % len(spec["cracks"])]
+    if spec["pressurize"]:
+        lines.append("-pressurize")
+    for i, c in enumerate(spec["cracks"]):
+        lines.append("-pc%d_center %r,%r,%r -pc%d_r %r -pc%d_theta %r -pc%d_phi %r -pc%d_thickness %r" % (
+            i, c["center"][0], c["center"][1], c["center"][2], i, c["r"], i, c["theta"], i, c["phi"], i,
+            c["thickness"]))
+    lines.append(CLAMPED_OPTIONS)
+    lines.append(extra)
+    return "\n".join(lines)
+
+
+def make_host(spec, u_rtol=1e-10, extra=""):
+    """A C-host context (vfh_ctx) holding the problem; V/VIrrev from the cracks,
+    zero clamped displacement, VFTimeStepPrepare done."""
+    from .host import Host, options
+    H = Host(options(options_text(spec, u_rtol, extra)))
+    f = build_fields(spec)
+    H.set_inputs(E_elem=f.get("E_elem"), theta=f["theta"], thetaRef=f["thetaRef"], pressure=f["pressure"])
+    H.fields_initialize()
+    H.set_bcu(1.0)
+    H.timestep_prepare()
+    return H, f
