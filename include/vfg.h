@@ -180,6 +180,31 @@ int vfg_u_solve(vfg_handle *h, double *U, const double *BCU, const vfg_solve_opt
 int vfg_v_solve(vfg_handle *h, const double *U, double *V, const double *ub,
                 const vfg_solve_opts *o, vfg_solve_info *info, int loc);
 
+/* -------------------------------------------------------------- ensembles */
+/* Independent problems of different sizes solved together.  The reference
+ * solves one problem per run (VF_StepU, VFMech.c:1436-1451); an ensemble
+ * runs that u-solve for every member at once: one persistent launch applies
+ * every member's operator, CG iterations of all members are captured in a
+ * HIP graph, and each member keeps its own scalars, so its result does not
+ * depend on the other members.  Members must outlive the batch. */
+typedef struct vfg_batch vfg_batch;
+int vfg_batch_create(vfg_handle *const *members, int n, vfg_batch **out);
+int vfg_batch_destroy(vfg_batch *b);
+const char *vfg_batch_last_error(const vfg_batch *b);
+int vfg_batch_size(const vfg_batch *b);
+/* y[p] = A_p x[p] for every member (device pointers) in one persistent launch */
+int vfg_batch_u_apply(vfg_batch *b, const double *const *x, double *const *y);
+int vfg_batch_u_apply_f32(vfg_batch *b, const float *const *x, float *const *y);
+/* times nrep persistent applies on internal vectors (HIP events), total ms */
+int vfg_batch_time_u_apply(vfg_batch *b, int fp32, int nrep, double *ms);
+/* picks z-chunk length and persistent grid size by timing; best ms per apply */
+int vfg_batch_autotune(vfg_batch *b, int fp32, double *best_ms);
+/* graph capture of `chunk` CG iterations per replay (enable = 0: plain launches) */
+int vfg_batch_set_graph(vfg_batch *b, int enable, int chunk);
+/* VF_StepU for every member; info has one entry per member */
+int vfg_batch_u_solve(vfg_batch *b, double *const *U, const double *const *BCU, const vfg_solve_opts *o,
+                      vfg_solve_info *info, int loc);
+
 /* ------------------------------------------------- device vector helpers */
 /* The PETSc Vec operations the host loop of the reference performs on the
  * fields (VecCopy, VecPointwiseMin, VecAXPY+VecNorm(NORM_INFINITY), VecScale,

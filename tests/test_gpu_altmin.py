@@ -14,9 +14,20 @@ from tests.scenarios import relerr
 pytestmark = pytest.mark.gpu
 
 
+def test_penny32_biot_cancellation():
+    """beta = 1, uniform p with -pressurize: the poro load and the crack-pressure
+    load cancel at every node, so the u-solve returns U = 0 to rounding."""
+    from vpfhf_amd import workloads
+    H, f = workloads.make_host(workloads.spec_penny32(beta=1.0), u_rtol=1e-10)
+    H.altmin(maxit=1)
+    U, V, _ = H.state()
+    assert np.abs(U).max() < 1e-14
+    H.close()
+
+
 def test_penny32_altmin_step_matches_oracle(oracle):
     from vpfhf_amd import workloads
-    spec = workloads.spec_penny32()
+    spec = workloads.spec_penny32(beta=0.0)
     H, f = workloads.make_host(spec, u_rtol=1e-10)
     U0, V0, VI0 = H.state()
     info = H.altmin(maxit=1)
@@ -26,7 +37,7 @@ def test_penny32_altmin_step_matches_oracle(oracle):
 
     n = spec["n"] + 1
     bc = [oracle.make_bc({q: oracle.ZERO for q in range(6)})] * 3
-    op = oracle.OracleProblem((n, n, n), E=1., nu=.2, alpha=1e-5, beta=1., Gc=1., eta=1e-5,
+    op = oracle.OracleProblem((n, n, n), E=1., nu=.2, alpha=1e-5, beta=0., Gc=1., eta=1e-5,
                               epsilon=spec["eps"], atnum=1, pressurize=True, bcU=bc)
     # the GPU penny builder agrees with the oracle's restatement of VFCracks.c
     c = spec["cracks"][0]
@@ -37,6 +48,7 @@ def test_penny32_altmin_step_matches_oracle(oracle):
     oracle.u_solve(op, F, U, np.zeros(3 * op.nn), rtol=1e-12)
     F = op.fields(V=V, VIrrev=VI0, theta=f["theta"], thetaRef=f["thetaRef"], pressure=f["pressure"])
     oracle.v_solve(op, F, U, V, ub=VI0.copy(), tol=1e-12)
+    assert np.linalg.norm(U) > 1e-8
     assert relerr(Ug, U) < 1e-6
     assert relerr(Vg, V) < 1e-6
     se = oracle.v_energy(op, V)

@@ -86,6 +86,17 @@ PROTOS = {
     "vfg_solve_opts_default": (None, [C.POINTER(SolveOpts)]),
     "vfg_u_solve": (C.c_int, [H, vp, vp, C.POINTER(SolveOpts), C.POINTER(SolveInfo), C.c_int]),
     "vfg_v_solve": (C.c_int, [H, vp, vp, vp, C.POINTER(SolveOpts), C.POINTER(SolveInfo), C.c_int]),
+    "vfg_batch_create": (C.c_int, [C.POINTER(H), C.c_int, C.POINTER(vp)]),
+    "vfg_batch_destroy": (C.c_int, [vp]),
+    "vfg_batch_last_error": (C.c_char_p, [vp]),
+    "vfg_batch_size": (C.c_int, [vp]),
+    "vfg_batch_u_apply": (C.c_int, [vp, C.POINTER(vp), C.POINTER(vp)]),
+    "vfg_batch_u_apply_f32": (C.c_int, [vp, C.POINTER(vp), C.POINTER(vp)]),
+    "vfg_batch_time_u_apply": (C.c_int, [vp, C.c_int, C.c_int, dp]),
+    "vfg_batch_autotune": (C.c_int, [vp, C.c_int, dp]),
+    "vfg_batch_set_graph": (C.c_int, [vp, C.c_int, C.c_int]),
+    "vfg_batch_u_solve": (C.c_int, [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(SolveOpts),
+                                    C.POINTER(SolveInfo), C.c_int]),
     "vfg_vec_copy": (C.c_int, [H, vp, vp, C.c_int64]),
     "vfg_vec_pointwise_min": (C.c_int, [H, vp, vp, vp, C.c_int64]),
     "vfg_vec_maxabs_diff": (C.c_int, [H, vp, vp, C.c_int64, dp]),

@@ -69,13 +69,13 @@ __device__ __forceinline__ bool on_dirichlet(const Grid &G, uint8_t face, uint8_
 //   __device__ void store(int i,int j,int k, const T (&acc)[NC], const T (&own)[NF])
 // Corners of an element are a = (k*2+j)*2+i; xin[k][j*2+i][f].
 template <typename T, int TX, int TY, class Op>
-__device__ __forceinline__ void march(const Grid &G, const Op &op, int OZ) {
+__device__ __forceinline__ void march(const Grid &G, const Op &op, int OZ, int bx, int by, int bz) {
   constexpr int NF = Op::NF, NC = Op::NC;
   constexpr int NT = TX * TY, OX = TX - 1, OY = TY - 1, SX = TX + 1, SY = TY + 1;
   __shared__ T s_in[NF][SY * SX];
   __shared__ T s_red[2 * NC][NT];
   const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
-  const int ox0 = blockIdx.x * OX, oy0 = blockIdx.y * OY, oz0 = blockIdx.z * OZ;
+  const int ox0 = bx * OX, oy0 = by * OY, oz0 = bz * OZ;
   const int ex = ox0 - 1 + tx, ey = oy0 - 1 + ty;
   const bool exy_ok = ex >= 0 && ex < G.nex && ey >= 0 && ey < G.ney;
   const bool owner = tx < OX && ty < OY && (ox0 + tx) < G.nx && (oy0 + ty) < G.ny;

@@ -7,6 +7,11 @@
 
 namespace vfg {
 
+// one work item of the persistent multi-problem apply
+struct BItem {
+  int p, bx, by, bz;
+};
+
 // Everything a kernel needs about one problem.  Device pointers are owned by
 // the handle (vfg_api.hip); host copies of the BC tables travel by value.
 struct DevProblem {

@@ -227,6 +227,256 @@ static double dot_host(vfg_handle *h, const double *a, const double *b, int64_t 
   return v;
 }
 
+
+// ----------------------------------------------------------- FP32 inner PCG
+// Vectors in FP32, all inner products and CG scalars in FP64.
+__global__ __launch_bounds__(RB) void k_pcg32_init(const float *__restrict__ r, const float *__restrict__ z,
+                                                   float *__restrict__ p, int64_t n, double *part) {
+  double rz = 0., rr = 0.;
+  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB) {
+    double ri = r[i], zi = z[i];
+    p[i] = z[i];
+    rz += ri * zi;
+    rr += ri * ri;
+  }
+  rz = bsum(rz);
+  rr = bsum(rr);
+  if (threadIdx.x == 0) { part[2 * blockIdx.x] = rz; part[2 * blockIdx.x + 1] = rr; }
+}
+__global__ __launch_bounds__(RB) void k_dotp32(const float *__restrict__ a, const float *__restrict__ b, int64_t n,
+                                               double *part) {
+  double s = 0.;
+  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB)
+    s += (double)a[i] * (double)b[i];
+  s = bsum(s);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+// x += alpha p ; r -= alpha q   (z is refreshed by the preconditioner)
+__global__ __launch_bounds__(RB) void k_pcg32_xr(float *__restrict__ x, float *__restrict__ r,
+                                                 const float *__restrict__ p, const float *__restrict__ q,
+                                                 int64_t n, const double *sc) {
+  const double al = sc[S_ALPHA];
+  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB) {
+    x[i] = (float)((double)x[i] + al * p[i]);
+    r[i] = (float)((double)r[i] - al * q[i]);
+  }
+}
+// partial sums of r.z and r.r
+__global__ __launch_bounds__(RB) void k_rz32(const float *__restrict__ r, const float *__restrict__ z, int64_t n,
+                                             double *part) {
+  double rz = 0., rr = 0.;
+  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB) {
+    double ri = r[i];
+    rz += ri * z[i];
+    rr += ri * ri;
+  }
+  rz = bsum(rz);
+  rr = bsum(rr);
+  if (threadIdx.x == 0) { part[2 * blockIdx.x] = rz; part[2 * blockIdx.x + 1] = rr; }
+}
+__global__ __launch_bounds__(RB) void k_pcg32_p(float *__restrict__ p, const float *__restrict__ z, int64_t n,
+                                                const double *sc) {
+  const double be = sc[S_BETA];
+  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB)
+    p[i] = (float)((double)z[i] + be * p[i]);
+}
+__global__ void k_jacobi32(const float *__restrict__ dinv, const float *__restrict__ r, float *__restrict__ z,
+                           int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) z[i] = dinv[i] * r[i];
+}
+// Chebyshev step: res = r - Az ; d = c1 d + c2 D^-1 res ; z += d
+__global__ void k_cheb32(const float *__restrict__ r, const float *__restrict__ Az, const float *__restrict__ dinv,
+                         float *__restrict__ d, float *__restrict__ z, float c1, float c2, int first, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float res = first ? r[i] : r[i] - Az[i];
+  float di = (first ? 0.f : c1 * d[i]) + c2 * dinv[i] * res;
+  d[i] = di;
+  z[i] = first ? di : z[i] + di;
+}
+// scale r64 by 1/s into FP32 ; x64 += s * d32
+__global__ void k_r64to32(const double *r, float *r32, double inv, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) r32[i] = (float)(r[i] * inv);
+}
+__global__ void k_fill32(float *x, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = 0.f;
+}
+__global__ void k_axpy3264(double *x, const float *d, double s, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] += s * (double)d[i];
+}
+
+struct Cheb {
+  int degree;
+  double lmin, lmax;
+};
+
+// z = M^-1 r  (Jacobi, or Chebyshev(degree) polynomial in D^-1 A)
+static void precond32(vfg_handle *h, const std::function<void(const float *, float *)> &A, const float *dinv,
+                      const float *r, float *z, float *d, float *t, int64_t n, const Cheb &cb) {
+  hipStream_t s = h->stream;
+  if (cb.degree <= 1) {
+    hipLaunchKernelGGL(k_jacobi32, dim3(nb_(n)), dim3(256), 0, s, dinv, r, z, n);
+    return;
+  }
+  const double th = .5 * (cb.lmax + cb.lmin), de = .5 * (cb.lmax - cb.lmin);
+  const double s1 = th / de;
+  double rho_old = 1. / s1;
+  hipLaunchKernelGGL(k_cheb32, dim3(nb_(n)), dim3(256), 0, s, r, (const float *)nullptr, dinv, d, z, 0.f,
+                     (float)(1. / th), 1, n);
+  for (int k = 1; k < cb.degree; k++) {
+    A(z, t);
+    double rho = 1. / (2. * s1 - rho_old);
+    hipLaunchKernelGGL(k_cheb32, dim3(nb_(n)), dim3(256), 0, s, r, (const float *)t, dinv, d, z,
+                       (float)(rho * rho_old), (float)(2. * rho / de), 0, n);
+    rho_old = rho;
+  }
+}
+
+// Largest eigenvalue of the Lanczos tridiagonal built from CG coefficients
+// (T_kk = 1/a_k + b_{k-1}/a_{k-1}, T_k,k+1 = sqrt(b_k)/a_k), by bisection with
+// Sturm counts; an estimate of lambda_max(M^-1 A) from below.
+static double lanczos_lmax(const double *ab, int m) {
+  if (m < 1) return 0.;
+  double d[64], e[64], hi = 0.;
+  for (int k = 0; k < m; k++) {
+    double a = ab[2 * k], ap = k ? ab[2 * (k - 1)] : 0., bp = k ? ab[2 * (k - 1) + 1] : 0.;
+    d[k] = (a > 0 ? 1. / a : 0.) + (k && ap > 0 ? bp / ap : 0.);
+    e[k] = (a > 0 ? sqrt(fmax(ab[2 * k + 1], 0.)) / a : 0.);
+  }
+  for (int k = 0; k < m; k++) hi = fmax(hi, d[k] + (k ? e[k - 1] : 0.) + (k < m - 1 ? e[k] : 0.));
+  double lo = 0.;
+  for (int it = 0; it < 100; it++) {
+    double x = .5 * (lo + hi), q = 1.;
+    int cnt = 0;  // eigenvalues below x
+    for (int k = 0; k < m; k++) {
+      q = d[k] - x - (k ? e[k - 1] * e[k - 1] / q : 0.);
+      if (q == 0.) q = 1e-300;
+      if (q < 0.) cnt++;
+    }
+    if (cnt >= m) hi = x; else lo = x;
+  }
+  return hi;
+}
+
+// Inner PCG in FP32 on A32 d = r (d0 = 0); returns iterations; lanczos
+// estimates of the extreme eigenvalues of M^-1 A are accumulated from the CG
+// coefficients when est != nullptr (used to size the Chebyshev interval).
+static int pcg32(vfg_handle *h, int64_t n, const std::function<void(const float *, float *)> &A, const float *dinv,
+                 float *x, float *r, float *z, float *p, float *q, float *w1, float *w2, const Cheb &cb,
+                 double rtol, int maxit, double *lmax_est, int *status) {
+  hipStream_t s = h->stream;
+  double *sc = h->scal, *part = h->part;
+  const int nb = red_blocks(n);
+  precond32(h, A, dinv, r, z, w1, w2, n, cb);
+  hipLaunchKernelGGL(k_pcg32_init, dim3(nb), dim3(RB), 0, s, (const float *)r, (const float *)z, p, n, part);
+  hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(RB), 0, s, (const double *)part, nb, 0, sc);
+  double hs[8];
+  if (hipMemcpyAsync(hs, sc, 64, hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+    *status = VFG_E_HIP;
+    return 0;
+  }
+  const double rn0 = sqrt(hs[S_RR0]);
+  double rn = rn0, ab[128];
+  int it = 0, n_ab = 0;
+  *status = VFG_OK;
+  const int check = lmax_est ? 1 : 10;
+  while (rn > rtol * rn0 && it < maxit) {
+    int todo = check < maxit - it ? check : maxit - it;
+    for (int k = 0; k < todo; k++) {
+      A(p, q);
+      hipLaunchKernelGGL(k_dotp32, dim3(nb), dim3(RB), 0, s, (const float *)p, (const float *)q, n, part);
+      hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(RB), 0, s, (const double *)part, nb, 1, sc);
+      hipLaunchKernelGGL(k_pcg32_xr, dim3(nb), dim3(RB), 0, s, x, r, (const float *)p, (const float *)q, n,
+                         (const double *)sc);
+      precond32(h, A, dinv, r, z, w1, w2, n, cb);
+      hipLaunchKernelGGL(k_rz32, dim3(nb), dim3(RB), 0, s, (const float *)r, (const float *)z, n, part);
+      hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(RB), 0, s, (const double *)part, nb, 2, sc);
+      hipLaunchKernelGGL(k_pcg32_p, dim3(nb), dim3(RB), 0, s, p, (const float *)z, n, (const double *)sc);
+    }
+    it += todo;
+    if (hipMemcpyAsync(hs, sc, 64, hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+      *status = VFG_E_HIP;
+      return it;
+    }
+    rn = sqrt(hs[S_RR]);
+    if (lmax_est && n_ab < 64) {
+      ab[2 * n_ab] = hs[S_ALPHA];
+      ab[2 * n_ab + 1] = hs[S_BETA];
+      n_ab++;
+    }
+    if (hs[S_FLAG] == 2. || !isfinite(rn)) { *status = VFG_E_NAN; break; }
+    if (hs[S_FLAG] == 1.) { *status = rn <= rtol * rn0 ? VFG_OK : VFG_E_BREAKDOWN; break; }
+  }
+  if (lmax_est) *lmax_est = lanczos_lmax(ab, n_ab);
+  return it;
+}
+
+
+// FP64 iterative refinement around the FP32 inner PCG (precision form):
+// r64 = b - A64 x in FP64; solve A32 d = r64/|r64| in FP32 to inner_rtol;
+// x += |r64| d in FP64; repeat until |r64| <= rtol |r64_0|.
+static int u_solve_mixed(vfg_handle *h, double *x, double *r64, double *t64, const double *dinv64, const double *B,
+                         const vfg_solve_opts *o, vfg_solve_info *inf) {
+  DevProblem &P = h->P;
+  hipStream_t s = h->stream;
+  const int64_t n = 3 * P.nn;
+  for (int i = 0; i < 7; i++)
+    if (!h->wf[i]) {
+      int rc = vfg_alloc(h, (void **)&h->wf[i], 4 * (size_t)n);
+      if (rc) return rc;
+    }
+  float *d = h->wf[0], *r = h->wf[1], *z = h->wf[2], *p = h->wf[3], *q = h->wf[4], *w1 = h->wf[5], *w2 = h->wf[6];
+  float *dinv = (float *)t64;  // FP32 reciprocal diagonal lives in the first half of t64
+  launch_copy_d2f(dinv64, dinv, n, s);
+  std::function<void(const float *, float *)> A32 = [&](const float *in, float *out) {
+    launch_u_apply_f32(P, in, out, s);
+  };
+  Cheb cb = {1, 0., 0.};
+  const double inner_rtol = 1e-4;
+  int rc = VFG_OK;
+  double rn0 = -1.;
+  for (int outer = 0; outer < 60; outer++) {
+    int e2 = 0;
+    double rn = sqrt(dot_host(h, r64, r64, n, &e2));
+    if (e2) return e2;
+    if (!isfinite(rn)) return VFG_E_NAN;
+    if (rn0 < 0.) rn0 = rn;
+    inf->relres = rn0 > 0. ? rn / rn0 : 0.;
+    if (rn <= fmax(o->rtol * rn0, o->atol) || rn == 0.) {
+      inf->outer = outer;
+      return VFG_OK;
+    }
+    hipLaunchKernelGGL(k_r64to32, dim3(nb_(n)), dim3(256), 0, s, (const double *)r64, r, 1. / rn, n);
+    hipLaunchKernelGGL(k_fill32, dim3(nb_(n)), dim3(256), 0, s, d, n);
+    if (o->pc == VFG_PC_CHEBYSHEV && cb.degree == 1) {
+      // lambda_max(D^-1 A) from 12 Jacobi-PCG steps on this residual, then restart
+      double lmax = 0.;
+      int st = 0;
+      pcg32(h, n, A32, dinv, d, r, z, p, q, w1, w2, cb, 0., 12, &lmax, &st);
+      inf->iterations += 12;
+      if (st == VFG_E_HIP) return st;
+      cb.degree = o->cheb_degree > 1 ? o->cheb_degree : 4;
+      cb.lmax = 1.1 * lmax;
+      cb.lmin = cb.lmax / 30.;
+      hipLaunchKernelGGL(k_r64to32, dim3(nb_(n)), dim3(256), 0, s, (const double *)r64, r, 1. / rn, n);
+      hipLaunchKernelGGL(k_fill32, dim3(nb_(n)), dim3(256), 0, s, d, n);
+    }
+    int st = 0;
+    int its = pcg32(h, n, A32, dinv, d, r, z, p, q, w1, w2, cb, inner_rtol, o->maxit, nullptr, &st);
+    inf->iterations += its * (cb.degree > 1 ? cb.degree : 1);
+    if (st == VFG_E_HIP || st == VFG_E_NAN) return st;
+    hipLaunchKernelGGL(k_axpy3264, dim3(nb_(n)), dim3(256), 0, s, x, (const float *)d, rn, n);
+    launch_u_rhs(P, x, B, r64, 1, s);  // F(x) in FP64
+    hipLaunchKernelGGL(k_neg, dim3(nb_(n)), dim3(256), 0, s, (const double *)r64, r64, n);
+    inf->outer = outer + 1;
+  }
+  return rc ? rc : VFG_E_MAXIT;
+}
+
 }  // namespace vfg
 
 using namespace vfg;
@@ -263,9 +513,15 @@ int vfg_u_solve(vfg_handle *h, double *U, const double *BCU, const vfg_solve_opt
   launch_u_diag(h->P, q, s);
   hipLaunchKernelGGL(k_recip, dim3(nb_(n)), dim3(256), 0, s, (const double *)q, dinv, n);
   const DevProblem &P = h->P;
-  ApplyFn A = [&](const double *in, double *out) { launch_u_apply(P, in, out, s); };
-  rc = pcg(h, n, A, dinv, x, r, z, p, q, o->rtol, o->atol, o->maxit, 20, &inf);
-  inf.outer = 1;
+  if (o->precision == VFG_PREC_MIXED) {
+    rc = u_solve_mixed(h, x, r, q, dinv, B, o, &inf);
+  } else if (o->pc == VFG_PC_CHEBYSHEV) {
+    rc = vfg_fail(h, VFG_E_ARG, "vfg_u_solve: the Chebyshev preconditioner is provided for the mixed-precision solve");
+  } else {
+    ApplyFn A = [&](const double *in, double *out) { launch_u_apply(P, in, out, s); };
+    rc = pcg(h, n, A, dinv, x, r, z, p, q, o->rtol, o->atol, o->maxit, 20, &inf);
+    inf.outer = 1;
+  }
   if (info) *info = inf;
   if (rc == VFG_E_HIP) return rc;
   hipError_t e = hipMemcpyAsync(U, x, 8 * n, loc == VFG_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, s);

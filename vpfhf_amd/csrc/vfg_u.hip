@@ -12,6 +12,8 @@
 // takes 9 distinct values per component.
 #include "vfg_internal.h"
 #include "vfg_kernels.h"
+#include "vfg_batch_apply.h"
+#include "vfg_handle.h"
 
 namespace vfg {
 
@@ -183,7 +185,7 @@ struct UApplyOp {
 
 template <typename T, int TX, int TY>
 __global__ __launch_bounds__(TX *TY) void k_u_apply(UApplyOp<T> op, int OZ) {
-  march<T, TX, TY>(op.G, op, OZ);
+  march<T, TX, TY>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 // ---------------------------------------------------------------- diagonal
@@ -226,7 +228,7 @@ struct UDiagOp {
   }
 };
 
-__global__ __launch_bounds__(256) void k_u_diag(UDiagOp op, int OZ) { march<double, 16, 16>(op.G, op, OZ); }
+__global__ __launch_bounds__(256) void k_u_diag(UDiagOp op, int OZ) { march<double, 16, 16>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z); }
 
 // ------------------------------------------------------- load vector / residual
 // b_{a,c} = sum_g W d_c phi_a [(3lam+2mu) alpha dth s + beta p v]          (thermo-poro)
@@ -309,18 +311,45 @@ struct URhsOp {
   }
 };
 
-__global__ __launch_bounds__(256) void k_u_rhs(URhsOp op, int OZ) { march<double, 16, 16>(op.G, op, OZ); }
+__global__ __launch_bounds__(256) void k_u_rhs(URhsOp op, int OZ) { march<double, 16, 16>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z); }
 
 // ------------------------------------------------------------ host launchers
 static dim3 march_grid(const Grid &G, int TX, int TY, int OZ) {
   return dim3((G.nx + TX - 2) / (TX - 1), (G.ny + TY - 2) / (TY - 1), (G.nz + OZ - 1) / OZ);
 }
 
-static int choose_oz(const Grid &G, int tiles_xy) {
-  // enough workgroups to cover 256 CUs several times, chunks of >= 8 planes
-  int oz = G.nz;
-  while (oz > 8 && (int64_t)tiles_xy * ((G.nz + oz - 1) / oz) < 2048) oz = (oz + 1) / 2;
-  return oz;
+// z-chunk length of the marching tiles: every chunk recomputes one element
+// layer below it, so long chunks waste less; the grid should still fill every
+// workgroup slot of the device.  Minimise rounds * (layers per workgroup),
+// rounds = ceil(workgroups / resident slots).
+int choose_oz(const Grid &G, int tiles_xy, int slots) {
+  if (slots < 1) slots = 256;
+  int best = G.nz;
+  double best_cost = 1e300;
+  for (int oz = 2; oz <= G.nz; oz++) {
+    int chunks = (G.nz + oz - 1) / oz;
+    if (oz > 2 && (G.nz + oz - 2) / (oz - 1) == chunks) continue;  // same chunk count, shorter is free
+    int64_t wg = (int64_t)tiles_xy * chunks;
+    double rounds = (double)((wg + slots - 1) / slots);
+    double cost = rounds * (oz + 1);
+    if (cost < best_cost - 1e-9) { best_cost = cost; best = oz; }
+  }
+  return best;
+}
+
+// resident workgroups of a kernel on the whole device (occupancy x CUs)
+template <typename K>
+static int device_slots(K kernel, int threads) {
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
+    hipGetDevice(&dev);
+    hipDeviceProp_t pr;
+    cus = hipGetDeviceProperties(&pr, dev) == hipSuccess ? pr.multiProcessorCount : 256;
+  }
+  int per = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0) != hipSuccess || per < 1) per = 1;
+  return per * cus;
 }
 
 template <typename T>
@@ -333,7 +362,8 @@ static void launch_u_apply_t(const DevProblem &P, const T *x, T *y, const T *v, 
   op.vol = T(P.G.hx * P.G.hy * P.G.hz);
   op.ihx = T(1. / P.G.hx); op.ihy = T(1. / P.G.hy); op.ihz = T(1. / P.G.hz);
   int tiles = ((P.G.nx + 14) / 15) * ((P.G.ny + 14) / 15);
-  int OZ = choose_oz(P.G, tiles);
+  static int slots = device_slots(k_u_apply<T, 16, 16>, 256);
+  int OZ = choose_oz(P.G, tiles, slots);
   hipLaunchKernelGGL((k_u_apply<T, 16, 16>), march_grid(P.G, 16, 16, OZ), dim3(256), 0, s, op, OZ);
 }
 
@@ -350,7 +380,8 @@ void launch_u_diag(const DevProblem &P, double *d, hipStream_t s) {
   op.E = P.E; op.lam1 = P.lam1; op.mu1 = P.mu1; op.eta = P.eta;
   op.vol = P.G.hx * P.G.hy * P.G.hz;
   int tiles = ((P.G.nx + 14) / 15) * ((P.G.ny + 14) / 15);
-  int OZ = choose_oz(P.G, tiles);
+  static int slots = device_slots(k_u_diag, 256);
+  int OZ = choose_oz(P.G, tiles, slots);
   hipLaunchKernelGGL(k_u_diag, march_grid(P.G, 16, 16, OZ), dim3(256), 0, s, op, OZ);
 }
 
@@ -367,8 +398,79 @@ void launch_u_rhs(const DevProblem &P, const double *U, const double *BCU, doubl
   op.vol = P.G.hx * P.G.hy * P.G.hz; op.alpha = P.alpha; op.beta = P.beta;
   op.crack = P.hasCrackPressure; op.mode = mode;
   int tiles = ((P.G.nx + 14) / 15) * ((P.G.ny + 14) / 15);
-  int OZ = choose_oz(P.G, tiles);
+  static int slots = device_slots(k_u_rhs, 256);
+  int OZ = choose_oz(P.G, tiles, slots);
   hipLaunchKernelGGL(k_u_rhs, march_grid(P.G, 16, 16, OZ), dim3(256), 0, s, op, OZ);
+}
+
+}  // namespace vfg
+
+// ------------------------------------------------- persistent multi-problem apply
+namespace vfg {
+
+template <typename T>
+struct BApply {
+  UApplyOp<T> op;
+  int OZ;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_batch_u_apply(const BApply<T> *__restrict__ probs,
+                                                       const BItem *__restrict__ items, int nitems) {
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const BItem w = items[it];
+    const UApplyOp<T> op = probs[w.p].op;
+    const int OZ = probs[w.p].OZ;
+    __syncthreads();  // LDS of the previous item is free
+    march<T, 16, 16>(op.G, op, OZ, w.bx, w.by, w.bz);
+  }
+}
+
+size_t batch_op_bytes(int fp32) { return fp32 ? sizeof(BApply<float>) : sizeof(BApply<double>); }
+
+int choose_oz_pub(const Grid &G, int tiles_xy, int slots) { return choose_oz(G, tiles_xy, slots); }
+
+int batch_slots() {
+  static int s = 0;
+  if (!s) s = device_slots(k_batch_u_apply<double>, 256);
+  return s;
+}
+
+template <typename T>
+static void fill_ops_t(const std::vector<vfg_handle *> &m, const void *const *x, void *const *y,
+                       const std::vector<int> &oz, void *d_ops, hipStream_t s) {
+  std::vector<BApply<T>> ops(m.size());
+  for (size_t p = 0; p < m.size(); p++) {
+    const DevProblem &P = m[p]->P;
+    UApplyOp<T> &op = ops[p].op;
+    op.x = (const T *)x[p];
+    op.y = (T *)y[p];
+    op.v = sizeof(T) == 4 ? (const T *)(const void *)P.V32 : (const T *)(const void *)P.V;
+    op.E_elem = P.E_elem;
+    op.G = P.G;
+    op.bc = P.bcU;
+    op.E = T(P.E); op.lam1 = T(P.lam1); op.mu1 = T(P.mu1); op.eta = T(P.eta);
+    op.vol = T(P.G.hx * P.G.hy * P.G.hz);
+    op.ihx = T(1. / P.G.hx); op.ihy = T(1. / P.G.hy); op.ihz = T(1. / P.G.hz);
+    ops[p].OZ = oz[p];
+  }
+  hipStreamSynchronize(s);  // no kernel of the previous call still reads d_ops
+  hipMemcpy(d_ops, ops.data(), sizeof(BApply<T>) * ops.size(), hipMemcpyHostToDevice);
+}
+
+void fill_ops(const std::vector<vfg_handle *> &m, const void *const *x, void *const *y, int fp32,
+              const std::vector<int> &oz, void *d_ops, hipStream_t s) {
+  if (fp32) fill_ops_t<float>(m, x, y, oz, d_ops, s);
+  else fill_ops_t<double>(m, x, y, oz, d_ops, s);
+}
+
+void launch_batch_apply(int fp32, const void *d_ops, const BItem *items, int nitems, int grid, hipStream_t s) {
+  int g = grid > 0 && grid < nitems ? grid : nitems;
+  if (fp32)
+    hipLaunchKernelGGL(k_batch_u_apply<float>, dim3(g), dim3(256), 0, s, (const BApply<float> *)d_ops, items, nitems);
+  else
+    hipLaunchKernelGGL(k_batch_u_apply<double>, dim3(g), dim3(256), 0, s, (const BApply<double> *)d_ops, items,
+                       nitems);
 }
 
 }  // namespace vfg

@@ -210,7 +210,7 @@ struct VApplyOp {
   }
 };
 
-__global__ __launch_bounds__(256) void k_v_apply(VApplyOp op, int OZ) { march<double, 16, 16>(op.G, op, OZ); }
+__global__ __launch_bounds__(256) void k_v_apply(VApplyOp op, int OZ) { march<double, 16, 16>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z); }
 
 // ---------------------------------------------------------------- diagonal
 struct VDiagOp {
@@ -241,7 +241,7 @@ struct VDiagOp {
   }
 };
 
-__global__ __launch_bounds__(256) void k_v_diag(VDiagOp op, int OZ) { march<double, 16, 16>(op.G, op, OZ); }
+__global__ __launch_bounds__(256) void k_v_diag(VDiagOp op, int OZ) { march<double, 16, 16>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z); }
 
 // ------------------------------------------------------------ load vector
 // b_a = crhs sum_g W phi_a + [pressurize] sum_g W p (u . grad phi_a)
@@ -285,17 +285,16 @@ struct VRhsOp {
   }
 };
 
-__global__ __launch_bounds__(256) void k_v_rhs(VRhsOp op, int OZ) { march<double, 16, 16>(op.G, op, OZ); }
+__global__ __launch_bounds__(256) void k_v_rhs(VRhsOp op, int OZ) { march<double, 16, 16>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z); }
 
 // ------------------------------------------------------------ launchers
 static dim3 mgrid(const Grid &G, int OZ) {
   return dim3((G.nx + 14) / 15, (G.ny + 14) / 15, (G.nz + OZ - 1) / OZ);
 }
+int choose_oz(const Grid &G, int tiles_xy, int slots);
 static int oz_for(const Grid &G) {
-  int tiles = ((G.nx + 14) / 15) * ((G.ny + 14) / 15);
-  int oz = G.nz;
-  while (oz > 8 && (int64_t)tiles * ((G.nz + oz - 1) / oz) < 2048) oz = (oz + 1) / 2;
-  return oz;
+  // the V kernels run at 2-4 workgroups per CU
+  return choose_oz(G, ((G.nx + 14) / 15) * ((G.ny + 14) / 15), 768);
 }
 
 void launch_v_setup(const DevProblem &P, const double *U, hipStream_t s) {

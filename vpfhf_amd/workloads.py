@@ -22,10 +22,13 @@ from . import inputs
 CLAMPED = [{f: 1 for f in range(6)}] * 3
 
 
-def spec_penny32(seed=0):
+def spec_penny32(seed=0, beta=1.0):
+    """With beta = 1 and the same uniform pressure in pores and crack the two
+    loads cancel node by node (p * integral of grad(v phi_a) = 0), so U = 0;
+    beta = 0 keeps the crack-pressure load alone (as ValidationTests/test2.c does)."""
     n = 32
     h = 1.0 / n
-    return dict(name="penny32", n=n, E=1.0, nu=0.2, beta=1.0, alpha=1e-5, Gc=1.0, eps=2 * h, eta=1e-5, atnum=1,
+    return dict(name="penny32", n=n, E=1.0, nu=0.2, beta=beta, alpha=1e-5, Gc=1.0, eps=2 * h, eta=1e-5, atnum=1,
                 pressurize=True, cracks=[dict(center=(.5, .5, .5), r=.2, theta=0., phi=0., thickness=3 * h)],
                 E_elem=None, pressure=("const", 1e-3), dtheta=0.0, seed=seed)
 
