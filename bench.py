@@ -34,14 +34,27 @@ def apply_bytes(nn, ne, fp32=False, hetero=True):
 
 
 def read_traffic(name):
+    """HBM bytes per launch of the apply kernel from a committed counter
+    summary: the line 'hbm_bytes_per_launch: <int>'."""
     p = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(p):
         return None
-    try:
-        d = json.load(open(p))
-        return d.get("hbm_bytes_per_launch")
-    except Exception:
-        return None
+    for line in open(p):
+        if line.startswith("hbm_bytes_per_launch:"):
+            return int(float(line.split(":", 1)[1]))
+    return None
+
+
+def copy_calibration(L, h, n=1 << 26, reps=4):
+    """Plain streaming kernel of known size for counter calibration:
+    out = min(a, b) over n doubles reads 16 n and writes 8 n bytes."""
+    import vpfhf_amd as V
+    bufs = [V.DeviceBuffer(8 * n) for _ in range(3)]
+    for _ in range(reps):
+        V.check(L.vfg_vec_pointwise_min(h, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, n), h)
+    V.check(L.vfg_synchronize(h), h)
+    for b in bufs:
+        b.free()
 
 
 def cpu_baseline(n_sample=48, seconds=3.0):
@@ -84,6 +97,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solves", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--calib", action="store_true", help="also run a plain streaming kernel of known size")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: every box here has one MI355X and this path has no multi-GPU "
@@ -112,7 +126,9 @@ def main():
     gdofs = 3 * nn / t_step / 1e9
     byt = apply_bytes(nn, ne)
     achieved = byt / t_step / 1e9
-    traffic = read_traffic("u_apply_hetero%d_traffic.json" % a.n)
+    traffic = read_traffic("u_apply_hetero%d_counters.txt" % a.n)
+    if a.calib:
+        copy_calibration(L, h)
 
     # FP32 apply (precision form) on the same operator
     xs32 = [V.DeviceBuffer(12 * nn).upload(rng.standard_normal(3 * nn).astype(np.float32)) for _ in range(nbuf)]

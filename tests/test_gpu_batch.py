@@ -40,7 +40,8 @@ def test_batch_apply_equals_single():
     y32 = B.u_apply([x.astype(np.float32) for x in xs], fp32=True)
     for h, x, y in zip(hosts, xs, y32):
         yy = np.empty(3 * h.nn, np.float32)
-        V.check(L.vfg_u_apply_f32(h.h, x.astype(np.float32).ctypes.data, yy.ctypes.data, V.HOST), h.h)
+        x32 = x.astype(np.float32)  # keep the array alive across the C call
+        V.check(L.vfg_u_apply_f32(h.h, x32.ctypes.data, yy.ctypes.data, V.HOST), h.h)
         assert np.array_equal(y, yy)
     B.close()
 
@@ -63,7 +64,8 @@ def test_batch_u_solve(mixed):
         Ur = np.array(U0)
         info = _lib.SolveInfo()
         o = _lib.SolveOpts(1e-10, 0., 100000, 0, 0, 4)
-        V.check(L.vfg_u_solve(h.h, Ur.ctypes.data, np.zeros_like(U0).ctypes.data, o, info, V.HOST), h.h)
+        Z = np.zeros_like(U0)
+        V.check(L.vfg_u_solve(h.h, Ur.ctypes.data, Z.ctypes.data, o, info, V.HOST), h.h)
         assert relerr(Ug, Ur) < (1e-7 if mixed else 1e-8)
     # alone vs in the mix: member 2 batched on its own
     B2 = Batch([hosts[2]])

@@ -26,10 +26,20 @@
 namespace vfg {
 
 struct BChunk {
-  int p;
-  int64_t start;
+  int p;          // member
+  int64_t start;  // offset inside the member's vectors
   int len;
 };
+// per-member vector table: slots 0..6 FP64 (X,R,Z,P,Q,Dinv,B), 7..12 FP32
+// (X,R,Z,P,Q,Dinv); the vectors belong to the member handle
+enum { VX = 0, VR, VZ, VP, VQ, VD, VB, VX32, VR32, VZ32, VP32, VQ32, VD32, NV };
+struct BVec {
+  void *v[NV];
+};
+template <typename T>
+__device__ __forceinline__ T *vec(const BVec *t, const BChunk &c, int slot) {
+  return (T *)t[c.p].v[slot] + c.start;
+}
 constexpr int CH = 4096;  // vector elements per chunk
 constexpr int BRB = 256;
 
@@ -40,14 +50,14 @@ __device__ __forceinline__ double bbsum(double v) {
   return block_sum<BRB>(v, sh);
 }
 
+// slot offset o = 0 (FP64 set) or VX32 (FP32 set)
 template <typename T>
-__global__ __launch_bounds__(BRB) void k_binit(const T *__restrict__ r, T *__restrict__ z, T *__restrict__ p,
-                                               const T *__restrict__ dinv, const BChunk *__restrict__ ch,
-                                               double *part) {
+__global__ __launch_bounds__(BRB) void k_binit(const BVec *vt, int o, const BChunk *__restrict__ ch, double *part) {
   const BChunk c = ch[blockIdx.x];
+  const T *r = vec<T>(vt, c, o + VR), *dinv = vec<T>(vt, c, o + VD);
+  T *z = vec<T>(vt, c, o + VZ), *p = vec<T>(vt, c, o + VP);
   double rz = 0., rr = 0.;
-  for (int i = threadIdx.x; i < c.len; i += BRB) {
-    int64_t k = c.start + i;
+  for (int k = threadIdx.x; k < c.len; k += BRB) {
     double ri = r[k];
     T zi = (T)(dinv[k] * r[k]);
     z[k] = zi;
@@ -61,11 +71,12 @@ __global__ __launch_bounds__(BRB) void k_binit(const T *__restrict__ r, T *__res
 }
 
 template <typename T>
-__global__ __launch_bounds__(BRB) void k_bdot(const T *__restrict__ a, const T *__restrict__ b,
-                                              const BChunk *__restrict__ ch, double *part) {
+__global__ __launch_bounds__(BRB) void k_bdot(const BVec *vt, int sa, int sb, const BChunk *__restrict__ ch,
+                                              double *part) {
   const BChunk c = ch[blockIdx.x];
+  const T *a = vec<T>(vt, c, sa), *b = vec<T>(vt, c, sb);
   double s = 0.;
-  for (int i = threadIdx.x; i < c.len; i += BRB) s += (double)a[c.start + i] * (double)b[c.start + i];
+  for (int i = threadIdx.x; i < c.len; i += BRB) s += (double)a[i] * (double)b[i];
   s = bbsum(s);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
@@ -113,15 +124,14 @@ __global__ __launch_bounds__(BRB) void k_bscalar(const double *part, const int *
 }
 
 template <typename T>
-__global__ __launch_bounds__(BRB) void k_bxr(T *__restrict__ x, T *__restrict__ r, T *__restrict__ z,
-                                             const T *__restrict__ p, const T *__restrict__ q,
-                                             const T *__restrict__ dinv, const BChunk *__restrict__ ch,
-                                             const double *sc, double *part) {
+__global__ __launch_bounds__(BRB) void k_bxr(const BVec *vt, int o, const BChunk *__restrict__ ch, const double *sc,
+                                             double *part) {
   const BChunk c = ch[blockIdx.x];
+  T *x = vec<T>(vt, c, o + VX), *r = vec<T>(vt, c, o + VR), *z = vec<T>(vt, c, o + VZ);
+  const T *p = vec<T>(vt, c, o + VP), *q = vec<T>(vt, c, o + VQ), *dinv = vec<T>(vt, c, o + VD);
   const double al = sc[(int64_t)c.p * B_NS + B_ALPHA];
   double rz = 0., rr = 0.;
-  for (int i = threadIdx.x; i < c.len; i += BRB) {
-    int64_t k = c.start + i;
+  for (int k = threadIdx.x; k < c.len; k += BRB) {
     x[k] = (T)((double)x[k] + al * (double)p[k]);
     T ri = (T)((double)r[k] - al * (double)q[k]);
     r[k] = ri;
@@ -136,47 +146,49 @@ __global__ __launch_bounds__(BRB) void k_bxr(T *__restrict__ x, T *__restrict__ 
 }
 
 template <typename T>
-__global__ __launch_bounds__(BRB) void k_bp(T *__restrict__ p, const T *__restrict__ z, const BChunk *__restrict__ ch,
-                                            const double *sc) {
+__global__ __launch_bounds__(BRB) void k_bp(const BVec *vt, int o, const BChunk *__restrict__ ch, const double *sc) {
   const BChunk c = ch[blockIdx.x];
+  T *p = vec<T>(vt, c, o + VP);
+  const T *z = vec<T>(vt, c, o + VZ);
   const double be = sc[(int64_t)c.p * B_NS + B_BETA];
-  for (int i = threadIdx.x; i < c.len; i += BRB) {
-    int64_t k = c.start + i;
-    p[k] = (T)((double)z[k] + be * (double)p[k]);
-  }
+  for (int k = threadIdx.x; k < c.len; k += BRB) p[k] = (T)((double)z[k] + be * (double)p[k]);
 }
 
-// r32 = r64 * scale_p ; d32 = 0
-__global__ __launch_bounds__(BRB) void k_b64to32(const double *r64, float *r32, float *d32,
-                                                 const BChunk *__restrict__ ch, const double *scale) {
+// R32 = R * scale_p ; X32 = 0
+__global__ __launch_bounds__(BRB) void k_b64to32(const BVec *vt, const BChunk *__restrict__ ch, const double *scale) {
   const BChunk c = ch[blockIdx.x];
+  const double *r64 = vec<double>(vt, c, VR);
+  float *r32 = vec<float>(vt, c, VR32), *d32 = vec<float>(vt, c, VX32);
   const double s = scale[c.p];
-  for (int i = threadIdx.x; i < c.len; i += BRB) {
-    int64_t k = c.start + i;
+  for (int k = threadIdx.x; k < c.len; k += BRB) {
     r32[k] = (float)(r64[k] * s);
     d32[k] = 0.f;
   }
 }
-// x64 += inv_scale_p * d32
-__global__ __launch_bounds__(BRB) void k_baxpy(double *x, const float *d, const BChunk *__restrict__ ch,
-                                               const double *inv_scale) {
+// X += inv_scale_p * X32
+__global__ __launch_bounds__(BRB) void k_baxpy(const BVec *vt, const BChunk *__restrict__ ch, const double *inv_scale) {
   const BChunk c = ch[blockIdx.x];
+  double *x = vec<double>(vt, c, VX);
+  const float *d = vec<float>(vt, c, VX32);
   const double s = inv_scale[c.p];
-  for (int i = threadIdx.x; i < c.len; i += BRB) x[c.start + i] += s * (double)d[c.start + i];
+  for (int k = threadIdx.x; k < c.len; k += BRB) x[k] += s * (double)d[k];
 }
-__global__ void k_bprep(double *R, double *D, int64_t n) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  R[i] = -R[i];
-  D[i] = D[i] != 0. ? 1. / D[i] : 1.;
+// R = -R ; Dinv = 1/Dinv ; Dinv32 = Dinv
+__global__ __launch_bounds__(BRB) void k_bprep(const BVec *vt, const BChunk *__restrict__ ch) {
+  const BChunk c = ch[blockIdx.x];
+  double *R = vec<double>(vt, c, VR), *D = vec<double>(vt, c, VD);
+  float *D32 = vec<float>(vt, c, VD32);
+  for (int k = threadIdx.x; k < c.len; k += BRB) {
+    R[k] = -R[k];
+    double di = D[k] != 0. ? 1. / D[k] : 1.;
+    D[k] = di;
+    D32[k] = (float)di;
+  }
 }
-__global__ void k_bneg(double *R, int64_t n) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) R[i] = -R[i];
-}
-__global__ void k_bd2f(const double *D, float *F, int64_t n) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) F[i] = (float)D[i];
+__global__ __launch_bounds__(BRB) void k_bneg(const BVec *vt, const BChunk *__restrict__ ch) {
+  const BChunk c = ch[blockIdx.x];
+  double *R = vec<double>(vt, c, VR);
+  for (int k = threadIdx.x; k < c.len; k += BRB) R[k] = -R[k];
 }
 __global__ __launch_bounds__(BRB) void k_bmember_sum(const double *part, const int *first, const int *nch,
                                                      double *out) {
@@ -186,18 +198,8 @@ __global__ __launch_bounds__(BRB) void k_bmember_sum(const double *part, const i
   a = bbsum(a);
   if (threadIdx.x == 0) out[p] = a;
 }
-static unsigned bnb(int64_t n) { return (unsigned)((n + 255) / 256); }
-void batch_prepare_vectors(double *R, double *D, int64_t n, hipStream_t s) {
-  hipLaunchKernelGGL(k_bprep, dim3(bnb(n)), dim3(256), 0, s, R, D, n);
-}
-void batch_dinv_to_f32(const double *D, float *D32, int64_t n, hipStream_t s) {
-  hipLaunchKernelGGL(k_bd2f, dim3(bnb(n)), dim3(256), 0, s, D, D32, n);
-}
 void batch_member_sums(const double *part, const int *first, const int *nch, int np, double *out, hipStream_t s) {
   hipLaunchKernelGGL(k_bmember_sum, dim3(np), dim3(BRB), 0, s, part, first, nch, out);
-}
-void batch_negate(double *R, int64_t n, hipStream_t s) {
-  hipLaunchKernelGGL(k_bneg, dim3(bnb(n)), dim3(256), 0, s, R, n);
 }
 
 }  // namespace vfg
@@ -212,18 +214,16 @@ struct vfg_batch {
   int np = 0;
   hipStream_t stream = nullptr;
   std::string err;
-  std::vector<int64_t> off;  // member offsets (doubles) of 3*nn vectors
-  int64_t n3 = 0;
   std::vector<BItem> items;
   std::vector<int> oz;
   BItem *d_items = nullptr;
   int nitems = 0;
   BChunk *d_ch = nullptr;
   int nch = 0;
+  BVec *d_vt = nullptr;  // per-member vector table (member work buffers)
+  std::vector<BVec> vt;
   int *d_first = nullptr, *d_nch = nullptr;
   double *d_part = nullptr, *d_sc = nullptr, *d_scale = nullptr, *d_iscale = nullptr;
-  double *X = nullptr, *R = nullptr, *Z = nullptr, *Pv = nullptr, *Q = nullptr, *Dinv = nullptr, *B = nullptr;
-  float *X32 = nullptr, *R32 = nullptr, *Z32 = nullptr, *P32 = nullptr, *Q32 = nullptr, *Dinv32 = nullptr;
   void *d_ops = nullptr, *d_ops32 = nullptr, *d_opsx = nullptr;
   int grid = 0;  // persistent grid size (0: one block per item)
   int graph_on = 1, graph_chunk = 10;
@@ -245,13 +245,24 @@ static int balloc(vfg_batch *b, void **p, size_t bytes) {
   b->allocs.push_back(*p);
   return VFG_OK;
 }
+static void bfree(vfg_batch *b, void *p) {
+  if (!p) return;
+  hipFree(p);
+  for (auto &q : b->allocs)
+    if (q == p) q = nullptr;
+}
 #define BCHK(b, call)                                                   \
   do {                                                                  \
     hipError_t e_ = (call);                                             \
     if (e_ != hipSuccess) return bfail(b, VFG_E_HIP, hipGetErrorString(e_)); \
   } while (0)
 
-static void build_items(vfg_batch *b, int slots, double oz_scale) {
+static void drop_graphs(vfg_batch *b) {
+  if (b->gexec) { hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+  if (b->gexec32) { hipGraphExecDestroy(b->gexec32); b->gexec32 = nullptr; }
+}
+
+static int build_items(vfg_batch *b, int slots, double oz_scale) {
   b->items.clear();
   b->oz.assign(b->np, 1);
   for (int p = 0; p < b->np; p++) {
@@ -268,6 +279,12 @@ static void build_items(vfg_batch *b, int slots, double oz_scale) {
         for (int x = 0; x < tx; x++) b->items.push_back({p, x, y, z});
   }
   b->nitems = (int)b->items.size();
+  bfree(b, b->d_items);
+  b->d_items = nullptr;
+  int rc = balloc(b, (void **)&b->d_items, sizeof(BItem) * b->items.size());
+  if (rc) return rc;
+  BCHK(b, hipMemcpy(b->d_items, b->items.data(), sizeof(BItem) * b->items.size(), hipMemcpyHostToDevice));
+  return VFG_OK;
 }
 
 extern "C" {
@@ -277,55 +294,54 @@ const char *vfg_batch_last_error(const vfg_batch *b) { return b ? b->err.c_str()
 int vfg_batch_create(vfg_handle *const *members, int n, vfg_batch **out) {
   if (!out || !members || n < 1) return VFG_E_ARG;
   *out = nullptr;
+  for (int i = 0; i < n; i++)
+    if (!members[i]) return VFG_E_ARG;
   vfg_batch *b = new vfg_batch();
   b->np = n;
-  for (int i = 0; i < n; i++) {
-    if (!members[i]) { delete b; return VFG_E_ARG; }
-    b->m.push_back(members[i]);
+  b->m.assign(members, members + n);
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete b;
+    return VFG_E_HIP;
   }
-  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) { delete b; return VFG_E_HIP; }
-  // layout of concatenated vectors and reduction chunks
+  int rc = VFG_OK;
+  // member work vectors and the pointer table
+  b->vt.resize(n);
+  for (int p = 0; p < n && !rc; p++) {
+    vfg_handle *h = members[p];
+    if ((rc = vfg_work(h, 7, 6))) {
+      bfail(b, rc, h->err.c_str());
+      break;
+    }
+    for (int i = 0; i < 7; i++) b->vt[p].v[VX + i] = h->w[i];
+    for (int i = 0; i < 6; i++) b->vt[p].v[VX32 + i] = h->wf[i];
+  }
+  // reduction chunks: never straddle two members
   std::vector<BChunk> ch;
   std::vector<int> first(n), nch(n);
-  int64_t o = 0;
   for (int p = 0; p < n; p++) {
-    b->off.push_back(o);
     int64_t len = 3 * members[p]->P.nn;
     first[p] = (int)ch.size();
-    for (int64_t s = 0; s < len; s += CH) ch.push_back({p, o + s, (int)(len - s < CH ? len - s : CH)});
+    for (int64_t s = 0; s < len; s += CH) ch.push_back({p, s, (int)(len - s < CH ? len - s : CH)});
     nch[p] = (int)ch.size() - first[p];
-    o += len;
   }
-  b->n3 = o;
   b->nch = (int)ch.size();
-  int rc = 0;
-  const size_t n3 = (size_t)b->n3;
-  if ((rc = balloc(b, (void **)&b->d_ch, sizeof(BChunk) * ch.size())) ||
+  if (rc || (rc = balloc(b, (void **)&b->d_ch, sizeof(BChunk) * ch.size())) ||
+      (rc = balloc(b, (void **)&b->d_vt, sizeof(BVec) * n)) ||
       (rc = balloc(b, (void **)&b->d_first, sizeof(int) * n)) || (rc = balloc(b, (void **)&b->d_nch, sizeof(int) * n)) ||
       (rc = balloc(b, (void **)&b->d_part, 16 * ch.size())) || (rc = balloc(b, (void **)&b->d_sc, 8 * B_NS * n)) ||
       (rc = balloc(b, (void **)&b->d_scale, 8 * n)) || (rc = balloc(b, (void **)&b->d_iscale, 8 * n)) ||
-      (rc = balloc(b, (void **)&b->X, 8 * n3)) || (rc = balloc(b, (void **)&b->R, 8 * n3)) ||
-      (rc = balloc(b, (void **)&b->Z, 8 * n3)) || (rc = balloc(b, (void **)&b->Pv, 8 * n3)) ||
-      (rc = balloc(b, (void **)&b->Q, 8 * n3)) || (rc = balloc(b, (void **)&b->Dinv, 8 * n3)) ||
-      (rc = balloc(b, (void **)&b->B, 8 * n3)) || (rc = balloc(b, (void **)&b->X32, 4 * n3)) ||
-      (rc = balloc(b, (void **)&b->R32, 4 * n3)) || (rc = balloc(b, (void **)&b->Z32, 4 * n3)) ||
-      (rc = balloc(b, (void **)&b->P32, 4 * n3)) || (rc = balloc(b, (void **)&b->Q32, 4 * n3)) ||
-      (rc = balloc(b, (void **)&b->Dinv32, 4 * n3)) ||
       (rc = balloc(b, &b->d_ops, batch_op_bytes(0) * n)) || (rc = balloc(b, &b->d_ops32, batch_op_bytes(1) * n)) ||
-      (rc = balloc(b, &b->d_opsx, batch_op_bytes(0) * n))) {
+      (rc = balloc(b, &b->d_opsx, batch_op_bytes(0) * n)) || (rc = build_items(b, batch_slots(), 1.0))) {
     vfg_batch_destroy(b);
     return rc;
   }
-  hipMemcpy(b->d_ch, ch.data(), sizeof(BChunk) * ch.size(), hipMemcpyHostToDevice);
-  hipMemcpy(b->d_first, first.data(), sizeof(int) * n, hipMemcpyHostToDevice);
-  hipMemcpy(b->d_nch, nch.data(), sizeof(int) * n, hipMemcpyHostToDevice);
-  int slots = batch_slots();
-  build_items(b, slots, 1.0);
-  if ((rc = balloc(b, (void **)&b->d_items, sizeof(BItem) * b->items.size()))) {
+  if (hipMemcpy(b->d_ch, ch.data(), sizeof(BChunk) * ch.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->d_vt, b->vt.data(), sizeof(BVec) * n, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->d_first, first.data(), sizeof(int) * n, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->d_nch, nch.data(), sizeof(int) * n, hipMemcpyHostToDevice) != hipSuccess) {
     vfg_batch_destroy(b);
-    return rc;
+    return VFG_E_HIP;
   }
-  hipMemcpy(b->d_items, b->items.data(), sizeof(BItem) * b->items.size(), hipMemcpyHostToDevice);
   b->grid = 0;
   *out = b;
   return VFG_OK;
@@ -334,9 +350,9 @@ int vfg_batch_create(vfg_handle *const *members, int n, vfg_batch **out) {
 int vfg_batch_destroy(vfg_batch *b) {
   if (!b) return VFG_OK;
   if (b->stream) hipStreamSynchronize(b->stream);
-  if (b->gexec) hipGraphExecDestroy(b->gexec);
-  if (b->gexec32) hipGraphExecDestroy(b->gexec32);
-  for (void *p : b->allocs) hipFree(p);
+  drop_graphs(b);
+  for (void *p : b->allocs)
+    if (p) hipFree(p);
   if (b->stream) hipStreamDestroy(b->stream);
   delete b;
   return VFG_OK;
@@ -365,19 +381,20 @@ int vfg_batch_u_apply_f32(vfg_batch *b, const float *const *x, float *const *y) 
   return VFG_OK;
 }
 
+// times nrep persistent applies p -> q over the members' CG direction vectors
 int vfg_batch_time_u_apply(vfg_batch *b, int fp32, int nrep, double *ms) {
   if (!b || nrep < 1 || !ms) return VFG_E_ARG;
   std::vector<const void *> xs(b->np);
   std::vector<void *> ys(b->np);
   for (int p = 0; p < b->np; p++) {
-    xs[p] = fp32 ? (const void *)(b->P32 + b->off[p]) : (const void *)(b->Pv + b->off[p]);
-    ys[p] = fp32 ? (void *)(b->Q32 + b->off[p]) : (void *)(b->Q + b->off[p]);
+    xs[p] = b->vt[p].v[fp32 ? VP32 : VP];
+    ys[p] = b->vt[p].v[fp32 ? VQ32 : VQ];
   }
   void *ops = fp32 ? b->d_ops32 : b->d_opsx;
   fill_ops(b->m, xs.data(), ys.data(), fp32, b->oz, ops, b->stream);
   hipEvent_t e0, e1;
-  hipEventCreate(&e0);
-  hipEventCreate(&e1);
+  BCHK(b, hipEventCreate(&e0));
+  BCHK(b, hipEventCreate(&e1));
   hipEventRecord(e0, b->stream);
   for (int i = 0; i < nrep; i++) launch_batch_apply(fp32, ops, b->d_items, b->nitems, b->grid, b->stream);
   hipEventRecord(e1, b->stream);
@@ -391,40 +408,27 @@ int vfg_batch_time_u_apply(vfg_batch *b, int fp32, int nrep, double *ms) {
   return VFG_OK;
 }
 
-// Autotuner: times the persistent apply for a few (z-chunk scale, grid) pairs
-// and keeps the fastest; returns the chosen grid (0 = one block per item).
+// Autotuner: times the persistent apply for each (z-chunk scale, grid) pair
+// and keeps the fastest; best_ms receives the time of one apply.
 int vfg_batch_autotune(vfg_batch *b, int fp32, double *best_ms) {
   if (!b) return VFG_E_ARG;
   const int slots = batch_slots();
   const double scales[3] = {0.5, 1.0, 2.0};
   const int grids[3] = {0, slots, 2 * slots};
-  double best = 1e300;
-  double bscale = 1.0;
-  int bgrid = 0;
+  double best = 1e300, bscale = 1.0;
+  int bgrid = 0, rc;
   for (double sc : scales) {
-    build_items(b, slots, sc);
-    hipFree(b->d_items);
-    for (auto &p : b->allocs)
-      if (p == b->d_items) p = nullptr;
-    if (balloc(b, (void **)&b->d_items, sizeof(BItem) * b->items.size())) return VFG_E_NOMEM;
-    hipMemcpy(b->d_items, b->items.data(), sizeof(BItem) * b->items.size(), hipMemcpyHostToDevice);
+    if ((rc = build_items(b, slots, sc))) return rc;
     for (int g : grids) {
       b->grid = g;
       double ms = 0.;
-      vfg_batch_time_u_apply(b, fp32, 2, &ms);
-      vfg_batch_time_u_apply(b, fp32, 5, &ms);
+      if ((rc = vfg_batch_time_u_apply(b, fp32, 2, &ms)) || (rc = vfg_batch_time_u_apply(b, fp32, 5, &ms))) return rc;
       if (ms < best) { best = ms; bscale = sc; bgrid = g; }
     }
   }
-  build_items(b, slots, bscale);
-  hipFree(b->d_items);
-  for (auto &p : b->allocs)
-    if (p == b->d_items) p = nullptr;
-  if (balloc(b, (void **)&b->d_items, sizeof(BItem) * b->items.size())) return VFG_E_NOMEM;
-  hipMemcpy(b->d_items, b->items.data(), sizeof(BItem) * b->items.size(), hipMemcpyHostToDevice);
+  if ((rc = build_items(b, slots, bscale))) return rc;
   b->grid = bgrid;
-  if (b->gexec) { hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
-  if (b->gexec32) { hipGraphExecDestroy(b->gexec32); b->gexec32 = nullptr; }
+  drop_graphs(b);
   if (best_ms) *best_ms = best / 5.;
   return VFG_OK;
 }
@@ -433,37 +437,34 @@ int vfg_batch_set_graph(vfg_batch *b, int enable, int chunk) {
   if (!b || chunk < 1) return VFG_E_ARG;
   b->graph_on = enable;
   b->graph_chunk = chunk;
-  if (b->gexec) { hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
-  if (b->gexec32) { hipGraphExecDestroy(b->gexec32); b->gexec32 = nullptr; }
+  drop_graphs(b);
   return VFG_OK;
 }
 
 }  // extern "C"
 
-// one CG iteration of the whole ensemble on vectors of type T
+// one CG iteration of the whole ensemble; o = 0 (FP64 vectors) or VX32
 template <typename T>
-static void cg_iter(vfg_batch *b, T *x, T *r, T *z, T *p, T *q, const T *dinv, void *ops, int fp32, double rtol,
-                    int maxit) {
+static void cg_iter(vfg_batch *b, int o, void *ops, int fp32, double rtol, int maxit) {
   hipStream_t s = b->stream;
+  const BChunk *ch = b->d_ch;
   launch_batch_apply(fp32, ops, b->d_items, b->nitems, b->grid, s);
-  hipLaunchKernelGGL(k_bdot<T>, dim3(b->nch), dim3(BRB), 0, s, (const T *)p, (const T *)q, (const BChunk *)b->d_ch,
-                     b->d_part);
+  hipLaunchKernelGGL(k_bdot<T>, dim3(b->nch), dim3(BRB), 0, s, (const BVec *)b->d_vt, o + VP, o + VQ, ch, b->d_part);
   hipLaunchKernelGGL(k_bscalar, dim3(b->np), dim3(BRB), 0, s, (const double *)b->d_part, (const int *)b->d_first,
                      (const int *)b->d_nch, 1, b->d_sc, rtol, maxit);
-  hipLaunchKernelGGL(k_bxr<T>, dim3(b->nch), dim3(BRB), 0, s, x, r, z, (const T *)p, (const T *)q, dinv,
-                     (const BChunk *)b->d_ch, (const double *)b->d_sc, b->d_part);
+  hipLaunchKernelGGL(k_bxr<T>, dim3(b->nch), dim3(BRB), 0, s, (const BVec *)b->d_vt, o, ch, (const double *)b->d_sc,
+                     b->d_part);
   hipLaunchKernelGGL(k_bscalar, dim3(b->np), dim3(BRB), 0, s, (const double *)b->d_part, (const int *)b->d_first,
                      (const int *)b->d_nch, 2, b->d_sc, rtol, maxit);
-  hipLaunchKernelGGL(k_bp<T>, dim3(b->nch), dim3(BRB), 0, s, p, (const T *)z, (const BChunk *)b->d_ch,
-                     (const double *)b->d_sc);
+  hipLaunchKernelGGL(k_bp<T>, dim3(b->nch), dim3(BRB), 0, s, (const BVec *)b->d_vt, o, ch, (const double *)b->d_sc);
 }
 
 // runs CG on all members to rtol; the iterate state (x, r) is prepared by the caller
 template <typename T>
-static int cg_run(vfg_batch *b, T *x, T *r, T *z, T *p, T *q, const T *dinv, void *ops, int fp32, double rtol,
-                  int maxit, hipGraphExec_t *gexec, std::vector<double> &S) {
+static int cg_run(vfg_batch *b, int o, void *ops, int fp32, double rtol, int maxit, hipGraphExec_t *gexec,
+                  std::vector<double> &S) {
   hipStream_t s = b->stream;
-  hipLaunchKernelGGL(k_binit<T>, dim3(b->nch), dim3(BRB), 0, s, (const T *)r, z, p, dinv, (const BChunk *)b->d_ch,
+  hipLaunchKernelGGL(k_binit<T>, dim3(b->nch), dim3(BRB), 0, s, (const BVec *)b->d_vt, o, (const BChunk *)b->d_ch,
                      b->d_part);
   hipLaunchKernelGGL(k_bscalar, dim3(b->np), dim3(BRB), 0, s, (const double *)b->d_part, (const int *)b->d_first,
                      (const int *)b->d_nch, 0, b->d_sc, rtol, maxit);
@@ -478,17 +479,20 @@ static int cg_run(vfg_batch *b, T *x, T *r, T *z, T *p, T *q, const T *dinv, voi
     b->g_maxit[gi] = maxit;
     hipGraph_t g;
     BCHK(b, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < K; k++) cg_iter<T>(b, x, r, z, p, q, dinv, ops, fp32, rtol, maxit);
+    for (int k = 0; k < K; k++) cg_iter<T>(b, o, ops, fp32, rtol, maxit);
     BCHK(b, hipStreamEndCapture(s, &g));
     hipError_t e = hipGraphInstantiate(gexec, g, nullptr, nullptr, 0);
     hipGraphDestroy(g);
-    if (e != hipSuccess) return bfail(b, VFG_E_HIP, "hipGraphInstantiate failed");
+    if (e != hipSuccess) {
+      *gexec = nullptr;
+      return bfail(b, VFG_E_HIP, "hipGraphInstantiate failed");
+    }
   }
   S.assign((size_t)b->np * B_NS, 0.);
   for (int done = 0; done < maxit + K; done += K) {
     if (b->graph_on) BCHK(b, hipGraphLaunch(*gexec, s));
     else
-      for (int k = 0; k < K; k++) cg_iter<T>(b, x, r, z, p, q, dinv, ops, fp32, rtol, maxit);
+      for (int k = 0; k < K; k++) cg_iter<T>(b, o, ops, fp32, rtol, maxit);
     BCHK(b, hipMemcpyAsync(S.data(), b->d_sc, 8 * S.size(), hipMemcpyDeviceToHost, s));
     BCHK(b, hipStreamSynchronize(s));
     bool any = false;
@@ -505,34 +509,38 @@ extern "C" int vfg_batch_u_solve(vfg_batch *b, double *const *U, const double *c
   vfg_solve_opts_default(&d);
   if (!o) o = &d;
   hipStream_t s = b->stream;
-  // per member: x = U with Dirichlet values, r = -F(x), dinv = 1/diag
-  for (int p = 0; p < b->np; p++) {
+  const int np = b->np;
+  const dim3 gch(b->nch), blk(BRB);
+  const BVec *vt = b->d_vt;
+  const BChunk *ch = b->d_ch;
+  auto V64 = [&](int p, int slot) { return (double *)b->vt[p].v[slot]; };
+  // per member: x = U with Dirichlet values, r = F(x), dinv = diag
+  hipMemcpyKind kin = loc == VFG_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+  for (int p = 0; p < np; p++) {
     vfg_handle *h = b->m[p];
     const int64_t n = 3 * h->P.nn;
-    double *x = b->X + b->off[p], *r = b->R + b->off[p], *B = b->B + b->off[p], *dv = b->Dinv + b->off[p];
-    hipMemcpyKind k = loc == VFG_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
-    BCHK(b, hipMemcpyAsync(x, U[p], 8 * n, k, s));
-    BCHK(b, hipMemcpyAsync(B, BCU[p], 8 * n, k, s));
-    launch_vec_apply_dirichlet(h->P, 0, x, B, s);
-    launch_u_rhs(h->P, x, B, r, 1, s);
-    launch_u_diag(h->P, dv, s);
+    BCHK(b, hipMemcpyAsync(V64(p, VX), U[p], 8 * n, kin, s));
+    BCHK(b, hipMemcpyAsync(V64(p, VB), BCU[p], 8 * n, kin, s));
+    launch_vec_apply_dirichlet(h->P, 0, V64(p, VX), V64(p, VB), s);
+    launch_u_rhs(h->P, V64(p, VX), V64(p, VB), V64(p, VR), 1, s);
+    launch_u_diag(h->P, V64(p, VD), s);
   }
-  batch_prepare_vectors(b->R, b->Dinv, b->n3, s);  // r = -F, dinv = 1/diag
-  std::vector<const void *> xs(b->np), xs32(b->np);
-  std::vector<void *> ys(b->np), ys32(b->np);
-  for (int p = 0; p < b->np; p++) {
-    xs[p] = b->Pv + b->off[p];
-    ys[p] = b->Q + b->off[p];
-    xs32[p] = b->P32 + b->off[p];
-    ys32[p] = b->Q32 + b->off[p];
+  hipLaunchKernelGGL(k_bprep, gch, blk, 0, s, vt, ch);  // r = -F, dinv = 1/diag (+ FP32 copy)
+  std::vector<const void *> xs(np), xs32(np);
+  std::vector<void *> ys(np), ys32(np);
+  for (int p = 0; p < np; p++) {
+    xs[p] = b->vt[p].v[VP];
+    ys[p] = b->vt[p].v[VQ];
+    xs32[p] = b->vt[p].v[VP32];
+    ys32[p] = b->vt[p].v[VQ32];
   }
-  std::vector<int> status(b->np, 0), iters(b->np, 0), outer(b->np, 0);
-  std::vector<double> relres(b->np, 0.), S;
+  std::vector<int> status(np, 0), iters(np, 0), outer(np, 0);
+  std::vector<double> relres(np, 0.), S;
   int rc = VFG_OK;
   if (o->precision != VFG_PREC_MIXED) {
     fill_ops(b->m, xs.data(), ys.data(), 0, b->oz, b->d_ops, s);
-    rc = cg_run<double>(b, b->X, b->R, b->Z, b->Pv, b->Q, b->Dinv, b->d_ops, 0, o->rtol, o->maxit, &b->gexec, S);
-    for (int p = 0; p < b->np && !rc; p++) {
+    rc = cg_run<double>(b, 0, b->d_ops, 0, o->rtol, o->maxit, &b->gexec, S);
+    for (int p = 0; p < np && !rc; p++) {
       const double *Sp = &S[(size_t)p * B_NS];
       iters[p] = (int)Sp[B_ITS];
       status[p] = (int)Sp[B_STATUS];
@@ -542,57 +550,55 @@ extern "C" int vfg_batch_u_solve(vfg_batch *b, double *const *U, const double *c
   } else {
     // FP64 refinement around the FP32 inner CG
     fill_ops(b->m, xs32.data(), ys32.data(), 1, b->oz, b->d_ops32, s);
-    batch_dinv_to_f32(b->Dinv, b->Dinv32, b->n3, s);
-    std::vector<double> rn0(b->np, -1.), scale(b->np), iscale(b->np), nrm(b->np);
-    std::vector<int> done(b->np, 0);
+    std::vector<double> rn0(np, -1.), scale(np), iscale(np), nrm(np);
+    std::vector<int> done(np, 0);
     const double inner_rtol = 1e-4;
-    for (int it = 0; it < 60 && !rc; it++) {
+    const int max_outer = 60;
+    for (int it = 0; it < max_outer && !rc; it++) {
       // per-member FP64 residual norms
-      hipLaunchKernelGGL(k_bdot<double>, dim3(b->nch), dim3(BRB), 0, s, (const double *)b->R, (const double *)b->R,
-                         (const BChunk *)b->d_ch, b->d_part);
-      batch_member_sums(b->d_part, b->d_first, b->d_nch, b->np, b->d_scale, s);
-      BCHK(b, hipMemcpyAsync(nrm.data(), b->d_scale, 8 * b->np, hipMemcpyDeviceToHost, s));
+      hipLaunchKernelGGL(k_bdot<double>, gch, blk, 0, s, vt, (int)VR, (int)VR, ch, b->d_part);
+      batch_member_sums(b->d_part, b->d_first, b->d_nch, np, b->d_scale, s);
+      BCHK(b, hipMemcpyAsync(nrm.data(), b->d_scale, 8 * np, hipMemcpyDeviceToHost, s));
       BCHK(b, hipStreamSynchronize(s));
       bool any = false;
-      for (int p = 0; p < b->np; p++) {
+      for (int p = 0; p < np; p++) {
         double rn = sqrt(nrm[p]);
         if (rn0[p] < 0.) rn0[p] = rn;
-        relres[p] = rn0[p] > 0. ? rn / rn0[p] : 0.;
-        if (!isfinite(rn)) { status[p] = VFG_E_NAN; done[p] = 1; }
+        if (!done[p]) relres[p] = rn0[p] > 0. ? rn / rn0[p] : 0.;
+        if (!done[p] && !isfinite(rn)) { status[p] = VFG_E_NAN; done[p] = 1; }
         if (!done[p] && (rn <= fmax(o->rtol * rn0[p], o->atol) || rn == 0.)) done[p] = 1;
         if (!done[p]) { outer[p] = it + 1; any = true; }
         scale[p] = done[p] ? 0. : 1. / rn;
         iscale[p] = done[p] ? 0. : rn;
       }
       if (!any) break;
-      BCHK(b, hipMemcpyAsync(b->d_scale, scale.data(), 8 * b->np, hipMemcpyHostToDevice, s));
-      BCHK(b, hipMemcpyAsync(b->d_iscale, iscale.data(), 8 * b->np, hipMemcpyHostToDevice, s));
-      hipLaunchKernelGGL(k_b64to32, dim3(b->nch), dim3(BRB), 0, s, (const double *)b->R, b->R32, b->X32,
-                         (const BChunk *)b->d_ch, (const double *)b->d_scale);
-      rc = cg_run<float>(b, b->X32, b->R32, b->Z32, b->P32, b->Q32, b->Dinv32, b->d_ops32, 1, inner_rtol, o->maxit,
-                         &b->gexec32, S);
-      for (int p = 0; p < b->np; p++) {
-        iters[p] += (int)S[(size_t)p * B_NS + B_ITS];
-        int st = (int)S[(size_t)p * B_NS + B_STATUS];
-        if (st == VFG_E_NAN) { status[p] = st; done[p] = 1; }
+      if (it == max_outer - 1) {
+        for (int p = 0; p < np; p++)
+          if (!done[p]) status[p] = VFG_E_MAXIT;
+        break;
       }
-      hipLaunchKernelGGL(k_baxpy, dim3(b->nch), dim3(BRB), 0, s, b->X, (const float *)b->X32, (const BChunk *)b->d_ch,
-                         (const double *)b->d_iscale);
-      for (int p = 0; p < b->np; p++) {
+      BCHK(b, hipMemcpyAsync(b->d_scale, scale.data(), 8 * np, hipMemcpyHostToDevice, s));
+      BCHK(b, hipMemcpyAsync(b->d_iscale, iscale.data(), 8 * np, hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(k_b64to32, gch, blk, 0, s, vt, ch, (const double *)b->d_scale);
+      rc = cg_run<float>(b, VX32, b->d_ops32, 1, inner_rtol, o->maxit, &b->gexec32, S);
+      for (int p = 0; p < np && !rc; p++) {
+        if (done[p]) continue;
+        iters[p] += (int)S[(size_t)p * B_NS + B_ITS];
+        if ((int)S[(size_t)p * B_NS + B_STATUS] == VFG_E_NAN) { status[p] = VFG_E_NAN; done[p] = 1; }
+      }
+      hipLaunchKernelGGL(k_baxpy, gch, blk, 0, s, vt, ch, (const double *)b->d_iscale);
+      for (int p = 0; p < np; p++) {
         if (done[p]) continue;
         vfg_handle *h = b->m[p];
-        launch_u_rhs(h->P, b->X + b->off[p], b->B + b->off[p], b->R + b->off[p], 1, s);
+        launch_u_rhs(h->P, V64(p, VX), V64(p, VB), V64(p, VR), 1, s);
       }
-      batch_negate(b->R, b->n3, s);
-      if (it == 59)
-        for (int p = 0; p < b->np; p++)
-          if (!done[p]) status[p] = VFG_E_MAXIT;
+      hipLaunchKernelGGL(k_bneg, gch, blk, 0, s, vt, ch);
     }
   }
   if (rc) return rc;
-  for (int p = 0; p < b->np; p++) {
-    hipMemcpyKind k = loc == VFG_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
-    BCHK(b, hipMemcpyAsync(U[p], b->X + b->off[p], 8 * 3 * b->m[p]->P.nn, k, s));
+  hipMemcpyKind kout = loc == VFG_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+  for (int p = 0; p < np; p++) {
+    BCHK(b, hipMemcpyAsync(U[p], V64(p, VX), 8 * 3 * b->m[p]->P.nn, kout, s));
     if (info) {
       info[p].iterations = iters[p];
       info[p].outer = outer[p];
@@ -602,7 +608,7 @@ extern "C" int vfg_batch_u_solve(vfg_batch *b, double *const *U, const double *c
   }
   BCHK(b, hipStreamSynchronize(s));
   BCHK(b, hipGetLastError());
-  for (int p = 0; p < b->np; p++)
+  for (int p = 0; p < np; p++)
     if (status[p]) return bfail(b, status[p], "vfg_batch_u_solve: a member did not converge (see info[].status)");
   return VFG_OK;
 }

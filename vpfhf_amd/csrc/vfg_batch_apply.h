@@ -16,9 +16,6 @@ void fill_ops(const std::vector<vfg_handle *> &m, const void *const *x, void *co
 // grid == 0: one workgroup per item; otherwise a persistent grid striding the items
 void launch_batch_apply(int fp32, const void *d_ops, const BItem *items, int nitems, int grid, hipStream_t s);
 
-void batch_prepare_vectors(double *R, double *D, int64_t n, hipStream_t s);
-void batch_dinv_to_f32(const double *D, float *D32, int64_t n, hipStream_t s);
 void batch_member_sums(const double *part, const int *first, const int *nch, int np, double *out, hipStream_t s);
-void batch_negate(double *R, int64_t n, hipStream_t s);
 
 }  // namespace vfg

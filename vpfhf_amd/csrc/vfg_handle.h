@@ -27,4 +27,6 @@ struct vfg_handle {
 int vfg_fail(vfg_handle *h, int code, const char *fmt, ...);
 int vfg_alloc(vfg_handle *h, void **p, size_t bytes);
 int vfg_scratch(vfg_handle *h, int64_t ndoubles);
+// solver work vectors of 3*nn entries: w[0..n64) FP64 and wf[0..n32) FP32
+int vfg_work(vfg_handle *h, int n64, int n32);
 int vfg_in(vfg_handle *h, void *dst, const void *src, size_t bytes, int loc);

@@ -173,6 +173,14 @@ static int work(vfg_handle *h, int nvec, int64_t n) {
   }
   return VFG_OK;
 }
+}  // namespace vfg
+int vfg_work(vfg_handle *h, int n64, int n32) {
+  const int64_t n = 3 * h->P.nn;
+  int rc = vfg::work(h, n64, n);
+  for (int i = 0; i < n32 && !rc; i++) rc = vfg_alloc(h, (void **)&h->wf[i], 4 * (size_t)n);
+  return rc;
+}
+namespace vfg {
 
 // ---------------------------------------------------------------- V helpers
 __global__ void k_v_clamp(double *V, const uint8_t *fix, const double *ub, int64_t n) {

@@ -81,10 +81,6 @@ def options(text=""):
     return o
 
 
-def _p(a):
-    return None if a is None else np.ascontiguousarray(a, np.float64).ctypes.data
-
-
 class Host:
     """One problem driven by the C host (device-resident state)."""
 
