@@ -205,6 +205,56 @@ int vfg_batch_set_graph(vfg_batch *b, int enable, int chunk);
 int vfg_batch_u_solve(vfg_batch *b, double *const *U, const double *const *BCU, const vfg_solve_opts *o,
                       vfg_solve_info *info, int loc);
 
+/* ---------------------------------------------------------------- serving */
+/* Continuous batching of alternate-minimisation requests.  The reference runs
+ * one problem per process and its loop (VFCommon.c:1192-1231, test1.c:53-69)
+ * alternates VF_StepU and VF_StepV until ||V - Vold||_inf < altmintol; the
+ * engine keeps up to max_slots such loops resident on one GPU.  Each call of
+ * vfg_engine_step advances every resident request: one graph replay runs
+ * `graph_chunk` CG iterations of every request currently in a u- or V-CG
+ * phase (one persistent apply per operator over all of them), and between
+ * replays each request takes its own refinement / active-set / line-search
+ * decisions from its own device scalars, so its result is bitwise the same
+ * alone and in any mix.  A request leaves after am_maxit iterations or when
+ * ||V - Vold||_inf < altmintol; requests join at the next step.
+ * Submitted arrays are device pointers owned by the caller (U, BCU of
+ * 3*nn entries, V, VIrrev of nn), which must stay valid until the request
+ * is reported by vfg_engine_poll; the handle keeps the request's fields. */
+typedef struct vfg_engine vfg_engine;
+typedef struct {
+  int    max_slots;   /* requests resident at once (64) */
+  int    precision;   /* VFG_PREC_MIXED (default): FP32 inner u-CG in FP64 refinement */
+  double u_rtol;      /* u-solve relative residual (1e-10) */
+  double v_rtol;      /* V-solve relative projected gradient (1e-8) */
+  int    maxit;       /* CG iteration cap of an inner solve (100000) */
+  int    graph_chunk; /* CG iterations per replay (4) */
+  int    use_graph;   /* 1: captured hipGraph replays; 0: plain launches */
+} vfg_engine_opts;
+typedef struct {
+  int64_t tag;
+  int     status;         /* 0 or the VFG_E_* of the first failing solve */
+  int     am_iterations;  /* alternate-minimisation iterations done */
+  int     u_its, v_outer, v_its;
+  double  errV;           /* last ||V - Vold||_inf */
+} vfg_engine_done;
+void vfg_engine_opts_default(vfg_engine_opts *o);
+int  vfg_engine_create(const vfg_engine_opts *o, vfg_engine **out);
+int  vfg_engine_destroy(vfg_engine *e);
+const char *vfg_engine_last_error(const vfg_engine *e);
+int  vfg_engine_free_slots(const vfg_engine *e);
+/* VIrrev is the AT1 upper bound (ignored for AT2); returns VFG_E_BUDGET when
+ * no slot is free */
+int  vfg_engine_submit(vfg_engine *e, vfg_handle *h, double *U, const double *BCU, double *V,
+                       const double *VIrrev, int am_maxit, double altmintol, int64_t tag);
+/* one scheduling step; returns the number of resident requests (>= 0) */
+int  vfg_engine_step(vfg_engine *e);
+/* finished requests since the last poll (at most max); returns the count */
+int  vfg_engine_poll(vfg_engine *e, vfg_engine_done *out, int max);
+int  vfg_engine_synchronize(vfg_engine *e);
+int  vfg_engine_stats(const vfg_engine *e, int64_t *ticks, int64_t *replays);
+/* device memory held by a handle (fields, coefficients, solver vectors) */
+int64_t vfg_device_bytes(const vfg_handle *h);
+
 /* ------------------------------------------------- device vector helpers */
 /* The PETSc Vec operations the host loop of the reference performs on the
  * fields (VecCopy, VecPointwiseMin, VecAXPY+VecNorm(NORM_INFINITY), VecScale,

@@ -85,6 +85,52 @@ int  vfh_energies(vfh_ctx *ctx, double out[5]);
 int  vfh_get_state(vfh_ctx *ctx, double *U, double *V, double *VIrrev);
 int  vfh_set_state(vfh_ctx *ctx, const double *U, const double *V, const double *VIrrev);
 
+/* ---------------------------------------------------------------- serving
+ * A request loop over the vfg engine: requests (each a complete problem given
+ * by the reference's option text plus its input fields) arrive at their
+ * arrival times, are admitted first come first served while an engine slot is
+ * free and the HBM they will hold fits the budget, are set up on the GPU
+ * (VFGeometryInitialize ... VFTimeStepPrepare, as a test driver does) and run
+ * their alternate-minimisation iterations in the engine's continuous batch;
+ * their fields stay in HBM from admission to completion.  Latency is
+ * completion - arrival. */
+typedef struct {
+  const char   *options;                /* reference option text of the request */
+  const double *E_elem, *theta, *thetaRef, *pressure;  /* host inputs (NULL: defaults) */
+  double        arrival;                /* seconds after the start of the run */
+  int           am_maxit;               /* iterations (fewer once ||V - Vold|| < altmintol) */
+  double       *U_out, *V_out;          /* optional host copies of the final U and V */
+} vfh_request;
+
+typedef struct {
+  int     status;                       /* 0, VFG_E_BUDGET (larger than the budget) or a solver code */
+  int     am_iterations, u_its, v_outer, v_its;
+  double  errV;
+  double  t_admit, t_done, latency;     /* seconds since the start; completion - arrival */
+  double  energies[5];                  /* elastic, in-situ (0), pressure work, surface, crack volume */
+  int64_t device_bytes;                 /* HBM the request held */
+} vfh_request_result;
+
+typedef struct {
+  vfg_engine_opts engine;
+  double hbm_budget;                    /* bytes resident requests may hold (200e9) */
+  int    realtime;                      /* 1: honour arrival times; 0: all arrive at t = 0 */
+} vfh_serve_opts;
+
+typedef struct {
+  double  wall, solves, solves_per_s;   /* a solve is one u- or one V-solve */
+  double  p50, p99, mean_latency, max_latency;
+  int     completed, rejected, max_resident;
+  double  peak_bytes;                   /* largest admitted total (estimate) */
+  int64_t ticks, replays;
+} vfh_serve_stats;
+
+void    vfh_serve_opts_default(vfh_serve_opts *o);
+/* HBM one request of nx*ny*nz nodes holds while resident (admission estimate) */
+int64_t vfh_request_bytes(int nx, int ny, int nz);
+int     vfh_serve(const vfh_serve_opts *o, const vfh_request *req, int nreq, vfh_request_result *res,
+                  vfh_serve_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,92 @@
+"""Serving form: continuous batching of alternate-minimisation requests.
+
+Bars: a request's U and V are bitwise the same alone and in the mix (any
+admission order, slot count or neighbours); the mixed-precision engine agrees
+with the single-problem FP64 path (vfh_altmin: vfg_u_solve + vfg_v_solve) within
+1e-6 relative in U, V, surface energy and crack volume, the FP64 engine within
+1e-8 (same algorithm, different reduction order); admission never exceeds the HBM budget and rejects a request larger than
+it; graph replays and plain launches give identical results.
+"""
+import numpy as np
+import pytest
+
+from tests.scenarios import relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def _small(n, seed, am=(2, 3), rate=None, outputs=True):
+    from vpfhf_amd import serve
+    return serve.make_requests(n, rate=rate, seed=seed, outputs=outputs, lo=8, hi=16, am_range=am)
+
+
+def test_serve_alone_equals_mix():
+    from vpfhf_amd import serve
+    R = _small(5, seed=3)
+    res, st = serve.serve(R, max_slots=3, realtime=False)
+    assert all(r["status"] == 0 for r in res), res
+    assert st["completed"] == 5 and st["max_resident"] == 3
+    for i in (0, 2, 4):
+        A = serve.RequestSet([R.specs[i]], [R.ams[i]], [0.], outputs=True)
+        ra, _ = serve.serve(A, max_slots=1, realtime=False)
+        assert ra[0]["am_iterations"] == res[i]["am_iterations"]
+        assert ra[0]["u_its"] == res[i]["u_its"] and ra[0]["v_its"] == res[i]["v_its"]
+        assert np.array_equal(A.U[0], R.U[i]) and np.array_equal(A.V[0], R.V[i])
+
+
+@pytest.mark.parametrize("mixed", [True, False])
+def test_serve_matches_single_problem_path(mixed):
+    from vpfhf_amd import serve, workloads
+    R = _small(3, seed=5, am=(2, 2))
+    res, _ = serve.serve(R, max_slots=4, realtime=False, mixed=mixed)
+    tol = 1e-6 if mixed else 1e-8
+    for i, s in enumerate(R.specs):
+        H, _ = workloads.make_host(s, u_rtol=1e-10)
+        info = H.altmin(maxit=R.ams[i], stop_on_tol=True)
+        U, V, _ = H.state()
+        en = H.energies()
+        H.close()
+        assert res[i]["status"] == 0 and info.iterations == res[i]["am_iterations"]
+        assert relerr(R.U[i], U) < tol, (i, relerr(R.U[i], U))
+        assert relerr(R.V[i], V) < tol, (i, relerr(R.V[i], V))
+        e = res[i]["energies"]
+        assert abs(e[3] - en["surface"]) <= tol * abs(en["surface"])
+        assert abs(e[4] - en["crack_volume"]) <= tol * abs(en["crack_volume"]) + 1e-18
+
+
+def test_serve_graph_and_plain_launches_agree():
+    from vpfhf_amd import serve
+    R1 = _small(3, seed=9)
+    R2 = _small(3, seed=9)
+    serve.serve(R1, max_slots=2, realtime=False, use_graph=True, graph_chunk=3)
+    serve.serve(R2, max_slots=2, realtime=False, use_graph=False, graph_chunk=3)
+    for a, b in zip(R1.U + R1.V, R2.U + R2.V):
+        assert np.array_equal(a, b)
+
+
+def test_serve_budget_admission():
+    from vpfhf_amd import serve
+    from vpfhf_amd.host import load
+    L = load()
+    R = _small(4, seed=7, am=(1, 1), outputs=False)
+    need = [L.vfh_request_bytes(s["n"] + 1, s["n"] + 1, s["n"] + 1) for s in R.specs]
+    two = min(a + b for i, a in enumerate(need) for b in need[i + 1:])
+    budget = max(max(need), two - 1)
+    res, st = serve.serve(R, max_slots=8, hbm_budget=budget, realtime=False)
+    assert all(r["status"] == 0 for r in res)
+    assert st["max_resident"] == 1 and st["peak_bytes"] <= budget
+    for r, nd in zip(res, need):
+        assert 0 < r["device_bytes"] <= nd  # the admission estimate covers what the request holds
+    res2, st2 = serve.serve(R, hbm_budget=min(need) * 0.5, realtime=False)
+    assert all(r["status"] == -7 for r in res2) and st2["rejected"] == 4 and st2["completed"] == 0
+
+
+def test_serve_poisson_stream():
+    from vpfhf_amd import serve
+    R = _small(6, seed=11, am=(1, 2), rate=200.0, outputs=False)
+    res, st = serve.serve(R, max_slots=4, realtime=True)
+    assert st["completed"] == 6 and st["solves"] == 2 * sum(r["am_iterations"] for r in res)
+    for r, t in zip(res, R.arr):
+        assert r["status"] == 0
+        assert r["t_admit"] >= t.arrival - 1e-9 and r["t_done"] >= r["t_admit"] and r["latency"] > 0
+    assert 0 < st["p50"] <= st["p99"] <= st["max_latency"]

@@ -44,6 +44,16 @@ class SolveInfo(C.Structure):
                 ("status", C.c_int)]
 
 
+class EngineOpts(C.Structure):
+    _fields_ = [("max_slots", C.c_int), ("precision", C.c_int), ("u_rtol", C.c_double), ("v_rtol", C.c_double),
+                ("maxit", C.c_int), ("graph_chunk", C.c_int), ("use_graph", C.c_int)]
+
+
+class EngineDone(C.Structure):
+    _fields_ = [("tag", C.c_int64), ("status", C.c_int), ("am_iterations", C.c_int), ("u_its", C.c_int),
+                ("v_outer", C.c_int), ("v_its", C.c_int), ("errV", C.c_double)]
+
+
 class VfgError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("%s (%d): %s" % (ERRORS.get(code, "?"), code, msg))
@@ -97,6 +107,17 @@ PROTOS = {
     "vfg_batch_set_graph": (C.c_int, [vp, C.c_int, C.c_int]),
     "vfg_batch_u_solve": (C.c_int, [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(SolveOpts),
                                     C.POINTER(SolveInfo), C.c_int]),
+    "vfg_engine_opts_default": (None, [C.POINTER(EngineOpts)]),
+    "vfg_engine_create": (C.c_int, [C.POINTER(EngineOpts), C.POINTER(vp)]),
+    "vfg_engine_destroy": (C.c_int, [vp]),
+    "vfg_engine_last_error": (C.c_char_p, [vp]),
+    "vfg_engine_free_slots": (C.c_int, [vp]),
+    "vfg_engine_submit": (C.c_int, [vp, H, vp, vp, vp, vp, C.c_int, C.c_double, C.c_int64]),
+    "vfg_engine_step": (C.c_int, [vp]),
+    "vfg_engine_poll": (C.c_int, [vp, C.POINTER(EngineDone), C.c_int]),
+    "vfg_engine_synchronize": (C.c_int, [vp]),
+    "vfg_engine_stats": (C.c_int, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "vfg_device_bytes": (C.c_int64, [H]),
     "vfg_vec_copy": (C.c_int, [H, vp, vp, C.c_int64]),
     "vfg_vec_pointwise_min": (C.c_int, [H, vp, vp, vp, C.c_int64]),
     "vfg_vec_maxabs_diff": (C.c_int, [H, vp, vp, C.c_int64, dp]),

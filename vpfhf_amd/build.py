@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libvfg.so")
 SOURCES = ["vfg_u.hip", "vfg_v.hip", "vfg_misc.hip", "vfg_api.hip", "vfg_solve.hip",
-           "vfg_batch.hip"]
+           "vfg_batch.hip", "vfg_engine.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "--offload-arch=gfx950", "-fPIC", "-std=c++17",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -46,7 +46,7 @@ def build(verbose=False):
     # C host layer (plain C, gcc) over the C-ABI
     host = os.path.join(HERE, "libvfhost.so")
     cmd = ["gcc", "-O2", "-std=c11", "-fPIC", "-shared", "-Wall", "-o", host + ".tmp",
-           os.path.join(HERE, "host", "vfhost.c"), "-L" + HERE, "-lvfg", "-Wl,-rpath,$ORIGIN"]
+           os.path.join(HERE, "host", "vfhost.c"), "-L" + HERE, "-lvfg", "-lm", "-Wl,-rpath,$ORIGIN"]
     p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:
         raise RuntimeError("gcc failed for vfhost.c:\n%s" % p.stderr[-4000:])

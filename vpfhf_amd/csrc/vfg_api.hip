@@ -92,6 +92,8 @@ int vfg_scratch(vfg_handle *h, int64_t ndoubles) {
     for (auto &p : h->allocs)
       if (p == h->stage) p = nullptr;
     h->stage = nullptr;
+    h->bytes -= 8 * (size_t)h->stage_n;
+    h->stage_n = 0;
   }
   int rc = vfg_alloc(h, (void **)&h->stage, 8 * ndoubles);
   if (!rc) h->stage_n = ndoubles;
@@ -479,6 +481,8 @@ int vfg_penny_build_v(vfg_handle *h, double *V, const double center[3], double r
 }
 
 // ------------------------------------------------------ device vector helpers
+int64_t vfg_device_bytes(const vfg_handle *h) { return h ? (int64_t)h->bytes : -1; }
+
 int vfg_vec_copy(vfg_handle *h, const double *x, double *y, int64_t n) {
   NEED_H(h);
   if (!x || !y || n < 0) return vfg_fail(h, VFG_E_ARG, "vfg_vec_copy: bad arguments");

@@ -16,6 +16,17 @@ void fill_ops(const std::vector<vfg_handle *> &m, const void *const *x, void *co
 // grid == 0: one workgroup per item; otherwise a persistent grid striding the items
 void launch_batch_apply(int fp32, const void *d_ops, const BItem *items, int nitems, int grid, hipStream_t s);
 
+// serving engine: persistent applies over device-counted work lists; items of
+// members whose flags[p * fstride] is 0 are skipped
+void launch_engine_u_apply(int fp32, const void *d_ops, const BItem *items, const int *dcount, int grid,
+                           const double *flags, int fstride, hipStream_t s);
+void make_uop(const DevProblem &P, const void *x, void *y, int fp32, int OZ, void *out);  // host record
+size_t vop_bytes();
+int v_oz(const Grid &G);
+void make_vop(const DevProblem &P, const double *x, double *y, const uint8_t *mask, void *out);
+void launch_engine_v_apply(const void *d_ops, const BItem *items, const int *dcount, int grid, const double *flags,
+                           int fstride, hipStream_t s);
+
 void batch_member_sums(const double *part, const int *first, const int *nch, int np, double *out, hipStream_t s);
 
 }  // namespace vfg

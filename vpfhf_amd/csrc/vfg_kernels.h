@@ -73,4 +73,14 @@ void launch_maxabs_diff(const double *a, const double *b, int64_t n, double *par
 void launch_copy_d2f(const double *x, float *y, int64_t n, hipStream_t s);
 void launch_copy_f2d(const float *x, double *y, int64_t n, hipStream_t s);
 
+// V-solve pieces (vfg_solve.hip): projection onto [0, ub] off the fixed nodes;
+// active set + r = -g on the inactive set; projected step Vn = P(V + lam d),
+// D = Vn - V; g = 0 on fixed nodes
+void launch_v_clamp(double *V, const uint8_t *fix, const double *ub, int64_t n, hipStream_t s);
+void launch_v_active(const double *V, const double *g, const uint8_t *fix, const double *ub, uint8_t *act,
+                     double *r, int64_t n, double *part, hipStream_t s);
+void launch_v_step(const double *V, const double *d, const uint8_t *act, const uint8_t *fix, const double *ub,
+                   double lam, double *Vn, double *D, int64_t n, hipStream_t s);
+void launch_zero_fixed(double *g, const uint8_t *fix, int64_t n, hipStream_t s);
+
 }  // namespace vfg

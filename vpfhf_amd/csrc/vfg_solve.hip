@@ -226,6 +226,22 @@ __global__ void k_zero_fixed(double *g, const uint8_t *fix, int64_t n) {
   if (i < n && fix[i]) g[i] = 0.;
 }
 
+static unsigned nbv(int64_t n) { return (unsigned)((n + 255) / 256); }
+void launch_v_clamp(double *V, const uint8_t *fix, const double *ub, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_v_clamp, dim3(nbv(n)), dim3(256), 0, s, V, fix, ub, n);
+}
+void launch_v_active(const double *V, const double *g, const uint8_t *fix, const double *ub, uint8_t *act,
+                     double *r, int64_t n, double *part, hipStream_t s) {
+  hipLaunchKernelGGL(k_v_active, dim3(red_blocks(n)), dim3(RB), 0, s, V, g, fix, ub, act, r, n, part);
+}
+void launch_v_step(const double *V, const double *d, const uint8_t *act, const uint8_t *fix, const double *ub,
+                   double lam, double *Vn, double *D, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_v_step, dim3(nbv(n)), dim3(256), 0, s, V, d, act, fix, ub, lam, Vn, D, n);
+}
+void launch_zero_fixed(double *g, const uint8_t *fix, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_fixed, dim3(nbv(n)), dim3(256), 0, s, g, fix, n);
+}
+
 static double dot_host(vfg_handle *h, const double *a, const double *b, int64_t n, int *rc) {
   launch_dot(a, b, n, h->part, h->scal + 16, h->stream);
   double v = 0.;

@@ -414,11 +414,18 @@ struct BApply {
   int OZ;
 };
 
+// Persistent walk over a work list.  dcount (optional) holds the item count on
+// the device, so a captured graph follows a list that changes between replays;
+// flags (optional) skip items of members whose flags[p * fstride] is 0.
 template <typename T>
 __global__ __launch_bounds__(256) void k_batch_u_apply(const BApply<T> *__restrict__ probs,
-                                                       const BItem *__restrict__ items, int nitems) {
-  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+                                                       const BItem *__restrict__ items, int nitems,
+                                                       const int *__restrict__ dcount,
+                                                       const double *__restrict__ flags, int fstride) {
+  const int n = dcount ? *dcount : nitems;
+  for (int it = blockIdx.x; it < n; it += gridDim.x) {
     const BItem w = items[it];
+    if (flags && flags[(int64_t)w.p * fstride] == 0.) continue;  // uniform per block
     const UApplyOp<T> op = probs[w.p].op;
     const int OZ = probs[w.p].OZ;
     __syncthreads();  // LDS of the previous item is free
@@ -467,10 +474,41 @@ void fill_ops(const std::vector<vfg_handle *> &m, const void *const *x, void *co
 void launch_batch_apply(int fp32, const void *d_ops, const BItem *items, int nitems, int grid, hipStream_t s) {
   int g = grid > 0 && grid < nitems ? grid : nitems;
   if (fp32)
-    hipLaunchKernelGGL(k_batch_u_apply<float>, dim3(g), dim3(256), 0, s, (const BApply<float> *)d_ops, items, nitems);
+    hipLaunchKernelGGL(k_batch_u_apply<float>, dim3(g), dim3(256), 0, s, (const BApply<float> *)d_ops, items, nitems,
+                       (const int *)nullptr, (const double *)nullptr, 0);
   else
     hipLaunchKernelGGL(k_batch_u_apply<double>, dim3(g), dim3(256), 0, s, (const BApply<double> *)d_ops, items,
-                       nitems);
+                       nitems, (const int *)nullptr, (const double *)nullptr, 0);
+}
+
+void launch_engine_u_apply(int fp32, const void *d_ops, const BItem *items, const int *dcount, int grid,
+                           const double *flags, int fstride, hipStream_t s) {
+  if (fp32)
+    hipLaunchKernelGGL(k_batch_u_apply<float>, dim3(grid), dim3(256), 0, s, (const BApply<float> *)d_ops, items, 0,
+                       dcount, flags, fstride);
+  else
+    hipLaunchKernelGGL(k_batch_u_apply<double>, dim3(grid), dim3(256), 0, s, (const BApply<double> *)d_ops, items, 0,
+                       dcount, flags, fstride);
+}
+
+// one member's operator entry for a persistent list (host memory `out`)
+void make_uop(const DevProblem &P, const void *x, void *y, int fp32, int OZ, void *out) {
+  auto fill = [&](auto *b, auto one) {
+    using T = decltype(one);
+    UApplyOp<T> &op = b->op;
+    op.x = (const T *)x;
+    op.y = (T *)y;
+    op.v = sizeof(T) == 4 ? (const T *)(const void *)P.V32 : (const T *)(const void *)P.V;
+    op.E_elem = P.E_elem;
+    op.G = P.G;
+    op.bc = P.bcU;
+    op.E = T(P.E); op.lam1 = T(P.lam1); op.mu1 = T(P.mu1); op.eta = T(P.eta);
+    op.vol = T(P.G.hx * P.G.hy * P.G.hz);
+    op.ihx = T(1. / P.G.hx); op.ihy = T(1. / P.G.hy); op.ihz = T(1. / P.G.hz);
+    b->OZ = OZ;
+  };
+  if (fp32) fill((BApply<float> *)out, 0.f);
+  else fill((BApply<double> *)out, 0.);
 }
 
 }  // namespace vfg

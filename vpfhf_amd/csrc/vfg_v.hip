@@ -323,6 +323,46 @@ void launch_v_apply(const DevProblem &P, const double *x, double *y, int mode, h
   hipLaunchKernelGGL(k_v_apply, mgrid(P.G, OZ), dim3(256), 0, s, op, OZ);
 }
 
+// ------------------------------------------------ persistent multi-problem apply
+// Masked Jacobian applies of several problems in one launch (serving engine):
+// a work list of (member, tile x, tile y, z-chunk); the item count lives on the
+// device and members whose flags[p * fstride] is 0 are skipped.
+struct BVApply {
+  VApplyOp op;
+  int OZ;
+};
+__global__ __launch_bounds__(256) void k_batch_v_apply(const BVApply *__restrict__ probs,
+                                                       const BItem *__restrict__ items,
+                                                       const int *__restrict__ dcount,
+                                                       const double *__restrict__ flags, int fstride) {
+  const int n = *dcount;
+  for (int it = blockIdx.x; it < n; it += gridDim.x) {
+    const BItem w = items[it];
+    if (flags[(int64_t)w.p * fstride] == 0.) continue;  // uniform per block
+    const VApplyOp op = probs[w.p].op;
+    const int OZ = probs[w.p].OZ;
+    __syncthreads();  // LDS of the previous item is free
+    march<double, 16, 16>(op.G, op, OZ, w.bx, w.by, w.bz);
+  }
+}
+size_t vop_bytes() { return sizeof(BVApply); }
+int v_oz(const Grid &G) { return oz_for(G); }
+void make_vop(const DevProblem &P, const double *x, double *y, const uint8_t *mask, void *out) {
+  BVApply *b = (BVApply *)out;
+  VApplyOp &op = b->op;
+  op.x = x; op.y = y; op.q = P.Vq; op.bV = P.bV; op.VIrrev = P.VIrrev; op.vfix = mask ? mask : P.vfix;
+  op.G = P.G; op.ne = P.ne; op.C = make_vconst(P); op.bc = P.bcV;
+  for (int f = 0; f < 6; f++) op.ftype[f] = (uint8_t)P.hbcV.face[f];
+  for (int v = 0; v < 8; v++) op.vtype[v] = (uint8_t)P.hbcV.vertex[v];
+  op.mode = 0; op.atnum = P.atnum; op.irrevtol = P.irrevtol;
+  b->OZ = oz_for(P.G);
+}
+void launch_engine_v_apply(const void *d_ops, const BItem *items, const int *dcount, int grid, const double *flags,
+                           int fstride, hipStream_t s) {
+  hipLaunchKernelGGL(k_batch_v_apply, dim3(grid), dim3(256), 0, s, (const BVApply *)d_ops, items, dcount, flags,
+                     fstride);
+}
+
 void launch_v_diag(const DevProblem &P, double *d, hipStream_t s, const uint8_t *mask) {
   VDiagOp op;
   op.d = d; op.q = P.Vq; op.vfix = mask ? mask : P.vfix; op.G = P.G; op.ne = P.ne; op.C = make_vconst(P);

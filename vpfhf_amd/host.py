@@ -34,6 +34,29 @@ class AltminInfo(C.Structure):
                 ("errV", C.c_double), ("status", C.c_int)]
 
 
+class Request(C.Structure):
+    _fields_ = [("options", C.c_char_p), ("E_elem", C.c_void_p), ("theta", C.c_void_p), ("thetaRef", C.c_void_p),
+                ("pressure", C.c_void_p), ("arrival", C.c_double), ("am_maxit", C.c_int),
+                ("U_out", C.c_void_p), ("V_out", C.c_void_p)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("am_iterations", C.c_int), ("u_its", C.c_int), ("v_outer", C.c_int),
+                ("v_its", C.c_int), ("errV", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double),
+                ("latency", C.c_double), ("energies", C.c_double * 5), ("device_bytes", C.c_int64)]
+
+
+class ServeOpts(C.Structure):
+    _fields_ = [("engine", _lib.EngineOpts), ("hbm_budget", C.c_double), ("realtime", C.c_int)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("wall", C.c_double), ("solves", C.c_double), ("solves_per_s", C.c_double), ("p50", C.c_double),
+                ("p99", C.c_double), ("mean_latency", C.c_double), ("max_latency", C.c_double),
+                ("completed", C.c_int), ("rejected", C.c_int), ("max_resident", C.c_int),
+                ("peak_bytes", C.c_double), ("ticks", C.c_int64), ("replays", C.c_int64)]
+
+
 _hl = None
 
 
@@ -61,6 +84,10 @@ def load():
             "vfh_energies": (C.c_int, [vp, dp]),
             "vfh_get_state": (C.c_int, [vp, vp, vp, vp]),
             "vfh_set_state": (C.c_int, [vp, vp, vp, vp]),
+            "vfh_serve_opts_default": (None, [C.POINTER(ServeOpts)]),
+            "vfh_request_bytes": (C.c_int64, [C.c_int, C.c_int, C.c_int]),
+            "vfh_serve": (C.c_int, [C.POINTER(ServeOpts), C.POINTER(Request), C.c_int, C.POINTER(RequestResult),
+                                    C.POINTER(ServeStats)]),
         }
         for name, (res, args) in protos.items():
             f = getattr(L, name)

@@ -15,6 +15,8 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <math.h>
+#include <time.h>
 
 struct vfh_ctx {
   vfh_options o;
@@ -349,4 +351,199 @@ int vfh_set_state(vfh_ctx *c, const double *U, const double *V, const double *VI
   if (VIrrev) CHK(c, vfg_memcpy(c->VIrrev, VIrrev, 8 * (size_t)c->nn, 0));
   CHK(c, vfg_set_fields(c->h, V ? c->V : NULL, VIrrev ? c->VIrrev : NULL, NULL, NULL, NULL, NULL, VFG_DEVICE));
   return VFG_OK;
+}
+
+/* ------------------------------------------------------------------ serving */
+static double now_s(void) {
+  struct timespec t;
+  clock_gettime(CLOCK_MONOTONIC, &t);
+  return (double)t.tv_sec + 1e-9 * (double)t.tv_nsec;
+}
+
+void vfh_serve_opts_default(vfh_serve_opts *o) {
+  if (!o) return;
+  vfg_engine_opts_default(&o->engine);
+  o->hbm_budget = 200e9;
+  o->realtime = 1;
+}
+
+/* context arrays 80 B/node; handle fields, staging and solver vectors
+ * 446 B/node; E and the V coupling table 224 B/cell; reduction scratch */
+int64_t vfh_request_bytes(int nx, int ny, int nz) {
+  const int64_t nn = (int64_t)nx * ny * nz, ne = (int64_t)(nx - 1) * (ny - 1) * (nz - 1);
+  return 530 * nn + 224 * ne + ((int64_t)5 << 20);
+}
+
+static int request_setup(const vfh_request *r, const vfh_options *o, vfh_ctx **out) {
+  vfh_ctx *c;
+  int rc = vfh_create(o, &c);
+  if (rc) return rc;
+  if ((rc = vfh_set_inputs(c, r->E_elem, r->theta, r->thetaRef, r->pressure)) || (rc = vfh_fields_initialize(c)) ||
+      (rc = vfh_set_bcu(c, 1.0)) || (rc = vfh_timestep_prepare(c)) || (rc = vfg_synchronize(c->h))) {
+    snprintf(g_err, sizeof g_err, "%s", c->err);
+    vfh_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return VFG_OK;
+}
+
+static int cmp_dbl(const void *a, const void *b) {
+  double x = *(const double *)a, y = *(const double *)b;
+  return x < y ? -1 : x > y;
+}
+static double pct(const double *sorted, int n, double q) { /* nearest rank */
+  if (n < 1) return 0.;
+  int k = (int)ceil(q * n) - 1;
+  if (k < 0) k = 0;
+  if (k >= n) k = n - 1;
+  return sorted[k];
+}
+
+int vfh_serve(const vfh_serve_opts *so, const vfh_request *req, int nreq, vfh_request_result *res,
+              vfh_serve_stats *stats) {
+  if (!req || nreq < 1 || !res) return fail(NULL, VFG_E_ARG, "vfh_serve: bad arguments");
+  vfh_serve_opts d;
+  if (!so) {
+    vfh_serve_opts_default(&d);
+    so = &d;
+  }
+  memset(res, 0, sizeof(*res) * (size_t)nreq);
+  int *order = malloc(sizeof(int) * (size_t)nreq);
+  vfh_ctx **ctx = calloc((size_t)nreq, sizeof *ctx);
+  int64_t *need = calloc((size_t)nreq, sizeof *need);
+  vfh_options *opt = malloc(sizeof(vfh_options) * (size_t)nreq);
+  double *lat = malloc(sizeof(double) * (size_t)nreq);
+  if (!order || !ctx || !need || !opt || !lat) {
+    free(order); free(ctx); free(need); free(opt); free(lat);
+    return fail(NULL, VFG_E_NOMEM, "vfh_serve: host allocation failed");
+  }
+  int rc = VFG_OK;
+  for (int i = 0; i < nreq && !rc; i++) {
+    order[i] = i;
+    vfh_options_default(&opt[i]);
+    if (!req[i].options || (rc = vfh_options_parse(&opt[i], req[i].options))) {
+      if (!rc) rc = fail(NULL, VFG_E_ARG, "vfh_serve: request without options");
+      break;
+    }
+    if (req[i].am_maxit < 1) rc = fail(NULL, VFG_E_ARG, "vfh_serve: am_maxit < 1");
+    need[i] = vfh_request_bytes(opt[i].grid.nx, opt[i].grid.ny, opt[i].grid.nz);
+  }
+  /* arrival order (stable insertion sort) */
+  for (int i = 1; i < nreq; i++) {
+    int k = order[i], j = i - 1;
+    while (j >= 0 && req[order[j]].arrival > req[k].arrival) {
+      order[j + 1] = order[j];
+      j--;
+    }
+    order[j + 1] = k;
+  }
+  vfg_engine *e = NULL;
+  if (!rc && (rc = vfg_engine_create(&so->engine, &e))) fail(NULL, rc, "vfh_serve: vfg_engine_create failed");
+  int next = 0, head = 0, done = 0, resident = 0, max_res = 0, rejected = 0;
+  double used = 0., peak = 0.;
+  vfg_engine_done dn[64];
+  const double t0 = now_s();
+  while (!rc && done < nreq) {
+    const double t = now_s() - t0;
+    while (next < nreq && (!so->realtime || req[order[next]].arrival <= t)) next++;
+    while (head < next) { /* first come, first served */
+      const int i = order[head];
+      if ((double)need[i] > so->hbm_budget) {
+        res[i].status = VFG_E_BUDGET;
+        res[i].t_admit = res[i].t_done = now_s() - t0;
+        head++;
+        done++;
+        rejected++;
+        continue;
+      }
+      if (vfg_engine_free_slots(e) < 1 || used + (double)need[i] > so->hbm_budget) break;
+      if ((rc = request_setup(&req[i], &opt[i], &ctx[i]))) break;
+      vfh_ctx *c = ctx[i];
+      if ((rc = vfg_engine_submit(e, c->h, c->U, c->BCU, c->V, c->VIrrev, req[i].am_maxit, opt[i].altmintol, i))) {
+        fail(NULL, rc, vfg_engine_last_error(e));
+        break;
+      }
+      res[i].t_admit = now_s() - t0;
+      used += (double)need[i];
+      if (used > peak) peak = used;
+      head++;
+      if (++resident > max_res) max_res = resident;
+    }
+    if (rc) break;
+    if (resident == 0) {
+      if (next < nreq && head == next && so->realtime) { /* idle until the next arrival */
+        double wait = req[order[next]].arrival - (now_s() - t0);
+        if (wait > 0.) {
+          struct timespec ts;
+          ts.tv_sec = (time_t)wait;
+          ts.tv_nsec = (long)((wait - (double)ts.tv_sec) * 1e9);
+          nanosleep(&ts, NULL);
+        }
+      }
+      continue;
+    }
+    int r = vfg_engine_step(e);
+    if (r < 0) {
+      rc = fail(NULL, r, vfg_engine_last_error(e));
+      break;
+    }
+    int nd;
+    while (!rc && (nd = vfg_engine_poll(e, dn, 64)) > 0) {
+      for (int k = 0; k < nd && !rc; k++) {
+        const int i = (int)dn[k].tag;
+        vfh_request_result *q = &res[i];
+        vfh_ctx *c = ctx[i];
+        q->status = dn[k].status;
+        q->am_iterations = dn[k].am_iterations;
+        q->u_its = dn[k].u_its;
+        q->v_outer = dn[k].v_outer;
+        q->v_its = dn[k].v_its;
+        q->errV = dn[k].errV;
+        q->device_bytes = vfg_device_bytes(c->h) + 80 * c->nn;
+        if ((rc = vfh_energies(c, q->energies)) ||
+            ((req[i].U_out || req[i].V_out) && (rc = vfh_get_state(c, req[i].U_out, req[i].V_out, NULL)))) {
+          snprintf(g_err, sizeof g_err, "%s", c->err);
+          break;
+        }
+        q->t_done = now_s() - t0;
+        q->latency = q->t_done - (so->realtime ? req[i].arrival : 0.);
+        vfh_destroy(c);
+        ctx[i] = NULL;
+        used -= (double)need[i];
+        resident--;
+        done++;
+      }
+    }
+  }
+  const double wall = now_s() - t0;
+  if (stats) {
+    memset(stats, 0, sizeof *stats);
+    int n = 0;
+    double solves = 0., sum = 0.;
+    for (int i = 0; i < nreq; i++) {
+      if (res[i].status == VFG_E_BUDGET) continue;
+      lat[n++] = res[i].latency;
+      sum += res[i].latency;
+      solves += 2. * res[i].am_iterations;
+    }
+    qsort(lat, (size_t)n, sizeof(double), cmp_dbl);
+    stats->wall = wall;
+    stats->solves = solves;
+    stats->solves_per_s = wall > 0. ? solves / wall : 0.;
+    stats->p50 = pct(lat, n, .50);
+    stats->p99 = pct(lat, n, .99);
+    stats->mean_latency = n ? sum / n : 0.;
+    stats->max_latency = n ? lat[n - 1] : 0.;
+    stats->completed = n;
+    stats->rejected = rejected;
+    stats->max_resident = max_res;
+    stats->peak_bytes = peak;
+    if (e) vfg_engine_stats(e, &stats->ticks, &stats->replays);
+  }
+  for (int i = 0; i < nreq; i++)
+    if (ctx[i]) vfh_destroy(ctx[i]);
+  if (e) vfg_engine_destroy(e);
+  free(order); free(ctx); free(need); free(opt); free(lat);
+  return rc;
 }
