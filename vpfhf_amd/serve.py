@@ -7,7 +7,8 @@ material, penny cracks, BCfiles clamped faces) plus its input fields
 (per-element E, temperature, pressure) -- and a number of alternate-
 minimisation iterations.  ``make_requests`` draws the stream: side
 log-uniform in [lo, hi] elements, lognormal E with log-std in [0.2, 0.8],
-1-3 cracks, uniform pressure in [5e-4, 5e-3], 1-20 iterations, exponential
+1-3 cracks, uniform injection pressure in [5e-4, 5e-3] on the crack faces
+(Biot beta = 0), 1-20 iterations, exponential
 inter-arrival gaps at a given rate; all from the counter RNG, so a request is
 the same bytes in every stream it appears in.
 """
@@ -30,6 +31,7 @@ def spec_request(i, seed=0, lo=24, hi=96, am_range=(1, 20), max_cracks=3):
     p = 5e-4 + (5e-3 - 5e-4) * u[4]
     s = workloads.spec_hetero(n, seed=seed * 100000 + i, sigma=sigma, ncracks=nc, dtheta=0.0)
     s["pressure"] = ("const", p)
+    s["beta"] = 0.0  # injection pressure on the crack faces (a uniform Biot load would cancel it)
     s["name"] = "request%d" % i
     return s, am
 

@@ -47,7 +47,9 @@ def spec_hetero(n=128, seed=0, sigma=0.5, corr=8.0, ncracks=3, p_range=(0., 2e-3
 def spec_ensemble_member(i, seed=0, lo=16, hi=64, uniform_p=True):
     """Member i of the batched ensemble: side log-uniform in [lo, hi], its own
     lognormal E (log-std in [0.2, 0.8]), 1-4 cracks, injection pressure in
-    [5e-4, 5e-3]."""
+    [5e-4, 5e-3] acting on the crack faces (Biot beta = 0, as
+    ValidationTests/test2.c: with beta = 1 a uniform pressure loads pores and
+    crack equally and the two loads cancel, U = 0)."""
     u = inputs.uniform(seed, 100 + i, 4)
     n = int(round(math.exp(math.log(lo) + (math.log(hi) - math.log(lo)) * u[0])))
     sigma = 0.2 + 0.6 * u[1]
@@ -55,6 +57,7 @@ def spec_ensemble_member(i, seed=0, lo=16, hi=64, uniform_p=True):
     p = 5e-4 + (5e-3 - 5e-4) * u[3]
     s = spec_hetero(n, seed=seed * 1000 + i, sigma=sigma, ncracks=nc, dtheta=0.0)
     s["pressure"] = ("const", p)
+    s["beta"] = 0.0
     s["name"] = "member%d" % i
     return s
 
