@@ -88,6 +88,133 @@ def cpu_baseline(n_sample=48, seconds=3.0):
                       "after a %.1f s assembly; oracle C restatement of VF_UIJacobian" % (n, reps, t_asm)}
 
 
+def _relerr(a, b):
+    d = float(np.linalg.norm(b))
+    return float(np.linalg.norm(a - b)) / (d if d > 0 else 1.0)
+
+
+def ensemble_bench(L, n_members=64, seed=0):
+    """Batching form: 64 independent problems, sides log-uniform in [16, 64]
+    elements, own lognormal E, 1-4 cracks, injection pressure.  The plain FP64
+    GPU drop-in solves them one after another (vfg_u_solve, Jacobi-PCG to
+    1e-10); the batched form solves all at once (FP32 persistent apply inside
+    FP64 refinement, graph-captured CG, autotuned tiles).  Each path runs once
+    untimed, then timed from the same initial state."""
+    import ctypes as C
+    from vpfhf_amd import _lib, workloads
+    from vpfhf_amd.batch import Batch
+    hosts = [workloads.make_host(workloads.spec_ensemble_member(i, seed), u_rtol=1e-10)[0]
+             for i in range(n_members)]
+    ptr = [H.device_fields() for H in hosts]
+    dof = sum(3 * H.nn for H in hosts)
+
+    def reset():
+        for H, (U, _, _, _) in zip(hosts, ptr):
+            _lib.check(L.vfg_vec_set(H.h, U, 0., 3 * H.nn), H.h)
+        for H in hosts:
+            _lib.check(L.vfg_synchronize(H.h), H.h)
+
+    def download():
+        out = []
+        for H, (U, _, _, _) in zip(hosts, ptr):
+            a = np.empty(3 * H.nn)
+            _lib.check(L.vfg_memcpy(a.ctypes.data, U, a.nbytes, 1))
+            out.append(a)
+        return out
+
+    o64 = _lib.SolveOpts(1e-10, 0., 100000, _lib.PC_JACOBI, _lib.PREC_FP64, 4)
+    info = _lib.SolveInfo()
+
+    def single():
+        for H, (U, B, _, _) in zip(hosts, ptr):
+            _lib.check(L.vfg_u_solve(H.h, U, B, C.byref(o64), C.byref(info), _lib.DEVICE), H.h)
+    reset(); single(); reset()
+    t0 = time.time(); single(); t64 = time.time() - t0
+    U64 = download()
+
+    bt = Batch(hosts)
+    apply_ms = bt.autotune(fp32=True)
+    bt.set_graph(True, 10)
+    Ua = (C.c_void_p * n_members)(*[p[0] for p in ptr])
+    Ba = (C.c_void_p * n_members)(*[p[1] for p in ptr])
+    infos = (_lib.SolveInfo * n_members)()
+    res = {}
+    for mixed in (False, True):
+        om = _lib.SolveOpts(1e-10, 0., 100000, _lib.PC_JACOBI, _lib.PREC_MIXED if mixed else _lib.PREC_FP64, 4)
+
+        def batched():
+            rc = L.vfg_batch_u_solve(bt.b, Ua, Ba, C.byref(om), infos, _lib.DEVICE)
+            bt._c(rc)
+        reset(); batched(); reset()
+        t0 = time.time(); batched(); tb = time.time() - t0
+        Ub = download()
+        res[mixed] = (tb, max(_relerr(a, b) for a, b in zip(Ub, U64)))
+    bt.close()
+    for H in hosts:
+        H.close()
+    return {"members": n_members, "dof": dof, "elements": sum(s ** 3 for s in
+                                                            [workloads.spec_ensemble_member(i, seed)["n"]
+                                                             for i in range(n_members)]),
+            "fp64_single_solves_per_s": round(n_members / t64, 2),
+            "batched_fp64_solves_per_s": round(n_members / res[False][0], 2),
+            "batched_mixed_solves_per_s": round(n_members / res[True][0], 2),
+            "speedup_mixed_batched_vs_fp64_single": round(t64 / res[True][0], 2),
+            "max_rel_err_mixed_vs_fp64": res[True][1], "max_rel_err_batched_fp64_vs_single": res[False][1],
+            "persistent_fp32_apply_ms": round(apply_ms, 4)}
+
+
+def precision_bench(n=256, seed=0):
+    """Precision form at n^3 elements: one alternate-minimisation iteration with
+    the FP64 u-solve and with FP32 Chebyshev(4)-Jacobi PCG inside FP64
+    refinement (both to 1e-10); relative errors of the mixed result in U, V,
+    crack volume and surface energy against the FP64 one."""
+    from vpfhf_amd import workloads
+    spec = workloads.spec_hetero(n, seed=seed)
+    out = {}
+    for name, extra in (("fp64", ""), ("mixed", "-vfg_precision mixed -vfg_pc chebyshev")):
+        H, _ = workloads.make_host(spec, u_rtol=1e-10, extra=extra)
+        t0 = time.time()
+        info = H.altmin(maxit=1)
+        dt = time.time() - t0
+        U, V, _ = H.state()
+        out[name] = (dt, info.u_its, U, V, H.energies())
+        H.close()
+    d64, dm = out["fp64"], out["mixed"]
+    e64, em = d64[4], dm[4]
+    return {"elements": n ** 3, "dof": 3 * (n + 1) ** 3,
+            "fp64_seconds": round(d64[0], 3), "mixed_seconds": round(dm[0], 3),
+            "fp64_u_cg_iterations": d64[1], "mixed_u_inner_iterations": dm[1],
+            "rel_err_U": _relerr(dm[2], d64[2]), "rel_err_V": _relerr(dm[3], d64[3]),
+            "rel_err_crack_volume": abs(em["crack_volume"] - e64["crack_volume"]) / abs(e64["crack_volume"]),
+            "rel_err_surface_energy": abs(em["surface"] - e64["surface"]) / abs(e64["surface"])}
+
+
+def serving_bench(rates=(50., 100., 200., 500.), n_req=16, seed=0):
+    """Serving form: a Poisson stream of n_req alternate-minimisation requests
+    (sides 24-96 elements, 1-20 iterations, 1-3 cracks) at each rate, 64
+    slots, 200 GB HBM budget, FP32 u-CG inside FP64 refinement.  The same
+    requests re-run on the FP64 engine give the error of every result."""
+    from vpfhf_amd import serve
+    R = serve.make_requests(n_req, rate=None, seed=seed, outputs=True)
+    base_t = serve.arrival_times(n_req, 1.0, seed)  # unit-rate arrivals, scaled per rate
+    rows = []
+    for k, rate in enumerate(rates):
+        for i in range(n_req):
+            R.arr[i].arrival = float(base_t[i] / rate)
+        res, st = serve.serve(R, max_slots=64, hbm_budget=200e9, realtime=True)
+        if k == 0:
+            Um = [u.copy() for u in R.U]
+            Vm = [v.copy() for v in R.V]
+        rows.append({"rate_per_s": rate, "p50_s": round(st["p50"], 4), "p99_s": round(st["p99"], 4),
+                     "solves_per_s": round(st["solves_per_s"], 2), "wall_s": round(st["wall"], 3),
+                     "max_resident": st["max_resident"], "solves": st["solves"]})
+    serve.serve(R, max_slots=64, mixed=False, realtime=False)
+    err_u = max(_relerr(a, b) for a, b in zip(Um, R.U))
+    err_v = max(_relerr(a, b) for a, b in zip(Vm, R.V))
+    return {"requests": n_req, "sides": [s["n"] for s in R.specs], "am_iterations": R.ams, "rates": rows,
+            "max_rel_err_U_vs_fp64": err_u, "max_rel_err_V_vs_fp64": err_v}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,6 +225,8 @@ def main():
     ap.add_argument("--no-solves", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--calib", action="store_true", help="also run a plain streaming kernel of known size")
+    ap.add_argument("--no-forms", action="store_true", help="skip the ensemble, precision and serving forms")
+    ap.add_argument("--serve-requests", type=int, default=16)
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: every box here has one MI355X and this path has no multi-GPU "
@@ -175,9 +304,15 @@ def main():
                        "achieved_GBs": round(apply_bytes(nn, ne, fp32=True) / t32 / 1e9, 1)},
     }
     out.update(extra)
+    H.close()
+    if not (a.no_solves or a.no_forms):
+        for key, fn in (("ensemble", lambda: ensemble_bench(L)), ("precision", precision_bench),
+                        ("serving", lambda: serving_bench(n_req=a.serve_requests))):
+            t0 = time.time()
+            out[key] = fn()
+            print("bench: %s form done in %.1f s" % (key, time.time() - t0), file=sys.stderr, flush=True)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
-    H.close()
     print(json.dumps(out), flush=True)
 
 
