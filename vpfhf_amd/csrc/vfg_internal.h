@@ -16,6 +16,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 
 namespace vfg {
 
@@ -67,11 +68,20 @@ __device__ __forceinline__ bool on_dirichlet(const Grid &G, uint8_t face, uint8_
 //   __device__ T load(int f, int64_t node)  -- value of staged field f at a node
 //   __device__ void element(int ex,int ey,int ek, const T (&xin)[2][4][NF], T (&r)[8][NC])
 //   __device__ void store(int i,int j,int k, const T (&acc)[NC], const T (&own)[NF])
+// Optional per-element coefficient read one layer ahead (hides its latency):
+//   __device__ T aux(int ex,int ey,int ek); element(ex,ey,ek, xin, aux, r)
 // Corners of an element are a = (k*2+j)*2+i; xin[k][j*2+i][f].
+template <class Op, class = void>
+struct has_aux : std::false_type {};
+template <class Op>
+struct has_aux<Op, std::void_t<decltype(&Op::aux)>> : std::true_type {};
+
 template <typename T, int TX, int TY, class Op>
 __device__ __forceinline__ void march(const Grid &G, const Op &op, int OZ, int bx, int by, int bz) {
   constexpr int NF = Op::NF, NC = Op::NC;
   constexpr int NT = TX * TY, OX = TX - 1, OY = TY - 1, SX = TX + 1, SY = TY + 1;
+  constexpr int NPF = (SX * SY + NT - 1) / NT;  // staged nodes per thread
+  constexpr bool AUX = has_aux<Op>::value;
   __shared__ T s_in[NF][SY * SX];
   __shared__ T s_red[2 * NC][NT];
   const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
@@ -79,50 +89,81 @@ __device__ __forceinline__ void march(const Grid &G, const Op &op, int OZ, int b
   const int ex = ox0 - 1 + tx, ey = oy0 - 1 + ty;
   const bool exy_ok = ex >= 0 && ex < G.nex && ey >= 0 && ey < G.ney;
   const bool owner = tx < OX && ty < OY && (ox0 + tx) < G.nx && (oy0 + ty) < G.ny;
+  const int64_t plane = (int64_t)G.nx * G.ny;
 
-  auto stage = [&](int iz) {
-    for (int n = tid; n < SX * SY; n += NT) {
-      int lx = n % SX, ly = n / SX;
-      int gx = ox0 - 1 + lx, gy = oy0 - 1 + ly;
-      bool ok = gx >= 0 && gx < G.nx && gy >= 0 && gy < G.ny && iz >= 0 && iz < G.nz;
-      int64_t nd = ok ? node_id(G, gx, gy, iz) : 0;
+  // staged nodes of this thread: in-plane offsets, fixed for the whole march
+  int64_t soff[NPF];
+  bool sok[NPF];
 #pragma unroll
-      for (int f = 0; f < NF; f++) s_in[f][n] = ok ? op.load(f, nd) : T(0);
+  for (int q = 0; q < NPF; q++) {
+    const int n = tid + q * NT;
+    const int lx = n % SX, ly = n / SX;
+    const int gx = ox0 - 1 + lx, gy = oy0 - 1 + ly;
+    sok[q] = n < SX * SY && gx >= 0 && gx < G.nx && gy >= 0 && gy < G.ny;
+    soff[q] = sok[q] ? (int64_t)gx + (int64_t)G.nx * gy : 0;
+  }
+  // The next node plane is loaded into registers while the current element
+  // layer computes, and written to LDS after the next barrier.
+  T pf[NPF][NF];
+  auto fetch = [&](int iz) {
+    const bool zok = iz >= 0 && iz < G.nz;  // uniform
+#pragma unroll
+    for (int q = 0; q < NPF; q++) {
+      const bool ok = zok && sok[q];
+      const int64_t nd = soff[q] + plane * iz;
+#pragma unroll
+      for (int f = 0; f < NF; f++) pf[q][f] = ok ? op.load(f, nd) : T(0);
     }
   };
-  T xin[2][4][NF];
+  auto commit = [&]() {
+#pragma unroll
+    for (int q = 0; q < NPF; q++) {
+      const int n = tid + q * NT;
+      if (n < SX * SY)
+#pragma unroll
+        for (int f = 0; f < NF; f++) s_in[f][n] = pf[q][f];
+    }
+  };
+  auto aux_at = [&](int ek) -> T {
+    if constexpr (AUX) return (exy_ok && ek >= 0 && ek < G.nez) ? op.aux(ex, ey, ek) : T(0);
+    else return T(0);
+  };
+  auto read_plane = [&](T (&xp)[4][NF]) {
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+      for (int f = 0; f < NF; f++) xp[q][f] = s_in[f][(ty + (q >> 1)) * SX + tx + (q & 1)];
+  };
   T carry[NC];
 #pragma unroll
   for (int c = 0; c < NC; c++) carry[c] = T(0);
 
-  stage(oz0 - 1);
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < 4; q++)
-#pragma unroll
-    for (int f = 0; f < NF; f++) xin[1][q][f] = s_in[f][(ty + (q >> 1)) * SX + tx + (q & 1)];
-
-  const int ek_end = min(oz0 + OZ - 1, G.nz - 1);
-  for (int ek = oz0 - 1; ek <= ek_end; ek++) {
-#pragma unroll
-    for (int q = 0; q < 4; q++)
-#pragma unroll
-      for (int f = 0; f < NF; f++) xin[0][q][f] = xin[1][q][f];
+  // one element layer: bottom plane xb (node plane ek), top plane xt (ek + 1)
+  auto layer = [&](int ek, const T (&xb)[4][NF], T (&xt)[4][NF], T &aux_next) {
+    const T aux_cur = aux_next;
     __syncthreads();
-    stage(ek + 1);
+    commit();       // plane ek + 1
+    fetch(ek + 2);  // in flight during this layer
+    aux_next = aux_at(ek + 1);
     __syncthreads();
-#pragma unroll
-    for (int q = 0; q < 4; q++)
-#pragma unroll
-      for (int f = 0; f < NF; f++) xin[1][q][f] = s_in[f][(ty + (q >> 1)) * SX + tx + (q & 1)];
-
+    read_plane(xt);
     T r[8][NC];
 #pragma unroll
     for (int a = 0; a < 8; a++)
 #pragma unroll
       for (int c = 0; c < NC; c++) r[a][c] = T(0);
-    if (exy_ok && ek >= 0 && ek < G.nez) op.element(ex, ey, ek, xin, r);
-
+    if (exy_ok && ek >= 0 && ek < G.nez) {
+      T xin[2][4][NF];
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+          xin[0][q][f] = xb[q][f];
+          xin[1][q][f] = xt[q][f];
+        }
+      if constexpr (AUX) op.element(ex, ey, ek, xin, aux_cur, r);
+      else op.element(ex, ey, ek, xin, r);
+    }
     // x-combine: node (ex+1) gets corner i=1 of this element and corner i=0
     // of the element to the right (next lane inside the 16-lane row).
     T cx[2][2][NC];
@@ -149,8 +190,22 @@ __device__ __forceinline__ void march(const Grid &G, const Op &op, int OZ, int b
         bot[c] = carry[c] + (cx[0][1][c] + s_red[c][tid + TX]);
         carry[c] = cx[1][1][c] + s_red[NC + c][tid + TX];
       }
-      if (ek >= oz0 && ek >= 0) op.store(ox0 + tx, oy0 + ty, ek, bot, xin[0][3]);
+      if (ek >= oz0 && ek >= 0) op.store(ox0 + tx, oy0 + ty, ek, bot, xb[3]);
     }
+  };
+
+  T xa[4][NF], xb2[4][NF];  // ping-pong planes: no register copies between layers
+  fetch(oz0 - 1);
+  commit();
+  fetch(oz0);
+  T aux_next = aux_at(oz0 - 1);
+  __syncthreads();
+  read_plane(xa);
+  const int ek_end = min(oz0 + OZ - 1, G.nz - 1);
+  for (int ek = oz0 - 1; ek <= ek_end; ek += 2) {
+    layer(ek, xa, xb2, aux_next);
+    if (ek + 1 > ek_end) break;
+    layer(ek + 1, xb2, xa, aux_next);
   }
 }
 

@@ -19,10 +19,15 @@ namespace vfg {
 
 // Apply of one element: xin holds (x0,x1,x2,v) at the 8 corners.
 // cE = E_e * hx*hy*hz; lam1/mu1 are the Lame constants for E = 1.
+// The gradients are carried pre-scaled by mu (H' = mu H), so that
+//   sigma / (W s cE) = lr tr(H') I + H' + H'^T,  lr = lam / mu,
+// and each stress entry is accumulated with one FMA by its weight W s cE.
 template <typename T>
 __device__ __forceinline__ void u_elem_apply(const T (&X)[2][4][4], T cE, T lam1, T mu1, T eta,
                                              T ihx, T ihy, T ihz, T (&r)[8][3]) {
-  // differences along each direction, scaled by 1/h
+  const T lr = lam1 / mu1;
+  const T mx = mu1 * ihx, my = mu1 * ihy, mz = mu1 * ihz;
+  // differences along each direction, scaled by mu/h
   T Dx[3][2][2], Dy[3][2][2], Dz[3][2][2];  // Dx[c][k][j], Dy[c][k][i], Dz[c][j][i]
 #pragma unroll
   for (int c = 0; c < 3; c++)
@@ -30,11 +35,11 @@ __device__ __forceinline__ void u_elem_apply(const T (&X)[2][4][4], T cE, T lam1
     for (int p = 0; p < 2; p++)
 #pragma unroll
       for (int q = 0; q < 2; q++) {
-        Dx[c][p][q] = (X[p][q * 2 + 1][c] - X[p][q * 2 + 0][c]) * ihx;
-        Dy[c][p][q] = (X[p][2 + q][c] - X[p][q][c]) * ihy;
-        Dz[c][p][q] = (X[1][p * 2 + q][c] - X[0][p * 2 + q][c]) * ihz;
+        Dx[c][p][q] = (X[p][q * 2 + 1][c] - X[p][q * 2 + 0][c]) * mx;
+        Dy[c][p][q] = (X[p][2 + q][c] - X[p][q][c]) * my;
+        Dz[c][p][q] = (X[1][p * 2 + q][c] - X[0][p * 2 + q][c]) * mz;
       }
-  // d_z u_c at (gx,gy): constant in z
+  // mu d_z u_c at (gx,gy): constant in z
   T Gz[3][3][3];  // [c][gy][gx]
 #pragma unroll
   for (int c = 0; c < 3; c++)
@@ -97,19 +102,17 @@ __device__ __forceinline__ void u_elem_apply(const T (&X)[2][4][4], T cE, T lam1
       const T wzy = wz * T(gW(gy));
 #pragma unroll
       for (int gx = 0; gx < 3; gx++) {
-        T v = interp<T>(gz, vxy[0][gy][gx], vxy[1][gy][gx]);
-        T cq = wzy * T(gW(gx)) * (v * v + eta);
-        T H00 = Gx[0][gy], H01 = Gy[0][gx], H02 = Gz[0][gy][gx];
-        T H10 = Gx[1][gy], H11 = Gy[1][gx], H12 = Gz[1][gy][gx];
-        T H20 = Gx[2][gy], H21 = Gy[2][gx], H22 = Gz[2][gy][gx];
-        T m = mu1 * cq;
-        T lt = lam1 * cq * (H00 + H11 + H22);
-        T m2 = m + m;
-        T s00 = lt + m2 * H00, s11 = lt + m2 * H11, s22 = lt + m2 * H22;
-        T s01 = m * (H01 + H10), s02 = m * (H02 + H20), s12 = m * (H12 + H21);
-        Sx[0][gy] += s00; Sx[1][gy] += s01; Sx[2][gy] += s02;
-        Sy[0][gx] += s01; Sy[1][gx] += s11; Sy[2][gx] += s12;
-        Sz[0][gy][gx] += s02; Sz[1][gy][gx] += s12; Sz[2][gy][gx] += s22;
+        const T v = interp<T>(gz, vxy[0][gy][gx], vxy[1][gy][gx]);
+        const T cq = (wzy * T(gW(gx))) * (v * v + eta);
+        const T H00 = Gx[0][gy], H01 = Gy[0][gx], H02 = Gz[0][gy][gx];
+        const T H10 = Gx[1][gy], H11 = Gy[1][gx], H12 = Gz[1][gy][gx];
+        const T H20 = Gx[2][gy], H21 = Gy[2][gx], H22 = Gz[2][gy][gx];
+        const T lt = lr * (H00 + H11 + H22);
+        const T t00 = lt + T(2) * H00, t11 = lt + T(2) * H11, t22 = lt + T(2) * H22;
+        const T t01 = H01 + H10, t02 = H02 + H20, t12 = H12 + H21;
+        Sx[0][gy] += cq * t00; Sx[1][gy] += cq * t01; Sx[2][gy] += cq * t02;
+        Sy[0][gx] += cq * t01; Sy[1][gx] += cq * t11; Sy[2][gx] += cq * t12;
+        Sz[0][gy][gx] += cq * t02; Sz[1][gy][gx] += cq * t12; Sz[2][gy][gx] += cq * t22;
       }
     }
     const T pz0 = T(gP0(gz)), pz1 = T(gP1(gz));
@@ -127,7 +130,7 @@ __device__ __forceinline__ void u_elem_apply(const T (&X)[2][4][4], T cE, T lam1
       Ry[c][1][0] += pz1 * ay0; Ry[c][1][1] += pz1 * ay1;
     }
   }
-  // project Sz onto the bilinear basis in (x,y)
+  // project Sz onto the bilinear basis in (x,y); scale each direction by 1/h once
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     T b[3][2];  // [gy][i]
@@ -144,18 +147,23 @@ __device__ __forceinline__ void u_elem_apply(const T (&X)[2][4][4], T cE, T lam1
       T a0 = T(0), a1 = T(0);
 #pragma unroll
       for (int gy = 0; gy < 3; gy++) { a0 += T(gP0(gy)) * b[gy][i]; a1 += T(gP1(gy)) * b[gy][i]; }
-      Rz[0][i] = a0; Rz[1][i] = a1;
+      Rz[0][i] = a0 * ihz; Rz[1][i] = a1 * ihz;
     }
+    T sx[2][2], sy[2][2];
+#pragma unroll
+    for (int p = 0; p < 2; p++)
+#pragma unroll
+      for (int q = 0; q < 2; q++) { sx[p][q] = Rx[c][p][q] * ihx; sy[p][q] = Ry[c][p][q] * ihy; }
 #pragma unroll
     for (int k = 0; k < 2; k++)
 #pragma unroll
       for (int j = 0; j < 2; j++)
 #pragma unroll
         for (int i = 0; i < 2; i++) {
-          T vxv = (i ? Rx[c][k][j] : -Rx[c][k][j]) * ihx;
-          T vyv = (j ? Ry[c][k][i] : -Ry[c][k][i]) * ihy;
-          T vzv = (k ? Rz[j][i] : -Rz[j][i]) * ihz;
-          r[(k * 2 + j) * 2 + i][c] = vxv + vyv + vzv;
+          const T vx = i ? sx[k][j] : -sx[k][j];
+          const T vy = j ? sy[k][i] : -sy[k][i];
+          const T vz = k ? Rz[j][i] : -Rz[j][i];
+          r[(k * 2 + j) * 2 + i][c] = vx + vy + vz;
         }
   }
 }
@@ -171,8 +179,8 @@ struct UApplyOp {
   BcMask bc;
   T E, lam1, mu1, eta, vol, ihx, ihy, ihz;
   __device__ T load(int f, int64_t n) const { return f < 3 ? x[3 * n + f] : v[n]; }
-  __device__ void element(int ex, int ey, int ek, const T (&xin)[2][4][4], T (&r)[8][3]) const {
-    T Ee = E_elem ? T(E_elem[cell_id(G, ex, ey, ek)]) : E;
+  __device__ T aux(int ex, int ey, int ek) const { return E_elem ? T(E_elem[cell_id(G, ex, ey, ek)]) : E; }
+  __device__ void element(int, int, int, const T (&xin)[2][4][4], T Ee, T (&r)[8][3]) const {
     u_elem_apply<T>(xin, Ee * vol, lam1, mu1, eta, ihx, ihy, ihz, r);
   }
   __device__ void store(int i, int j, int k, const T (&acc)[3], const T (&own)[4]) const {
@@ -183,8 +191,9 @@ struct UApplyOp {
   }
 };
 
+// FP32: at most 256 registers, two waves per SIMD
 template <typename T, int TX, int TY>
-__global__ __launch_bounds__(TX *TY) void k_u_apply(UApplyOp<T> op, int OZ) {
+__global__ __launch_bounds__(TX *TY, sizeof(T) == 4 ? 2 : 1) void k_u_apply(UApplyOp<T> op, int OZ) {
   march<T, TX, TY>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
@@ -418,7 +427,7 @@ struct BApply {
 // the device, so a captured graph follows a list that changes between replays;
 // flags (optional) skip items of members whose flags[p * fstride] is 0.
 template <typename T>
-__global__ __launch_bounds__(256) void k_batch_u_apply(const BApply<T> *__restrict__ probs,
+__global__ __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) void k_batch_u_apply(const BApply<T> *__restrict__ probs,
                                                        const BItem *__restrict__ items, int nitems,
                                                        const int *__restrict__ dcount,
                                                        const double *__restrict__ flags, int fstride) {

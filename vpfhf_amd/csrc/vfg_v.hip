@@ -339,7 +339,7 @@ __global__ __launch_bounds__(256) void k_batch_v_apply(const BVApply *__restrict
   for (int it = blockIdx.x; it < n; it += gridDim.x) {
     const BItem w = items[it];
     if (flags[(int64_t)w.p * fstride] == 0.) continue;  // uniform per block
-    const VApplyOp op = probs[w.p].op;
+    const VApplyOp &op = probs[w.p].op;  // read in place: the record is large
     const int OZ = probs[w.p].OZ;
     __syncthreads();  // LDS of the previous item is free
     march<double, 16, 16>(op.G, op, OZ, w.bx, w.by, w.bz);
