@@ -226,6 +226,7 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--calib", action="store_true", help="also run a plain streaming kernel of known size")
     ap.add_argument("--no-forms", action="store_true", help="skip the ensemble, precision and serving forms")
+    ap.add_argument("--forms", default="ensemble,precision,serving", help="comma-separated subset of the forms")
     ap.add_argument("--serve-requests", type=int, default=16)
     a = ap.parse_args()
     if a.gpus != 1:
@@ -308,6 +309,8 @@ def main():
     if not (a.no_solves or a.no_forms):
         for key, fn in (("ensemble", lambda: ensemble_bench(L)), ("precision", precision_bench),
                         ("serving", lambda: serving_bench(n_req=a.serve_requests))):
+            if key not in a.forms.split(","):
+                continue
             t0 = time.time()
             out[key] = fn()
             print("bench: %s form done in %.1f s" % (key, time.time() - t0), file=sys.stderr, flush=True)

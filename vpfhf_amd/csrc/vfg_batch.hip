@@ -70,10 +70,12 @@ __global__ __launch_bounds__(BRB) void k_binit(const BVec *vt, int o, const BChu
   if (threadIdx.x == 0) { part[2 * blockIdx.x] = rz; part[2 * blockIdx.x + 1] = rr; }
 }
 
+// sc (optional): per-member scalars; chunks of frozen members are skipped
 template <typename T>
 __global__ __launch_bounds__(BRB) void k_bdot(const BVec *vt, int sa, int sb, const BChunk *__restrict__ ch,
-                                              double *part) {
+                                              const double *sc, double *part) {
   const BChunk c = ch[blockIdx.x];
+  if (sc && sc[(int64_t)c.p * B_NS + B_ACTIVE] == 0.) return;  // frozen member
   const T *a = vec<T>(vt, c, sa), *b = vec<T>(vt, c, sb);
   double s = 0.;
   for (int i = threadIdx.x; i < c.len; i += BRB) s += (double)a[i] * (double)b[i];
@@ -127,6 +129,7 @@ template <typename T>
 __global__ __launch_bounds__(BRB) void k_bxr(const BVec *vt, int o, const BChunk *__restrict__ ch, const double *sc,
                                              double *part) {
   const BChunk c = ch[blockIdx.x];
+  if (sc[(int64_t)c.p * B_NS + B_ACTIVE] == 0.) return;  // frozen member
   T *x = vec<T>(vt, c, o + VX), *r = vec<T>(vt, c, o + VR), *z = vec<T>(vt, c, o + VZ);
   const T *p = vec<T>(vt, c, o + VP), *q = vec<T>(vt, c, o + VQ), *dinv = vec<T>(vt, c, o + VD);
   const double al = sc[(int64_t)c.p * B_NS + B_ALPHA];
@@ -148,6 +151,7 @@ __global__ __launch_bounds__(BRB) void k_bxr(const BVec *vt, int o, const BChunk
 template <typename T>
 __global__ __launch_bounds__(BRB) void k_bp(const BVec *vt, int o, const BChunk *__restrict__ ch, const double *sc) {
   const BChunk c = ch[blockIdx.x];
+  if (sc[(int64_t)c.p * B_NS + B_ACTIVE] == 0.) return;  // frozen member
   T *p = vec<T>(vt, c, o + VP);
   const T *z = vec<T>(vt, c, o + VZ);
   const double be = sc[(int64_t)c.p * B_NS + B_BETA];
@@ -218,6 +222,7 @@ struct vfg_batch {
   std::vector<int> oz;
   BItem *d_items = nullptr;
   int nitems = 0;
+  int *d_nitems = nullptr;  // nitems on the device (the flagged persistent apply reads it)
   BChunk *d_ch = nullptr;
   int nch = 0;
   BVec *d_vt = nullptr;  // per-member vector table (member work buffers)
@@ -284,6 +289,9 @@ static int build_items(vfg_batch *b, int slots, double oz_scale) {
   int rc = balloc(b, (void **)&b->d_items, sizeof(BItem) * b->items.size());
   if (rc) return rc;
   BCHK(b, hipMemcpy(b->d_items, b->items.data(), sizeof(BItem) * b->items.size(), hipMemcpyHostToDevice));
+  if (!b->d_nitems && (rc = balloc(b, (void **)&b->d_nitems, sizeof(int)))) return rc;
+  BCHK(b, hipMemcpy(b->d_nitems, &b->nitems, sizeof(int), hipMemcpyHostToDevice));
+  drop_graphs(b);  // captured graphs hold the old item list
   return VFG_OK;
 }
 
@@ -448,8 +456,11 @@ template <typename T>
 static void cg_iter(vfg_batch *b, int o, void *ops, int fp32, double rtol, int maxit) {
   hipStream_t s = b->stream;
   const BChunk *ch = b->d_ch;
-  launch_batch_apply(fp32, ops, b->d_items, b->nitems, b->grid, s);
-  hipLaunchKernelGGL(k_bdot<T>, dim3(b->nch), dim3(BRB), 0, s, (const BVec *)b->d_vt, o + VP, o + VQ, ch, b->d_part);
+  // frozen members are skipped by the apply and the vector kernels
+  const int g = b->grid > 0 && b->grid < b->nitems ? b->grid : b->nitems;
+  launch_engine_u_apply(fp32, ops, b->d_items, b->d_nitems, g, b->d_sc + B_ACTIVE, B_NS, s);
+  hipLaunchKernelGGL(k_bdot<T>, dim3(b->nch), dim3(BRB), 0, s, (const BVec *)b->d_vt, o + VP, o + VQ, ch,
+                     (const double *)b->d_sc, b->d_part);
   hipLaunchKernelGGL(k_bscalar, dim3(b->np), dim3(BRB), 0, s, (const double *)b->d_part, (const int *)b->d_first,
                      (const int *)b->d_nch, 1, b->d_sc, rtol, maxit);
   hipLaunchKernelGGL(k_bxr<T>, dim3(b->nch), dim3(BRB), 0, s, (const BVec *)b->d_vt, o, ch, (const double *)b->d_sc,
@@ -556,7 +567,8 @@ extern "C" int vfg_batch_u_solve(vfg_batch *b, double *const *U, const double *c
     const int max_outer = 60;
     for (int it = 0; it < max_outer && !rc; it++) {
       // per-member FP64 residual norms
-      hipLaunchKernelGGL(k_bdot<double>, gch, blk, 0, s, vt, (int)VR, (int)VR, ch, b->d_part);
+      hipLaunchKernelGGL(k_bdot<double>, gch, blk, 0, s, vt, (int)VR, (int)VR, ch, (const double *)nullptr,
+                         b->d_part);
       batch_member_sums(b->d_part, b->d_first, b->d_nch, np, b->d_scale, s);
       BCHK(b, hipMemcpyAsync(nrm.data(), b->d_scale, 8 * np, hipMemcpyDeviceToHost, s));
       BCHK(b, hipStreamSynchronize(s));
