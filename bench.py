@@ -282,6 +282,20 @@ def main():
         extra["solves"] = {"u_cg_iterations": info.u_its, "v_newton": info.v_outer, "v_cg": info.v_its,
                            "seconds_per_altmin_iteration": round(dt, 3), "solves_per_s": round(2.0 / dt, 3),
                            "surface_energy": en["surface"], "crack_volume": en["crack_volume"]}
+        # the V-Jacobian apply of the same problem (coupling set by the V-solve)
+        vx = [V.DeviceBuffer(8 * nn).upload(rng.standard_normal(nn)) for _ in range(nbuf)]
+        vy = [V.DeviceBuffer(8 * nn) for _ in range(nbuf)]
+        va = (C.c_void_p * nbuf)(*[b.ptr for b in vx])
+        vb = (C.c_void_p * nbuf)(*[b.ptr for b in vy])
+        _lib.check(L.vfg_time_v_apply(h, va, vb, nbuf, max(a.warmup, 1), C.byref(ms)), h)
+        _lib.check(L.vfg_time_v_apply(h, va, vb, nbuf, a.steps, C.byref(ms)), h)
+        tv = ms.value * 1e-3 / a.steps
+        vbytes = 16 * nn + 216 * ne
+        extra["v_apply"] = {"ms": round(tv * 1e3, 4), "algorithmic_bytes": vbytes,
+                            "achieved_GBs": round(vbytes / tv / 1e9, 1),
+                            "frac_of_hbm_peak": round(vbytes / tv / 1e9 / PEAK_HBM_GBS, 4)}
+        for b in vx + vy:
+            b.free()
 
     out = {
         "metric": "GDoF/s elasticity apply; solves/s and p99 latency at 1e-6 rel err vs FP64",

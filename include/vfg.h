@@ -276,6 +276,9 @@ int vfg_memcpy(void *dst, const void *src, size_t bytes, int kind);
  * on the handle's stream between two HIP events; returns total ms. */
 int vfg_time_u_apply(vfg_handle *h, const double *const *x, double *const *y, int nbuf,
                      int nrep, int fp32, double *ms);
+/* The same for the masked V-Jacobian apply (needs vfg_v_set_u first). */
+int vfg_time_v_apply(vfg_handle *h, const double *const *x, double *const *y, int nbuf,
+                     int nrep, double *ms);
 
 #ifdef __cplusplus
 }

@@ -127,6 +127,7 @@ PROTOS = {
     "vfg_device_free": (C.c_int, [vp]),
     "vfg_memcpy": (C.c_int, [vp, vp, C.c_size_t, C.c_int]),
     "vfg_time_u_apply": (C.c_int, [H, C.POINTER(vp), C.POINTER(vp), C.c_int, C.c_int, C.c_int, dp]),
+    "vfg_time_v_apply": (C.c_int, [H, C.POINTER(vp), C.POINTER(vp), C.c_int, C.c_int, dp]),
 }
 
 

@@ -559,4 +559,24 @@ int vfg_time_u_apply(vfg_handle *h, const double *const *x, double *const *y, in
   return VFG_OK;
 }
 
+int vfg_time_v_apply(vfg_handle *h, const double *const *x, double *const *y, int nbuf, int nrep, double *ms) {
+  NEED_H(h);
+  if (!x || !y || nbuf < 1 || nrep < 1 || !ms) return vfg_fail(h, VFG_E_ARG, "vfg_time_v_apply: bad arguments");
+  if (!h->v_ready) return vfg_fail(h, VFG_E_ARG, "vfg_time_v_apply: V operator used before vfg_v_set_u");
+  hipEvent_t e0, e1;
+  HIPCHK(h, hipEventCreate(&e0));
+  HIPCHK(h, hipEventCreate(&e1));
+  HIPCHK(h, hipEventRecord(e0, h->stream));
+  for (int i = 0; i < nrep; i++) launch_v_apply(h->P, x[i % nbuf], y[i % nbuf], 0, h->stream);
+  HIPCHK(h, hipEventRecord(e1, h->stream));
+  HIPCHK(h, hipEventSynchronize(e1));
+  float t = 0.f;
+  HIPCHK(h, hipEventElapsedTime(&t, e0, e1));
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  *ms = t;
+  HIPCHK(h, hipGetLastError());
+  return VFG_OK;
+}
+
 }  // extern "C"

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <type_traits>
+#include <utility>
 
 namespace vfg {
 
@@ -68,13 +69,22 @@ __device__ __forceinline__ bool on_dirichlet(const Grid &G, uint8_t face, uint8_
 //   __device__ T load(int f, int64_t node)  -- value of staged field f at a node
 //   __device__ void element(int ex,int ey,int ek, const T (&xin)[2][4][NF], T (&r)[8][NC])
 //   __device__ void store(int i,int j,int k, const T (&acc)[NC], const T (&own)[NF])
-// Optional per-element coefficient read one layer ahead (hides its latency):
-//   __device__ T aux(int ex,int ey,int ek); element(ex,ey,ek, xin, aux, r)
+// Optional per-element coefficients read one layer ahead (hides their latency):
+//   __device__ A aux(int ex,int ey,int ek); element(ex,ey,ek, xin, aux, r)
+// with A any trivially copyable type (a value, or a struct of values).
 // Corners of an element are a = (k*2+j)*2+i; xin[k][j*2+i][f].
 template <class Op, class = void>
 struct has_aux : std::false_type {};
 template <class Op>
 struct has_aux<Op, std::void_t<decltype(&Op::aux)>> : std::true_type {};
+template <class Op, class = void>
+struct aux_of {
+  using type = int;
+};
+template <class Op>
+struct aux_of<Op, std::void_t<decltype(&Op::aux)>> {
+  using type = decltype(std::declval<const Op &>().aux(0, 0, 0));
+};
 
 template <typename T, int TX, int TY, class Op>
 __device__ __forceinline__ void march(const Grid &G, const Op &op, int OZ, int bx, int by, int bz) {
@@ -124,9 +134,12 @@ __device__ __forceinline__ void march(const Grid &G, const Op &op, int OZ, int b
         for (int f = 0; f < NF; f++) s_in[f][n] = pf[q][f];
     }
   };
-  auto aux_at = [&](int ek) -> T {
-    if constexpr (AUX) return (exy_ok && ek >= 0 && ek < G.nez) ? op.aux(ex, ey, ek) : T(0);
-    else return T(0);
+  using AuxT = typename aux_of<Op>::type;
+  auto aux_at = [&](int ek) -> AuxT {
+    AuxT a{};
+    if constexpr (AUX)
+      if (exy_ok && ek >= 0 && ek < G.nez) a = op.aux(ex, ey, ek);
+    return a;
   };
   auto read_plane = [&](T (&xp)[4][NF]) {
 #pragma unroll
@@ -139,8 +152,8 @@ __device__ __forceinline__ void march(const Grid &G, const Op &op, int OZ, int b
   for (int c = 0; c < NC; c++) carry[c] = T(0);
 
   // one element layer: bottom plane xb (node plane ek), top plane xt (ek + 1)
-  auto layer = [&](int ek, const T (&xb)[4][NF], T (&xt)[4][NF], T &aux_next) {
-    const T aux_cur = aux_next;
+  auto layer = [&](int ek, const T (&xb)[4][NF], T (&xt)[4][NF], AuxT &aux_next) {
+    const AuxT aux_cur = aux_next;
     __syncthreads();
     commit();       // plane ek + 1
     fetch(ek + 2);  // in flight during this layer
@@ -198,7 +211,7 @@ __device__ __forceinline__ void march(const Grid &G, const Op &op, int OZ, int b
   fetch(oz0 - 1);
   commit();
   fetch(oz0);
-  T aux_next = aux_at(oz0 - 1);
+  AuxT aux_next = aux_at(oz0 - 1);
   __syncthreads();
   read_plane(xa);
   const int ek_end = min(oz0 + OZ - 1, G.nz - 1);

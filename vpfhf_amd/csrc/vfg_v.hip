@@ -104,9 +104,19 @@ __global__ void k_v_fixmask(DevProblem P) {
 }
 
 // ------------------------------------------------------------------- apply
+// the 27 coupling weights of one element, read one layer ahead by the march
+struct VQ {
+  double q[27];
+};
+__device__ __forceinline__ VQ load_vq(const double *__restrict__ Vq, int64_t e, int64_t ne) {
+  VQ a;
+#pragma unroll
+  for (int g = 0; g < 27; g++) a.q[g] = Vq[(int64_t)g * ne + e];
+  return a;
+}
+
 // y_a = ks sum_b L[a][b] x_b + sum_g q_g phi_a(g) x(g)
-__device__ __forceinline__ void v_elem_apply(const double (&x)[8], const double *__restrict__ q,
-                                             int64_t e, int64_t ne, const VConst &C,
+__device__ __forceinline__ void v_elem_apply(const double (&x)[8], const VQ &Q, const VConst &C,
                                              double (&r)[8]) {
 #pragma unroll
   for (int a = 0; a < 8; a++) {
@@ -139,7 +149,7 @@ __device__ __forceinline__ void v_elem_apply(const double (&x)[8], const double 
         double a0 = interp<double>(gy, ax[0][0][gx], ax[0][1][gx]);
         double a1 = interp<double>(gy, ax[1][0][gx], ax[1][1][gx]);
         double xv = interp<double>(gz, a0, a1);
-        double f = q[(int64_t)((gz * 3 + gy) * 3 + gx) * ne + e] * xv;
+        double f = Q.q[(gz * 3 + gy) * 3 + gx] * xv;
         double f0 = gP0(gz) * f, f1 = gP1(gz) * f;
         bz[0][0][gx] += gP0(gy) * f0; bz[0][1][gx] += gP1(gy) * f0;
         bz[1][0][gx] += gP0(gy) * f1; bz[1][1][gx] += gP1(gy) * f1;
@@ -172,11 +182,13 @@ struct VApplyOp {
   double irrevtol;
   __device__ double load(int, int64_t n) const { return (mode == 0 && vfix[n]) ? 0. : x[n]; }
   // mode 0: masked Jacobian; 1: VF_VResidual; 2: plain K x
+  // the coupling weights are read inside the layer: prefetching them a layer
+  // ahead costs a wave of occupancy and is slower
   __device__ void element(int ex, int ey, int ek, const double (&xin)[2][4][1], double (&r)[8][1]) const {
     double xe[8], re[8];
 #pragma unroll
     for (int a = 0; a < 8; a++) xe[a] = xin[a >> 2][a & 3][0];
-    v_elem_apply(xe, q, cell_id(G, ex, ey, ek), ne, C, re);
+    v_elem_apply(xe, load_vq(q, cell_id(G, ex, ey, ek), ne), C, re);
 #pragma unroll
     for (int a = 0; a < 8; a++) r[a][0] = re[a];
   }
@@ -210,7 +222,7 @@ struct VApplyOp {
   }
 };
 
-__global__ __launch_bounds__(256) void k_v_apply(VApplyOp op, int OZ) { march<double, 16, 16>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z); }
+__global__ __launch_bounds__(256, 3) void k_v_apply(VApplyOp op, int OZ) { march<double, 16, 16>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z); }
 
 // ---------------------------------------------------------------- diagonal
 struct VDiagOp {
@@ -222,17 +234,46 @@ struct VDiagOp {
   int64_t ne;
   VConst C;
   __device__ double load(int, int64_t) const { return 0.; }
-  __device__ void element(int ex, int ey, int ek, const double (&)[2][4][1], double (&r)[8][1]) const {
-    int64_t e = cell_id(G, ex, ey, ek);
-    for (int a = 0; a < 8; a++) {
-      int i = a & 1, j = (a >> 1) & 1, k = a >> 2;
-      double s = C.ks * C.L[a * 9];
-      for (int g = 0; g < 27; g++) {
-        int gx = g % 3, gy = (g / 3) % 3, gz = g / 9;
-        double ph = (i ? gP1(gx) : gP0(gx)) * (j ? gP1(gy) : gP0(gy)) * (k ? gP1(gz) : gP0(gz));
-        s += q[(int64_t)g * ne + e] * ph * ph;
+  __device__ VQ aux(int ex, int ey, int ek) const { return load_vq(q, cell_id(G, ex, ey, ek), ne); }
+  // d_a = ks L[a][a] + sum_g q_g phi_a(g)^2, phi_a^2 = P_i(gx)^2 P_j(gy)^2 P_k(gz)^2:
+  // contracted direction by direction (x, then y, then z)
+  __device__ void element(int, int, int, const double (&)[2][4][1], const VQ &Q, double (&r)[8][1]) const {
+    double bx[3][3][2];  // [gz][gy][i]
+#pragma unroll
+    for (int gz = 0; gz < 3; gz++)
+#pragma unroll
+      for (int gy = 0; gy < 3; gy++) {
+        double s0 = 0., s1 = 0.;
+#pragma unroll
+        for (int gx = 0; gx < 3; gx++) {
+          const double qv = Q.q[(gz * 3 + gy) * 3 + gx];
+          s0 += gP0(gx) * gP0(gx) * qv;
+          s1 += gP1(gx) * gP1(gx) * qv;
+        }
+        bx[gz][gy][0] = s0;
+        bx[gz][gy][1] = s1;
       }
-      r[a][0] = s;
+    double by[3][2][2];  // [gz][j][i]
+#pragma unroll
+    for (int gz = 0; gz < 3; gz++)
+#pragma unroll
+      for (int i = 0; i < 2; i++) {
+        double s0 = 0., s1 = 0.;
+#pragma unroll
+        for (int gy = 0; gy < 3; gy++) {
+          s0 += gP0(gy) * gP0(gy) * bx[gz][gy][i];
+          s1 += gP1(gy) * gP1(gy) * bx[gz][gy][i];
+        }
+        by[gz][0][i] = s0;
+        by[gz][1][i] = s1;
+      }
+#pragma unroll
+    for (int a = 0; a < 8; a++) {
+      const int i = a & 1, j = (a >> 1) & 1, k = a >> 2;
+      double s = 0.;
+#pragma unroll
+      for (int gz = 0; gz < 3; gz++) s += (k ? gP1(gz) * gP1(gz) : gP0(gz) * gP0(gz)) * by[gz][j][i];
+      r[a][0] = C.ks * C.L[a * 9] + s;
     }
   }
   __device__ void store(int i, int j, int k, const double (&acc)[1], const double (&)[1]) const {
@@ -331,7 +372,7 @@ struct BVApply {
   VApplyOp op;
   int OZ;
 };
-__global__ __launch_bounds__(256) void k_batch_v_apply(const BVApply *__restrict__ probs,
+__global__ __launch_bounds__(256, 3) void k_batch_v_apply(const BVApply *__restrict__ probs,
                                                        const BItem *__restrict__ items,
                                                        const int *__restrict__ dcount,
                                                        const double *__restrict__ flags, int fstride) {
