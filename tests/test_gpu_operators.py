@@ -141,3 +141,34 @@ def test_larger_grid_apply(oracle):
     J = oracle.v_jacobian(op, F, s["U"])
     assert relerr(P.VIJacobian_mult(s["xv"]), J @ s["xv"]) < 1e-12
     P.close()
+
+
+EDGE_GRIDS = [(2, 2, 2), (3, 2, 2), (2, 40, 3), (64, 2, 2), (16, 16, 16), (31, 31, 5), (46, 17, 4), (9, 9, 97)]
+
+
+@pytest.mark.parametrize("n", EDGE_GRIDS, ids=["x".join(map(str, g)) for g in EDGE_GRIDS])
+def test_apply_edge_grids(oracle, n):
+    """One element, slabs one element thick, exact multiples of the 15-node
+    tile, and a tall grid crossing many z-chunks: u- and V-apply against the
+    assembled oracle operators (FP64 1e-12, FP32 1e-5), device pointers too."""
+    import vpfhf_amd as V
+    s = S.make(n=n, l=(1.0, 0.7, 1.3), seed=sum(n), bcU=S.MIXED_U, bcV=S.MIXED_V, atnum=2)
+    op, F = S.to_oracle(oracle, s)
+    P = S.to_gpu(s)
+    A = oracle.u_jacobian(op, F)
+    ref = A @ s["x"]
+    assert relerr(P.UIJacobian_mult(s["x"]), ref) < 1e-12
+    assert relerr(P.UIJacobian_mult(s["x"], fp32=True), ref) < 1e-5
+    L = V.load()
+    dx = V.DeviceBuffer(s["x"].nbytes).upload(s["x"])
+    dy = V.DeviceBuffer(s["x"].nbytes)
+    V.check(L.vfg_u_apply(P.h, dx.ptr, dy.ptr, V.DEVICE), P.h)
+    assert relerr(dy.download(), ref) < 1e-12
+    dx.free()
+    dy.free()
+    P.VIJacobian_setup(s["U"])
+    J = oracle.v_jacobian(op, F, s["U"])
+    assert relerr(P.VIJacobian_mult(s["xv"]), J @ s["xv"]) < 1e-12
+    assert relerr(P.v_diag(), J.diagonal()) < 1e-12
+    assert relerr(P.u_diag(), A.diagonal()) < 1e-12
+    P.close()

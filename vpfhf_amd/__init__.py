@@ -156,13 +156,14 @@ class Problem:
 
     def VF_UResidual(self, U, BCU):
         F = np.empty(3 * self.nn)
-        self._c(self.L.vfg_u_residual(self.h, _ptr(np.ascontiguousarray(U, np.float64)),
-                                      _ptr(np.ascontiguousarray(BCU, np.float64)), _ptr(F), HOST))
+        U, BCU = np.ascontiguousarray(U, np.float64), np.ascontiguousarray(BCU, np.float64)  # alive for the call
+        self._c(self.L.vfg_u_residual(self.h, _ptr(U), _ptr(BCU), _ptr(F), HOST))
         return F
 
     # -- V operator ----------------------------------------------------------
     def VIJacobian_setup(self, U):
-        self._c(self.L.vfg_v_set_u(self.h, _ptr(np.ascontiguousarray(U, np.float64)), HOST))
+        U = np.ascontiguousarray(U, np.float64)
+        self._c(self.L.vfg_v_set_u(self.h, _ptr(U), HOST))
 
     def VIJacobian_mult(self, x):
         x = np.ascontiguousarray(x, np.float64)
@@ -182,7 +183,8 @@ class Problem:
 
     def VF_VResidual(self, V):
         F = np.empty(self.nn)
-        self._c(self.L.vfg_v_residual(self.h, _ptr(np.ascontiguousarray(V, np.float64)), _ptr(F), HOST))
+        V = np.ascontiguousarray(V, np.float64)
+        self._c(self.L.vfg_v_residual(self.h, _ptr(V), _ptr(F), HOST))
         return F
 
     # -- BC kit --------------------------------------------------------------
@@ -203,23 +205,23 @@ class Problem:
         (VecApplyDirichletBC(V, V, bcV) of VFTimeStepPrepare)."""
         w = 0 if which == "U" else 1
         vec = np.ascontiguousarray(vec, np.float64)
-        Bp = _ptr(vec) if B is None else _ptr(np.ascontiguousarray(B, np.float64))
-        self._c(self.L.vfg_vec_apply_dirichlet(self.h, w, _ptr(vec), Bp, HOST))
+        B = vec if B is None else np.ascontiguousarray(B, np.float64)  # alive for the call
+        self._c(self.L.vfg_vec_apply_dirichlet(self.h, w, _ptr(vec), _ptr(B), HOST))
         return vec
 
     # -- diagnostics ---------------------------------------------------------
     def energies(self, U, V):
         """(elastic, insitu work, pressure work, surface) -- VF_UEnergy3D + VF_VEnergy3D."""
         out = (C.c_double * 4)()
-        self._c(self.L.vfg_energies(self.h, _ptr(np.ascontiguousarray(U, np.float64)),
-                                    _ptr(np.ascontiguousarray(V, np.float64)), out, HOST))
+        U, V = np.ascontiguousarray(U, np.float64), np.ascontiguousarray(V, np.float64)
+        self._c(self.L.vfg_energies(self.h, _ptr(U), _ptr(V), out, HOST))
         return tuple(out)
 
     def VolumetricCrackOpening(self, U, V, cellfield=False):
         vol = C.c_double()
         cf = np.empty(self.ne) if cellfield else None
-        self._c(self.L.vfg_crack_volume(self.h, _ptr(np.ascontiguousarray(U, np.float64)),
-                                        _ptr(np.ascontiguousarray(V, np.float64)), C.byref(vol), _ptr(cf), HOST))
+        U, V = np.ascontiguousarray(U, np.float64), np.ascontiguousarray(V, np.float64)
+        self._c(self.L.vfg_crack_volume(self.h, _ptr(U), _ptr(V), C.byref(vol), _ptr(cf), HOST))
         return (vol.value, cf) if cellfield else vol.value
 
     def penny_v(self, center, r, theta, phi, thickness):
@@ -238,8 +240,8 @@ class Problem:
         U = np.array(U, np.float64, copy=True)
         info = _lib.SolveInfo()
         o = self.opts(**kw)
-        self._c(self.L.vfg_u_solve(self.h, _ptr(U), _ptr(np.ascontiguousarray(BCU, np.float64)),
-                                   C.byref(o), C.byref(info), HOST))
+        BCU = np.ascontiguousarray(BCU, np.float64)
+        self._c(self.L.vfg_u_solve(self.h, _ptr(U), _ptr(BCU), C.byref(o), C.byref(info), HOST))
         return U, info
 
     def VF_StepV(self, U, V, ub=None, **kw):
@@ -248,6 +250,6 @@ class Problem:
         info = _lib.SolveInfo()
         o = self.opts(**kw)
         ubp = None if ub is None else np.ascontiguousarray(ub, np.float64)
-        self._c(self.L.vfg_v_solve(self.h, _ptr(np.ascontiguousarray(U, np.float64)), _ptr(V), _ptr(ubp),
-                                   C.byref(o), C.byref(info), HOST))
+        U = np.ascontiguousarray(U, np.float64)
+        self._c(self.L.vfg_v_solve(self.h, _ptr(U), _ptr(V), _ptr(ubp), C.byref(o), C.byref(info), HOST))
         return V, info

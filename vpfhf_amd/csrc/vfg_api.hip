@@ -331,7 +331,7 @@ int vfg_v_set_u(vfg_handle *h, const double *U, int loc) {
   if (!U) return vfg_fail(h, VFG_E_ARG, "vfg_v_set_u: NULL U");
   DevProblem &P = h->P;
   int rc;
-  if ((rc = vfg_alloc(h, (void **)&P.Vq, 8 * 27 * P.ne))) return rc;
+  if ((rc = vfg_alloc(h, (void **)&P.Vq, 8 * (size_t)vq_doubles(P.G)))) return rc;
   IO io(h, loc, U, 24 * P.nn, nullptr, 0);
   if (io.rc) return io.rc;
   launch_v_setup(P, (const double *)io.din, h->stream);

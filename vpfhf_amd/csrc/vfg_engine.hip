@@ -480,7 +480,7 @@ static int advance(vfg_engine *e, int p) {
       }
       case ST_V_BEGIN: {  // VF_StepV set-up (vfg_v_solve): coupling from U, Dirichlet V, projection
         int rc;
-        if ((rc = vfg_alloc(h, (void **)&P.Vq, 8 * 27 * (size_t)P.ne)) || (rc = vfg_alloc(h, (void **)&h->act, nn)))
+        if ((rc = vfg_alloc(h, (void **)&P.Vq, 8 * (size_t)vq_doubles(P.G))) || (rc = vfg_alloc(h, (void **)&h->act, nn)))
           return efail(e, rc, h->err);
         if (!q.vop_made) {
           make_vop(P, h->w[QP], h->w[QQ], h->act, &e->h_vop[(size_t)p * e->vop_sz]);

@@ -77,6 +77,20 @@ template <class Op, class = void>
 struct has_aux : std::false_type {};
 template <class Op>
 struct has_aux<Op, std::void_t<decltype(&Op::aux)>> : std::true_type {};
+// Ops that keep per-element data in a tile-blocked layout (one 16x16 slot
+// block per workgroup tile and element layer; halo elements stored in every
+// tile that computes them) declare TILED and take (tile, slot, ek) instead of
+// (ex, ey, ek): a wavefront then reads each coefficient as one aligned,
+// contiguous 512-byte segment.
+template <class Op, class = void>
+struct is_tiled : std::false_type {};
+template <class Op>
+struct is_tiled<Op, std::void_t<decltype(Op::TILED)>> : std::true_type {};
+// tile-blocked layout: [tile][element layer][coefficient][TX*TY slots]
+__host__ __device__ inline int64_t tiled_index(int tile, int ek, int nez, int ncoef, int slot, int coef, int nslot) {
+  return (((int64_t)tile * nez + ek) * ncoef + coef) * nslot + slot;
+}
+
 template <class Op, class = void>
 struct aux_of {
   using type = int;
@@ -92,6 +106,7 @@ __device__ __forceinline__ void march(const Grid &G, const Op &op, int OZ, int b
   constexpr int NT = TX * TY, OX = TX - 1, OY = TY - 1, SX = TX + 1, SY = TY + 1;
   constexpr int NPF = (SX * SY + NT - 1) / NT;  // staged nodes per thread
   constexpr bool AUX = has_aux<Op>::value;
+  constexpr bool TILED = is_tiled<Op>::value;
   __shared__ T s_in[NF][SY * SX];
   __shared__ T s_red[2 * NC][NT];
   const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
@@ -100,6 +115,7 @@ __device__ __forceinline__ void march(const Grid &G, const Op &op, int OZ, int b
   const bool exy_ok = ex >= 0 && ex < G.nex && ey >= 0 && ey < G.ney;
   const bool owner = tx < OX && ty < OY && (ox0 + tx) < G.nx && (oy0 + ty) < G.ny;
   const int64_t plane = (int64_t)G.nx * G.ny;
+  const int tile = bx + ((G.nx + OX - 1) / OX) * by;  // workgroup tile, for tiled ops
 
   // staged nodes of this thread: in-plane offsets, fixed for the whole march
   int64_t soff[NPF];
@@ -137,8 +153,12 @@ __device__ __forceinline__ void march(const Grid &G, const Op &op, int OZ, int b
   using AuxT = typename aux_of<Op>::type;
   auto aux_at = [&](int ek) -> AuxT {
     AuxT a{};
-    if constexpr (AUX)
-      if (exy_ok && ek >= 0 && ek < G.nez) a = op.aux(ex, ey, ek);
+    if constexpr (AUX) {
+      if (exy_ok && ek >= 0 && ek < G.nez) {
+        if constexpr (TILED) a = op.aux(tile, tid, ek);
+        else a = op.aux(ex, ey, ek);
+      }
+    }
     return a;
   };
   auto read_plane = [&](T (&xp)[4][NF]) {
@@ -174,8 +194,9 @@ __device__ __forceinline__ void march(const Grid &G, const Op &op, int OZ, int b
           xin[0][q][f] = xb[q][f];
           xin[1][q][f] = xt[q][f];
         }
-      if constexpr (AUX) op.element(ex, ey, ek, xin, aux_cur, r);
-      else op.element(ex, ey, ek, xin, r);
+      const int e0 = TILED ? tile : ex, e1 = TILED ? tid : ey;
+      if constexpr (AUX) op.element(e0, e1, ek, xin, aux_cur, r);
+      else op.element(e0, e1, ek, xin, r);
     }
     // x-combine: node (ex+1) gets corner i=1 of this element and corner i=0
     // of the element to the right (next lane inside the 16-lane row).

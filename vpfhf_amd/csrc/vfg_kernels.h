@@ -27,7 +27,7 @@ struct DevProblem {
   float *V32;                             // FP32 copy of V for the FP32 apply
   double *zero3;                          // 3*nn zeros
   // V-subproblem state, refreshed by vfg_v_set_u
-  double *Vq;      // [27][ne] W_g * (c_AT/eps [AT2] + 2 W^el_g)
+  double *Vq;      // W_g * (c_AT/eps [AT2] + 2 W^el_g), tile-blocked (vq_doubles)
   double *bV;      // V load vector (nn)
   uint8_t *vfix;   // fixed V rows/cols (Dirichlet + AT2 irreversibility)
 };
@@ -42,6 +42,8 @@ void launch_u_rhs(const DevProblem &P, const double *U, const double *BCU, doubl
 
 // V operator (vfg_v.hip)
 void launch_v_setup(const DevProblem &P, const double *U, hipStream_t s);  // Vq, bV
+// Vq size: tile-blocked [tile][element layer][27][256] (vfg_v.hip)
+int64_t vq_doubles(const Grid &G);
 void launch_v_fixmask(const DevProblem &P, hipStream_t s);                 // vfix
 // mode 0: masked Jacobian apply; mode 1: residual K x - bV with BC rows
 // mode 2: plain K x (no masks, no BC).  `mask` overrides vfix (reduced systems).

@@ -579,7 +579,7 @@ int vfg_v_solve(vfg_handle *h, const double *U, double *V, const double *ub, con
     if ((rc = vfg_in(h, ubd, ub, 8 * n, loc))) return rc;
   }
   // VF_VIJacobian refresh: coupling from U; Dirichlet values on V
-  if ((rc = vfg_alloc(h, (void **)&P.Vq, 8 * 27 * (size_t)P.ne))) return rc;
+  if ((rc = vfg_alloc(h, (void **)&P.Vq, 8 * (size_t)vq_doubles(P.G)))) return rc;
   launch_v_setup(P, Ud, s);
   h->v_ready = true;
   launch_v_fixmask(P, s);
