@@ -70,6 +70,7 @@ __global__ __launch_bounds__(256) void k_v_setup(DevProblem P, const double *__r
   const double ih[3] = {1. / G.hx, 1. / G.hy, 1. / G.hz};
   const double vol = G.hx * G.hy * G.hz;
   const double cm = P.atnum == 2 ? P.Gc / P.atcv * .5 / P.eps : 0.;
+  double q[27];
   for (int g = 0; g < 27; g++) {
     int gx = g % 3, gy = (g / 3) % 3, gz = g / 9;
     double H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, t = 0.;
@@ -89,8 +90,31 @@ __global__ __launch_bounds__(256) void k_v_setup(DevProblem P, const double *__r
     double s33 = lam * e11 + lam * e22 + (lam + 2. * mu) * e33;
     double W = (s11 * e11 + s22 * e22 + s33 * e33) * .5 + 2. * mu * (e12 * e12 + e23 * e23 + e13 * e13);
     double Wg = vol * gW(gx) * gW(gy) * gW(gz);
-    P.Vq[(int64_t)g * P.ne + e] = Wg * (cm + 2. * W);
+    q[g] = Wg * (cm + 2. * W);
   }
+  // every workgroup tile whose element block contains this element: tile b
+  // covers elements 15 b - 1 ... 15 b + 14 in each of x and y
+  const int tiles_x = (G.nx + 14) / 15, tiles_y = (G.ny + 14) / 15;
+  int bxs[2], lxs[2], nbx = 0, bys[2], lys[2], nby = 0;
+  {
+    const int b = (ei + 1) / 15, l = (ei + 1) % 15;
+    if (b < tiles_x) { bxs[nbx] = b; lxs[nbx++] = l; }
+    if (l == 0 && b >= 1) { bxs[nbx] = b - 1; lxs[nbx++] = 15; }
+  }
+  {
+    const int b = (ej + 1) / 15, l = (ej + 1) % 15;
+    if (b < tiles_y) { bys[nby] = b; lys[nby++] = l; }
+    if (l == 0 && b >= 1) { bys[nby] = b - 1; lys[nby++] = 15; }
+  }
+  for (int a = 0; a < nbx; a++)
+    for (int c = 0; c < nby; c++) {
+      const int tile = bxs[a] + tiles_x * bys[c], slot = lys[c] * 16 + lxs[a];
+      for (int g = 0; g < 27; g++) P.Vq[tiled_index(tile, ek, G.nez, 27, slot, g, 256)] = q[g];
+    }
+}
+
+int64_t vq_doubles(const Grid &G) {
+  return (int64_t)((G.nx + 14) / 15) * ((G.ny + 14) / 15) * G.nez * 27 * 256;
 }
 
 __global__ void k_v_fixmask(DevProblem P) {
@@ -108,16 +132,22 @@ __global__ void k_v_fixmask(DevProblem P) {
 struct VQ {
   double q[27];
 };
-__device__ __forceinline__ VQ load_vq(const double *__restrict__ Vq, int64_t e, int64_t ne) {
+__device__ __forceinline__ VQ load_vq(const double *__restrict__ Vq, int tile, int slot, int ek, int nez) {
   VQ a;
+  const double *b = Vq + tiled_index(tile, ek, nez, 27, slot, 0, 256);
 #pragma unroll
-  for (int g = 0; g < 27; g++) a.q[g] = Vq[(int64_t)g * ne + e];
+  for (int g = 0; g < 27; g++) a.q[g] = b[g * 256];
   return a;
 }
 
-// y_a = ks sum_b L[a][b] x_b + sum_g q_g phi_a(g) x(g)
-__device__ __forceinline__ void v_elem_apply(const double (&x)[8], const VQ &Q, const VConst &C,
-                                             double (&r)[8]) {
+// y_a = ks sum_b L[a][b] x_b + sum_g q_g phi_a(g) x(g); the weights are read
+// one Gauss plane (9 values) at a time, the next plane's loads in flight while
+// the current one is used (keeps the register count at four waves per SIMD)
+__device__ __forceinline__ void v_elem_apply_qp(const double (&x)[8], const double *__restrict__ qb,
+                                                const VConst &C, double (&r)[8]) {
+  double qa[9], qn[9];
+#pragma unroll
+  for (int g = 0; g < 9; g++) qa[g] = qb[g * 256];
 #pragma unroll
   for (int a = 0; a < 8; a++) {
     double s = 0.;
@@ -125,7 +155,6 @@ __device__ __forceinline__ void v_elem_apply(const double (&x)[8], const VQ &Q, 
     for (int b = 0; b < 8; b++) s += C.L[a * 8 + b] * x[b];
     r[a] = C.ks * s;
   }
-  // interpolate x to the 27 points: x-direction, then y, then z
   double ax[2][2][3];  // [k][j][gx]
 #pragma unroll
   for (int k = 0; k < 2; k++)
@@ -133,7 +162,7 @@ __device__ __forceinline__ void v_elem_apply(const double (&x)[8], const VQ &Q, 
     for (int j = 0; j < 2; j++)
 #pragma unroll
       for (int gx = 0; gx < 3; gx++) ax[k][j][gx] = interp<double>(gx, x[(k * 2 + j) * 2], x[(k * 2 + j) * 2 + 1]);
-  double bz[2][2][3];  // projected back: [k][j][gx] accumulators
+  double bz[2][2][3];
 #pragma unroll
   for (int k = 0; k < 2; k++)
 #pragma unroll
@@ -141,20 +170,25 @@ __device__ __forceinline__ void v_elem_apply(const double (&x)[8], const VQ &Q, 
 #pragma unroll
       for (int gx = 0; gx < 3; gx++) bz[k][j][gx] = 0.;
 #pragma unroll
-  for (int gz = 0; gz < 3; gz++)
+  for (int gz = 0; gz < 3; gz++) {
+    if (gz < 2)
 #pragma unroll
-    for (int gy = 0; gy < 3; gy++) {
+      for (int g = 0; g < 9; g++) qn[g] = qb[((gz + 1) * 9 + g) * 256];
+#pragma unroll
+    for (int gy = 0; gy < 3; gy++)
 #pragma unroll
       for (int gx = 0; gx < 3; gx++) {
         double a0 = interp<double>(gy, ax[0][0][gx], ax[0][1][gx]);
         double a1 = interp<double>(gy, ax[1][0][gx], ax[1][1][gx]);
         double xv = interp<double>(gz, a0, a1);
-        double f = Q.q[(gz * 3 + gy) * 3 + gx] * xv;
+        double f = qa[gy * 3 + gx] * xv;
         double f0 = gP0(gz) * f, f1 = gP1(gz) * f;
         bz[0][0][gx] += gP0(gy) * f0; bz[0][1][gx] += gP1(gy) * f0;
         bz[1][0][gx] += gP0(gy) * f1; bz[1][1][gx] += gP1(gy) * f1;
       }
-    }
+#pragma unroll
+    for (int g = 0; g < 9; g++) qa[g] = qn[g];
+  }
 #pragma unroll
   for (int k = 0; k < 2; k++)
 #pragma unroll
@@ -169,6 +203,7 @@ __device__ __forceinline__ void v_elem_apply(const double (&x)[8], const VQ &Q, 
 
 struct VApplyOp {
   static constexpr int NF = 1, NC = 1;
+  static constexpr bool TILED = true;  // element(tile, slot, ek, ...): q is tile-blocked
   const double *x;
   double *y;
   const double *q, *bV, *VIrrev;
@@ -184,11 +219,11 @@ struct VApplyOp {
   // mode 0: masked Jacobian; 1: VF_VResidual; 2: plain K x
   // the coupling weights are read inside the layer: prefetching them a layer
   // ahead costs a wave of occupancy and is slower
-  __device__ void element(int ex, int ey, int ek, const double (&xin)[2][4][1], double (&r)[8][1]) const {
+  __device__ void element(int tile, int slot, int ek, const double (&xin)[2][4][1], double (&r)[8][1]) const {
     double xe[8], re[8];
 #pragma unroll
     for (int a = 0; a < 8; a++) xe[a] = xin[a >> 2][a & 3][0];
-    v_elem_apply(xe, load_vq(q, cell_id(G, ex, ey, ek), ne), C, re);
+    v_elem_apply_qp(xe, q + tiled_index(tile, ek, G.nez, 27, slot, 0, 256), C, re);
 #pragma unroll
     for (int a = 0; a < 8; a++) r[a][0] = re[a];
   }
@@ -227,6 +262,7 @@ __global__ __launch_bounds__(256, 3) void k_v_apply(VApplyOp op, int OZ) { march
 // ---------------------------------------------------------------- diagonal
 struct VDiagOp {
   static constexpr int NF = 1, NC = 1;
+  static constexpr bool TILED = true;
   double *d;
   const double *q;
   const uint8_t *vfix;
@@ -234,7 +270,7 @@ struct VDiagOp {
   int64_t ne;
   VConst C;
   __device__ double load(int, int64_t) const { return 0.; }
-  __device__ VQ aux(int ex, int ey, int ek) const { return load_vq(q, cell_id(G, ex, ey, ek), ne); }
+  __device__ VQ aux(int tile, int slot, int ek) const { return load_vq(q, tile, slot, ek, G.nez); }
   // d_a = ks L[a][a] + sum_g q_g phi_a(g)^2, phi_a^2 = P_i(gx)^2 P_j(gy)^2 P_k(gz)^2:
   // contracted direction by direction (x, then y, then z)
   __device__ void element(int, int, int, const double (&)[2][4][1], const VQ &Q, double (&r)[8][1]) const {
@@ -332,10 +368,21 @@ __global__ __launch_bounds__(256) void k_v_rhs(VRhsOp op, int OZ) { march<double
 static dim3 mgrid(const Grid &G, int OZ) {
   return dim3((G.nx + 14) / 15, (G.ny + 14) / 15, (G.nz + OZ - 1) / OZ);
 }
-int choose_oz(const Grid &G, int tiles_xy, int slots);
+// z-chunk length of the V kernels (bandwidth-bound, three workgroups per CU):
+// each chunk re-reads one halo layer, so once the chunks fill the device the
+// total layer count (oz + 1) * wg / slots is what to minimise.
 static int oz_for(const Grid &G) {
-  // the V kernels run at 2-4 workgroups per CU
-  return choose_oz(G, ((G.nx + 14) / 15) * ((G.ny + 14) / 15), 768);
+  const int tiles = ((G.nx + 14) / 15) * ((G.ny + 14) / 15);
+  const double slots = 768.;
+  int best = G.nz;
+  double best_cost = 1e300;
+  for (int oz = 2; oz <= G.nz; oz++) {
+    const int chunks = (G.nz + oz - 1) / oz;
+    const double wg = (double)tiles * chunks;
+    const double cost = (oz + 1) * (wg > slots ? wg / slots : 1.);
+    if (cost < best_cost - 1e-9) { best_cost = cost; best = oz; }
+  }
+  return best;
 }
 
 void launch_v_setup(const DevProblem &P, const double *U, hipStream_t s) {
