@@ -368,22 +368,10 @@ __global__ __launch_bounds__(256) void k_v_rhs(VRhsOp op, int OZ) { march<double
 static dim3 mgrid(const Grid &G, int OZ) {
   return dim3((G.nx + 14) / 15, (G.ny + 14) / 15, (G.nz + OZ - 1) / OZ);
 }
-// z-chunk length of the V kernels (bandwidth-bound, three workgroups per CU):
-// each chunk re-reads one halo layer, so once the chunks fill the device the
-// total layer count (oz + 1) * wg / slots is what to minimise.
-static int oz_for(const Grid &G) {
-  const int tiles = ((G.nx + 14) / 15) * ((G.ny + 14) / 15);
-  const double slots = 768.;
-  int best = G.nz;
-  double best_cost = 1e300;
-  for (int oz = 2; oz <= G.nz; oz++) {
-    const int chunks = (G.nz + oz - 1) / oz;
-    const double wg = (double)tiles * chunks;
-    const double cost = (oz + 1) * (wg > slots ? wg / slots : 1.);
-    if (cost < best_cost - 1e-9) { best_cost = cost; best = oz; }
-  }
-  return best;
-}
+int choose_oz(const Grid &G, int tiles_xy, int slots);
+// z-chunk length of the V kernels (three workgroups per CU): whole rounds of
+// workgroups, as for the u kernels; a partial last round costs a full one.
+static int oz_for(const Grid &G) { return choose_oz(G, ((G.nx + 14) / 15) * ((G.ny + 14) / 15), 768); }
 
 void launch_v_setup(const DevProblem &P, const double *U, hipStream_t s) {
   hipLaunchKernelGGL(k_v_setup, dim3((unsigned)((P.ne + 255) / 256)), dim3(256), 0, s, P, U);
