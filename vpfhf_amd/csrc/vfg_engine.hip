@@ -523,7 +523,9 @@ static int advance(vfg_engine *e, int p) {
         launch_v_diag(P, h->w[QD], s, h->act);
         hipLaunchKernelGGL(k_e_prep, dim3(enb(nn)), dim3(256), 0, s, h->w[QD], (float *)nullptr, nn);
         launch_fill(h->w[QX], 0., nn, s);
-        cg_init<double>(e, p, h->w[QR], h->w[QD], h->w[QZ], h->w[QP], nn, fmin(e->o.v_rtol, 1e-12), 20000);
+        // inexact Newton forcing term, as vfg_v_solve
+        const double eta = fmax(fmin(e->o.v_rtol, 1e-12), fmin(0.1, 0.1 * pg / q.pg0));
+        cg_init<double>(e, p, h->w[QR], h->w[QD], h->w[QZ], h->w[QP], nn, eta, 20000);
         enter_cg(e, p, K_V64);
         q.st = ST_V_CG;
         return VFG_OK;

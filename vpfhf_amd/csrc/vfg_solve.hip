@@ -585,7 +585,9 @@ int vfg_v_solve(vfg_handle *h, const double *U, double *V, const double *ub, con
   launch_v_fixmask(P, s);
   launch_vec_apply_dirichlet(P, 1, Vd, Vd, s);
   hipLaunchKernelGGL(k_v_clamp, dim3(nb_(n)), dim3(256), 0, s, Vd, (const uint8_t *)P.vfix, (const double *)ubd, n);
-  const double inner_rtol = o->rtol < 1e-12 ? o->rtol : 1e-12;
+  // inexact Newton: the reduced system is solved to eta_k = 0.1 |g_k| / |g_0|
+  // (between the final tolerance and 0.1), tight only as the iteration converges
+  const double inner_floor = o->rtol < 1e-12 ? o->rtol : 1e-12;
   double pg0 = -1.;
   int it;
   const int nb = red_blocks(n);
@@ -608,6 +610,7 @@ int vfg_v_solve(vfg_handle *h, const double *U, double *V, const double *ub, con
     hipLaunchKernelGGL(k_recip, dim3(nb_(n)), dim3(256), 0, s, (const double *)q, dinv, n);
     launch_fill(x, 0., n, s);
     ApplyFn A = [&](const double *in, double *out) { launch_v_apply(P, in, out, 0, s, act); };
+    const double inner_rtol = fmax(inner_floor, fmin(0.1, 0.1 * pg / pg0));
     int prc = pcg(h, n, A, dinv, x, r, z, p, q, inner_rtol, 0., 20000, 20, &inf);
     if (prc == VFG_E_HIP || prc == VFG_E_NAN) { status = prc; break; }
     // projected step with Armijo backtracking on f(V) = 1/2 V.KV - b.V
