@@ -19,7 +19,7 @@ struct vfg_handle {
   double *scal = nullptr;      // device scalars (64)
   bool v_ready = false;
   // solver work space (allocated on first use)
-  double *w[12] = {nullptr};
+  double *w[16] = {nullptr};
   float *wf[8] = {nullptr};
   uint8_t *act = nullptr;
 };

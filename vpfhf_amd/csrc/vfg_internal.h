@@ -62,6 +62,25 @@ __device__ __forceinline__ bool on_dirichlet(const Grid &G, uint8_t face, uint8_
   return m;
 }
 
+// wave64 / block reductions in FP64 with a fixed order (deterministic)
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double *sh /* NT/64 */) {
+  v = wave_sum(v);
+  const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  double s = 0.;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < NT / 64; i++) s += sh[i];
+  __syncthreads();
+  return s;  // valid in thread 0
+}
+
 // ---------------------------------------------------------------- marching
 // Op interface:
 //   static constexpr int NF  -- staged nodal scalar fields
@@ -91,6 +110,16 @@ __host__ __device__ inline int64_t tiled_index(int tile, int ek, int nez, int nc
   return (((int64_t)tile * nez + ek) * ncoef + coef) * nslot + slot;
 }
 
+// Ops that also reduce over their owned nodes define
+//   __device__ double store(...)      (returns the node's contribution)
+//   __device__ void partial(int slot, double sum)
+// the march sums the contributions of each thread, reduces them over the
+// workgroup in a fixed order and hands the sum to partial(rslot, .).
+template <class Op, class = void>
+struct has_red : std::false_type {};
+template <class Op>
+struct has_red<Op, std::void_t<decltype(&Op::partial)>> : std::true_type {};
+
 template <class Op, class = void>
 struct aux_of {
   using type = int;
@@ -101,12 +130,13 @@ struct aux_of<Op, std::void_t<decltype(&Op::aux)>> {
 };
 
 template <typename T, int TX, int TY, class Op>
-__device__ __forceinline__ void march(const Grid &G, const Op &op, int OZ, int bx, int by, int bz) {
+__device__ __forceinline__ void march(const Grid &G, const Op &op, int OZ, int bx, int by, int bz, int rslot = 0) {
   constexpr int NF = Op::NF, NC = Op::NC;
   constexpr int NT = TX * TY, OX = TX - 1, OY = TY - 1, SX = TX + 1, SY = TY + 1;
   constexpr int NPF = (SX * SY + NT - 1) / NT;  // staged nodes per thread
   constexpr bool AUX = has_aux<Op>::value;
   constexpr bool TILED = is_tiled<Op>::value;
+  constexpr bool RED = has_red<Op>::value;
   __shared__ T s_in[NF][SY * SX];
   __shared__ T s_red[2 * NC][NT];
   const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
@@ -170,6 +200,7 @@ __device__ __forceinline__ void march(const Grid &G, const Op &op, int OZ, int b
   T carry[NC];
 #pragma unroll
   for (int c = 0; c < NC; c++) carry[c] = T(0);
+  double racc = 0.;  // owned-node contributions (reducing ops)
 
   // one element layer: bottom plane xb (node plane ek), top plane xt (ek + 1)
   auto layer = [&](int ek, const T (&xb)[4][NF], T (&xt)[4][NF], AuxT &aux_next) {
@@ -224,7 +255,10 @@ __device__ __forceinline__ void march(const Grid &G, const Op &op, int OZ, int b
         bot[c] = carry[c] + (cx[0][1][c] + s_red[c][tid + TX]);
         carry[c] = cx[1][1][c] + s_red[NC + c][tid + TX];
       }
-      if (ek >= oz0 && ek >= 0) op.store(ox0 + tx, oy0 + ty, ek, bot, xb[3]);
+      if (ek >= oz0 && ek >= 0) {
+        if constexpr (RED) racc += op.store(ox0 + tx, oy0 + ty, ek, bot, xb[3]);
+        else op.store(ox0 + tx, oy0 + ty, ek, bot, xb[3]);
+      }
     }
   };
 
@@ -241,30 +275,16 @@ __device__ __forceinline__ void march(const Grid &G, const Op &op, int OZ, int b
     if (ek + 1 > ek_end) break;
     layer(ek + 1, xb2, xa, aux_next);
   }
+  if constexpr (RED) {
+    __shared__ double s_rs[NT / 64];
+    const double v = block_sum<NT>(racc, s_rs);
+    if (tid == 0) op.partial(rslot, v);
+  }
 }
 
 template <typename T>
 __device__ __forceinline__ T interp(int g, T f0, T f1) {
   return T(gP0(g)) * f0 + T(gP1(g)) * f1;
-}
-
-// wave64 / block reductions in FP64 with a fixed order (deterministic)
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-template <int NT>
-__device__ __forceinline__ double block_sum(double v, double *sh /* NT/64 */) {
-  v = wave_sum(v);
-  const int w = threadIdx.x / 64, l = threadIdx.x % 64;
-  if (l == 0) sh[w] = v;
-  __syncthreads();
-  double s = 0.;
-  if (threadIdx.x == 0)
-    for (int i = 0; i < NT / 64; i++) s += sh[i];
-  __syncthreads();
-  return s;  // valid in thread 0
 }
 
 }  // namespace vfg

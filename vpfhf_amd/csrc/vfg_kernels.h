@@ -50,6 +50,11 @@ void launch_v_fixmask(const DevProblem &P, hipStream_t s);                 // vf
 void launch_v_apply(const DevProblem &P, const double *x, double *y, int mode, hipStream_t s,
                     const uint8_t *mask = nullptr);
 void launch_v_diag(const DevProblem &P, double *d, hipStream_t s, const uint8_t *mask = nullptr);
+// CG-fused masked V apply (mask: fixed or active nodes): p = z + beta p_old
+// staged, q = K p, p and q written, one partial of p.q per workgroup; returns
+// the number of partials
+int launch_v_apply_cg(const DevProblem &P, const double *pold, double *p, const double *z, const double *beta,
+                      double *q, double *part, const uint8_t *mask, hipStream_t s);
 
 // misc (vfg_misc.hip)
 void launch_bc_mask(const DevProblem &P, int which, uint8_t *mask, hipStream_t s);

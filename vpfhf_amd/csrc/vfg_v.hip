@@ -257,6 +257,29 @@ struct VApplyOp {
   }
 };
 
+// CG-fused masked apply (mode 0): stages p = z + beta p_old off the mask, writes
+// p and q = K p, one partial of p.q per workgroup; p_old and p are different
+// arrays (neighbouring workgroups still read p_old at their halo nodes)
+struct VApplyCgOp : VApplyOp {
+  const double *z;
+  const double *beta;
+  double *pw;
+  double *part;
+  __device__ double load(int, int64_t n) const { return vfix[n] ? 0. : z[n] + *beta * x[n]; }
+  __device__ double store(int i, int j, int k, const double (&acc)[1], const double (&own)[1]) const {
+    const int64_t n = node_id(G, i, j, k);
+    const double yv = vfix[n] ? own[0] : acc[0];
+    y[n] = yv;
+    pw[n] = own[0];
+    return own[0] * yv;
+  }
+  __device__ void partial(int slot, double v) const { part[slot] = v; }
+};
+__global__ __launch_bounds__(256, 3) void k_v_apply_cg(VApplyCgOp op, int OZ) {
+  march<double, 16, 16>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z,
+                        blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
+}
+
 __global__ __launch_bounds__(256, 3) void k_v_apply(VApplyOp op, int OZ) { march<double, 16, 16>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z); }
 
 // ---------------------------------------------------------------- diagonal
@@ -437,6 +460,21 @@ void launch_engine_v_apply(const void *d_ops, const BItem *items, const int *dco
                            int fstride, hipStream_t s) {
   hipLaunchKernelGGL(k_batch_v_apply, dim3(grid), dim3(256), 0, s, (const BVApply *)d_ops, items, dcount, flags,
                      fstride);
+}
+
+int launch_v_apply_cg(const DevProblem &P, const double *pold, double *p, const double *z, const double *beta,
+                      double *q, double *part, const uint8_t *mask, hipStream_t s) {
+  VApplyCgOp op;
+  op.x = pold; op.y = q; op.q = P.Vq; op.bV = P.bV; op.VIrrev = P.VIrrev; op.vfix = mask ? mask : P.vfix;
+  op.G = P.G; op.ne = P.ne; op.C = make_vconst(P); op.bc = P.bcV;
+  for (int f = 0; f < 6; f++) op.ftype[f] = (uint8_t)P.hbcV.face[f];
+  for (int v = 0; v < 8; v++) op.vtype[v] = (uint8_t)P.hbcV.vertex[v];
+  op.mode = 0; op.atnum = P.atnum; op.irrevtol = P.irrevtol;
+  op.z = z; op.beta = beta; op.pw = p; op.part = part;
+  const int OZ = oz_for(P.G);
+  const dim3 g = mgrid(P.G, OZ);
+  hipLaunchKernelGGL(k_v_apply_cg, g, dim3(256), 0, s, op, OZ);
+  return (int)(g.x * g.y * g.z);
 }
 
 void launch_v_diag(const DevProblem &P, double *d, hipStream_t s, const uint8_t *mask) {
