@@ -90,3 +90,36 @@ def test_serve_poisson_stream():
         assert r["status"] == 0
         assert r["t_admit"] >= t.arrival - 1e-9 and r["t_done"] >= r["t_admit"] and r["latency"] > 0
     assert 0 < st["p50"] <= st["p99"] <= st["max_latency"]
+
+
+def test_serve_failing_request_is_isolated():
+    """A request whose pressure field holds a NaN stops with VFG_E_NAN; the
+    others in the same mix finish and are bitwise equal to their runs alone."""
+    from vpfhf_amd import serve
+    R = _small(4, seed=13, am=(2, 2))
+    bad = 1
+    p = R.keep[5 * bad + 4]  # payload per request: options, E, theta, thetaRef, pressure
+    p[len(p) // 2] = float("nan")  # the C loop reads this array in place
+    res, st = serve.serve(R, max_slots=4, realtime=False)
+    assert res[bad]["status"] == -6
+    for i in (0, 2, 3):
+        assert res[i]["status"] == 0
+        A = serve.RequestSet([R.specs[i]], [R.ams[i]], [0.], outputs=True)
+        serve.serve(A, max_slots=1, realtime=False)
+        assert np.array_equal(A.U[0], R.U[i]) and np.array_equal(A.V[0], R.V[i])
+
+
+def test_serve_at2_requests_match_single_problem_path():
+    from vpfhf_amd import serve, workloads
+    sa = [serve.spec_request(i, seed=17, lo=8, hi=14, am_range=(2, 2)) for i in range(2)]
+    for s, _ in sa:
+        s["atnum"] = 2
+    R = serve.RequestSet([s for s, _ in sa], [a for _, a in sa], [0., 0.], outputs=True)
+    res, _ = serve.serve(R, max_slots=2, realtime=False)
+    for i, (s, am) in enumerate(sa):
+        H, _ = workloads.make_host(s, u_rtol=1e-10)
+        info = H.altmin(maxit=am, stop_on_tol=True)
+        U, V, _ = H.state()
+        H.close()
+        assert res[i]["status"] == 0 and info.iterations == res[i]["am_iterations"]
+        assert relerr(R.U[i], U) < 1e-6 and relerr(R.V[i], V) < 1e-6
