@@ -145,6 +145,19 @@ int vfg_energies(vfg_handle *h, const double *U, const double *V, double out[4],
 /* VolumetricCrackOpening (VFPermfield.c:75-150); cellfield (N_e) optional */
 int vfg_crack_volume(vfg_handle *h, const double *U, const double *V, double *vol,
                      double *cellfield, int loc);
+/* per-cell energy terms, 3 per cell interleaved (cell3[3e + c]): c = 0 the
+ * element's elastic energy (VF_ElasticEnergy3D_local, VFMech.c:725-806), 1 its
+ * crack pressure work (VF_PressureWork3D_local, VFMech.c:920-974), 2 its
+ * surface energy (VF_AT1/AT2SurfaceEnergy3D_local, VFMech.c:2394-2465); the
+ * terms VF_UEnergy3D and VF_VEnergy3D sum (VFMech.c:1111-1427, 2474-2526).
+ * Feeds vfg_cell_to_node with dof 3 for nodal energy fields. */
+int vfg_energy_fields(vfg_handle *h, const double *U, const double *V, double *cell3, int loc);
+/* CellToNodeInterpolation (VFPermfield.c:1156-1327): node (dof*N_n) = the
+ * volume-weighted mean of the adjacent cells of cell (dof*N_e), components
+ * interleaved, 1 <= dof <= 6.  Overwrites node; every reference caller zeroes
+ * it first (VecSet(...,0.) at VFPermfield.c:144, 563, 1398, 1922 and
+ * VFFlow_SNESStandardFEM.c:228, 340), so the results agree. */
+int vfg_cell_to_node(vfg_handle *h, const double *cell, double *node, int dof, int loc);
 
 /* ---------------------------------------------------------- crack geometry */
 /* VFPennyCrackBuildVAT1/AT2 (VFCracks.c:136-227) for one crack into V */

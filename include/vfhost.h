@@ -82,6 +82,12 @@ int  vfh_timestep_prepare(vfh_ctx *ctx);
 int  vfh_altmin(vfh_ctx *ctx, int maxit, int stop_on_tol, vfh_altmin_info *info);
 /* out: elastic, in-situ work (0), pressure work, surface energy, crack volume */
 int  vfh_energies(vfh_ctx *ctx, double out[5]);
+/* fields->VolCrackOpening, the nodal crack-opening field of
+ * VolumetricCrackOpening (VFPermfield.c:75-146): host array of N_n */
+int  vfh_crack_opening_field(vfh_ctx *ctx, double *node);
+/* per-cell (elastic, pressure work, surface) energy terms, host array of
+ * 3 N_e interleaved (element terms of VFMech.c:1111-1427, 2474-2526) */
+int  vfh_energy_fields(vfh_ctx *ctx, double *cell3);
 int  vfh_get_state(vfh_ctx *ctx, double *U, double *V, double *VIrrev);
 int  vfh_set_state(vfh_ctx *ctx, const double *U, const double *V, const double *VIrrev);
 
@@ -100,6 +106,8 @@ typedef struct {
   double        arrival;                /* seconds after the start of the run */
   int           am_maxit;               /* iterations (fewer once ||V - Vold|| < altmintol) */
   double       *U_out, *V_out;          /* optional host copies of the final U and V */
+  double       *opening_out;            /* optional nodal crack opening (N_n) */
+  double       *energy_out;             /* optional per-cell energy terms (3 N_e) */
 } vfh_request;
 
 typedef struct {

@@ -1005,6 +1005,58 @@ double vfo_crack_volume(const vfo_problem *P, const double *U, const double *V, 
   return total;
 }
 
+/* element terms of VF_UEnergy3D (VFMech.c:1111-1427) and VF_VEnergy3D
+ * (VFMech.c:2474-2526), one triple per cell, all three from the V argument */
+void vfo_energy_fields(const vfo_problem *P, const vfo_fields *F, const double *U, const double *V,
+                       double *cell3)
+{
+  coords_t C = mkcoords(P);
+  vfo_elem e;
+  elemdata d;
+  ELEM_LOOP(P, C.X, C.Y, C.Z)
+  {
+    ELEM_SETUP(P, C.X, C.Y, C.Z, e, cell);
+    double lambda, mu, pw = 0., se = 0., v8[8];
+    elem_lame(P, cell, &lambda, &mu);
+    gather(P, F, U, ei, ej, ek, &d);
+    for (int a = 0; a < 8; a++) v8[a] = V[NODE(P, ei + (a & 1), ej + ((a >> 1) & 1), ek + (a >> 2))];
+    cell3[3 * cell] = vfo_elastic_energy_local(d.u24, v8, d.dth8, d.p8, lambda, mu, P->alpha, P->beta,
+                                               P->eta, &e);
+    if (P->hasCrackPressure) pressure_work_acc(&pw, d.u24, v8, d.p8, &e);
+    cell3[3 * cell + 1] = pw;
+    surface_energy_acc(&se, v8, P->atnum, P->Gc, vfo_atcv(P->atnum), P->epsilon, &e);
+    cell3[3 * cell + 2] = se;
+  }
+  freecoords(C);
+}
+
+/* VFPermfield.c:1156-1327: scatter cell * volume and volume to the 8 nodes of
+ * every cell (in cell order), then divide node by node */
+void vfo_cell_to_node(const vfo_problem *P, const double *cell, double *node, int dof)
+{
+  coords_t C = mkcoords(P);
+  const int64_t nn = vfo_nnodes(P);
+  double *vol = (double *)calloc((size_t)nn, sizeof(double));
+  memset(node, 0, sizeof(double) * (size_t)dof * (size_t)nn);
+  for (int ek = 0; ek < P->nz - 1; ek++)
+    for (int ej = 0; ej < P->ny - 1; ej++)
+      for (int ei = 0; ei < P->nx - 1; ei++) {
+        const double hx = C.X[ei + 1] - C.X[ei], hy = C.Y[ej + 1] - C.Y[ej], hz = C.Z[ek + 1] - C.Z[ek];
+        const int64_t c = (int64_t)ei + (int64_t)(P->nx - 1) * ((int64_t)ej + (int64_t)(P->ny - 1) * ek);
+        for (int k = 0; k < 2; k++)
+          for (int j = 0; j < 2; j++)
+            for (int i = 0; i < 2; i++) {
+              const int64_t n = NODE(P, ei + i, ej + j, ek + k);
+              vol[n] += hz * hy * hx;
+              for (int d = 0; d < dof; d++) node[dof * n + d] += cell[dof * c + d] * hx * hy * hz;
+            }
+      }
+  for (int64_t n = 0; n < nn; n++)
+    for (int d = 0; d < dof; d++) node[dof * n + d] = node[dof * n + d] / vol[n];
+  free(vol);
+  freecoords(C);
+}
+
 /* ----------------------------------------------------------------- solvers */
 
 static double dot(const double *a, const double *b, int64_t n)

@@ -173,6 +173,20 @@ double vfo_v_energy(const vfo_problem *P, const double *V);
 double vfo_crack_volume(const vfo_problem *P, const double *U, const double *V,
                         double *cellfield);
 
+/* per-cell energy contributions, 3 per cell interleaved: elastic
+ * (VF_ElasticEnergy3D_local, VFMech.c:725-806), crack pressure work
+ * (VF_PressureWork3D_local, VFMech.c:920-974, 0 without -pressurize) and
+ * surface energy
+ * (VF_AT1/AT2SurfaceEnergy3D_local, VFMech.c:2394-2465): the element terms
+ * whose sums VF_UEnergy3D / VF_VEnergy3D report */
+void vfo_energy_fields(const vfo_problem *P, const vfo_fields *F, const double *U, const double *V,
+                       double *cell3);
+
+/* CellToNodeInterpolation (VFPermfield.c:1156-1327): node value = sum over the
+ * adjacent cells of cell value * cell volume / sum of those volumes; dof 1 or
+ * 3 values per cell / node, interleaved */
+void vfo_cell_to_node(const vfo_problem *P, const double *cell, double *node, int dof);
+
 /* ---- solvers (semantics of VFCommon.c:1005-1088, 1132-1157, 1192-1231) ---- */
 /* Jacobi PCG on an ELL matrix; returns iterations, <0 on breakdown */
 int vfo_pcg(const vfo_ell *A, const double *b, double *x, double rtol,

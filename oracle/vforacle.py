@@ -91,6 +91,10 @@ def lib():
         L.vfo_v_energy.restype = C.c_double
         L.vfo_crack_volume.argtypes = [P, dp, dp, dp]
         L.vfo_crack_volume.restype = C.c_double
+        L.vfo_energy_fields.argtypes = [P, Fp, dp, dp, dp]
+        L.vfo_energy_fields.restype = None
+        L.vfo_cell_to_node.argtypes = [P, dp, dp, C.c_int]
+        L.vfo_cell_to_node.restype = None
         L.vfo_bc_mask.argtypes = [P, C.POINTER(BC), C.c_int, C.POINTER(C.c_uint8)]
         L.vfo_vec_set_from_bc.argtypes = [P, C.POINTER(BC), C.c_int, dp]
         L.vfo_vec_apply_dirichlet.argtypes = [P, C.POINTER(BC), C.c_int, dp, dp]
@@ -254,6 +258,21 @@ def crack_volume(op, U, V, cellfield=False):
     cf = np.zeros(op.ne) if cellfield else None
     vol = lib().vfo_crack_volume(op.ref, _dp(U), _dp(V), _dp(cf))
     return (vol, cf) if cellfield else vol
+
+
+def energy_fields(op, F, U, V):
+    """Per-cell (elastic, pressure work, surface) energy terms, shape (N_e, 3)."""
+    out = np.zeros(3 * op.ne)
+    lib().vfo_energy_fields(op.ref, C.byref(F), _dp(U), _dp(V), _dp(out))
+    return out.reshape(op.ne, 3)
+
+
+def cell_to_node(op, cell, dof=1):
+    """CellToNodeInterpolation (VFPermfield.c:1156-1327)."""
+    cell = np.ascontiguousarray(cell, np.float64)
+    node = np.zeros(dof * op.nn)
+    lib().vfo_cell_to_node(op.ref, _dp(cell), _dp(node), dof)
+    return node
 
 
 def bc_mask(op, which="U"):

@@ -117,6 +117,49 @@ def test_energies_and_crack_volume(pair, oracle):
     assert relerr(cf, rcf) < 1e-12
 
 
+def test_energy_fields(pair, oracle):
+    """Per-cell energy terms against the oracle (1e-12 per column, 2-norm) and
+    their sums against the GPU's own whole-domain energies."""
+    s, op, F, P = pair
+    ef = P.energy_fields(s["U"], s["V"])
+    ref = oracle.energy_fields(op, F, s["U"], s["V"])
+    assert ef.shape == (op.ne, 3)
+    for c in range(3):
+        assert relerr(ef[:, c], ref[:, c]) < 1e-12, c
+    el, _, pw, se = P.energies(s["U"], s["V"])
+    assert abs(ef[:, 0].sum() - el) <= 1e-12 * abs(el)
+    assert abs(ef[:, 1].sum() - pw) <= 1e-12 * max(np.abs(ef[:, 1]).sum(), 1e-300)
+    assert abs(ef[:, 2].sum() - se) <= 1e-12 * abs(se)
+    nodal = P.CellToNodeInterpolation(ef.ravel(), 3)
+    assert relerr(nodal, oracle.cell_to_node(op, ref.ravel(), 3)) < 1e-12
+
+
+@pytest.mark.parametrize("n,l", [((9, 8, 7), (1., 1., 1.)), ((2, 2, 2), (1., 2., .5)), ((17, 3, 11), (3., .2, 1.7)),
+                                 ((41, 37, 33), (1., 1.3, .6))])
+def test_cell_to_node(oracle, n, l):
+    """CellToNodeInterpolation (VFPermfield.c:1156-1327) for 1, 3 and 6
+    components against the oracle: same cell order, same products, no
+    contraction, so agreement is to the last bit or an ulp of rounding in
+    the coordinate divisions."""
+    s = S.make(n=n, l=l, seed=3)
+    op, F = S.to_oracle(oracle, s)
+    P = S.to_gpu(s)
+    rng = np.random.default_rng(11)
+    for dof in (1, 3, 6):
+        cell = rng.standard_normal(dof * op.ne) * rng.lognormal(0, 2, dof * op.ne)
+        got = P.CellToNodeInterpolation(cell, dof)
+        ref = oracle.cell_to_node(op, cell, dof)
+        assert got.shape == ref.shape
+        assert np.max(np.abs(got - ref)) <= 4e-16 * np.max(np.abs(ref)), dof
+    # the nodal crack-opening field of VolumetricCrackOpening (VFPermfield.c:144-146)
+    vol, cf, nf = P.VolumetricCrackOpening(s["U"], s["V"], cellfield=True, nodefield=True)
+    rvol, rcf = oracle.crack_volume(op, s["U"], s["V"], cellfield=True)
+    assert relerr(nf, oracle.cell_to_node(op, rcf)) < 1e-12
+    with pytest.raises(Exception):
+        P.CellToNodeInterpolation(np.zeros(op.ne), dof=7)
+    P.close()
+
+
 @pytest.mark.parametrize("atnum", [1, 2])
 def test_penny_v(oracle, atnum):
     s = S.make(n=(17, 15, 13), atnum=atnum)

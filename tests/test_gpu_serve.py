@@ -123,3 +123,24 @@ def test_serve_at2_requests_match_single_problem_path():
         H.close()
         assert res[i]["status"] == 0 and info.iterations == res[i]["am_iterations"]
         assert relerr(R.U[i], U) < 1e-6 and relerr(R.V[i], V) < 1e-6
+
+
+def test_serve_field_outputs():
+    """Nodal crack opening (fields->VolCrackOpening, VFPermfield.c:144-146) and
+    per-cell energy terms returned with each request equal those of the same
+    final state loaded into a single-problem context, and the energy terms sum
+    to the request's reported energies."""
+    from vpfhf_amd import serve, workloads
+    R = serve.make_requests(3, seed=21, outputs=True, fields=True, lo=8, hi=16, am_range=(2, 2))
+    res, _ = serve.serve(R, max_slots=3, realtime=False)
+    for i, s in enumerate(R.specs):
+        assert res[i]["status"] == 0
+        H, _ = workloads.make_host(s)
+        H.set_state(U=R.U[i], V=R.V[i])
+        assert np.array_equal(H.crack_opening_field(), R.opening[i])
+        assert np.array_equal(H.energy_fields(), R.energy[i])
+        H.close()
+        e = res[i]["energies"]
+        assert abs(R.energy[i][:, 0].sum() - e[0]) <= 1e-12 * abs(e[0])
+        assert abs(R.energy[i][:, 2].sum() - e[3]) <= 1e-12 * abs(e[3])
+        assert np.all(np.isfinite(R.opening[i])) and R.opening[i].shape == R.V[i].shape

@@ -169,6 +169,43 @@ def test_elastic_energy_uniaxial_bar(oracle):
         assert abs(st["elastic"] - 0.2 * st["t"] ** 2) < 1e-9
 
 
+def test_energy_fields_closed_forms(oracle):
+    """Per-cell energy terms (VFMech.c:725-806, 920-974, 2394-2465):
+    uniaxial strain gives 1/2 E eps^2 (1+eta) vol in every cell; a constant u
+    across a linear v under uniform p gives p (u . grad v) vol; uniform V = 0
+    gives Gc/(4 atCv) vol/epsilon; the cells sum to VF_UEnergy3D / VF_VEnergy3D."""
+    n, l = (6, 5, 4), (1.5, 2., .75)
+    nn = n[0] * n[1] * n[2]
+    vol = np.prod([l[d] / (n[d] - 1) for d in range(3)])
+    x = np.tile(np.arange(n[0]) * l[0] / (n[0] - 1), n[1] * n[2])
+    E, eta, eps0, Gc, epsl = 3.0, 1e-3, 0.01, 1.3, 0.2
+    for atnum, atcv in ((1, 2 / 3), (2, .5)):
+        op = oracle.OracleProblem(n, l, E=E, nu=0.0, alpha=0., beta=0., Gc=Gc, eta=eta, epsilon=epsl,
+                                  atnum=atnum, pressurize=True)
+        zero, one = np.zeros(nn), np.ones(nn)
+        U = np.zeros(3 * nn); U[0::3] = eps0 * x
+        F = op.fields(V=one, VIrrev=one, theta=zero, thetaRef=zero, pressure=zero)
+        ef = oracle.energy_fields(op, F, U, one)
+        assert np.allclose(ef[:, 0], .5 * E * eps0 ** 2 * (1 + eta) * vol, rtol=1e-12)
+        assert np.all(ef[:, 1] == 0) and np.allclose(ef[:, 2], 0, atol=1e-15)
+        # pressure work of a constant displacement over a linear V
+        U = np.tile([.3, -.2, .1], nn); s, p = .7, 2.5
+        F = op.fields(V=s * x, VIrrev=one, theta=zero, thetaRef=zero, pressure=np.full(nn, p))
+        ef = oracle.energy_fields(op, F, U, s * x)
+        assert np.allclose(ef[:, 1], p * .3 * s * vol, rtol=1e-12)
+        # surface energy of V = 0
+        ef = oracle.energy_fields(op, F, U, zero)
+        assert np.allclose(ef[:, 2], Gc / (4 * atcv) * vol / epsl, rtol=1e-13)
+    # sums against the whole-domain functionals on a random state
+    s = S.make(n=(7, 6, 5), seed=9, atnum=2)
+    op, F = S.to_oracle(oracle, s)
+    ef = oracle.energy_fields(op, F, s["U"], s["V"])
+    el, pw = oracle.u_energy(op, F, s["U"])
+    assert abs(ef[:, 0].sum() - el) < 1e-12 * abs(el)
+    assert abs(ef[:, 1].sum() - pw) < 1e-12 * np.abs(ef[:, 1]).sum()
+    assert abs(ef[:, 2].sum() - oracle.v_energy(op, s["V"])) < 1e-12 * oracle.v_energy(op, s["V"])
+
+
 def test_AT_matrix_closed_form(oracle):
     h = (0.3, 0.2, 0.5)
     e = oracle.elem(*h)
@@ -296,6 +333,52 @@ def test_penny_profiles(oracle):
     V2 = oracle.penny_v(op2, (.5, .5, .5), .2, 0., 0., .09375)
     ref2 = np.where(dist <= 0, 0., 1 - np.exp(-dist / 2 / .0625))
     assert np.allclose(V2.reshape(33, 33, 33)[:, 16, 16], ref2, atol=1e-15)
+
+
+# ------------------------------------------------------ cell -> node fields
+def _adjacent_centroid(n, l):
+    """Mean centroid coordinate of the cells next to each node along one axis."""
+    h = l / (n - 1)
+    x = np.arange(n) * h
+    x[0], x[-1] = h / 2, l - h / 2
+    return x
+
+
+@pytest.mark.parametrize("n,l", [((6, 5, 4), (1.5, 2., .75)), ((2, 3, 2), (1., .5, 2.)), ((9, 2, 7), (3., 1., 1.))])
+def test_cell_to_node_closed_forms(oracle, n, l):
+    """CellToNodeInterpolation (VFPermfield.c:1156-1327) pinned by what the
+    volume-weighted mean fixes: constants stay, a field linear in the cell
+    centroids gives the mean adjacent centroid (the node itself inside), and
+    sum_n node_n * (adjacent volume)/8 equals sum_c cell_c * volume_c."""
+    op = oracle.OracleProblem(n, l)
+    h = [l[d] / (n[d] - 1) for d in range(3)]
+    cz, cy, cx = np.meshgrid(*[(np.arange(n[d] - 1) + .5) * h[d] for d in (2, 1, 0)], indexing="ij")
+    cx, cy, cz = cx.ravel(), cy.ravel(), cz.ravel()
+    nz_, ny_, nx_ = np.meshgrid(*[_adjacent_centroid(n[d], l[d]) for d in (2, 1, 0)], indexing="ij")
+    nx_, ny_, nz_ = nx_.ravel(), ny_.ravel(), nz_.ravel()
+    # constants, one per component
+    const = np.tile([2.5, -1., 7.], op.ne)
+    assert np.allclose(oracle.cell_to_node(op, const, 3), np.tile([2.5, -1., 7.], op.nn), rtol=1e-14, atol=0)
+    # linear fields: scalar in all three axes, vector with one axis per component
+    lin = .3 + 1.7 * cx - 2.9 * cy + .45 * cz
+    assert np.allclose(oracle.cell_to_node(op, lin), .3 + 1.7 * nx_ - 2.9 * ny_ + .45 * nz_, rtol=1e-13, atol=1e-14)
+    vec = np.stack([1 + cx, 2 - 3 * cy, 5 * cz], axis=1).ravel()
+    ref = np.stack([1 + nx_, 2 - 3 * ny_, 5 * nz_], axis=1).ravel()
+    assert np.allclose(oracle.cell_to_node(op, vec, 3), ref, rtol=1e-13, atol=1e-14)
+    # conservation with random data
+    rng = np.random.default_rng(5)
+    cell = rng.standard_normal(op.ne)
+    node = oracle.cell_to_node(op, cell)
+    cnt = np.ones(op.ne)
+    adj = oracle.cell_to_node(op, cnt)          # == 1; use the count of adjacent cells instead
+    assert np.allclose(adj, 1.0, rtol=1e-15)
+    ncells = np.ones(n[::-1])
+    for ax in range(3):
+        m = np.full(n[::-1][ax], 2.0)
+        m[0] = m[-1] = 1.0
+        ncells = ncells * m.reshape([-1 if a == ax else 1 for a in range(3)])
+    vc = np.prod(h)
+    assert abs((node * ncells.ravel() * vc / 8).sum() - (cell * vc).sum()) < 1e-13 * np.abs(cell).sum() * vc
 
 
 # ------------------------------------------------------------- solvers

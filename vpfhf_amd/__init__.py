@@ -217,12 +217,38 @@ class Problem:
         self._c(self.L.vfg_energies(self.h, _ptr(U), _ptr(V), out, HOST))
         return tuple(out)
 
-    def VolumetricCrackOpening(self, U, V, cellfield=False):
+    def VolumetricCrackOpening(self, U, V, cellfield=False, nodefield=False):
+        """Crack volume (VFPermfield.c:75-150); with cellfield the per-cell
+        opening, with nodefield also the nodal field the reference stores in
+        fields->VolCrackOpening (VFPermfield.c:144-146)."""
         vol = C.c_double()
-        cf = np.empty(self.ne) if cellfield else None
+        cf = np.empty(self.ne) if (cellfield or nodefield) else None
         U, V = np.ascontiguousarray(U, np.float64), np.ascontiguousarray(V, np.float64)
         self._c(self.L.vfg_crack_volume(self.h, _ptr(U), _ptr(V), C.byref(vol), _ptr(cf), HOST))
-        return (vol.value, cf) if cellfield else vol.value
+        out = (vol.value,)
+        if cellfield:
+            out += (cf,)
+        if nodefield:
+            out += (self.CellToNodeInterpolation(cf),)
+        return out if len(out) > 1 else vol.value
+
+    def energy_fields(self, U, V):
+        """Per-cell (elastic, pressure work, surface) energy terms, shape (N_e, 3):
+        the element terms of VF_UEnergy3D / VF_VEnergy3D (VFMech.c:1111-1427,
+        2474-2526)."""
+        out = np.empty(3 * self.ne)
+        U, V = np.ascontiguousarray(U, np.float64), np.ascontiguousarray(V, np.float64)
+        self._c(self.L.vfg_energy_fields(self.h, _ptr(U), _ptr(V), _ptr(out), HOST))
+        return out.reshape(self.ne, 3)
+
+    def CellToNodeInterpolation(self, cell, dof=1):
+        """Volume-weighted cell -> node averaging (VFPermfield.c:1156-1327)."""
+        cell = np.ascontiguousarray(cell, np.float64)
+        if cell.size != dof * self.ne:
+            raise ValueError(f"cell field has {cell.size} values, expected {dof * self.ne}")
+        node = np.empty(dof * self.nn)
+        self._c(self.L.vfg_cell_to_node(self.h, _ptr(cell), _ptr(node), dof, HOST))
+        return node
 
     def penny_v(self, center, r, theta, phi, thickness):
         V = np.empty(self.nn)

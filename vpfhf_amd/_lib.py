@@ -92,6 +92,8 @@ PROTOS = {
     "vfg_vec_apply_dirichlet": (C.c_int, [H, C.c_int, vp, vp, C.c_int]),
     "vfg_energies": (C.c_int, [H, vp, vp, dp, C.c_int]),
     "vfg_crack_volume": (C.c_int, [H, vp, vp, dp, vp, C.c_int]),
+    "vfg_cell_to_node": (C.c_int, [H, vp, vp, C.c_int, C.c_int]),
+    "vfg_energy_fields": (C.c_int, [H, vp, vp, vp, C.c_int]),
     "vfg_penny_build_v": (C.c_int, [H, vp, dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int]),
     "vfg_solve_opts_default": (None, [C.POINTER(SolveOpts)]),
     "vfg_u_solve": (C.c_int, [H, vp, vp, C.POINTER(SolveOpts), C.POINTER(SolveInfo), C.c_int]),

@@ -423,7 +423,8 @@ int vfg_vec_apply_dirichlet(vfg_handle *h, int which, double *vec, const double 
 }
 
 // ------------------------------------------------------------- diagnostics
-static int energies_(vfg_handle *h, const double *U, const double *V, double *cellfield, double out[4], int loc) {
+static int energies_(vfg_handle *h, const double *U, const double *V, double *cellfield, double *efield,
+                     double out[4], int loc) {
   DevProblem &P = h->P;
   int rc;
   int nb = energy_blocks(P);
@@ -432,18 +433,21 @@ static int energies_(vfg_handle *h, const double *U, const double *V, double *ce
   }
   double *part = nb > 65536 ? h->part_big : h->part;
   const double *dU = U, *dV = V;
-  double *dcell = cellfield;
+  double *dcell = cellfield, *deng = efield;
   if (loc == VFG_HOST) {
-    if ((rc = vfg_scratch(h, 4 * P.nn + P.ne))) return rc;
+    if ((rc = vfg_scratch(h, 4 * P.nn + 4 * P.ne))) return rc;
     double *sU = h->stage, *sV = h->stage + 3 * P.nn;
     if ((rc = vfg_in(h, sU, U, 24 * P.nn, VFG_HOST)) || (rc = vfg_in(h, sV, V, 8 * P.nn, VFG_HOST))) return rc;
     dU = sU; dV = sV;
     dcell = cellfield ? h->stage + 4 * P.nn : nullptr;
+    deng = efield ? h->stage + 4 * P.nn + P.ne : nullptr;
   }
-  launch_energies(P, dU, dV, part, dcell, h->scal, h->stream);
+  launch_energies(P, dU, dV, part, dcell, deng, h->scal, h->stream);
   HIPCHK(h, hipMemcpyAsync(out, h->scal, 32, hipMemcpyDeviceToHost, h->stream));
   if (loc == VFG_HOST && cellfield)
     HIPCHK(h, hipMemcpyAsync(cellfield, dcell, 8 * P.ne, hipMemcpyDeviceToHost, h->stream));
+  if (loc == VFG_HOST && efield)
+    HIPCHK(h, hipMemcpyAsync(efield, deng, 24 * P.ne, hipMemcpyDeviceToHost, h->stream));
   return vfg_synchronize(h);
 }
 
@@ -451,7 +455,7 @@ int vfg_energies(vfg_handle *h, const double *U, const double *V, double out[4],
   NEED_H(h);
   if (!U || !V || !out) return vfg_fail(h, VFG_E_ARG, "vfg_energies: NULL array");
   double r[4];
-  int rc = energies_(h, U, V, nullptr, r, loc);
+  int rc = energies_(h, U, V, nullptr, nullptr, r, loc);
   if (rc) return rc;
   out[0] = r[0];
   out[1] = 0.;
@@ -464,10 +468,28 @@ int vfg_crack_volume(vfg_handle *h, const double *U, const double *V, double *vo
   NEED_H(h);
   if (!U || !V || !vol) return vfg_fail(h, VFG_E_ARG, "vfg_crack_volume: NULL array");
   double r[4];
-  int rc = energies_(h, U, V, cellfield, r, loc);
+  int rc = energies_(h, U, V, cellfield, nullptr, r, loc);
   if (rc) return rc;
   *vol = r[3];
   return VFG_OK;
+}
+
+int vfg_energy_fields(vfg_handle *h, const double *U, const double *V, double *cell3, int loc) {
+  NEED_H(h);
+  if (!U || !V || !cell3) return vfg_fail(h, VFG_E_ARG, "vfg_energy_fields: NULL array");
+  double r[4];
+  return energies_(h, U, V, nullptr, cell3, r, loc);
+}
+
+int vfg_cell_to_node(vfg_handle *h, const double *cell, double *node, int dof, int loc) {
+  NEED_H(h);
+  if (!cell || !node) return vfg_fail(h, VFG_E_ARG, "vfg_cell_to_node: NULL array");
+  if (dof < 1 || dof > 6) return vfg_fail(h, VFG_E_ARG, "vfg_cell_to_node: dof must be 1..6");
+  IO io(h, loc, cell, 8 * (size_t)dof * h->P.ne, node, 8 * (size_t)dof * h->P.nn);
+  if (io.rc) return io.rc;
+  launch_cell_to_node(h->P, h->grid.lx, h->grid.ly, h->grid.lz, (const double *)io.din, (double *)io.dout, dof,
+                      h->stream);
+  return io.finish();
 }
 
 int vfg_penny_build_v(vfg_handle *h, double *V, const double center[3], double r, double theta_deg,

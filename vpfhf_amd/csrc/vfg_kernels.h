@@ -64,7 +64,9 @@ void launch_vec_apply_dirichlet(const DevProblem &P, int which, double *vec, con
 // partial sums per block: out[b*4 + {elastic, pressure work, surface, crack volume}]
 int energy_blocks(const DevProblem &P);
 void launch_energies(const DevProblem &P, const double *U, const double *V, double *partials,
-                     double *cellfield, double *result4, hipStream_t s);
+                     double *cellfield, double *efield, double *result4, hipStream_t s);
+void launch_cell_to_node(const DevProblem &P, double lx, double ly, double lz, const double *cell, double *node,
+                         int dof, hipStream_t s);
 void launch_penny(const DevProblem &P, double *V, const double c[3], double r, double theta,
                   double phi, double thick, hipStream_t s);
 

@@ -95,7 +95,7 @@ int energy_blocks(const DevProblem &P) { return (int)((P.ne + EB - 1) / EB); }
 
 __global__ __launch_bounds__(EB) void k_energies(DevProblem P, const double *__restrict__ U,
                                                  const double *__restrict__ V, double *partials,
-                                                 double *cellfield) {
+                                                 double *cellfield, double *efield) {
   __shared__ double sh[EB / 64];
   const int64_t e = (int64_t)blockIdx.x * EB + threadIdx.x;
   double el = 0., pw = 0., se = 0., cv = 0.;
@@ -150,6 +150,11 @@ __global__ __launch_bounds__(EB) void k_energies(DevProblem P, const double *__r
       cell += ugv * W;
     }
     if (cellfield) cellfield[e] = cell / vol;
+    if (efield) {
+      efield[3 * e] = el;
+      efield[3 * e + 1] = pw;
+      efield[3 * e + 2] = se;
+    }
   }
   double r0 = block_sum<EB>(el, sh), r1 = block_sum<EB>(pw, sh), r2 = block_sum<EB>(se, sh),
          r3 = block_sum<EB>(cv, sh);
@@ -171,9 +176,9 @@ __global__ __launch_bounds__(256) void k_reduce_partials(const double *partials,
 }
 
 void launch_energies(const DevProblem &P, const double *U, const double *V, double *partials,
-                     double *cellfield, double *result4, hipStream_t s) {
+                     double *cellfield, double *efield, double *result4, hipStream_t s) {
   int nb = energy_blocks(P);
-  hipLaunchKernelGGL(k_energies, dim3(nb), dim3(EB), 0, s, P, U, V, partials, cellfield);
+  hipLaunchKernelGGL(k_energies, dim3(nb), dim3(EB), 0, s, P, U, V, partials, cellfield, efield);
   hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, s, (const double *)partials, nb, 4, result4);
 }
 
@@ -209,6 +214,48 @@ void launch_penny(const DevProblem &P, double *V, const double c[3], double r, d
   double lx = P.G.hx * P.G.nex, ly = P.G.hy * P.G.ney, lz = P.G.hz * P.G.nez;
   hipLaunchKernelGGL(k_penny, dim3(nblk(P.nn)), dim3(256), 0, s, P.G, lx, ly, lz, P.atnum, P.eps, V,
                      c[0], c[1], c[2], r, theta, phi, thick);
+}
+
+// ------------------------------------------------- cell -> node averaging
+// CellToNodeInterpolation (VFPermfield.c:1156-1327): every node takes the
+// volume-weighted mean of its adjacent cells, node = sum(cell*hx*hy*hz) /
+// sum(hz*hy*hx).  The reference scatters cell by cell; here each thread
+// gathers one (node, component) pair, visiting the adjacent cells in the
+// order the scatter reaches the node (ek, ej, ei ascending) with the same
+// products and no contraction, so the sums round as the reference's do.
+// Cell sizes come from the node coordinates X[i] = i*l/(n-1)
+// (VFCommon.c:395-397), as the reference reads them from ctx->coordinates.
+__device__ __forceinline__ double coord_of(int i, double l, int n) { return i == 0 ? 0. : i * l / (n - 1.); }
+
+__global__ __launch_bounds__(256) void k_cell_to_node(Grid G, double lx, double ly, double lz, const double *cell,
+                                                      double *node, int dof) {
+#pragma clang fp contract(off)
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nn = (int64_t)G.nx * G.ny * G.nz;
+  if (t >= nn * dof) return;
+  const int64_t n = t / dof;
+  const int d = (int)(t - n * dof);
+  const int i = (int)(n % G.nx), j = (int)((n / G.nx) % G.ny), k = (int)(n / ((int64_t)G.nx * G.ny));
+  double acc = 0., vol = 0.;
+  for (int ek = k > 0 ? k - 1 : 0; ek <= k && ek < G.nez; ek++) {
+    const double hz = coord_of(ek + 1, lz, G.nz) - coord_of(ek, lz, G.nz);
+    for (int ej = j > 0 ? j - 1 : 0; ej <= j && ej < G.ney; ej++) {
+      const double hy = coord_of(ej + 1, ly, G.ny) - coord_of(ej, ly, G.ny);
+      for (int ei = i > 0 ? i - 1 : 0; ei <= i && ei < G.nex; ei++) {
+        const double hx = coord_of(ei + 1, lx, G.nx) - coord_of(ei, lx, G.nx);
+        vol = vol + hz * hy * hx;
+        acc = acc + cell[dof * cell_id(G, ei, ej, ek) + d] * hx * hy * hz;
+      }
+    }
+  }
+  node[t] = acc / vol;
+}
+
+void launch_cell_to_node(const DevProblem &P, double lx, double ly, double lz, const double *cell, double *node,
+                         int dof, hipStream_t s) {
+  const int64_t n = P.nn * dof;
+  hipLaunchKernelGGL(k_cell_to_node, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, P.G, lx, ly, lz, cell,
+                     node, dof);
 }
 
 // ------------------------------------------------------------ vector ops

@@ -37,7 +37,8 @@ class AltminInfo(C.Structure):
 class Request(C.Structure):
     _fields_ = [("options", C.c_char_p), ("E_elem", C.c_void_p), ("theta", C.c_void_p), ("thetaRef", C.c_void_p),
                 ("pressure", C.c_void_p), ("arrival", C.c_double), ("am_maxit", C.c_int),
-                ("U_out", C.c_void_p), ("V_out", C.c_void_p)]
+                ("U_out", C.c_void_p), ("V_out", C.c_void_p), ("opening_out", C.c_void_p),
+                ("energy_out", C.c_void_p)]
 
 
 class RequestResult(C.Structure):
@@ -83,6 +84,8 @@ def load():
             "vfh_altmin": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(AltminInfo)]),
             "vfh_energies": (C.c_int, [vp, dp]),
             "vfh_get_state": (C.c_int, [vp, vp, vp, vp]),
+            "vfh_crack_opening_field": (C.c_int, [vp, vp]),
+            "vfh_energy_fields": (C.c_int, [vp, vp]),
             "vfh_set_state": (C.c_int, [vp, vp, vp, vp]),
             "vfh_serve_opts_default": (None, [C.POINTER(ServeOpts)]),
             "vfh_request_bytes": (C.c_int64, [C.c_int, C.c_int, C.c_int]),
@@ -166,6 +169,18 @@ class Host:
         out = (C.c_double * 5)()
         self._c(self.L.vfh_energies(self.ctx, out))
         return dict(elastic=out[0], insitu=out[1], pressure_work=out[2], surface=out[3], crack_volume=out[4])
+
+    def crack_opening_field(self):
+        """Nodal crack opening, fields->VolCrackOpening (VFPermfield.c:75-146)."""
+        out = np.empty(self.nn)
+        self._c(self.L.vfh_crack_opening_field(self.ctx, out.ctypes.data))
+        return out
+
+    def energy_fields(self):
+        """Per-cell (elastic, pressure work, surface) energy terms, shape (N_e, 3)."""
+        out = np.empty(3 * self.ne)
+        self._c(self.L.vfh_energy_fields(self.ctx, out.ctypes.data))
+        return out.reshape(-1, 3)
 
     def state(self):
         U = np.empty(3 * self.nn)

@@ -334,6 +334,30 @@ int vfh_energies(vfh_ctx *c, double out[5]) {
   return VFG_OK;
 }
 
+int vfh_crack_opening_field(vfh_ctx *c, double *node) {
+  if (!c || !node) return fail(c, VFG_E_ARG, "vfh_crack_opening_field: NULL argument");
+  const int64_t ne = (int64_t)(c->o.grid.nx - 1) * (c->o.grid.ny - 1) * (c->o.grid.nz - 1);
+  double *d = NULL, vol = 0.;
+  CHK(c, vfg_device_alloc(8 * (size_t)(ne + c->nn), (void **)&d));
+  int rc = vfg_crack_volume(c->h, c->U, c->V, &vol, d, VFG_DEVICE);
+  if (!rc) rc = vfg_cell_to_node(c->h, d, d + ne, 1, VFG_DEVICE);
+  if (!rc) rc = vfg_synchronize(c->h);
+  if (!rc) rc = vfg_memcpy(node, d + ne, 8 * (size_t)c->nn, 1);
+  vfg_device_free(d);
+  return vchk(c, rc);
+}
+
+int vfh_energy_fields(vfh_ctx *c, double *cell3) {
+  if (!c || !cell3) return fail(c, VFG_E_ARG, "vfh_energy_fields: NULL argument");
+  const int64_t ne = (int64_t)(c->o.grid.nx - 1) * (c->o.grid.ny - 1) * (c->o.grid.nz - 1);
+  double *d = NULL;
+  CHK(c, vfg_device_alloc(24 * (size_t)ne, (void **)&d));
+  int rc = vfg_energy_fields(c->h, c->U, c->V, d, VFG_DEVICE);
+  if (!rc) rc = vfg_memcpy(cell3, d, 24 * (size_t)ne, 1);
+  vfg_device_free(d);
+  return vchk(c, rc);
+}
+
 int vfh_get_state(vfh_ctx *c, double *U, double *V, double *VIrrev) {
   if (!c) return fail(NULL, VFG_E_ARG, "NULL context");
   CHK(c, vfg_synchronize(c->h));
@@ -504,7 +528,9 @@ int vfh_serve(const vfh_serve_opts *so, const vfh_request *req, int nreq, vfh_re
         q->errV = dn[k].errV;
         q->device_bytes = vfg_device_bytes(c->h) + 80 * c->nn;
         if ((rc = vfh_energies(c, q->energies)) ||
-            ((req[i].U_out || req[i].V_out) && (rc = vfh_get_state(c, req[i].U_out, req[i].V_out, NULL)))) {
+            ((req[i].U_out || req[i].V_out) && (rc = vfh_get_state(c, req[i].U_out, req[i].V_out, NULL))) ||
+            (req[i].opening_out && (rc = vfh_crack_opening_field(c, req[i].opening_out))) ||
+            (req[i].energy_out && (rc = vfh_energy_fields(c, req[i].energy_out)))) {
           snprintf(g_err, sizeof g_err, "%s", c->err);
           break;
         }

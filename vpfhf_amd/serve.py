@@ -45,13 +45,15 @@ def arrival_times(n, rate, seed=0):
 class RequestSet:
     """Host payloads of a request stream (kept alive for the C loop)."""
 
-    def __init__(self, specs, ams, arrivals, outputs=False):
+    def __init__(self, specs, ams, arrivals, outputs=False, fields=False):
         self.specs, self.ams = list(specs), list(ams)
         n = len(self.specs)
         self.arr = (Request * n)()
         self.keep = []
         self.U = [None] * n
         self.V = [None] * n
+        self.opening = [None] * n
+        self.energy = [None] * n
         for i, (s, am, t) in enumerate(zip(self.specs, self.ams, arrivals)):
             f = workloads.build_fields(s)
             opt = workloads.options_text(s).encode()
@@ -69,16 +71,21 @@ class RequestSet:
                 self.V[i] = np.zeros(nn)
                 r.U_out = self.U[i].ctypes.data
                 r.V_out = self.V[i].ctypes.data
+            if fields:
+                self.opening[i] = np.zeros((s["n"] + 1) ** 3)
+                self.energy[i] = np.zeros((s["n"] ** 3, 3))
+                r.opening_out = self.opening[i].ctypes.data
+                r.energy_out = self.energy[i].ctypes.data
 
     def __len__(self):
         return len(self.specs)
 
 
-def make_requests(n, rate=None, seed=0, outputs=False, **kw):
+def make_requests(n, rate=None, seed=0, outputs=False, fields=False, **kw):
     """n requests of stream `seed`; rate None: all arrive at t = 0."""
     sa = [spec_request(i, seed, **kw) for i in range(n)]
     t = arrival_times(n, rate, seed) if rate else np.zeros(n)
-    return RequestSet([s for s, _ in sa], [a for _, a in sa], t, outputs)
+    return RequestSet([s for s, _ in sa], [a for _, a in sa], t, outputs, fields)
 
 
 def serve(reqs, max_slots=64, hbm_budget=200e9, mixed=True, realtime=True, u_rtol=1e-10, v_rtol=1e-8,
