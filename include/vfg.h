@@ -165,7 +165,11 @@ int vfg_penny_build_v(vfg_handle *h, double *V, const double center[3], double r
                       double theta_deg, double phi_deg, double thickness, int loc);
 
 /* ----------------------------------------------------------------- solvers */
-enum { VFG_PC_JACOBI = 0, VFG_PC_CHEBYSHEV = 1 };
+/* VFG_PC_MG (u-solve): geometric multigrid V-cycle in FP32, the matrix-free
+ * counterpart of the reference's PCHYPRE / GAMG preconditioners
+ * (VFCommon.c:1041-1047); grids whose element counts are not all even fall
+ * back to its coarsest-level smoother (Chebyshev(8)-Jacobi). */
+enum { VFG_PC_JACOBI = 0, VFG_PC_CHEBYSHEV = 1, VFG_PC_MG = 2 };
 enum { VFG_PREC_FP64 = 0, VFG_PREC_MIXED = 1 };
 typedef struct {
   double rtol;      /* relative residual tolerance (default 1e-10) */

@@ -14,7 +14,7 @@ NONE, ZERO, ONE, FIXED = 0, 1, 2, 3
 X0, X1, Y0, Y1, Z0, Z1 = range(6)
 ERRORS = {0: "VFG_OK", -1: "VFG_E_ARG", -2: "VFG_E_HIP", -3: "VFG_E_NOMEM", -4: "VFG_E_BREAKDOWN",
           -5: "VFG_E_MAXIT", -6: "VFG_E_NAN", -7: "VFG_E_BUDGET", -8: "VFG_E_NODEV"}
-PC_JACOBI, PC_CHEBYSHEV = 0, 1
+PC_JACOBI, PC_CHEBYSHEV, PC_MG = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
 
 

@@ -160,6 +160,7 @@ int vfg_destroy(vfg_handle *h) {
   if (h->stream) hipStreamSynchronize(h->stream);
   for (void *p : h->allocs) hipFree(p);
   if (h->stream) hipStreamDestroy(h->stream);
+  delete h->mg;
   delete h;
   return VFG_OK;
 }

@@ -4,6 +4,26 @@
 #include <vector>
 #include "vfg_kernels.h"
 
+namespace vfg {
+// Multigrid hierarchy of the u-operator (allocated once per handle, its
+// coefficients refreshed at every u-solve).  Level 0 is the problem itself;
+// level l+1 halves the element counts of level l.
+struct MgLevel {
+  DevProblem P;                     // level operator (level >= 1: E_elem = coefficient, V = 1, eta = 0)
+  int64_t n = 0;                    // 3 * nodes
+  float *dinv = nullptr;            // FP32 inverse diagonal
+  float *b = nullptr, *x = nullptr; // rhs / iterate (level 0: the caller's arrays)
+  float *y = nullptr, *d = nullptr; // A x, Chebyshev direction
+  double lmax = 0.;                 // upper bound of the D^-1 A spectrum used by the smoother
+};
+struct MgHier {
+  static constexpr int MAXL = 12;
+  int L = 0;
+  MgLevel lv[MAXL];
+  double *dtmp = nullptr;  // FP64 diagonal of level 1 (largest coarse level)
+};
+}  // namespace vfg
+
 struct vfg_handle {
   vfg::DevProblem P;
   vfg_grid grid;
@@ -22,6 +42,7 @@ struct vfg_handle {
   double *w[16] = {nullptr};
   float *wf[8] = {nullptr};
   uint8_t *act = nullptr;
+  vfg::MgHier *mg = nullptr;  // u-solve multigrid (VFG_PC_MG)
 };
 
 int vfg_fail(vfg_handle *h, int code, const char *fmt, ...);

@@ -65,6 +65,18 @@ void launch_vec_apply_dirichlet(const DevProblem &P, int which, double *vec, con
 int energy_blocks(const DevProblem &P);
 void launch_energies(const DevProblem &P, const double *U, const double *V, double *partials,
                      double *cellfield, double *efield, double *result4, hipStream_t s);
+// geometric multigrid pieces (vfg_mg.hip)
+void launch_mg_coef0(const DevProblem &Pf, const Grid &Gc, double *coef, hipStream_t s);
+void launch_mg_coarsen(const Grid &Gf, const double *cf, const Grid &Gc, double *cc, hipStream_t s);
+void launch_mg_restrict(const Grid &Gf, const Grid &Gc, const BcMask &bc, const float *b, const float *y,
+                        float *bcoarse, hipStream_t s);
+void launch_mg_prolong(const Grid &Gf, const Grid &Gc, const BcMask &bc, const float *xc, float *xf,
+                       hipStream_t s);
+void launch_mg_cheb(const float *b, const float *y, const float *dinv, float *d, float *x, float c1, float c2,
+                    int dfirst, int xzero, int64_t n, hipStream_t s);
+void launch_mg_recip32(const double *d, float *dinv, int64_t n, hipStream_t s);
+void launch_mg_fill32(float *x, float v, int64_t n, hipStream_t s);
+void launch_mg_seed(float *x, int64_t n, hipStream_t s);
 void launch_cell_to_node(const DevProblem &P, double lx, double ly, double lz, const double *cell, double *node,
                          int dof, hipStream_t s);
 void launch_penny(const DevProblem &P, double *V, const double c[3], double r, double theta,

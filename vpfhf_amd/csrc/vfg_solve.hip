@@ -16,6 +16,7 @@
 // few scalars every `check` iterations to test convergence.
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <string.h>
 #include <functional>
 #include "vfg_handle.h"
 
@@ -397,10 +398,17 @@ struct Cheb {
   double lmin, lmax;
 };
 
-// z = M^-1 r  (Jacobi, or Chebyshev(degree) polynomial in D^-1 A)
+static void mg_apply(vfg_handle *h, const float *r, float *z);
+
+// z = M^-1 r  (Jacobi, Chebyshev(degree) polynomial in D^-1 A, or, with
+// degree < 0, one multigrid V-cycle)
 static void precond32(vfg_handle *h, const std::function<void(const float *, float *)> &A, const float *dinv,
                       const float *r, float *z, float *d, float *t, int64_t n, const Cheb &cb) {
   hipStream_t s = h->stream;
+  if (cb.degree < 0) {
+    mg_apply(h, r, z);
+    return;
+  }
   if (cb.degree <= 1) {
     hipLaunchKernelGGL(k_jacobi32, dim3(nb_(n)), dim3(256), 0, s, dinv, r, z, n);
     return;
@@ -499,6 +507,229 @@ static int pcg32(vfg_handle *h, int64_t n, const std::function<void(const float 
 }
 
 
+// ---------------------------------------------------------------- multigrid
+// V-cycle: Chebyshev(MG_SMOOTH)-Jacobi pre- and post-smoothing on
+// [lmax/11, lmax] (lmax = 1.1 x the Lanczos estimate of D^-1 A), restriction
+// of the residual, recursion, prolongation; the coarsest level gets
+// Chebyshev(MG_COARSE_DEG) on [lmax/30, lmax].  All in FP32 with the level
+// operators of vfg_mg.hip.  Coarsening stops before a level would have fewer
+// than MG_MIN elements in some direction or an odd element count.
+constexpr int MG_SMOOTH = 2, MG_COARSE_DEG = 8, MG_MIN = 4;
+
+static int mg_build(vfg_handle *h) {
+  if (h->mg) return VFG_OK;
+  MgHier *m = new MgHier();
+  int rc = VFG_OK;
+  MgLevel &l0 = m->lv[0];
+  l0.P = h->P;
+  l0.n = 3 * h->P.nn;
+  rc = vfg_alloc(h, (void **)&l0.dinv, 4 * (size_t)l0.n);
+  if (!rc) rc = vfg_alloc(h, (void **)&l0.y, 4 * (size_t)l0.n);
+  if (!rc) rc = vfg_alloc(h, (void **)&l0.d, 4 * (size_t)l0.n);
+  int L = 1;
+  Grid G = h->P.G;
+  while (!rc && L < MgHier::MAXL && G.nex % 2 == 0 && G.ney % 2 == 0 && G.nez % 2 == 0 &&
+         G.nex / 2 >= MG_MIN && G.ney / 2 >= MG_MIN && G.nez / 2 >= MG_MIN) {
+    Grid Gc;
+    Gc.nex = G.nex / 2; Gc.ney = G.ney / 2; Gc.nez = G.nez / 2;
+    Gc.nx = Gc.nex + 1; Gc.ny = Gc.ney + 1; Gc.nz = Gc.nez + 1;
+    Gc.hx = 2. * G.hx; Gc.hy = 2. * G.hy; Gc.hz = 2. * G.hz;
+    MgLevel &lc = m->lv[L];
+    DevProblem &Pc = lc.P;
+    memset(&Pc, 0, sizeof Pc);
+    Pc.G = Gc;
+    Pc.nn = (int64_t)Gc.nx * Gc.ny * Gc.nz;
+    Pc.ne = (int64_t)Gc.nex * Gc.ney * Gc.nez;
+    lc.n = 3 * Pc.nn;
+    if ((rc = vfg_alloc(h, (void **)&Pc.E_elem, 8 * (size_t)Pc.ne)) ||
+        (rc = vfg_alloc(h, (void **)&Pc.V, 8 * (size_t)Pc.nn)) ||
+        (rc = vfg_alloc(h, (void **)&Pc.V32, 4 * (size_t)Pc.nn)))
+      break;
+    float **fv[5] = {&lc.dinv, &lc.b, &lc.x, &lc.y, &lc.d};
+    for (int i = 0; i < 5 && !rc; i++) rc = vfg_alloc(h, (void **)fv[i], 4 * (size_t)lc.n);
+    if (rc) break;
+    launch_fill(Pc.V, 1., Pc.nn, h->stream);
+    launch_mg_fill32(Pc.V32, 1.f, Pc.nn, h->stream);
+    G = Gc;
+    L++;
+  }
+  if (!rc && L > 1) rc = vfg_alloc(h, (void **)&m->dtmp, 8 * (size_t)m->lv[1].n);
+  if (rc) {
+    delete m;
+    return rc;
+  }
+  m->L = L;
+  h->mg = m;
+  return VFG_OK;
+}
+
+// Refreshes the hierarchy from the current problem (V, E, properties, BCs):
+// coarse coefficients, FP32 inverse diagonals and the smoother intervals.
+// dinv64: the problem's FP64 inverse diagonal.
+static int mg_setup(vfg_handle *h, const double *dinv64) {
+  int rc = mg_build(h);
+  if (rc) return rc;
+  MgHier &m = *h->mg;
+  hipStream_t s = h->stream;
+  const int64_t nf = 3 * h->P.nn;
+  for (int i = 0; i < 7; i++)
+    if (!h->wf[i] && (rc = vfg_alloc(h, (void **)&h->wf[i], 4 * (size_t)nf))) return rc;
+  m.lv[0].P = h->P;
+  launch_copy_d2f(dinv64, m.lv[0].dinv, nf, s);
+  for (int l = 1; l < m.L; l++) {
+    MgLevel &lc = m.lv[l], &lf = m.lv[l - 1];
+    DevProblem &Pc = lc.P;
+    Pc.E = 1.; Pc.nu = h->P.nu; Pc.lam1 = h->P.lam1; Pc.mu1 = h->P.mu1; Pc.eta = 0.;
+    Pc.bcU = h->P.bcU;
+    for (int c = 0; c < 3; c++) Pc.hbcU[c] = h->P.hbcU[c];
+    if (l == 1) launch_mg_coef0(h->P, Pc.G, Pc.E_elem, s);
+    else launch_mg_coarsen(lf.P.G, lf.P.E_elem, Pc.G, Pc.E_elem, s);
+    launch_u_diag(Pc, m.dtmp, s);
+    launch_mg_recip32(m.dtmp, lc.dinv, lc.n, s);
+  }
+  // lambda_max(D^-1 A) per level from 10 Jacobi-PCG steps (Lanczos)
+  float **w = h->wf;
+  for (int l = 0; l < m.L; l++) {
+    MgLevel &lv = m.lv[l];
+    launch_mg_seed(w[1], lv.n, s);
+    launch_mg_fill32(w[0], 0.f, lv.n, s);
+    std::function<void(const float *, float *)> A = [&](const float *in, float *out) {
+      launch_u_apply_f32(lv.P, in, out, s);
+    };
+    double lmax = 0.;
+    int st = 0;
+    pcg32(h, lv.n, A, lv.dinv, w[0], w[1], w[2], w[3], w[4], w[5], w[6], Cheb{1, 0., 0.}, 0., 10, &lmax, &st);
+    if (st == VFG_E_HIP || st == VFG_E_NAN || !(lmax > 0.))
+      return vfg_fail(h, st ? st : VFG_E_NAN, "multigrid set-up: eigenvalue estimate failed on level %d", l);
+    lv.lmax = 1.1 * lmax;
+  }
+  return VFG_OK;
+}
+
+static void mg_smooth(vfg_handle *h, MgLevel &L, int k, double lo, double hi, bool xzero) {
+  hipStream_t s = h->stream;
+  const double th = .5 * (hi + lo), de = .5 * (hi - lo), sg = th / de;
+  double rho = 1. / sg;
+  for (int i = 0; i < k; i++) {
+    const float *y = nullptr;
+    if (i > 0 || !xzero) {
+      launch_u_apply_f32(L.P, L.x, L.y, s);
+      y = L.y;
+    }
+    float c1 = 0.f, c2 = (float)(1. / th);
+    if (i > 0) {
+      const double rn = 1. / (2. * sg - rho);
+      c1 = (float)(rn * rho);
+      c2 = (float)(2. * rn / de);
+      rho = rn;
+    }
+    launch_mg_cheb(L.b, y, L.dinv, L.d, L.x, c1, c2, i == 0, i == 0 && xzero, L.n, s);
+  }
+}
+
+static void mg_vcycle(vfg_handle *h, int l) {
+  MgHier &m = *h->mg;
+  MgLevel &L = m.lv[l];
+  hipStream_t s = h->stream;
+  if (l == m.L - 1) {
+    mg_smooth(h, L, MG_COARSE_DEG, L.lmax / 30., L.lmax, true);
+    return;
+  }
+  MgLevel &C = m.lv[l + 1];
+  mg_smooth(h, L, MG_SMOOTH, L.lmax / 11., L.lmax, true);
+  launch_u_apply_f32(L.P, L.x, L.y, s);
+  launch_mg_restrict(L.P.G, C.P.G, L.P.bcU, L.b, L.y, C.b, s);
+  mg_vcycle(h, l + 1);
+  launch_mg_prolong(L.P.G, C.P.G, L.P.bcU, C.x, L.x, s);
+  mg_smooth(h, L, MG_SMOOTH, L.lmax / 11., L.lmax, false);
+}
+
+// z = V-cycle(r) on the problem's grid (r: FP32, zero on Dirichlet rows)
+static void mg_apply(vfg_handle *h, const float *r, float *z) {
+  MgLevel &L = h->mg->lv[0];
+  L.b = const_cast<float *>(r);
+  L.x = z;
+  mg_vcycle(h, 0);
+}
+
+// x += alpha p ; r -= alpha q ; r32 = r
+__global__ __launch_bounds__(RB) void k_mg_xr(double *__restrict__ x, double *__restrict__ r,
+                                              const double *__restrict__ p, const double *__restrict__ q,
+                                              float *__restrict__ r32, int64_t n, const double *sc) {
+  const double al = sc[S_ALPHA];
+  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB) {
+    x[i] += al * p[i];
+    const double ri = r[i] - al * q[i];
+    r[i] = ri;
+    r32[i] = (float)ri;
+  }
+}
+// z = z32 (p = z too when p != nullptr); partial sums of r.z and r.r
+__global__ __launch_bounds__(RB) void k_mg_rz(const double *__restrict__ r, const float *__restrict__ z32,
+                                              double *__restrict__ z, double *__restrict__ p, int64_t n,
+                                              double *part) {
+  double rz = 0., rr = 0.;
+  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB) {
+    const double ri = r[i], zi = (double)z32[i];
+    z[i] = zi;
+    if (p) p[i] = zi;
+    rz += ri * zi;
+    rr += ri * ri;
+  }
+  rz = bsum(rz);
+  rr = bsum(rr);
+  if (threadIdx.x == 0) { part[2 * blockIdx.x] = rz; part[2 * blockIdx.x + 1] = rr; }
+}
+
+// FP64 PCG with the FP32 V-cycle as preconditioner; same contract as pcg().
+static int pcg_mg(vfg_handle *h, int64_t n, const ApplyFn &A, double *x, double *r, double *z, double *p, double *q,
+                  float *r32, float *z32, double rtol, double atol, int maxit, int check, vfg_solve_info *info) {
+  hipStream_t s = h->stream;
+  double *sc = h->scal, *part = h->part;
+  const int nb = red_blocks(n);
+  launch_copy_d2f(r, r32, n, s);
+  mg_apply(h, r32, z32);
+  hipLaunchKernelGGL(k_mg_rz, dim3(nb), dim3(RB), 0, s, (const double *)r, (const float *)z32, z, p, n, part);
+  hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(RB), 0, s, (const double *)part, nb, 0, sc);
+  double hs[8];
+  if (hipMemcpyAsync(hs, sc, 64, hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+    return vfg_fail(h, VFG_E_HIP, "pcg: initial reduction failed");
+  const double rn0 = sqrt(hs[S_RR0]);
+  double rn = rn0;
+  int it = 0, status = VFG_OK;
+  const double tol = fmax(rtol * rn0, atol);
+  if (!isfinite(rn0)) status = VFG_E_NAN;
+  while (status == VFG_OK && rn > tol && it < maxit) {
+    int todo = check < maxit - it ? check : maxit - it;
+    for (int k = 0; k < todo; k++) {
+      A(p, q);
+      hipLaunchKernelGGL(k_dotp, dim3(nb), dim3(RB), 0, s, (const double *)p, (const double *)q, n, part);
+      hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(RB), 0, s, (const double *)part, nb, 1, sc);
+      hipLaunchKernelGGL(k_mg_xr, dim3(nb), dim3(RB), 0, s, x, r, (const double *)p, (const double *)q, r32, n,
+                         (const double *)sc);
+      mg_apply(h, r32, z32);
+      hipLaunchKernelGGL(k_mg_rz, dim3(nb), dim3(RB), 0, s, (const double *)r, (const float *)z32, z,
+                         (double *)nullptr, n, part);
+      hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(RB), 0, s, (const double *)part, nb, 2, sc);
+      hipLaunchKernelGGL(k_pcg_p, dim3(nb), dim3(RB), 0, s, p, (const double *)z, n, (const double *)sc);
+    }
+    it += todo;
+    if (hipMemcpyAsync(hs, sc, 64, hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+      return vfg_fail(h, VFG_E_HIP, "pcg: iteration failed");
+    rn = sqrt(hs[S_RR]);
+    if (hs[S_FLAG] == 2. || !isfinite(rn)) status = VFG_E_NAN;
+    else if (hs[S_FLAG] == 1.) status = rn <= tol ? VFG_OK : VFG_E_BREAKDOWN;
+    if (hs[S_FLAG] != 0.) break;
+  }
+  if (status == VFG_OK && rn > tol) status = VFG_E_MAXIT;
+  if (info) {
+    info->iterations += it;
+    info->relres = rn0 > 0. ? rn / rn0 : 0.;
+    info->status = status;
+  }
+  return status;
+}
+
 // FP64 iterative refinement around the FP32 inner PCG (precision form):
 // r64 = b - A64 x in FP64; solve A32 d = r64/|r64| in FP32 to inner_rtol;
 // x += |r64| d in FP64; repeat until |r64| <= rtol |r64_0|.
@@ -521,6 +752,10 @@ static int u_solve_mixed(vfg_handle *h, double *x, double *r64, double *t64, con
   Cheb cb = {1, 0., 0.};
   const double inner_rtol = 1e-4;
   int rc = VFG_OK;
+  if (o->pc == VFG_PC_MG) {
+    if ((rc = mg_setup(h, dinv64))) return rc;
+    cb.degree = -1;
+  }
   double rn0 = -1.;
   for (int outer = 0; outer < 60; outer++) {
     int e2 = 0;
@@ -550,7 +785,7 @@ static int u_solve_mixed(vfg_handle *h, double *x, double *r64, double *t64, con
     }
     int st = 0;
     int its = pcg32(h, n, A32, dinv, d, r, z, p, q, w1, w2, cb, inner_rtol, o->maxit, nullptr, &st);
-    inf->iterations += its * (cb.degree > 1 ? cb.degree : 1);
+    inf->iterations += its * (cb.degree > 1 ? cb.degree : 1);  // a V-cycle step counts once
     if (st == VFG_E_HIP || st == VFG_E_NAN) return st;
     hipLaunchKernelGGL(k_axpy3264, dim3(nb_(n)), dim3(256), 0, s, x, (const float *)d, rn, n);
     launch_u_rhs(P, x, B, r64, 1, s);  // F(x) in FP64
@@ -600,6 +835,11 @@ int vfg_u_solve(vfg_handle *h, double *U, const double *BCU, const vfg_solve_opt
     rc = u_solve_mixed(h, x, r, q, dinv, B, o, &inf);
   } else if (o->pc == VFG_PC_CHEBYSHEV) {
     rc = vfg_fail(h, VFG_E_ARG, "vfg_u_solve: the Chebyshev preconditioner is provided for the mixed-precision solve");
+  } else if (o->pc == VFG_PC_MG) {
+    ApplyFn A = [&](const double *in, double *out) { launch_u_apply(P, in, out, s); };
+    rc = mg_setup(h, dinv);
+    if (!rc) rc = pcg_mg(h, n, A, x, r, z, p, q, h->wf[0], h->wf[1], o->rtol, o->atol, o->maxit, 4, &inf);
+    inf.outer = 1;
   } else {
     // the u-operator is issue-bound at one wave per SIMD: fusing the direction
     // update into its staging, as the V-solve does, measured slower than the

@@ -170,7 +170,12 @@ int vfh_options_parse(vfh_options *o, const char *text) {
       o->npc = atoi(v);
       if (o->npc < 0 || o->npc > VFH_MAX_PENNY) rc = -1;
     } else if (ieq(k, "-vfg_precision") && v) o->precision = ieq(v, "mixed") ? VFG_PREC_MIXED : VFG_PREC_FP64;
-    else if (ieq(k, "-vfg_pc") && v) o->pc_type = ieq(v, "chebyshev") ? VFG_PC_CHEBYSHEV : VFG_PC_JACOBI;
+    else if (ieq(k, "-vfg_pc") && v)
+      o->pc_type = ieq(v, "chebyshev") ? VFG_PC_CHEBYSHEV : ieq(v, "mg") ? VFG_PC_MG : VFG_PC_JACOBI;
+    /* the reference's multigrid choices for the u-KSP (VFCommon.c:1041-1047,
+     * scripts/test16-*.sh) map to the matrix-free multigrid */
+    else if (ieq(k, "-U_pc_type") && v)
+      o->pc_type = (ieq(v, "hypre") || ieq(v, "gamg") || ieq(v, "mg") || ieq(v, "ml")) ? VFG_PC_MG : VFG_PC_JACOBI;
     else if (sscanf(k, "-pc%d_%63s", &pi, sub) == 2 && pi >= 0 && pi < VFH_MAX_PENNY) {
       vfh_penny *c = &o->pc[pi];
       if (ieq(sub, "center")) { if (parse_reals(v, c->center, 3) != 3) rc = -1; }
