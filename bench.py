@@ -97,7 +97,7 @@ def ensemble_bench(L, n_members=64, seed=0):
     """Batching form: 64 independent problems, sides log-uniform in [16, 64]
     elements, own lognormal E, 1-4 cracks, injection pressure.  The plain FP64
     GPU drop-in solves them one after another (vfg_u_solve, Jacobi-PCG to
-    1e-10); the batched form solves all at once (FP32 persistent apply inside
+    1e-10; also timed with the multigrid preconditioner); the batched form solves all at once (FP32 persistent apply inside
     FP64 refinement, graph-captured CG, autotuned tiles).  Each path runs once
     untimed, then timed from the same initial state."""
     import ctypes as C
@@ -131,6 +131,11 @@ def ensemble_bench(L, n_members=64, seed=0):
     reset(); single(); reset()
     t0 = time.time(); single(); t64 = time.time() - t0
     U64 = download()
+    o64.pc = _lib.PC_MG
+    reset(); single(); reset()
+    t0 = time.time(); single(); tmg = time.time() - t0
+    Umg = download()
+    o64.pc = _lib.PC_JACOBI
 
     bt = Batch(hosts)
     apply_ms = bt.autotune(fp32=True)
@@ -156,6 +161,8 @@ def ensemble_bench(L, n_members=64, seed=0):
                                                             [workloads.spec_ensemble_member(i, seed)["n"]
                                                              for i in range(n_members)]),
             "fp64_single_solves_per_s": round(n_members / t64, 2),
+            "fp64_single_multigrid_solves_per_s": round(n_members / tmg, 2),
+            "max_rel_err_multigrid_vs_jacobi": max(_relerr(a, b) for a, b in zip(Umg, U64)),
             "batched_fp64_solves_per_s": round(n_members / res[False][0], 2),
             "batched_mixed_solves_per_s": round(n_members / res[True][0], 2),
             "speedup_mixed_batched_vs_fp64_single": round(t64 / res[True][0], 2),
@@ -165,13 +172,13 @@ def ensemble_bench(L, n_members=64, seed=0):
 
 def precision_bench(n=256, seed=0):
     """Precision form at n^3 elements: one alternate-minimisation iteration with
-    the FP64 u-solve and with FP32 Chebyshev(4)-Jacobi PCG inside FP64
-    refinement (both to 1e-10); relative errors of the mixed result in U, V,
+    the FP64 u-solve (multigrid-preconditioned CG) and with FP32 CG under the
+    same V-cycle inside FP64 refinement (both to 1e-10); relative errors of the mixed result in U, V,
     crack volume and surface energy against the FP64 one."""
     from vpfhf_amd import workloads
     spec = workloads.spec_hetero(n, seed=seed)
     out = {}
-    for name, extra in (("fp64", ""), ("mixed", "-vfg_precision mixed -vfg_pc chebyshev")):
+    for name, extra in (("fp64", ""), ("mixed", "-vfg_precision mixed")):
         H, _ = workloads.make_host(spec, u_rtol=1e-10, extra=extra)
         t0 = time.time()
         info = H.altmin(maxit=1)
@@ -279,9 +286,21 @@ def main():
         info = H.altmin(maxit=1)
         dt = time.time() - t0
         en = H.energies()
-        extra["solves"] = {"u_cg_iterations": info.u_its, "v_newton": info.v_outer, "v_cg": info.v_its,
-                           "seconds_per_altmin_iteration": round(dt, 3), "solves_per_s": round(2.0 / dt, 3),
+        extra["solves"] = {"u_pc": "multigrid", "u_cg_iterations": info.u_its, "v_newton": info.v_outer,
+                           "v_cg": info.v_its, "seconds_per_altmin_iteration": round(dt, 3),
+                           "solves_per_s": round(2.0 / dt, 3),
                            "surface_energy": en["surface"], "crack_volume": en["crack_volume"]}
+        # the same iteration with the Jacobi-preconditioned u-CG
+        Hj, _ = workloads.make_host(spec, u_rtol=1e-10, extra="-vfg_pc jacobi")
+        t0 = time.time()
+        ij = Hj.altmin(maxit=1)
+        dtj = time.time() - t0
+        Uj, _, _ = Hj.state()
+        Um, _, _ = H.state()
+        Hj.close()
+        extra["solves"].update({"jacobi_u_cg_iterations": ij.u_its,
+                                "jacobi_seconds_per_altmin_iteration": round(dtj, 3),
+                                "rel_diff_U_multigrid_vs_jacobi": _relerr(Um, Uj)})
         # the V-Jacobian apply of the same problem (coupling set by the V-solve)
         vx = [V.DeviceBuffer(8 * nn).upload(rng.standard_normal(nn)) for _ in range(nbuf)]
         vy = [V.DeviceBuffer(8 * nn) for _ in range(nbuf)]

@@ -165,17 +165,17 @@ int vfg_penny_build_v(vfg_handle *h, double *V, const double center[3], double r
                       double theta_deg, double phi_deg, double thickness, int loc);
 
 /* ----------------------------------------------------------------- solvers */
-/* VFG_PC_MG (u-solve): geometric multigrid V-cycle in FP32, the matrix-free
- * counterpart of the reference's PCHYPRE / GAMG preconditioners
- * (VFCommon.c:1041-1047); grids whose element counts are not all even fall
- * back to its coarsest-level smoother (Chebyshev(8)-Jacobi). */
+/* VFG_PC_MG (u-solve, default): geometric multigrid V-cycle in FP32, the
+ * matrix-free counterpart of the reference's PCHYPRE / GAMG preconditioners
+ * (VFCommon.c:1041-1047); any grid (coarse levels take ceil(n/2) elements per
+ * direction down to 4). */
 enum { VFG_PC_JACOBI = 0, VFG_PC_CHEBYSHEV = 1, VFG_PC_MG = 2 };
 enum { VFG_PREC_FP64 = 0, VFG_PREC_MIXED = 1 };
 typedef struct {
   double rtol;      /* relative residual tolerance (default 1e-10) */
   double atol;      /* absolute residual tolerance (default 0) */
   int    maxit;     /* iteration cap (default 100000) */
-  int    pc;        /* VFG_PC_* */
+  int    pc;        /* VFG_PC_* (default VFG_PC_MG) */
   int    precision; /* VFG_PREC_* : mixed = FP32 inner solves inside FP64
                        iterative refinement */
   int    cheb_degree;

@@ -41,7 +41,9 @@ typedef struct {
   int       npc;           /* -npc */
   vfh_penny pc[VFH_MAX_PENNY]; /* -pc<i>_center -pc<i>_r -pc<i>_theta -pc<i>_phi -pc<i>_thickness */
   int       precision;     /* VFG_PREC_FP64 (default) or VFG_PREC_MIXED (-vfg_precision mixed) */
-  int       pc_type;       /* VFG_PC_JACOBI (default) or VFG_PC_CHEBYSHEV (-vfg_pc chebyshev) */
+  int       pc_type;       /* VFG_PC_MG (default; -U_pc_type hypre|gamg|mg, -vfg_pc mg),
+                              VFG_PC_JACOBI (-vfg_pc jacobi, -U_pc_type jacobi) or
+                              VFG_PC_CHEBYSHEV (-vfg_pc chebyshev, mixed precision) */
 } vfh_options;
 
 typedef struct vfh_ctx vfh_ctx;

@@ -3,8 +3,8 @@ of the reference's PCHYPRE / GAMG u-preconditioners (VFCommon.c:1041-1047).
 
 Bar: the same linear system as the Jacobi solve, so the solution agrees with
 the oracle's solve within 1e-8 relative (both to 1e-11..1e-12) and the FP64
-residual through VF_UResidual is at the solve tolerance; grids whose element
-counts cannot be halved run the coarsest-level smoother alone; the
+residual through VF_UResidual is at the solve tolerance, on nested (even) and
+non-nested (odd, flat) hierarchies; the
 mixed-precision solve with the V-cycle agrees with FP64 within 1e-7; the
 alternate-minimisation loop gives the same fields with -U_pc_type hypre as
 with Jacobi.  Iteration counts are checked against Jacobi's on the same system.
@@ -18,9 +18,11 @@ from tests.scenarios import relerr
 pytestmark = pytest.mark.gpu
 
 GRIDS = [
-    ((17, 17, 17), "clamped"),   # 16^3 -> 8^3 -> 4^3
-    ((17, 9, 13), "mixed"),      # 16x8x12 -> 8x4x6
-    ((11, 10, 9), "mixed"),      # odd counts: coarsest-level smoother only
+    ((17, 17, 17), "clamped"),   # 16^3 -> 8^3 -> 4^3 (nested)
+    ((17, 9, 13), "mixed"),      # 16x8x12 -> 8x4x6 -> 4x4x3
+    ((11, 10, 9), "mixed"),      # 10x9x8 -> 5x5x4 (non-nested)
+    ((26, 26, 26), "clamped"),   # 25^3 -> 13^3 -> 7^3 -> 4^3 (non-nested)
+    ((41, 21, 6), "test1"),      # 40x20x5 -> 20x10x5 -> 10x5x5 -> 5x5x5 (flat: z not coarsened)
     ((33, 33, 33), "test1"),     # 32^3 -> 16^3 -> 8^3 -> 4^3
 ]
 BCS = {"clamped": S.CLAMPED, "mixed": S.MIXED_U, "test1": S.TEST1_U}
@@ -37,7 +39,7 @@ def test_mg_u_solve_matches_oracle(oracle, n, bc):
     Uo = U0.copy()
     oracle.u_solve(op, F, Uo, BCU, rtol=1e-12)
     Um, im = P.VF_StepU(U0, BCU, rtol=1e-11, pc=_lib.PC_MG)
-    Uj, ij = P.VF_StepU(U0, BCU, rtol=1e-11)
+    Uj, ij = P.VF_StepU(U0, BCU, rtol=1e-11, pc=_lib.PC_JACOBI)
     assert im.status == 0 and ij.status == 0
     assert relerr(Um, Uo) < 1e-8, relerr(Um, Uo)
     r = oracle.u_residual(op, F, Um, BCU)
@@ -97,7 +99,7 @@ def test_mg_altmin_matches_jacobi():
     from vpfhf_amd import workloads
     spec = workloads.spec_penny32(beta=0.0)
     out = []
-    for extra in ("", "-U_pc_type hypre"):
+    for extra in ("-vfg_pc jacobi", "-U_pc_type hypre"):
         H, _ = workloads.make_host(spec, u_rtol=1e-11, extra=extra)
         info = H.altmin(maxit=2)
         U, V, _ = H.state()

@@ -258,7 +258,7 @@ class Problem:
 
     # -- solvers -------------------------------------------------------------
     @staticmethod
-    def opts(rtol=1e-10, atol=0.0, maxit=100000, pc=_lib.PC_JACOBI, precision=_lib.PREC_FP64, cheb_degree=4):
+    def opts(rtol=1e-10, atol=0.0, maxit=100000, pc=_lib.PC_MG, precision=_lib.PREC_FP64, cheb_degree=4):
         return _lib.SolveOpts(rtol, atol, maxit, pc, precision, cheb_degree)
 
     def VF_StepU(self, U, BCU, **kw):

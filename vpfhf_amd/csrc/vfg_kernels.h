@@ -66,7 +66,7 @@ int energy_blocks(const DevProblem &P);
 void launch_energies(const DevProblem &P, const double *U, const double *V, double *partials,
                      double *cellfield, double *efield, double *result4, hipStream_t s);
 // geometric multigrid pieces (vfg_mg.hip)
-void launch_mg_coef0(const DevProblem &Pf, const Grid &Gc, double *coef, hipStream_t s);
+void launch_mg_elem_coef(const DevProblem &P, double *out, hipStream_t s);
 void launch_mg_coarsen(const Grid &Gf, const double *cf, const Grid &Gc, double *cc, hipStream_t s);
 void launch_mg_restrict(const Grid &Gf, const Grid &Gc, const BcMask &bc, const float *b, const float *y,
                         float *bcoarse, hipStream_t s);

@@ -4,12 +4,12 @@
 // VFCommon.c:1041-1047) on the assembled Jacobian; its large runs use PETSc
 // GAMG with Chebyshev smoothers (-U_pc_type gamg, scripts/test16-*.sh).  On
 // the structured grid of this path the same role is played matrix-free by
-// geometric multigrid: 2:1 coarsening of the element grid, coarse operators
+// geometric multigrid: about 2:1 coarsening of the element grid, coarse operators
 // by rediscretisation, trilinear prolongation and its transpose.
 //
-//   level 1 coefficient of coarse element C: the mean over its 8 fine children
-//     of E_e * sbar_e, sbar_e = sum_g w_g (v_g^2 + eta) (weights summing to 1);
-//   level l+1 from level l: the mean over the 8 children;
+//   level 1 coefficient of coarse element C: the mean of E_e * sbar_e over the
+//     fine elements under C (weighted by overlap), sbar_e = sum_g w_g (v_g^2 +
+//     eta) (weights summing to 1); level l+1 from level l likewise;
 //   coarse operator: the Q1 elasticity operator with s = 1 and E = coefficient
 //     (the u-apply with V = 1, eta = 0), so every level runs the same kernels.
 // Dirichlet rows are excluded from restriction and prolongation on every
@@ -19,88 +19,139 @@
 
 namespace vfg {
 
+// Coarse grids cover the same box with ceil(n/2) elements per direction
+// (directions already at the minimum keep theirs), so the grids are nested
+// only where n is even.  Everything between two levels is built from the 1-D
+// map xi(i) = i * nc / nf of fine node i onto the coarse node axis:
+//   prolongation: trilinear interpolation at xi, weight 1 - |xi - I| for the
+//     two coarse nodes I bracketing xi;
+//   restriction: its transpose (the same weights, gathered per coarse node);
+//   coefficients: the overlap-weighted mean of the fine elements under each
+//     coarse element.
+__device__ __forceinline__ double mg_xi(int i, int nf, int nc) { return (double)i * (double)nc / (double)nf; }
+__device__ __forceinline__ float mg_w(double xi, int I) {
+  const double d = fabs(xi - (double)I);
+  return d < 1. ? (float)(1. - d) : 0.f;
+}
+
 // ------------------------------------------------------------ coefficients
-__global__ void k_mg_coef0(Grid G, const double *__restrict__ V, const double *__restrict__ E_elem, double E,
-                           double eta, Grid Gc, double *__restrict__ coef) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= (int64_t)Gc.nex * Gc.ney * Gc.nez) return;
-  const int ci = (int)(c % Gc.nex), cj = (int)((c / Gc.nex) % Gc.ney), ck = (int)(c / ((int64_t)Gc.nex * Gc.ney));
-  double acc = 0.;
-  for (int ch = 0; ch < 8; ch++) {
-    const int ei = 2 * ci + (ch & 1), ej = 2 * cj + ((ch >> 1) & 1), ek = 2 * ck + (ch >> 2);
-    double v[8];
-    for (int a = 0; a < 8; a++) v[a] = V[node_id(G, ei + (a & 1), ej + ((a >> 1) & 1), ek + (a >> 2))];
-    double s = 0.;
-    for (int gz = 0; gz < 3; gz++)
-      for (int gy = 0; gy < 3; gy++)
-        for (int gx = 0; gx < 3; gx++) {
-          double vg = 0.;
-          for (int a = 0; a < 8; a++)
-            vg += ((a & 1) ? gP1(gx) : gP0(gx)) * (((a >> 1) & 1) ? gP1(gy) : gP0(gy)) *
-                  ((a >> 2) ? gP1(gz) : gP0(gz)) * v[a];
-          s += gW(gx) * gW(gy) * gW(gz) * (vg * vg + eta);
-        }
-    acc += (E_elem ? E_elem[cell_id(G, ei, ej, ek)] : E) * s;
+// per fine element: E_e * sbar_e, sbar_e = sum_g w_g (v_g^2 + eta)
+__global__ void k_mg_elem_coef(Grid G, const double *__restrict__ V, const double *__restrict__ E_elem, double E,
+                               double eta, double *__restrict__ out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)G.nex * G.ney * G.nez) return;
+  const int ei = (int)(e % G.nex), ej = (int)((e / G.nex) % G.ney), ek = (int)(e / ((int64_t)G.nex * G.ney));
+  double v[8];
+  for (int a = 0; a < 8; a++) v[a] = V[node_id(G, ei + (a & 1), ej + ((a >> 1) & 1), ek + (a >> 2))];
+  double s = 0.;
+  for (int gz = 0; gz < 3; gz++)
+    for (int gy = 0; gy < 3; gy++)
+      for (int gx = 0; gx < 3; gx++) {
+        double vg = 0.;
+        for (int a = 0; a < 8; a++)
+          vg += ((a & 1) ? gP1(gx) : gP0(gx)) * (((a >> 1) & 1) ? gP1(gy) : gP0(gy)) * ((a >> 2) ? gP1(gz) : gP0(gz)) *
+                v[a];
+        s += gW(gx) * gW(gy) * gW(gz) * (vg * vg + eta);
+      }
+  out[e] = (E_elem ? E_elem[e] : E) * s;
+}
+
+// fine elements e overlapping coarse element C along one axis, with overlaps
+__device__ __forceinline__ int mg_overlap(int C, int nf, int nc, int *e0, double *ov) {
+  const double a = (double)C * nf / nc, b = (double)(C + 1) * nf / nc;
+  const int lo = (int)floor(a);
+  int m = 0;
+  for (int e = lo; e < nf && (double)e < b && m < 4; e++) {
+    const double o = fmin((double)(e + 1), b) - fmax((double)e, a);
+    if (o > 0.) {
+      if (m == 0) *e0 = e;
+      ov[m++] = o;
+    }
   }
-  coef[c] = acc * .125;
+  return m;
 }
 
 __global__ void k_mg_coarsen(Grid Gf, const double *__restrict__ cf, Grid Gc, double *__restrict__ cc) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= (int64_t)Gc.nex * Gc.ney * Gc.nez) return;
   const int ci = (int)(c % Gc.nex), cj = (int)((c / Gc.nex) % Gc.ney), ck = (int)(c / ((int64_t)Gc.nex * Gc.ney));
-  double acc = 0.;
-  for (int ch = 0; ch < 8; ch++)
-    acc += cf[cell_id(Gf, 2 * ci + (ch & 1), 2 * cj + ((ch >> 1) & 1), 2 * ck + (ch >> 2))];
-  cc[c] = acc * .125;
+  int x0 = 0, y0 = 0, z0 = 0;
+  double ox[4], oy[4], oz[4];
+  const int mx = mg_overlap(ci, Gf.nex, Gc.nex, &x0, ox);
+  const int my = mg_overlap(cj, Gf.ney, Gc.ney, &y0, oy);
+  const int mz = mg_overlap(ck, Gf.nez, Gc.nez, &z0, oz);
+  double acc = 0., wsum = 0.;
+  for (int k = 0; k < mz; k++)
+    for (int j = 0; j < my; j++)
+      for (int i = 0; i < mx; i++) {
+        const double w = ox[i] * oy[j] * oz[k];
+        acc += w * cf[cell_id(Gf, x0 + i, y0 + j, z0 + k)];
+        wsum += w;
+      }
+  cc[c] = acc / wsum;
 }
 
 // ------------------------------------------------- restriction / prolongation
-// coarse rhs = P^T (b - y) on the free fine rows; 27 fine nodes per coarse node
-// with weights (1/2, 1, 1/2) per direction; zero on coarse Dirichlet rows
+// fine nodes i of one axis that coarse node I gathers from, with weights
+__device__ __forceinline__ int mg_gather(int I, int nf, int nc, int *i0, float *w) {
+  const int lo = max(0, (int)floor((double)(I - 1) * nf / nc));
+  const int hi = min(nf, (int)ceil((double)(I + 1) * nf / nc));
+  int m = 0;
+  for (int i = lo; i <= hi && m < 6; i++) {
+    const float wi = mg_w(mg_xi(i, nf, nc), I);
+    if (wi > 0.f) {
+      if (m == 0) *i0 = i;
+      w[m++] = wi;
+    }
+  }
+  return m;
+}
+
+// coarse rhs = P^T (b - y) over the free fine rows; zero on coarse Dirichlet rows
 __global__ __launch_bounds__(256) void k_mg_restrict(Grid Gf, Grid Gc, BcMask bc, const float *__restrict__ b,
                                                      const float *__restrict__ y, float *__restrict__ bcoarse) {
   const int64_t N = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (N >= (int64_t)Gc.nx * Gc.ny * Gc.nz) return;
   const int I = (int)(N % Gc.nx), J = (int)((N / Gc.nx) % Gc.ny), K = (int)(N / ((int64_t)Gc.nx * Gc.ny));
+  int i0 = 0, j0 = 0, k0 = 0;
+  float wx[6], wy[6], wz[6];
+  const int mx = mg_gather(I, Gf.nex, Gc.nex, &i0, wx);
+  const int my = mg_gather(J, Gf.ney, Gc.ney, &j0, wy);
+  const int mz = mg_gather(K, Gf.nez, Gc.nez, &k0, wz);
   float acc[3] = {0.f, 0.f, 0.f};
-  for (int dk = -1; dk <= 1; dk++) {
-    const int k = 2 * K + dk;
-    if (k < 0 || k >= Gf.nz) continue;
-    for (int dj = -1; dj <= 1; dj++) {
-      const int j = 2 * J + dj;
-      if (j < 0 || j >= Gf.ny) continue;
-      for (int di = -1; di <= 1; di++) {
-        const int i = 2 * I + di;
-        if (i < 0 || i >= Gf.nx) continue;
-        const float w = (di ? .5f : 1.f) * (dj ? .5f : 1.f) * (dk ? .5f : 1.f);
+  for (int kk = 0; kk < mz; kk++)
+    for (int jj = 0; jj < my; jj++)
+      for (int ii = 0; ii < mx; ii++) {
+        const int i = i0 + ii, j = j0 + jj, k = k0 + kk;
+        const float w = wx[ii] * wy[jj] * wz[kk];
         const int64_t n = node_id(Gf, i, j, k);
         for (int c = 0; c < 3; c++) {
           if (on_dirichlet(Gf, bc.face[c], bc.vert[c], i, j, k)) continue;
           acc[c] += w * (b[3 * n + c] - y[3 * n + c]);
         }
       }
-    }
-  }
   for (int c = 0; c < 3; c++)
     bcoarse[3 * N + c] = on_dirichlet(Gc, bc.face[c], bc.vert[c], I, J, K) ? 0.f : acc[c];
 }
 
-// x_fine += P x_coarse on the free fine rows (trilinear interpolation)
+// x_fine += P x_coarse on the free fine rows
 __global__ __launch_bounds__(256) void k_mg_prolong(Grid Gf, Grid Gc, BcMask bc, const float *__restrict__ xc,
                                                     float *__restrict__ xf) {
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= (int64_t)Gf.nx * Gf.ny * Gf.nz) return;
   const int i = (int)(n % Gf.nx), j = (int)((n / Gf.nx) % Gf.ny), k = (int)(n / ((int64_t)Gf.nx * Gf.ny));
-  const int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
-  const int ni = (i & 1) ? 2 : 1, nj = (j & 1) ? 2 : 1, nk = (k & 1) ? 2 : 1;
-  const float wi = (i & 1) ? .5f : 1.f, wj = (j & 1) ? .5f : 1.f, wk = (k & 1) ? .5f : 1.f;
+  const double xi = mg_xi(i, Gf.nex, Gc.nex), yj = mg_xi(j, Gf.ney, Gc.ney), zk = mg_xi(k, Gf.nez, Gc.nez);
+  const int I0 = min((int)floor(xi), Gc.nex - 1), J0 = min((int)floor(yj), Gc.ney - 1),
+            K0 = min((int)floor(zk), Gc.nez - 1);
+  const float wx[2] = {mg_w(xi, I0), mg_w(xi, I0 + 1)}, wy[2] = {mg_w(yj, J0), mg_w(yj, J0 + 1)},
+              wz[2] = {mg_w(zk, K0), mg_w(zk, K0 + 1)};
   float acc[3] = {0.f, 0.f, 0.f};
-  for (int c3 = 0; c3 < nk; c3++)
-    for (int c2 = 0; c2 < nj; c2++)
-      for (int c1 = 0; c1 < ni; c1++) {
-        const int64_t N = node_id(Gc, i0 + c1, j0 + c2, k0 + c3);
-        const float w = wi * wj * wk;
+  for (int c3 = 0; c3 < 2; c3++)
+    for (int c2 = 0; c2 < 2; c2++)
+      for (int c1 = 0; c1 < 2; c1++) {
+        const float w = wx[c1] * wy[c2] * wz[c3];
+        if (w == 0.f) continue;
+        const int64_t N = node_id(Gc, I0 + c1, J0 + c2, K0 + c3);
         for (int c = 0; c < 3; c++) acc[c] += w * xc[3 * N + c];
       }
   for (int c = 0; c < 3; c++)
@@ -146,10 +197,9 @@ __global__ void k_mg_seed(float *x, int64_t n) {
 
 static unsigned nbm(int64_t n) { return (unsigned)((n + 255) / 256); }
 
-void launch_mg_coef0(const DevProblem &Pf, const Grid &Gc, double *coef, hipStream_t s) {
-  const int64_t ne = (int64_t)Gc.nex * Gc.ney * Gc.nez;
-  hipLaunchKernelGGL(k_mg_coef0, dim3(nbm(ne)), dim3(256), 0, s, Pf.G, (const double *)Pf.V,
-                     (const double *)Pf.E_elem, Pf.E, Pf.eta, Gc, coef);
+void launch_mg_elem_coef(const DevProblem &P, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_mg_elem_coef, dim3(nbm(P.ne)), dim3(256), 0, s, P.G, (const double *)P.V,
+                     (const double *)P.E_elem, P.E, P.eta, out);
 }
 void launch_mg_coarsen(const Grid &Gf, const double *cf, const Grid &Gc, double *cc, hipStream_t s) {
   const int64_t ne = (int64_t)Gc.nex * Gc.ney * Gc.nez;

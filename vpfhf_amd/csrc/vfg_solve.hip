@@ -512,8 +512,9 @@ static int pcg32(vfg_handle *h, int64_t n, const std::function<void(const float 
 // [lmax/11, lmax] (lmax = 1.1 x the Lanczos estimate of D^-1 A), restriction
 // of the residual, recursion, prolongation; the coarsest level gets
 // Chebyshev(MG_COARSE_DEG) on [lmax/30, lmax].  All in FP32 with the level
-// operators of vfg_mg.hip.  Coarsening stops before a level would have fewer
-// than MG_MIN elements in some direction or an odd element count.
+// operators of vfg_mg.hip.  Each level has ceil(n/2) elements per direction
+// where that leaves at least MG_MIN, else n; coarsening stops when no
+// direction changes.
 constexpr int MG_SMOOTH = 2, MG_COARSE_DEG = 8, MG_MIN = 4;
 
 static int mg_build(vfg_handle *h) {
@@ -528,12 +529,13 @@ static int mg_build(vfg_handle *h) {
   if (!rc) rc = vfg_alloc(h, (void **)&l0.d, 4 * (size_t)l0.n);
   int L = 1;
   Grid G = h->P.G;
-  while (!rc && L < MgHier::MAXL && G.nex % 2 == 0 && G.ney % 2 == 0 && G.nez % 2 == 0 &&
-         G.nex / 2 >= MG_MIN && G.ney / 2 >= MG_MIN && G.nez / 2 >= MG_MIN) {
+  auto halve = [](int n) { return (n + 1) / 2 >= MG_MIN ? (n + 1) / 2 : n; };
+  while (!rc && L < MgHier::MAXL) {
     Grid Gc;
-    Gc.nex = G.nex / 2; Gc.ney = G.ney / 2; Gc.nez = G.nez / 2;
+    Gc.nex = halve(G.nex); Gc.ney = halve(G.ney); Gc.nez = halve(G.nez);
+    if (Gc.nex == G.nex && Gc.ney == G.ney && Gc.nez == G.nez) break;
     Gc.nx = Gc.nex + 1; Gc.ny = Gc.ney + 1; Gc.nz = Gc.nez + 1;
-    Gc.hx = 2. * G.hx; Gc.hy = 2. * G.hy; Gc.hz = 2. * G.hz;
+    Gc.hx = G.hx * G.nex / Gc.nex; Gc.hy = G.hy * G.ney / Gc.ney; Gc.hz = G.hz * G.nez / Gc.nez;
     MgLevel &lc = m->lv[L];
     DevProblem &Pc = lc.P;
     memset(&Pc, 0, sizeof Pc);
@@ -574,6 +576,7 @@ static int mg_setup(vfg_handle *h, const double *dinv64) {
   const int64_t nf = 3 * h->P.nn;
   for (int i = 0; i < 7; i++)
     if (!h->wf[i] && (rc = vfg_alloc(h, (void **)&h->wf[i], 4 * (size_t)nf))) return rc;
+  if (m.L > 1 && (rc = work(h, 8, nf))) return rc;  // w[7]: per-element coefficients of the problem
   m.lv[0].P = h->P;
   launch_copy_d2f(dinv64, m.lv[0].dinv, nf, s);
   for (int l = 1; l < m.L; l++) {
@@ -582,8 +585,12 @@ static int mg_setup(vfg_handle *h, const double *dinv64) {
     Pc.E = 1.; Pc.nu = h->P.nu; Pc.lam1 = h->P.lam1; Pc.mu1 = h->P.mu1; Pc.eta = 0.;
     Pc.bcU = h->P.bcU;
     for (int c = 0; c < 3; c++) Pc.hbcU[c] = h->P.hbcU[c];
-    if (l == 1) launch_mg_coef0(h->P, Pc.G, Pc.E_elem, s);
-    else launch_mg_coarsen(lf.P.G, lf.P.E_elem, Pc.G, Pc.E_elem, s);
+    if (l == 1) {
+      launch_mg_elem_coef(h->P, h->w[7], s);
+      launch_mg_coarsen(h->P.G, h->w[7], Pc.G, Pc.E_elem, s);
+    } else {
+      launch_mg_coarsen(lf.P.G, lf.P.E_elem, Pc.G, Pc.E_elem, s);
+    }
     launch_u_diag(Pc, m.dtmp, s);
     launch_mg_recip32(m.dtmp, lc.dinv, lc.n, s);
   }
@@ -806,7 +813,7 @@ void vfg_solve_opts_default(vfg_solve_opts *o) {
   o->rtol = 1e-10;
   o->atol = 0.;
   o->maxit = 100000;
-  o->pc = VFG_PC_JACOBI;
+  o->pc = VFG_PC_MG;  // the reference preconditions its u-CG with multigrid (PCHYPRE)
   o->precision = VFG_PREC_FP64;
   o->cheb_degree = 4;
 }

@@ -60,7 +60,7 @@ void vfh_options_default(vfh_options *o) {
   o->u_rtol = 1e-8;
   o->v_rtol = 1e-8;
   o->precision = VFG_PREC_FP64;
-  o->pc_type = VFG_PC_JACOBI;
+  o->pc_type = VFG_PC_MG;
 }
 
 static int ieq(const char *a, const char *b) {
