@@ -170,6 +170,7 @@ int vfg_set_props(vfg_handle *h, const vfg_props *pr) {
   if (!check_props(pr)) return vfg_fail(h, VFG_E_ARG, "vfg_set_props: invalid properties");
   set_props_(h, pr);
   launch_v_fixmask(h->P, h->stream);
+  if (h->mg) h->mg->setups = 0;  // the multigrid smoother bounds depend on nu
   return VFG_OK;
 }
 
@@ -183,6 +184,7 @@ int vfg_set_bc(vfg_handle *h, const vfg_bc u_bc[3], const vfg_bc *v_bc) {
   if (u_bc) {
     for (int c = 0; c < 3; c++) h->P.hbcU[c] = u_bc[c];
     bcmask_from(u_bc, 3, &h->P.bcU);
+    if (h->mg) h->mg->setups = 0;  // and on the Dirichlet rows
   }
   if (v_bc) {
     h->P.hbcV = *v_bc;

@@ -20,6 +20,7 @@ struct MgHier {
   static constexpr int MAXL = 12;
   int L = 0;
   MgLevel lv[MAXL];
+  int setups = 0;          // smoother bounds are re-estimated on every 16th set-up
   double *dtmp = nullptr;  // FP64 diagonal of level 1 (largest coarse level)
 };
 }  // namespace vfg

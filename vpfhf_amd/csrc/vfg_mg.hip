@@ -107,31 +107,37 @@ __device__ __forceinline__ int mg_gather(int I, int nf, int nc, int *i0, float *
   return m;
 }
 
-// coarse rhs = P^T (b - y) over the free fine rows; zero on coarse Dirichlet rows
+// coarse rhs = P^T (b - y) over the free fine rows; zero on coarse Dirichlet
+// rows.  One thread per coarse (node, component).
 __global__ __launch_bounds__(256) void k_mg_restrict(Grid Gf, Grid Gc, BcMask bc, const float *__restrict__ b,
                                                      const float *__restrict__ y, float *__restrict__ bcoarse) {
-  const int64_t N = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (N >= (int64_t)Gc.nx * Gc.ny * Gc.nz) return;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * (int64_t)Gc.nx * Gc.ny * Gc.nz) return;
+  const int64_t N = t / 3;
+  const int c = (int)(t - 3 * N);
   const int I = (int)(N % Gc.nx), J = (int)((N / Gc.nx) % Gc.ny), K = (int)(N / ((int64_t)Gc.nx * Gc.ny));
+  if (on_dirichlet(Gc, bc.face[c], bc.vert[c], I, J, K)) {
+    bcoarse[t] = 0.f;
+    return;
+  }
   int i0 = 0, j0 = 0, k0 = 0;
   float wx[6], wy[6], wz[6];
   const int mx = mg_gather(I, Gf.nex, Gc.nex, &i0, wx);
   const int my = mg_gather(J, Gf.ney, Gc.ney, &j0, wy);
   const int mz = mg_gather(K, Gf.nez, Gc.nez, &k0, wz);
-  float acc[3] = {0.f, 0.f, 0.f};
+  float acc = 0.f;
   for (int kk = 0; kk < mz; kk++)
-    for (int jj = 0; jj < my; jj++)
+    for (int jj = 0; jj < my; jj++) {
+      float row = 0.f;
       for (int ii = 0; ii < mx; ii++) {
         const int i = i0 + ii, j = j0 + jj, k = k0 + kk;
-        const float w = wx[ii] * wy[jj] * wz[kk];
-        const int64_t n = node_id(Gf, i, j, k);
-        for (int c = 0; c < 3; c++) {
-          if (on_dirichlet(Gf, bc.face[c], bc.vert[c], i, j, k)) continue;
-          acc[c] += w * (b[3 * n + c] - y[3 * n + c]);
-        }
+        if (on_dirichlet(Gf, bc.face[c], bc.vert[c], i, j, k)) continue;
+        const int64_t n = 3 * node_id(Gf, i, j, k) + c;
+        row += wx[ii] * (b[n] - y[n]);
       }
-  for (int c = 0; c < 3; c++)
-    bcoarse[3 * N + c] = on_dirichlet(Gc, bc.face[c], bc.vert[c], I, J, K) ? 0.f : acc[c];
+      acc += wy[jj] * wz[kk] * row;
+    }
+  bcoarse[t] = acc;
 }
 
 // x_fine += P x_coarse on the free fine rows
@@ -207,8 +213,8 @@ void launch_mg_coarsen(const Grid &Gf, const double *cf, const Grid &Gc, double 
 }
 void launch_mg_restrict(const Grid &Gf, const Grid &Gc, const BcMask &bc, const float *b, const float *y,
                         float *bcoarse, hipStream_t s) {
-  const int64_t nn = (int64_t)Gc.nx * Gc.ny * Gc.nz;
-  hipLaunchKernelGGL(k_mg_restrict, dim3(nbm(nn)), dim3(256), 0, s, Gf, Gc, bc, b, y, bcoarse);
+  const int64_t n3 = 3 * (int64_t)Gc.nx * Gc.ny * Gc.nz;
+  hipLaunchKernelGGL(k_mg_restrict, dim3(nbm(n3)), dim3(256), 0, s, Gf, Gc, bc, b, y, bcoarse);
 }
 void launch_mg_prolong(const Grid &Gf, const Grid &Gc, const BcMask &bc, const float *xc, float *xf,
                        hipStream_t s) {

