@@ -63,7 +63,15 @@ typedef struct {
   double eta, epsilon, irrevtol;
   int    atnum;            /* 1: AT1, 2: AT2 */
   int    hasCrackPressure; /* -pressurize */
+  int    unilateral;       /* VFG_UNILATERAL_* (-unilateral, VFCommon.h:120-123) */
 } vfg_props;
+/* UNILATERAL_NOCOMPRESSION: where tr(eps(U)) < 0 the spherical part of the
+ * u-operator and of the thermo-poro load is not degraded, and the crack is
+ * not driven by compressive volumetric energy (VFMech.c:115-218, 299-376,
+ * 456-524, 819-908, 2125-2153).  The u-problem is then nonlinear: its
+ * Jacobian depends on the state set by vfg_u_set_state, and vfg_u_solve runs
+ * Newton iterations. */
+enum { VFG_UNILATERAL_NONE = 0, VFG_UNILATERAL_NOCOMPRESSION = 1 };
 
 /* VFBC (VFCartFE.h:129-136); edges are not parsed by the reference and are
  * absent here.  vertexValue is read for FIXED vertices. */
@@ -102,6 +110,11 @@ int vfg_synchronize(vfg_handle *h);
  * (VFMech.c:1897-2029) then MatApplyDirichletBC (VFCartFE.c:1147-1296):
  * y = K x on free rows, y_i = x_i on Dirichlet rows (columns kept). */
 int vfg_u_apply(vfg_handle *h, const double *x, double *y, int loc);
+/* VF_UIJacobian(U) under UNILATERAL_NOCOMPRESSION (VFMech.c:1942-2023): sets
+ * the state U (3*N_n) whose tension/compression pattern the u-Jacobian
+ * (vfg_u_apply, vfg_u_diag and their FP32 twins) uses; the pattern of U = 0
+ * (all tensile) until called.  No effect for UNILATERAL_NONE. */
+int vfg_u_set_state(vfg_handle *h, const double *U, int loc);
 int vfg_u_apply_f32(vfg_handle *h, const float *x, float *y, int loc);
 /* MATOP_GET_DIAGONAL of the same matrix (1 on Dirichlet rows) */
 int vfg_u_diag(vfg_handle *h, double *d, int loc);
@@ -189,7 +202,10 @@ typedef struct {
 void vfg_solve_opts_default(vfg_solve_opts *o);
 /* VF_StepU (VFMech.c:1436-1451): one Newton step of SNESNEWTONLS with KSPCG
  * on the linear u-problem.  U must carry its Dirichlet values on entry
- * (VFTimeStepPrepare, VFCommon.c:1132-1157). */
+ * (VFTimeStepPrepare, VFCommon.c:1132-1157).  UNILATERAL_NOCOMPRESSION:
+ * Newton iterations (Jacobian at each iterate, FP64 CG under the selected
+ * preconditioner, backtracking on ||F||) until ||F|| <= rtol ||F(U0)||;
+ * info->outer counts them, precision is ignored. */
 int vfg_u_solve(vfg_handle *h, double *U, const double *BCU, const vfg_solve_opts *o,
                 vfg_solve_info *info, int loc);
 /* VF_StepV (VFMech.c:3057-3082) with SNESVINEWTONRSLS semantics: bounds
@@ -203,7 +219,8 @@ int vfg_v_solve(vfg_handle *h, const double *U, double *V, const double *ub,
  * runs that u-solve for every member at once: one persistent launch applies
  * every member's operator, CG iterations of all members are captured in a
  * HIP graph, and each member keeps its own scalars, so its result does not
- * depend on the other members.  Members must outlive the batch. */
+ * depend on the other members.  Members must outlive the batch and be
+ * UNILATERAL_NONE problems (VFG_E_ARG otherwise). */
 typedef struct vfg_batch vfg_batch;
 int vfg_batch_create(vfg_handle *const *members, int n, vfg_batch **out);
 int vfg_batch_destroy(vfg_batch *b);
@@ -260,7 +277,7 @@ int  vfg_engine_destroy(vfg_engine *e);
 const char *vfg_engine_last_error(const vfg_engine *e);
 int  vfg_engine_free_slots(const vfg_engine *e);
 /* VIrrev is the AT1 upper bound (ignored for AT2); returns VFG_E_BUDGET when
- * no slot is free */
+ * no slot is free, VFG_E_ARG for a UNILATERAL_NOCOMPRESSION problem */
 int  vfg_engine_submit(vfg_engine *e, vfg_handle *h, double *U, const double *BCU, double *V,
                        const double *VIrrev, int am_maxit, double altmintol, int64_t tag);
 /* one scheduling step; returns the number of resident requests (>= 0) */

@@ -162,6 +162,118 @@ void vfo_elastic_energy_density(double W[27], const double u24[24], const double
   }
 }
 
+/* tr(eps(u)) at the 27 Gauss points, accumulated node by node as the
+ * reference does (VFMech.c:320-326) */
+static void trace_eps(double tr[27], const double u24[24], const vfo_elem *e)
+{
+  for (int g = 0; g < 27; g++) tr[g] = 0.;
+  for (int a = 0; a < 8; a++)
+    for (int g = 0; g < 27; g++)
+      tr[g] += DPHI(e, a, 0, g) * u24[3 * a] + DPHI(e, a, 1, g) * u24[3 * a + 1] +
+               DPHI(e, a, 2, g) * u24[3 * a + 2];
+}
+
+/* VF_BilinearFormUNoCompression3D_local: VFMech.c:299-376 */
+void vfo_bilinear_u_nocomp(double *Ke, const double u24[24], const double v8[8], double lambda,
+                           double mu, double eta, const vfo_elem *e)
+{
+  const double kappa = lambda + 2. * mu / 3.;
+  double s[27], tr[27];
+  trace_eps(tr, u24, e);
+  for (int g = 0; g < 27; g++) s[g] = 0.;
+  for (int a = 0; a < 8; a++)
+    for (int g = 0; g < 27; g++) s[g] += v8[a] * PHI(e, a, g);
+  for (int g = 0; g < 27; g++) s[g] = s[g] * s[g] + eta;
+  for (int l = 0; l < 576; l++) Ke[l] = 0.;
+  int l = 0;
+  for (int a = 0; a < 8; a++)
+    for (int c1 = 0; c1 < 3; c1++)
+      for (int b = 0; b < 8; b++)
+        for (int c2 = 0; c2 < 3; c2++, l++)
+          for (int g = 0; g < 27; g++) {
+            const double d12 = DPHI(e, a, c1, g) * DPHI(e, b, c2, g);
+            double mg = tr[g] < 0. ? kappa * (1. + eta) * d12 : kappa * s[g] * d12;
+            mg -= 2. * mu * s[g] * d12 / 3.;
+            mg += mu * s[g] * DPHI(e, a, c2, g) * DPHI(e, b, c1, g);
+            if (c1 == c2)
+              mg += mu * s[g] *
+                    (DPHI(e, a, 0, g) * DPHI(e, b, 0, g) + DPHI(e, a, 1, g) * DPHI(e, b, 1, g) +
+                     DPHI(e, a, 2, g) * DPHI(e, b, 2, g));
+            Ke[l] += e->weight[g] * mg;
+          }
+}
+
+/* VF_GradientUThermoPoroNoCompression3D_local: VFMech.c:456-524 */
+void vfo_grad_u_thermoporo_nocomp(double *b24, const double u24[24], const double v8[8],
+                                  const double dth8[8], const double p8[8], double lambda,
+                                  double mu, double alpha, double beta, double eta,
+                                  const vfo_elem *e)
+{
+  const double coefalpha = (3. * lambda + 2. * mu) * alpha;
+  double th[27], pr[27], vv[27], tr[27];
+  trace_eps(tr, u24, e);
+  for (int g = 0; g < 27; g++) { th[g] = 0; pr[g] = 0; vv[g] = 0.; }
+  for (int a = 0; a < 8; a++)
+    for (int g = 0; g < 27; g++) {
+      th[g] += PHI(e, a, g) * dth8[a];
+      pr[g] += PHI(e, a, g) * p8[a];
+      vv[g] += PHI(e, a, g) * v8[a];
+    }
+  for (int g = 0; g < 27; g++) {
+    const double f = tr[g] < 0. ? coefalpha * th[g] * (1. + eta) + beta * pr[g]
+                                : coefalpha * th[g] * (vv[g] * vv[g] + eta) + beta * pr[g] * vv[g];
+    for (int a = 0, l = 0; a < 8; a++)
+      for (int c = 0; c < 3; c++, l++) b24[l] += e->weight[g] * DPHI(e, a, c, g) * f;
+  }
+}
+
+/* ElasticEnergyDensitySphericalDeviatoricNoCompression3D_local:
+ * VFMech.c:115-218.  Stress from the strain less the thermal strain, energy
+ * density 1/2 sigma : eps with the total strain; S = kappa tr^2 / 2 where
+ * tr(eps) >= 0 (else 0), D = W - kappa tr^2 / 2. */
+void vfo_energy_density_sd_nocomp(double S[27], double D[27], const double u24[24],
+                                  const double dth8[8], double lambda, double mu, double alpha,
+                                  const vfo_elem *e)
+{
+  const double kappa = lambda + 2. * mu / 3.;
+  double e11[27], e22[27], e33[27], e12[27], e23[27], e13[27], e0[27];
+  for (int g = 0; g < 27; g++) e11[g] = e22[g] = e33[g] = e12[g] = e23[g] = e13[g] = e0[g] = 0.;
+  for (int a = 0; a < 8; a++)
+    for (int g = 0; g < 27; g++) {
+      const double *u = u24 + 3 * a;
+      e11[g] += DPHI(e, a, 0, g) * u[0];
+      e22[g] += DPHI(e, a, 1, g) * u[1];
+      e33[g] += DPHI(e, a, 2, g) * u[2];
+      e12[g] += (DPHI(e, a, 0, g) * u[1] + DPHI(e, a, 1, g) * u[0]) * .5;
+      e23[g] += (DPHI(e, a, 1, g) * u[2] + DPHI(e, a, 2, g) * u[1]) * .5;
+      e13[g] += (DPHI(e, a, 0, g) * u[2] + DPHI(e, a, 2, g) * u[0]) * .5;
+      e0[g] += alpha * PHI(e, a, g) * dth8[a];
+    }
+  for (int g = 0; g < 27; g++) {
+    const double a11 = e11[g] - e0[g], a22 = e22[g] - e0[g], a33 = e33[g] - e0[g];
+    const double s11 = (lambda + 2. * mu) * a11 + lambda * a22 + lambda * a33;
+    const double s22 = lambda * a11 + (lambda + 2. * mu) * a22 + lambda * a33;
+    const double s33 = lambda * a11 + lambda * a22 + (lambda + 2. * mu) * a33;
+    const double W = (s11 * e11[g] + s22 * e22[g] + s33 * e33[g]) * .5 + 2. * mu * e12[g] * e12[g] +
+                     2. * mu * e23[g] * e23[g] + 2. * mu * e13[g] * e13[g];
+    const double tr = e11[g] + e22[g] + e33[g];
+    S[g] = tr < 0. ? 0. : kappa * tr * tr * .5;
+    D[g] = W - kappa * tr * tr * .5;
+  }
+}
+
+/* VF_BilinearFormVCouplingNoCompression3D_local: VFMech.c:2125-2153 */
+void vfo_bilinear_v_coupling_nocomp(double *K, const double u24[24], const double dth8[8],
+                                    double lambda, double mu, double alpha, const vfo_elem *e)
+{
+  double S[27], D[27];
+  vfo_energy_density_sd_nocomp(S, D, u24, dth8, lambda, mu, alpha, e);
+  for (int a = 0, l = 0; a < 8; a++)
+    for (int b = 0; b < 8; b++, l++)
+      for (int g = 0; g < 27; g++)
+        K[l] += e->weight[g] * PHI(e, a, g) * PHI(e, b, g) * (S[g] + D[g]) * 2.;
+}
+
 /* VF_BilinearFormVAT13D_local / VF_BilinearFormVAT23D_local: VFMech.c:2040-2084 */
 void vfo_bilinear_v_at(double *K, int atnum, double Gc, double atCv, double eps,
                        const vfo_elem *e)
@@ -251,6 +363,50 @@ double vfo_elastic_energy_local(const double u24[24], const double v8[8], const 
           (D12[g] * D12[g] + D23[g] * D23[g] + D13[g] * D13[g]) * 2.) *
          ve[g] * ve[g] * (1. + eta) * mu * e->weight[g];
     double WD = (tr / 3. - alpha * te[g]) * ve[g] - coefbeta * pe[g];
+    E += 3. * threekappa / 2. * WD * WD * e->weight[g];
+    E += threekappa / 6. * tr * tr * eta * e->weight[g];
+  }
+  return E;
+}
+
+/* VF_ElasticEnergyNoCompression3D_local: VFMech.c:819-908 (the spherical
+ * term uses s = 1 where tr(eps) < 0, s = v elsewhere) */
+double vfo_elastic_energy_nocomp_local(const double u24[24], const double v8[8],
+                                       const double dth8[8], const double p8[8], double lambda,
+                                       double mu, double alpha, double beta, double eta,
+                                       const vfo_elem *e)
+{
+  double D11[27], D22[27], D33[27], D12[27], D23[27], D13[27], ve[27], pe[27], te[27];
+  double threekappa = 3. * lambda + 2. * mu;
+  double coefbeta = beta / threekappa;
+  for (int g = 0; g < 27; g++) {
+    D11[g] = D22[g] = D33[g] = D12[g] = D23[g] = D13[g] = 0.;
+    ve[g] = pe[g] = te[g] = 0.;
+  }
+  for (int a = 0; a < 8; a++)
+    for (int g = 0; g < 27; g++) {
+      const double *u = u24 + 3 * a;
+      D11[g] += DPHI(e, a, 0, g) * u[0];
+      D22[g] += DPHI(e, a, 1, g) * u[1];
+      D33[g] += DPHI(e, a, 2, g) * u[2];
+      D12[g] += (DPHI(e, a, 0, g) * u[1] + DPHI(e, a, 1, g) * u[0]) * .5;
+      D23[g] += (DPHI(e, a, 1, g) * u[2] + DPHI(e, a, 2, g) * u[1]) * .5;
+      D13[g] += (DPHI(e, a, 0, g) * u[2] + DPHI(e, a, 2, g) * u[0]) * .5;
+      ve[g] += PHI(e, a, g) * v8[a];
+      pe[g] += PHI(e, a, g) * p8[a];
+      te[g] += PHI(e, a, g) * dth8[a];
+    }
+  double E = 0.;
+  for (int g = 0; g < 27; g++) {
+    double tr = D11[g] + D22[g] + D33[g];
+    double s = tr < 0. ? 1. : ve[g];
+    D11[g] -= tr / 3.;
+    D22[g] -= tr / 3.;
+    D33[g] -= tr / 3.;
+    E += (D11[g] * D11[g] + D22[g] * D22[g] + D33[g] * D33[g] +
+          (D12[g] * D12[g] + D23[g] * D23[g] + D13[g] * D13[g]) * 2.) *
+         ve[g] * ve[g] * (1. + eta) * mu * e->weight[g];
+    double WD = (tr / 3. - alpha * te[g]) * s - coefbeta * pe[g];
     E += 3. * threekappa / 2. * WD * WD * e->weight[g];
     E += threekappa / 6. * tr * tr * eta * e->weight[g];
   }
@@ -744,21 +900,37 @@ static coords_t mkcoords(const vfo_problem *P)
 }
 static void freecoords(coords_t c) { free(c.X); free(c.Y); free(c.Z); }
 
+/* element stiffness at the state u24 (ignored for UNILATERAL_NONE) */
+static void u_elem_matrix(const vfo_problem *P, const elemdata *d, double lambda, double mu,
+                          const vfo_elem *e, double *Ke)
+{
+  if (P->unilateral)
+    vfo_bilinear_u_nocomp(Ke, d->u24, d->v8, lambda, mu, P->eta, e);
+  else
+    vfo_bilinear_u(Ke, d->v8, lambda, mu, P->eta, e);
+}
+
 /* VF_UIJacobian (VFMech.c:1897-2029) + MatApplyDirichletBC (VFCartFE.c:1147-1296) */
 vfo_ell *vfo_u_jacobian(const vfo_problem *P, const vfo_fields *F)
+{
+  return vfo_u_jacobian_at(P, F, NULL);
+}
+
+vfo_ell *vfo_u_jacobian_at(const vfo_problem *P, const vfo_fields *F, const double *U)
 {
   vfo_ell *A = pattern(P, 3);
   coords_t C = mkcoords(P);
   vfo_elem e;
   elemdata d;
+  memset(&d, 0, sizeof d);
   double Ke[576];
   ELEM_LOOP(P, C.X, C.Y, C.Z)
   {
     ELEM_SETUP(P, C.X, C.Y, C.Z, e, cell);
     double lambda, mu;
     elem_lame(P, cell, &lambda, &mu);
-    gather(P, F, NULL, ei, ej, ek, &d);
-    vfo_bilinear_u(Ke, d.v8, lambda, mu, P->eta, &e);
+    gather(P, F, U, ei, ej, ek, &d);
+    u_elem_matrix(P, &d, lambda, mu, &e, Ke);
     add_block(A, P, 3, ei, ej, ek, Ke);
   }
   uint8_t *mask = malloc(3 * vfo_nnodes(P));
@@ -774,7 +946,11 @@ static void u_elem_rhs(const vfo_problem *P, const elemdata *d, double lambda, d
                        const vfo_elem *e, double *b24)
 {
   for (int l = 0; l < 24; l++) b24[l] = 0.;
-  vfo_grad_u_thermoporo(b24, d->v8, d->dth8, d->p8, lambda, mu, P->alpha, P->beta, P->eta, e);
+  if (P->unilateral)
+    vfo_grad_u_thermoporo_nocomp(b24, d->u24, d->v8, d->dth8, d->p8, lambda, mu, P->alpha, P->beta,
+                                 P->eta, e);
+  else
+    vfo_grad_u_thermoporo(b24, d->v8, d->dth8, d->p8, lambda, mu, P->alpha, P->beta, P->eta, e);
   if (P->hasCrackPressure) vfo_grad_u_crackpressure(b24, d->v8, d->p8, e);
 }
 
@@ -794,7 +970,7 @@ void vfo_u_residual(const vfo_problem *P, const vfo_fields *F, const double *U,
     double lambda, mu;
     elem_lame(P, cell, &lambda, &mu);
     gather(P, F, U, ei, ej, ek, &d);
-    vfo_bilinear_u(Ke, d.v8, lambda, mu, P->eta, &e);
+    u_elem_matrix(P, &d, lambda, mu, &e, Ke);
     for (int r = 0, l = 0; r < 24; r++) {
       int a = r / 3;
       int64_t row = 3 * NODE(P, ei + (a & 1), ej + ((a >> 1) & 1), ek + (a >> 2)) + r % 3;
@@ -818,6 +994,7 @@ void vfo_u_rhs(const vfo_problem *P, const vfo_fields *F, double *b)
   coords_t C = mkcoords(P);
   vfo_elem e;
   elemdata d;
+  memset(&d, 0, sizeof d);  /* the load at U = 0 (NOCOMPRESSION: every point tensile) */
   double b24[24];
   ELEM_LOOP(P, C.X, C.Y, C.Z)
   {
@@ -840,7 +1017,10 @@ static void v_elem_matrix(const vfo_problem *P, const elemdata *d, double lambda
 {
   for (int l = 0; l < 64; l++) K64[l] = 0.;
   vfo_bilinear_v_at(K64, P->atnum, P->Gc, vfo_atcv(P->atnum), P->epsilon, e);
-  vfo_bilinear_v_coupling(K64, d->u24, d->dth8, lambda, mu, P->alpha, e);
+  if (P->unilateral)
+    vfo_bilinear_v_coupling_nocomp(K64, d->u24, d->dth8, lambda, mu, P->alpha, e);
+  else
+    vfo_bilinear_v_coupling(K64, d->u24, d->dth8, lambda, mu, P->alpha, e);
 }
 
 static uint8_t *v_fixed_mask(const vfo_problem *P, const vfo_fields *F)
@@ -958,8 +1138,8 @@ void vfo_u_energy(const vfo_problem *P, const vfo_fields *F, const double *U, do
     double lambda, mu;
     elem_lame(P, cell, &lambda, &mu);
     gather(P, F, U, ei, ej, ek, &d);
-    myE += vfo_elastic_energy_local(d.u24, d.v8, d.dth8, d.p8, lambda, mu, P->alpha, P->beta,
-                                    P->eta, &e);
+    myE += (P->unilateral ? vfo_elastic_energy_nocomp_local : vfo_elastic_energy_local)(
+        d.u24, d.v8, d.dth8, d.p8, lambda, mu, P->alpha, P->beta, P->eta, &e);
     if (P->hasCrackPressure) pressure_work_acc(&myP, d.u24, d.v8, d.p8, &e);
   }
   *elastic = myE;
@@ -1020,8 +1200,8 @@ void vfo_energy_fields(const vfo_problem *P, const vfo_fields *F, const double *
     elem_lame(P, cell, &lambda, &mu);
     gather(P, F, U, ei, ej, ek, &d);
     for (int a = 0; a < 8; a++) v8[a] = V[NODE(P, ei + (a & 1), ej + ((a >> 1) & 1), ek + (a >> 2))];
-    cell3[3 * cell] = vfo_elastic_energy_local(d.u24, v8, d.dth8, d.p8, lambda, mu, P->alpha, P->beta,
-                                               P->eta, &e);
+    cell3[3 * cell] = (P->unilateral ? vfo_elastic_energy_nocomp_local : vfo_elastic_energy_local)(
+        d.u24, v8, d.dth8, d.p8, lambda, mu, P->alpha, P->beta, P->eta, &e);
     if (P->hasCrackPressure) pressure_work_acc(&pw, d.u24, v8, d.p8, &e);
     cell3[3 * cell + 1] = pw;
     surface_energy_acc(&se, v8, P->atnum, P->Gc, vfo_atcv(P->atnum), P->epsilon, &e);
@@ -1114,11 +1294,65 @@ int vfo_pcg(const vfo_ell *A, const double *b, double *x, double rtol, double at
  * J d = -F(U), J = K with Dirichlet rows replaced by identity.  The
  * Dirichlet part of d is -F_D; the interior part solves K_II d_I =
  * -F_I - K_ID d_D (the identity rows decouple). */
+static double norm2(const double *a, int64_t n)
+{
+  double s = 0.;
+  for (int64_t i = 0; i < n; i++) s += a[i] * a[i];
+  return sqrt(s);
+}
+
+/* UNILATERAL_NOCOMPRESSION: SNESNEWTONLS on F(U) = K(U) U - b(U) (VFMech.c
+ * :1456-1887) with the Jacobian K(U) of VF_UIJacobian (VFMech.c:1897-2029);
+ * each step solved by the linear step below, then halved until ||F||
+ * decreases (the role of the line search). */
+static int u_solve_linear(const vfo_problem *P, const vfo_fields *F, double *U, const double *BCU,
+                          double rtol, int maxit, double *relres);
+static int u_solve_newton(const vfo_problem *P, const vfo_fields *F, double *U, const double *BCU,
+                          double rtol, int maxit, double *relres)
+{
+  int64_t nd = 3 * vfo_nnodes(P);
+  double *Fr = malloc(sizeof(double) * nd), *U0 = malloc(sizeof(double) * nd),
+         *D = malloc(sizeof(double) * nd);
+  vfo_u_residual(P, F, U, BCU, Fr);
+  const double rn0 = norm2(Fr, nd);
+  double rn = rn0, rr;
+  int its = 0;
+  for (int newton = 0; newton < 50 && rn > rtol * rn0 && rn > 0.; newton++) {
+    memcpy(U0, U, sizeof(double) * nd);
+    its += u_solve_linear(P, F, U, BCU, 1e-13, maxit, &rr);  /* U = U0 + D */
+    for (int64_t i = 0; i < nd; i++) D[i] = U[i] - U0[i];
+    double t = 1.;
+    for (int ls = 0; ls < 30; ls++) {
+      for (int64_t i = 0; i < nd; i++) U[i] = U0[i] + t * D[i];
+      vfo_u_residual(P, F, U, BCU, Fr);
+      double rt = norm2(Fr, nd);
+      if (rt < rn || ls == 29) {
+        rn = rt;
+        break;
+      }
+      t *= .5;
+    }
+  }
+  if (relres) *relres = rn0 > 0. ? rn / rn0 : 0.;
+  free(Fr);
+  free(U0);
+  free(D);
+  return its;
+}
+
 int vfo_u_solve(const vfo_problem *P, const vfo_fields *F, double *U, const double *BCU,
                 double rtol, int maxit, double *relres)
 {
+  if (P->unilateral) return u_solve_newton(P, F, U, BCU, rtol, maxit, relres);
+  return u_solve_linear(P, F, U, BCU, rtol, maxit, relres);
+}
+
+/* one Newton step: J d = -F(U) with J = K(U) (identity Dirichlet rows) */
+static int u_solve_linear(const vfo_problem *P, const vfo_fields *F, double *U, const double *BCU,
+                          double rtol, int maxit, double *relres)
+{
   int64_t nd = 3 * vfo_nnodes(P);
-  vfo_ell *J = vfo_u_jacobian(P, F);
+  vfo_ell *J = vfo_u_jacobian_at(P, F, U);
   double *Fr = malloc(sizeof(double) * nd), *d = calloc(nd, sizeof(double)),
          *rhs = malloc(sizeof(double) * nd), *t = malloc(sizeof(double) * nd);
   uint8_t *mask = malloc(nd);

@@ -60,6 +60,8 @@ typedef struct {
   const double *E_elem;         /* optional per-element E (N_e) */
   vfo_bc bcU[3];
   vfo_bc bcV;
+  int    unilateral;            /* 0 UNILATERAL_NONE, 1 UNILATERAL_NOCOMPRESSION
+                                   (VFCommon.h:120-123, -unilateral) */
 } vfo_problem;
 
 /* Nodal fields (all length N_n except U, BCU: 3*N_n) */
@@ -96,6 +98,25 @@ void vfo_grad_u_crackpressure(double *b24, const double v8[8], const double p8[8
 void vfo_elastic_energy_density(double W27[27], const double u24[24],
                                 const double dth8[8], double lambda, double mu,
                                 double alpha, const vfo_elem *e);  /* VFMech.c:32-106 */
+/* UNILATERAL_NOCOMPRESSION variants: the spherical part is not degraded
+ * where tr(eps(u)) < 0 (UNILATERAL_THRES = 0, VFMech.c:6) */
+void vfo_bilinear_u_nocomp(double *Ke576, const double u24[24], const double v8[8],
+                           double lambda, double mu, double eta,
+                           const vfo_elem *e);                     /* VFMech.c:299-376 */
+void vfo_grad_u_thermoporo_nocomp(double *b24, const double u24[24], const double v8[8],
+                                  const double dth8[8], const double p8[8], double lambda,
+                                  double mu, double alpha, double beta, double eta,
+                                  const vfo_elem *e);              /* VFMech.c:456-524 */
+void vfo_energy_density_sd_nocomp(double S27[27], double D27[27], const double u24[24],
+                                  const double dth8[8], double lambda, double mu,
+                                  double alpha, const vfo_elem *e); /* VFMech.c:115-218 */
+void vfo_bilinear_v_coupling_nocomp(double *K64, const double u24[24], const double dth8[8],
+                                    double lambda, double mu, double alpha,
+                                    const vfo_elem *e);            /* VFMech.c:2125-2153 */
+double vfo_elastic_energy_nocomp_local(const double u24[24], const double v8[8],
+                                       const double dth8[8], const double p8[8], double lambda,
+                                       double mu, double alpha, double beta, double eta,
+                                       const vfo_elem *e);         /* VFMech.c:819-908 */
 void vfo_bilinear_v_at(double *K64, int atnum, double Gc, double atCv,
                        double epsilon, const vfo_elem *e);        /* VFMech.c:2040-2084 */
 void vfo_bilinear_v_coupling(double *K64, const double u24[24], const double dth8[8],
@@ -152,6 +173,8 @@ int64_t  vfo_ell_to_csr(const vfo_ell *A, int64_t *rowptr, int64_t *colidx, doub
 
 /* VF_UIJacobian + MatApplyDirichletBC (rows zeroed, diag 1): VFMech.c:1897-2029 */
 vfo_ell *vfo_u_jacobian(const vfo_problem *P, const vfo_fields *F);
+/* the same at the state U (NOCOMPRESSION: the matrix depends on U) */
+vfo_ell *vfo_u_jacobian_at(const vfo_problem *P, const vfo_fields *F, const double *U);
 /* VF_UResidual: VFMech.c:1456-1887 (no in-situ) */
 void vfo_u_residual(const vfo_problem *P, const vfo_fields *F, const double *U,
                     const double *BCU, double *res);
@@ -191,7 +214,9 @@ void vfo_cell_to_node(const vfo_problem *P, const double *cell, double *node, in
 /* Jacobi PCG on an ELL matrix; returns iterations, <0 on breakdown */
 int vfo_pcg(const vfo_ell *A, const double *b, double *x, double rtol,
             double atol, int maxit, double *relres);
-/* u-solve: one Newton step of SNESNEWTONLS on the linear u-problem.
+/* u-solve: one Newton step of SNESNEWTONLS on the linear u-problem; with
+ * NOCOMPRESSION, Newton iterations (Jacobian at the iterate, backtracking on
+ * ||F||) until ||F|| <= rtol ||F(U0)||.  Returns CG iterations.
  * U must satisfy the BCs on entry (VFTimeStepPrepare). */
 int vfo_u_solve(const vfo_problem *P, const vfo_fields *F, double *U,
                 const double *BCU, double rtol, int maxit, double *relres);

@@ -41,7 +41,7 @@ class Problem(C.Structure):
                 ("eta", C.c_double), ("epsilon", C.c_double), ("irrevtol", C.c_double),
                 ("atnum", C.c_int), ("hasCrackPressure", C.c_int),
                 ("E_elem", C.POINTER(C.c_double)),
-                ("bcU", BC * 3), ("bcV", BC)]
+                ("bcU", BC * 3), ("bcV", BC), ("unilateral", C.c_int)]
 
 
 class Fields(C.Structure):
@@ -91,6 +91,15 @@ def lib():
         L.vfo_v_energy.restype = C.c_double
         L.vfo_crack_volume.argtypes = [P, dp, dp, dp]
         L.vfo_crack_volume.restype = C.c_double
+        L.vfo_u_jacobian_at.argtypes = [P, Fp, dp]
+        L.vfo_u_jacobian_at.restype = Ep
+        El = C.POINTER(Elem)
+        L.vfo_bilinear_u_nocomp.argtypes = [dp, dp, dp, C.c_double, C.c_double, C.c_double, El]
+        L.vfo_grad_u_thermoporo_nocomp.argtypes = [dp, dp, dp, dp, dp] + [C.c_double] * 5 + [El]
+        L.vfo_energy_density_sd_nocomp.argtypes = [dp, dp, dp, dp, C.c_double, C.c_double, C.c_double, El]
+        L.vfo_bilinear_v_coupling_nocomp.argtypes = [dp, dp, dp, C.c_double, C.c_double, C.c_double, El]
+        L.vfo_elastic_energy_nocomp_local.argtypes = [dp, dp, dp, dp] + [C.c_double] * 5 + [El]
+        L.vfo_elastic_energy_nocomp_local.restype = C.c_double
         L.vfo_energy_fields.argtypes = [P, Fp, dp, dp, dp]
         L.vfo_energy_fields.restype = None
         L.vfo_cell_to_node.argtypes = [P, dp, dp, C.c_int]
@@ -152,7 +161,7 @@ class OracleProblem:
 
     def __init__(self, n, l=(1.0, 1.0, 1.0), E=1.0, nu=0.0, alpha=1e-5, beta=1e-4, Gc=1.0,
                  eta=1e-5, epsilon=0.2, irrevtol=5e-2, atnum=2, pressurize=False,
-                 E_elem=None, bcU=None, bcV=None):
+                 E_elem=None, bcU=None, bcV=None, unilateral=0):
         self.P = Problem()
         self.P.nx, self.P.ny, self.P.nz = n
         self.P.lx, self.P.ly, self.P.lz = l
@@ -161,6 +170,7 @@ class OracleProblem:
             setattr(self.P, k, v)
         self.P.atnum = atnum
         self.P.hasCrackPressure = int(bool(pressurize))
+        self.P.unilateral = int(unilateral)
         self.E_elem = None if E_elem is None else np.ascontiguousarray(E_elem, np.float64)
         self.P.E_elem = _dp(self.E_elem) if self.E_elem is not None else C.POINTER(C.c_double)()
         for c in range(3):
@@ -195,9 +205,10 @@ def ell_to_csr(E):
     return sp.csr_matrix((va, ci, rp), shape=(n, n))
 
 
-def u_jacobian(op, F, csr=True):
+def u_jacobian(op, F, csr=True, U=None):
+    """VF_UIJacobian; U: the state (NOCOMPRESSION), None for U = 0."""
     L = lib()
-    E = L.vfo_u_jacobian(op.ref, C.byref(F))
+    E = L.vfo_u_jacobian_at(op.ref, C.byref(F), _dp(U)) if U is not None else L.vfo_u_jacobian(op.ref, C.byref(F))
     if not csr:
         return E
     A = ell_to_csr(E)

@@ -34,13 +34,14 @@ def _bc(mod, spec):
 
 def make(n=(9, 8, 7), l=(1.0, 0.9, 0.8), seed=0, nu=0.2, atnum=1, pressurize=True, hetero=True,
          bcU=CLAMPED, bcV=NONE_V, E=1.0, alpha=1e-5, beta=1.0, Gc=1.0, eta=1e-5, eps=None,
-         irrevtol=5e-2, thermal=True):
+         irrevtol=5e-2, thermal=True, unilateral=0):
     rng = np.random.default_rng(seed)
     nn = n[0] * n[1] * n[2]
     ne = (n[0] - 1) * (n[1] - 1) * (n[2] - 1)
     h = l[0] / (n[0] - 1)
     s = dict(n=n, l=l, nu=nu, atnum=atnum, pressurize=pressurize, E=E, alpha=alpha, beta=beta, Gc=Gc,
-             eta=eta, eps=eps if eps is not None else 2 * h, irrevtol=irrevtol, bcU=bcU, bcV=bcV)
+             eta=eta, eps=eps if eps is not None else 2 * h, irrevtol=irrevtol, bcU=bcU, bcV=bcV,
+             unilateral=unilateral)
     s["E_elem"] = np.exp(0.5 * rng.standard_normal(ne)) if hetero else None
     V = np.clip(rng.uniform(-0.2, 1.2, nn), 0.0, 1.0)
     s["V"] = V
@@ -58,7 +59,8 @@ def to_oracle(O, s):
     op = O.OracleProblem(s["n"], s["l"], E=s["E"], nu=s["nu"], alpha=s["alpha"], beta=s["beta"], Gc=s["Gc"],
                          eta=s["eta"], epsilon=s["eps"], irrevtol=s["irrevtol"], atnum=s["atnum"],
                          pressurize=s["pressurize"], E_elem=s["E_elem"],
-                         bcU=[_bc(O, s["bcU"].get(c, ({}, {}, {}))) for c in range(3)], bcV=_bc(O, s["bcV"]))
+                         bcU=[_bc(O, s["bcU"].get(c, ({}, {}, {}))) for c in range(3)], bcV=_bc(O, s["bcV"]),
+                         unilateral=s.get("unilateral", 0))
     F = op.fields(V=s["V"], VIrrev=s["VIrrev"], theta=s["theta"], thetaRef=s["thetaRef"], pressure=s["pressure"])
     return op, F
 
@@ -68,7 +70,7 @@ def to_gpu(s):
     from vpfhf_amd import _lib
     P = V.Problem(s["n"], s["l"], E=s["E"], nu=s["nu"], alpha=s["alpha"], beta=s["beta"], Gc=s["Gc"],
                   eta=s["eta"], epsilon=s["eps"], irrevtol=s["irrevtol"], atnum=s["atnum"],
-                  pressurize=s["pressurize"])
+                  pressurize=s["pressurize"], unilateral=s.get("unilateral", 0))
     P.set_bc([_bc(_lib, s["bcU"].get(c, ({}, {}, {}))) for c in range(3)], _bc(_lib, s["bcV"]))
     P.set_fields(V=s["V"], VIrrev=s["VIrrev"], theta=s["theta"], thetaRef=s["thetaRef"],
                  pressure=s["pressure"], E_elem=s["E_elem"])

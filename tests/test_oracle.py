@@ -500,3 +500,176 @@ def test_runtest1_goldens(oracle):
         else:
             assert abs(r["surface"] - g["surface"]) < 1e-4 * g["surface"]
             assert abs(r["crack_opening"] - g["crack_opening"]) < 2e-4 * g["crack_opening"]
+
+
+# ------------------------------------------------ UNILATERAL_NOCOMPRESSION
+def _q1_tables(h):
+    """3-point Gauss tables on one element: weights (27), phi (8x27), dphi (8x3x27)."""
+    gp = np.array([(1 - math.sqrt(.6)) / 2, .5, (1 + math.sqrt(.6)) / 2])
+    gw = np.array([5., 8., 5.]) / 18.
+    W, phi, dphi = np.zeros(27), np.zeros((8, 27)), np.zeros((8, 3, 27))
+    for gz in range(3):
+        for gy in range(3):
+            for gx in range(3):
+                g = (gz * 3 + gy) * 3 + gx
+                W[g] = h[0] * h[1] * h[2] * gw[gx] * gw[gy] * gw[gz]
+                x = (gp[gx], gp[gy], gp[gz])
+                for a in range(8):
+                    ijk = (a & 1, (a >> 1) & 1, a >> 2)
+                    f = [x[d] if ijk[d] else 1 - x[d] for d in range(3)]
+                    df = [(1 if ijk[d] else -1) / h[d] for d in range(3)]
+                    phi[a, g] = f[0] * f[1] * f[2]
+                    dphi[a, 0, g] = df[0] * f[1] * f[2]
+                    dphi[a, 1, g] = f[0] * df[1] * f[2]
+                    dphi[a, 2, g] = f[0] * f[1] * df[2]
+    return W, phi, dphi
+
+
+def _voigt_B(dphi, g):
+    """strain (e11, e22, e33, g12, g23, g13) of the 24 element dofs at point g"""
+    B = np.zeros((6, 24))
+    for a in range(8):
+        d = dphi[a, :, g]
+        B[0, 3 * a], B[1, 3 * a + 1], B[2, 3 * a + 2] = d
+        B[3, 3 * a], B[3, 3 * a + 1] = d[1], d[0]
+        B[4, 3 * a + 1], B[4, 3 * a + 2] = d[2], d[1]
+        B[5, 3 * a], B[5, 3 * a + 2] = d[2], d[0]
+    return B
+
+
+def _nocomp_elem(oracle, h, lam, mu, eta, v8, u24):
+    e = oracle.elem(*h)
+    K = np.zeros(576)
+    oracle.lib().vfo_bilinear_u_nocomp(oracle._dp(K), oracle._dp(u24), oracle._dp(v8), lam, mu, eta, e)
+    return K.reshape(24, 24), e
+
+
+def test_nocomp_element_independent(oracle):
+    """VF_BilinearFormUNoCompression3D_local (VFMech.c:299-376) against
+    B^T C_g B in Voigt form, C_g = s (lam m m^T + mu D) where tr eps >= 0 and
+    (kappa (1+eta) - 2 mu s / 3) m m^T + mu s D where tr eps < 0; all-tension
+    states reproduce the UNILATERAL_NONE matrix."""
+    h, lam, mu, eta = (0.3, 0.2, 0.5), 0.52, 0.61, 1e-3
+    kappa = lam + 2 * mu / 3
+    rng = np.random.default_rng(3)
+    v8 = rng.uniform(0, 1, 8)
+    W, phi, dphi = _q1_tables(h)
+    m = np.array([1., 1., 1., 0., 0., 0.])
+    Dm = np.diag([2., 2., 2., 1., 1., 1.])
+    x = np.array([[a & 1, (a >> 1) & 1, a >> 2] for a in range(8)], float) * np.array(h)
+    states = {"tension": (0.01 * x).ravel(), "compression": (-0.01 * x).ravel(),
+              "mixed": rng.standard_normal(24) * 0.01}
+    for name, u24 in states.items():
+        K, e = _nocomp_elem(oracle, h, lam, mu, eta, v8, u24)
+        ref = np.zeros((24, 24))
+        ncomp = 0
+        for g in range(27):
+            B = _voigt_B(dphi, g)
+            s = (phi[:, g] @ v8) ** 2 + eta
+            tr = sum(dphi[a, c, g] * u24[3 * a + c] for a in range(8) for c in range(3))
+            if tr < 0:
+                Cg = (kappa * (1 + eta) - 2 * mu * s / 3) * np.outer(m, m) + mu * s * Dm
+                ncomp += 1
+            else:
+                Cg = s * (lam * np.outer(m, m) + mu * Dm)
+            ref += W[g] * B.T @ Cg @ B
+        assert np.abs(K - ref).max() < 1e-13 * np.abs(ref).max(), name
+        if name == "tension":
+            assert ncomp == 0
+            K0 = np.zeros(576)
+            oracle.lib().vfo_bilinear_u(oracle._dp(K0), oracle._dp(v8), lam, mu, eta, e)
+            assert np.abs(K - K0.reshape(24, 24)).max() < 1e-14 * np.abs(K0).max()
+        if name == "compression":
+            assert ncomp == 27
+        if name == "mixed":
+            assert 0 < ncomp < 27
+
+
+def test_nocomp_loads_coupling_energy(oracle):
+    """Thermo-poro load (VFMech.c:456-524) per point from its branch; the
+    V coupling (VFMech.c:115-218, 2125-2153) vanishes under pure volumetric
+    compression and equals UNILATERAL_NONE's in tension; the elastic energy
+    (VFMech.c:819-908) of uniform compression eps = -e0 I is
+    (9 kappa / 2) e0^2 (1 + eta) per volume, whatever v is."""
+    h, lam, mu, eta, alpha, beta = (0.3, 0.2, 0.5), 0.52, 0.61, 1e-3, 1e-2, 0.7
+    kappa = lam + 2 * mu / 3
+    rng = np.random.default_rng(4)
+    v8, dth8, p8 = rng.uniform(0, 1, 8), rng.uniform(-2, 2, 8), rng.uniform(0, 1, 8)
+    u24 = rng.standard_normal(24) * 0.01
+    W, phi, dphi = _q1_tables(h)
+    e = oracle.elem(*h)
+    L = oracle.lib()
+    b = np.zeros(24)
+    L.vfo_grad_u_thermoporo_nocomp(oracle._dp(b), oracle._dp(u24), oracle._dp(v8), oracle._dp(dth8),
+                                   oracle._dp(p8), lam, mu, alpha, beta, eta, e)
+    ca = (3 * lam + 2 * mu) * alpha
+    ref = np.zeros(24)
+    for g in range(27):
+        th, pr, vv = phi[:, g] @ dth8, phi[:, g] @ p8, phi[:, g] @ v8
+        tr = sum(dphi[a, c, g] * u24[3 * a + c] for a in range(8) for c in range(3))
+        f = ca * th * (1 + eta) + beta * pr if tr < 0 else ca * th * (vv * vv + eta) + beta * pr * vv
+        for a in range(8):
+            for c in range(3):
+                ref[3 * a + c] += W[g] * dphi[a, c, g] * f
+    assert np.abs(b - ref).max() < 1e-13 * np.abs(ref).max()
+    x = np.array([[a & 1, (a >> 1) & 1, a >> 2] for a in range(8)], float) * np.array(h)
+    zero8 = np.zeros(8)
+    for sign in (-1., 1.):
+        u = (sign * 0.02 * x).ravel()
+        K = np.zeros(64)
+        L.vfo_bilinear_v_coupling_nocomp(oracle._dp(K), oracle._dp(u), oracle._dp(zero8), lam, mu, 0., e)
+        K0 = np.zeros(64)
+        L.vfo_bilinear_v_coupling(oracle._dp(K0), oracle._dp(u), oracle._dp(zero8), lam, mu, 0., e)
+        if sign < 0:
+            assert np.abs(K).max() < 1e-15 * np.abs(K0).max()
+        else:
+            assert np.abs(K - K0).max() < 1e-14 * np.abs(K0).max()
+    e0 = 0.02
+    u = (-e0 * x).ravel()
+    En = L.vfo_elastic_energy_nocomp_local(oracle._dp(u), oracle._dp(v8), oracle._dp(zero8), oracle._dp(zero8),
+                                           lam, mu, 0., 0., eta, e)
+    assert abs(En - 4.5 * kappa * e0 ** 2 * (1 + eta) * np.prod(h)) < 1e-14 * En
+
+
+def _compressed_points(s, U):
+    """number of Gauss points with tr eps(U) < 0 (numpy, element by element)"""
+    n = s["n"]
+    h = [s["l"][d] / (n[d] - 1) for d in range(3)]
+    _, _, dphi = _q1_tables(h)
+    Ug = U.reshape(n[2], n[1], n[0], 3)
+    cnt = 0
+    for ek in range(n[2] - 1):
+        for ej in range(n[1] - 1):
+            for ei in range(n[0] - 1):
+                u24 = np.array([Ug[ek + (a >> 2), ej + ((a >> 1) & 1), ei + (a & 1)] for a in range(8)]).ravel()
+                tr = np.einsum('acg,ac->g', dphi, u24.reshape(8, 3))
+                cnt += int((tr < 0).sum())
+    return cnt
+
+
+@pytest.mark.parametrize("scale", [-0.02, 0.02])
+def test_nocomp_newton_solve(oracle, scale):
+    """Newton on F(U) = K(U) U - b(U) with a continuous piecewise-linear F
+    (crack pressure, no thermal or Biot load, whose branches jump at
+    tr eps = 0): the residual reaches the tolerance; compressing gives both
+    branches and a solution away from UNILATERAL_NONE's; a stretched problem
+    with no compressed point gives UNILATERAL_NONE's solution."""
+    s = S.make(n=(7, 6, 5), seed=2, bcU=S.TEST1_U, unilateral=1, thermal=False, beta=0.0)
+    op, F = S.to_oracle(oracle, s)
+    BCU = oracle.vec_set_from_bc(op, "U") * scale
+    U0 = oracle.vec_apply_dirichlet(op, np.zeros(3 * op.nn), BCU, "U")
+    U = U0.copy()
+    oracle.u_solve(op, F, U, BCU, rtol=1e-11)
+    r = oracle.u_residual(op, F, U, BCU)
+    assert np.linalg.norm(r) <= 1e-10 * np.linalg.norm(oracle.u_residual(op, F, U0, BCU))
+    s0 = dict(s)
+    s0["unilateral"] = 0
+    op0, F0 = S.to_oracle(oracle, s0)
+    U1 = U0.copy()
+    oracle.u_solve(op0, F0, U1, BCU, rtol=1e-12)
+    nc = _compressed_points(s, U)
+    if scale < 0:
+        assert 0 < nc < 27 * op.ne
+        assert np.linalg.norm(U - U1) > 1e-3 * np.linalg.norm(U1)  # the compressed points stiffen
+    elif nc == 0:
+        assert np.linalg.norm(U - U1) < 1e-9 * np.linalg.norm(U1)

@@ -84,10 +84,11 @@ class Problem:
     """
 
     def __init__(self, n, l=(1.0, 1.0, 1.0), E=1.0, nu=0.0, alpha=1e-5, beta=1e-4, Gc=1.0,
-                 eta=1e-5, epsilon=0.2, irrevtol=5e-2, atnum=2, pressurize=False):
+                 eta=1e-5, epsilon=0.2, irrevtol=5e-2, atnum=2, pressurize=False, unilateral=0):
         self.L = load()
         g = _lib.Grid(n[0], n[1], n[2], l[0], l[1], l[2])
-        self.props = _lib.Props(E, nu, alpha, beta, Gc, eta, epsilon, irrevtol, atnum, int(bool(pressurize)))
+        self.props = _lib.Props(E, nu, alpha, beta, Gc, eta, epsilon, irrevtol, atnum, int(bool(pressurize)),
+                                int(unilateral))
         h = C.c_void_p()
         check(self.L.vfg_create(C.byref(g), C.byref(self.props), C.byref(h)))
         self.h = h.value
@@ -231,6 +232,11 @@ class Problem:
         if nodefield:
             out += (self.CellToNodeInterpolation(cf),)
         return out if len(out) > 1 else vol.value
+
+    def set_state(self, U):
+        """The u-Jacobian's state under UNILATERAL_NOCOMPRESSION (VF_UIJacobian(U))."""
+        U = np.ascontiguousarray(U, np.float64)
+        self._c(self.L.vfg_u_set_state(self.h, _ptr(U), HOST))
 
     def energy_fields(self, U, V):
         """Per-cell (elastic, pressure work, surface) energy terms, shape (N_e, 3):

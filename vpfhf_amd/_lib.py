@@ -15,6 +15,7 @@ X0, X1, Y0, Y1, Z0, Z1 = range(6)
 ERRORS = {0: "VFG_OK", -1: "VFG_E_ARG", -2: "VFG_E_HIP", -3: "VFG_E_NOMEM", -4: "VFG_E_BREAKDOWN",
           -5: "VFG_E_MAXIT", -6: "VFG_E_NAN", -7: "VFG_E_BUDGET", -8: "VFG_E_NODEV"}
 PC_JACOBI, PC_CHEBYSHEV, PC_MG = 0, 1, 2
+UNILATERAL_NONE, UNILATERAL_NOCOMPRESSION = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 
 
@@ -26,7 +27,8 @@ class Grid(C.Structure):
 class Props(C.Structure):
     _fields_ = [("E", C.c_double), ("nu", C.c_double), ("alpha", C.c_double), ("beta", C.c_double),
                 ("Gc", C.c_double), ("eta", C.c_double), ("epsilon", C.c_double),
-                ("irrevtol", C.c_double), ("atnum", C.c_int), ("hasCrackPressure", C.c_int)]
+                ("irrevtol", C.c_double), ("atnum", C.c_int), ("hasCrackPressure", C.c_int),
+                ("unilateral", C.c_int)]
 
 
 class BC(C.Structure):
@@ -93,6 +95,7 @@ PROTOS = {
     "vfg_energies": (C.c_int, [H, vp, vp, dp, C.c_int]),
     "vfg_crack_volume": (C.c_int, [H, vp, vp, dp, vp, C.c_int]),
     "vfg_cell_to_node": (C.c_int, [H, vp, vp, C.c_int, C.c_int]),
+    "vfg_u_set_state": (C.c_int, [H, vp, C.c_int]),
     "vfg_energy_fields": (C.c_int, [H, vp, vp, vp, C.c_int]),
     "vfg_penny_build_v": (C.c_int, [H, vp, dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int]),
     "vfg_solve_opts_default": (None, [C.POINTER(SolveOpts)]),

@@ -61,6 +61,7 @@ static void set_props_(vfg_handle *h, const vfg_props *pr) {
   P.E = pr->E; P.nu = pr->nu; P.alpha = pr->alpha; P.beta = pr->beta; P.Gc = pr->Gc;
   P.eta = pr->eta; P.eps = pr->epsilon; P.irrevtol = pr->irrevtol;
   P.atnum = pr->atnum; P.hasCrackPressure = pr->hasCrackPressure;
+  P.unilateral = pr->unilateral;
   // VFMatPropGet (VFCommon.c:583-586) for E = 1
   P.lam1 = 1. * pr->nu / (1. + pr->nu) / (1. - 2. * pr->nu);
   P.mu1 = 1. / (1. + pr->nu) * .5;
@@ -70,6 +71,7 @@ static void set_props_(vfg_handle *h, const vfg_props *pr) {
 static int check_props(const vfg_props *pr) {
   if (!pr) return 0;
   if (pr->atnum != 1 && pr->atnum != 2) return 0;
+  if (pr->unilateral != VFG_UNILATERAL_NONE && pr->unilateral != VFG_UNILATERAL_NOCOMPRESSION) return 0;
   if (!(pr->nu > -1. && pr->nu < .5)) return 0;
   if (!(pr->epsilon > 0.)) return 0;
   return 1;
@@ -133,7 +135,8 @@ int vfg_create(const vfg_grid *g, const vfg_props *pr, vfg_handle **out) {
       (rc = vfg_alloc(h, (void **)&P.dth, 8 * P.nn)) || (rc = vfg_alloc(h, (void **)&P.p, 8 * P.nn)) ||
       (rc = vfg_alloc(h, (void **)&P.V32, 4 * P.nn)) || (rc = vfg_alloc(h, (void **)&P.zero3, 24 * P.nn)) ||
       (rc = vfg_alloc(h, (void **)&P.vfix, P.nn)) || (rc = vfg_alloc(h, (void **)&P.bV, 8 * P.nn)) ||
-      (rc = vfg_alloc(h, (void **)&h->part, 8 * 4 * 65536)) || (rc = vfg_alloc(h, (void **)&h->scal, 8 * 64))) {
+      (rc = vfg_alloc(h, (void **)&h->part, 8 * 4 * 65536)) || (rc = vfg_alloc(h, (void **)&h->scal, 8 * 64)) ||
+      (rc = vfg_alloc(h, (void **)&P.cmask, 4 * P.ne))) {
     snprintf(g_err, sizeof g_err, "%s", h->err.c_str());
     vfg_destroy(h);
     return rc;
@@ -144,6 +147,7 @@ int vfg_create(const vfg_grid *g, const vfg_props *pr, vfg_handle **out) {
   launch_fill(P.p, 0., P.nn, h->stream);
   launch_fill(P.zero3, 0., 3 * P.nn, h->stream);
   launch_copy_d2f(P.V, P.V32, P.nn, h->stream);
+  hipMemsetAsync(P.cmask, 0, 4 * P.ne, h->stream);  // the state U = 0: every point tensile
   for (int c = 0; c < 3; c++) memset(&P.hbcU[c], 0, sizeof(vfg_bc));
   memset(&P.hbcV, 0, sizeof(vfg_bc));
   launch_v_fixmask(P, h->stream);
@@ -275,6 +279,16 @@ struct IO {
 
 #define NEED_H(h) \
   if (!h) return vfg_fail(nullptr, VFG_E_ARG, "NULL handle")
+
+int vfg_u_set_state(vfg_handle *h, const double *U, int loc) {
+  NEED_H(h);
+  if (!U) return vfg_fail(h, VFG_E_ARG, "vfg_u_set_state: NULL array");
+  if (!h->P.unilateral) return VFG_OK;
+  IO io(h, loc, U, 24 * h->P.nn, nullptr, 0);
+  if (io.rc) return io.rc;
+  launch_u_cmask(h->P, (const double *)io.din, h->stream);
+  return io.finish();
+}
 
 int vfg_u_apply(vfg_handle *h, const double *x, double *y, int loc) {
   NEED_H(h);

@@ -303,7 +303,7 @@ int vfg_batch_create(vfg_handle *const *members, int n, vfg_batch **out) {
   if (!out || !members || n < 1) return VFG_E_ARG;
   *out = nullptr;
   for (int i = 0; i < n; i++)
-    if (!members[i]) return VFG_E_ARG;
+    if (!members[i] || members[i]->P.unilateral) return VFG_E_ARG;  // linear u-problems only
   vfg_batch *b = new vfg_batch();
   b->np = n;
   b->m.assign(members, members + n);

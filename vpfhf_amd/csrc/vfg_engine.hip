@@ -785,6 +785,8 @@ int vfg_engine_submit(vfg_engine *e, vfg_handle *h, double *U, const double *BCU
                       int am_maxit, double altmintol, int64_t tag) {
   if (!e) return VFG_E_ARG;
   if (!h || !U || !BCU || !V || !VIrrev || am_maxit < 1) return efail(e, VFG_E_ARG, "vfg_engine_submit: bad arguments");
+  if (h->P.unilateral)
+    return efail(e, VFG_E_ARG, "vfg_engine_submit: UNILATERAL_NOCOMPRESSION requests run on the single-problem path");
   int p = -1;
   for (int i = 0; i < e->ns && p < 0; i++)
     if (e->slot[i].st == ST_FREE) p = i;

@@ -41,6 +41,30 @@ __device__ __forceinline__ int64_t cell_id(const Grid &G, int i, int j, int k) {
   return (int64_t)i + (int64_t)G.nex * ((int64_t)j + (int64_t)G.ney * k);
 }
 
+// tr(eps(u)) at Gauss point (gx, gy, gz) of an element from its 8 nodal
+// displacements (local node a = (k*2+j)*2+i), accumulated per component over
+// the nodes and summed as (d1u1 + d2u2) + d3u3: every kernel that branches on
+// its sign (UNILATERAL_NOCOMPRESSION) evaluates it this way.
+__device__ __forceinline__ double tr_eps_at(const double (&u)[8][3], int gx, int gy, int gz, double ihx,
+                                            double ihy, double ihz) {
+  double h0 = 0., h1 = 0., h2 = 0.;
+  for (int a = 0; a < 8; a++) {
+    const int i = a & 1, j = (a >> 1) & 1, k = a >> 2;
+    const double px = i ? gP1(gx) : gP0(gx), py = j ? gP1(gy) : gP0(gy), pz = k ? gP1(gz) : gP0(gz);
+    h0 += (i ? ihx : -ihx) * py * pz * u[a][0];
+    h1 += (j ? ihy : -ihy) * px * pz * u[a][1];
+    h2 += (k ? ihz : -ihz) * px * py * u[a][2];
+  }
+  return (h0 + h1) + h2;
+}
+// bit g = (gz*3+gy)*3+gx set where tr(eps(u)) < 0 (UNILATERAL_THRES = 0, VFMech.c:6)
+__device__ __forceinline__ uint32_t compression_mask(const double (&u)[8][3], double ihx, double ihy, double ihz) {
+  uint32_t m = 0u;
+  for (int g = 0; g < 27; g++)
+    if (tr_eps_at(u, g % 3, (g / 3) % 3, g / 9, ihx, ihy, ihz) < 0.) m |= 1u << g;
+  return m;
+}
+
 // Dirichlet row set of one component, computed from node indices
 // (MatApplyDirichletBC, VFCartFE.c:1147-1296): bit f of `face` set iff face f
 // is not NONE; bit v of `vert` set iff vertex v is not NONE.

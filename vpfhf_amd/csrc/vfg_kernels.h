@@ -20,6 +20,8 @@ struct DevProblem {
   // properties (VFMatProp/VFProp); lam1/mu1 are the Lame constants for E = 1
   double E, nu, lam1, mu1, alpha, beta, Gc, eta, eps, irrevtol, atcv;
   int atnum, hasCrackPressure;
+  int unilateral;   // VFG_UNILATERAL_*
+  uint32_t *cmask;  // per element: Gauss points where tr(eps(U_J)) < 0 (Jacobian state)
   BcMask bcU, bcV;
   vfg_bc hbcU[3], hbcV;
   // fields
@@ -36,6 +38,8 @@ struct DevProblem {
 void launch_u_apply(const DevProblem &P, const double *x, double *y, hipStream_t s);
 void launch_u_apply_f32(const DevProblem &P, const float *x, float *y, hipStream_t s);
 void launch_u_diag(const DevProblem &P, double *d, hipStream_t s);
+// cmask from the state U (UNILATERAL_NOCOMPRESSION)
+void launch_u_cmask(const DevProblem &P, const double *U, hipStream_t s);
 // mode 0: out = element load vector b; mode 1: out = VF_UResidual(U)
 void launch_u_rhs(const DevProblem &P, const double *U, const double *BCU, double *out, int mode,
                   hipStream_t s);

@@ -138,7 +138,9 @@ __global__ __launch_bounds__(EB) void k_energies(DevProblem P, const double *__r
       double d12 = (H[0][1] + H[1][0]) * .5, d23 = (H[1][2] + H[2][1]) * .5, d13 = (H[0][2] + H[2][0]) * .5;
       el += (d11 * d11 + d22 * d22 + d33 * d33 + (d12 * d12 + d23 * d23 + d13 * d13) * 2.) * vg * vg *
             (1. + P.eta) * mu * W;
-      double WD = (tr / 3. - P.alpha * tg) * vg - cb * pg;
+      // UNILATERAL_NOCOMPRESSION: the spherical term is not degraded where
+      // tr(eps) < 0 (VF_ElasticEnergyNoCompression3D_local, VFMech.c:871-903)
+      double WD = (tr / 3. - P.alpha * tg) * (P.unilateral && tr < 0. ? 1. : vg) - cb * pg;
       el += 3. * tk / 2. * WD * WD * W;
       el += tk / 6. * tr * tr * P.eta * W;
       double ugv = ug[0] * gv[0] + ug[1] * gv[1] + ug[2] * gv[2];

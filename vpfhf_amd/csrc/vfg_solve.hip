@@ -742,6 +742,72 @@ static int pcg_mg(vfg_handle *h, int64_t n, const ApplyFn &A, double *x, double 
   return status;
 }
 
+// x = x0 + t d
+__global__ void k_lin(double *__restrict__ x, const double *__restrict__ x0, const double *__restrict__ d, double t,
+                      int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = x0[i] + t * d[i];
+}
+
+// UNILATERAL_NOCOMPRESSION: SNESNEWTONLS on F(U) = K(U) U - b(U)
+// (VF_UResidual, VFMech.c:1456-1887) with the Jacobian K(U) of VF_UIJacobian
+// (VFMech.c:1897-2029).  Each step solves J d = -F by PCG to
+// min(0.1, tol / (2 ||F||)) relative (inexact Newton: the last step lands
+// below tol), then halves d until ||F|| decreases (the line search's role).
+// On entry r = -F(x); x carries its Dirichlet values.
+static int u_solve_newton(vfg_handle *h, double *x, double *r, double *z, double *p, double *q, double *dinv,
+                          const double *B, const vfg_solve_opts *o, vfg_solve_info *inf) {
+  DevProblem &P = h->P;
+  hipStream_t s = h->stream;
+  const int64_t n = 3 * P.nn;
+  int rc = work(h, 10, n);
+  if (rc) return rc;
+  double *d = h->w[8], *x0 = h->w[9];
+  int e2 = 0;
+  double rn = sqrt(dot_host(h, r, r, n, &e2));
+  if (e2) return e2;
+  const double rn0 = rn, tol = fmax(o->rtol * rn0, o->atol);
+  ApplyFn A = [&](const double *in, double *out) { launch_u_apply(P, in, out, s); };
+  int newton = 0, status = VFG_OK;
+  for (; newton < 50; newton++) {
+    if (!isfinite(rn)) { status = VFG_E_NAN; break; }
+    if (rn <= tol || rn == 0.) break;
+    launch_u_cmask(P, x, s);  // the Jacobian at the iterate
+    launch_u_diag(P, q, s);
+    hipLaunchKernelGGL(k_recip, dim3(nb_(n)), dim3(256), 0, s, (const double *)q, dinv, n);
+    launch_fill(d, 0., n, s);
+    const double inner = fmax(1e-14, fmin(0.1, .5 * tol / rn));
+    vfg_solve_info li = {0, 0, 0., 0};
+    int prc;
+    if (o->pc == VFG_PC_MG) {
+      if ((rc = mg_setup(h, dinv))) return rc;
+      prc = pcg_mg(h, n, A, d, r, z, p, q, h->wf[0], h->wf[1], inner, 0., o->maxit, 4, &li);
+    } else {
+      prc = pcg(h, n, A, dinv, d, r, z, p, q, inner, 0., o->maxit, 20, &li);
+    }
+    inf->iterations += li.iterations;
+    if (prc == VFG_E_HIP || prc == VFG_E_NAN) return prc;
+    hipMemcpyAsync(x0, x, 8 * n, hipMemcpyDeviceToDevice, s);
+    double t = 1.;
+    for (int ls = 0; ls < 20; ls++) {
+      hipLaunchKernelGGL(k_lin, dim3(nb_(n)), dim3(256), 0, s, x, (const double *)x0, (const double *)d, t, n);
+      launch_u_rhs(P, x, B, r, 1, s);
+      hipLaunchKernelGGL(k_neg, dim3(nb_(n)), dim3(256), 0, s, (const double *)r, r, n);
+      const double rt = sqrt(dot_host(h, r, r, n, &e2));
+      if (e2) return e2;
+      if (rt < rn || ls == 19) {
+        rn = rt;
+        break;
+      }
+      t *= .5;
+    }
+  }
+  inf->outer = newton;
+  inf->relres = rn0 > 0. ? rn / rn0 : 0.;
+  if (status == VFG_OK && rn > tol) status = VFG_E_MAXIT;
+  return status;
+}
+
 // FP64 iterative refinement around the FP32 inner PCG (precision form):
 // r64 = b - A64 x in FP64; solve A32 d = r64/|r64| in FP32 to inner_rtol;
 // x += |r64| d in FP64; repeat until |r64| <= rtol |r64_0|.
@@ -843,7 +909,9 @@ int vfg_u_solve(vfg_handle *h, double *U, const double *BCU, const vfg_solve_opt
   launch_u_diag(h->P, q, s);
   hipLaunchKernelGGL(k_recip, dim3(nb_(n)), dim3(256), 0, s, (const double *)q, dinv, n);
   const DevProblem &P = h->P;
-  if (o->precision == VFG_PREC_MIXED) {
+  if (P.unilateral) {
+    rc = u_solve_newton(h, x, r, z, p, q, dinv, B, o, &inf);
+  } else if (o->precision == VFG_PREC_MIXED) {
     rc = u_solve_mixed(h, x, r, q, dinv, B, o, &inf);
   } else if (o->pc == VFG_PC_CHEBYSHEV) {
     rc = vfg_fail(h, VFG_E_ARG, "vfg_u_solve: the Chebyshev preconditioner is provided for the mixed-precision solve");

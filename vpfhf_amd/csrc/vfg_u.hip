@@ -22,10 +22,15 @@ namespace vfg {
 // The gradients are carried pre-scaled by mu (H' = mu H), so that
 //   sigma / (W s cE) = lr tr(H') I + H' + H'^T,  lr = lam / mu,
 // and each stress entry is accumulated with one FMA by its weight W s cE.
-template <typename T>
+// UNI (UNILATERAL_NOCOMPRESSION, VFMech.c:299-376): at the Gauss points set in
+// cm the spherical part keeps kappa (1 + eta) instead of kappa s, i.e.
+//   sigma / (W cE) = (kr - 2 s / 3) tr(H') I + s (H' + H'^T),
+//   kr = kappa (1 + eta) / mu = (lr + 2/3)(1 + eta).
+template <typename T, bool UNI = false>
 __device__ __forceinline__ void u_elem_apply(const T (&X)[2][4][4], T cE, T lam1, T mu1, T eta,
-                                             T ihx, T ihy, T ihz, T (&r)[8][3]) {
+                                             T ihx, T ihy, T ihz, T (&r)[8][3], uint32_t cm = 0u) {
   const T lr = lam1 / mu1;
+  const T kr = (lr + T(2. / 3.)) * (T(1) + eta);
   const T mx = mu1 * ihx, my = mu1 * ihy, mz = mu1 * ihz;
   // differences along each direction, scaled by mu/h
   T Dx[3][2][2], Dy[3][2][2], Dz[3][2][2];  // Dx[c][k][j], Dy[c][k][i], Dz[c][j][i]
@@ -103,16 +108,27 @@ __device__ __forceinline__ void u_elem_apply(const T (&X)[2][4][4], T cE, T lam1
 #pragma unroll
       for (int gx = 0; gx < 3; gx++) {
         const T v = interp<T>(gz, vxy[0][gy][gx], vxy[1][gy][gx]);
-        const T cq = (wzy * T(gW(gx))) * (v * v + eta);
+        const T wq = wzy * T(gW(gx));
+        const T cq = wq * (v * v + eta);
         const T H00 = Gx[0][gy], H01 = Gy[0][gx], H02 = Gz[0][gy][gx];
         const T H10 = Gx[1][gy], H11 = Gy[1][gx], H12 = Gz[1][gy][gx];
         const T H20 = Gx[2][gy], H21 = Gy[2][gx], H22 = Gz[2][gy][gx];
-        const T lt = lr * (H00 + H11 + H22);
-        const T t00 = lt + T(2) * H00, t11 = lt + T(2) * H11, t22 = lt + T(2) * H22;
         const T t01 = H01 + H10, t02 = H02 + H20, t12 = H12 + H21;
-        Sx[0][gy] += cq * t00; Sx[1][gy] += cq * t01; Sx[2][gy] += cq * t02;
-        Sy[0][gx] += cq * t01; Sy[1][gx] += cq * t11; Sy[2][gx] += cq * t12;
-        Sz[0][gy][gx] += cq * t02; Sz[1][gy][gx] += cq * t12; Sz[2][gy][gx] += cq * t22;
+        if constexpr (UNI) {
+          const bool comp = (cm >> ((gz * 3 + gy) * 3 + gx)) & 1u;
+          const T lw = comp ? wq * (kr - T(2. / 3.) * (v * v + eta)) : cq * lr;
+          const T L = lw * (H00 + H11 + H22);
+          const T c2 = cq + cq;
+          Sx[0][gy] += L + c2 * H00; Sx[1][gy] += cq * t01; Sx[2][gy] += cq * t02;
+          Sy[0][gx] += cq * t01; Sy[1][gx] += L + c2 * H11; Sy[2][gx] += cq * t12;
+          Sz[0][gy][gx] += cq * t02; Sz[1][gy][gx] += cq * t12; Sz[2][gy][gx] += L + c2 * H22;
+        } else {
+          const T lt = lr * (H00 + H11 + H22);
+          const T t00 = lt + T(2) * H00, t11 = lt + T(2) * H11, t22 = lt + T(2) * H22;
+          Sx[0][gy] += cq * t00; Sx[1][gy] += cq * t01; Sx[2][gy] += cq * t02;
+          Sy[0][gx] += cq * t01; Sy[1][gx] += cq * t11; Sy[2][gx] += cq * t12;
+          Sz[0][gy][gx] += cq * t02; Sz[1][gy][gx] += cq * t12; Sz[2][gy][gx] += cq * t22;
+        }
       }
     }
     const T pz0 = T(gP0(gz)), pz1 = T(gP1(gz));
@@ -169,19 +185,33 @@ __device__ __forceinline__ void u_elem_apply(const T (&X)[2][4][4], T cE, T lam1
 }
 
 template <typename T>
+struct UAuxM {
+  T E;
+  uint32_t m;  // compressed Gauss points of the Jacobian state
+};
+
+template <typename T, bool UNI = false>
 struct UApplyOp {
   static constexpr int NF = 4, NC = 3;
   const T *x;
   T *y;
   const T *v;
   const double *E_elem;
+  const uint32_t *cmask;  // UNI only
   Grid G;
   BcMask bc;
   T E, lam1, mu1, eta, vol, ihx, ihy, ihz;
   __device__ T load(int f, int64_t n) const { return f < 3 ? x[3 * n + f] : v[n]; }
-  __device__ T aux(int ex, int ey, int ek) const { return E_elem ? T(E_elem[cell_id(G, ex, ey, ek)]) : E; }
-  __device__ void element(int, int, int, const T (&xin)[2][4][4], T Ee, T (&r)[8][3]) const {
-    u_elem_apply<T>(xin, Ee * vol, lam1, mu1, eta, ihx, ihy, ihz, r);
+  __device__ auto aux(int ex, int ey, int ek) const {
+    const int64_t c = cell_id(G, ex, ey, ek);
+    const T Ee = E_elem ? T(E_elem[c]) : E;
+    if constexpr (UNI) return UAuxM<T>{Ee, cmask[c]};
+    else return Ee;
+  }
+  template <typename A>
+  __device__ void element(int, int, int, const T (&xin)[2][4][4], A a, T (&r)[8][3]) const {
+    if constexpr (UNI) u_elem_apply<T, true>(xin, a.E * vol, lam1, mu1, eta, ihx, ihy, ihz, r, a.m);
+    else u_elem_apply<T>(xin, a * vol, lam1, mu1, eta, ihx, ihy, ihz, r);
   }
   __device__ void store(int i, int j, int k, const T (&acc)[3], const T (&own)[4]) const {
     int64_t n = node_id(G, i, j, k);
@@ -192,8 +222,8 @@ struct UApplyOp {
 };
 
 // FP32: at most 256 registers, two waves per SIMD
-template <typename T, int TX, int TY>
-__global__ __launch_bounds__(TX *TY, sizeof(T) == 4 ? 2 : 1) void k_u_apply(UApplyOp<T> op, int OZ) {
+template <typename T, int TX, int TY, bool UNI = false>
+__global__ __launch_bounds__(TX *TY, sizeof(T) == 4 ? 2 : 1) void k_u_apply(UApplyOp<T, UNI> op, int OZ) {
   march<T, TX, TY>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
@@ -204,6 +234,7 @@ struct UDiagOp {
   const double *v;
   double *d;
   const double *E_elem;
+  const uint32_t *cmask;  // UNILATERAL_NOCOMPRESSION state (nullptr: none)
   Grid G;
   BcMask bc;
   double E, lam1, mu1, eta, vol;
@@ -211,6 +242,8 @@ struct UDiagOp {
   __device__ void element(int ex, int ey, int ek, const double (&xin)[2][4][1], double (&r)[8][3]) const {
     double Ee = E_elem ? E_elem[cell_id(G, ex, ey, ek)] : E;
     double cE = Ee * vol;
+    const uint32_t cm = cmask ? cmask[cell_id(G, ex, ey, ek)] : 0u;
+    const double kap1 = (lam1 + 2. * mu1 / 3.) * (1. + eta);
     const double ih[3] = {1. / G.hx, 1. / G.hy, 1. / G.hz};
     for (int gz = 0; gz < 3; gz++)
       for (int gy = 0; gy < 3; gy++)
@@ -220,14 +253,17 @@ struct UDiagOp {
             int i = a & 1, j = (a >> 1) & 1, k = a >> 2;
             vv += (i ? gP1(gx) : gP0(gx)) * (j ? gP1(gy) : gP0(gy)) * (k ? gP1(gz) : gP0(gz)) * xin[k][j * 2 + i][0];
           }
-          double cq = cE * gW(gx) * gW(gy) * gW(gz) * (vv * vv + eta);
+          const double w = cE * gW(gx) * gW(gy) * gW(gz), s = vv * vv + eta;
+          // spherical weight: lam s, or kappa (1 + eta) - 2 mu s / 3 at a compressed point
+          const double lw = ((cm >> ((gz * 3 + gy) * 3 + gx)) & 1u) ? w * (kap1 - 2. * mu1 * s / 3.) : w * s * lam1;
+          const double mw = w * s * mu1;
           for (int a = 0; a < 8; a++) {
             int i = a & 1, j = (a >> 1) & 1, k = a >> 2;
             double px = i ? gP1(gx) : gP0(gx), py = j ? gP1(gy) : gP0(gy), pz = k ? gP1(gz) : gP0(gz);
             double dg[3] = {(i ? ih[0] : -ih[0]) * py * pz, (j ? ih[1] : -ih[1]) * px * pz,
                             (k ? ih[2] : -ih[2]) * px * py};
             double g2 = dg[0] * dg[0] + dg[1] * dg[1] + dg[2] * dg[2];
-            for (int c = 0; c < 3; c++) r[a][c] += cq * ((lam1 + mu1) * dg[c] * dg[c] + mu1 * g2);
+            for (int c = 0; c < 3; c++) r[a][c] += (lw + mw) * dg[c] * dg[c] + mw * g2;
           }
         }
   }
@@ -253,7 +289,7 @@ struct URhsOp {
   uint8_t ftype[3][6];   // face types for the residual BC values
   uint8_t vtype[3][8];
   double E, lam1, mu1, eta, vol, alpha, beta;
-  int crack, mode;
+  int crack, mode, uni;
   __device__ double load(int f, int64_t n) const {
     return f < 3 ? U[3 * n + f] : f == 3 ? v[n] : f == 4 ? dth[n] : p[n];
   }
@@ -261,12 +297,22 @@ struct URhsOp {
     double Ee = E_elem ? E_elem[cell_id(G, ex, ey, ek)] : E;
     double lam = lam1 * Ee, mu = mu1 * Ee;
     const double ih[3] = {1. / G.hx, 1. / G.hy, 1. / G.hz};
+    // UNILATERAL_NOCOMPRESSION: the branch of every Gauss point follows U itself
+    // (VFMech.c:1566-1615: VF_UResidual evaluates both terms at the iterate)
+    uint32_t cm = 0u;
+    if (uni) {
+      double u[8][3];
+      for (int a = 0; a < 8; a++)
+        for (int c = 0; c < 3; c++) u[a][c] = X[a >> 2][a & 3][c];
+      cm = compression_mask(u, ih[0], ih[1], ih[2]);
+    }
     if (mode == 1) {
       double Xa[2][4][4];
       for (int k = 0; k < 2; k++)
         for (int q = 0; q < 4; q++)
           for (int f = 0; f < 4; f++) Xa[k][q][f] = X[k][q][f];
-      u_elem_apply<double>(Xa, Ee * vol, lam1, mu1, eta, ih[0], ih[1], ih[2], r);
+      if (uni) u_elem_apply<double, true>(Xa, Ee * vol, lam1, mu1, eta, ih[0], ih[1], ih[2], r, cm);
+      else u_elem_apply<double>(Xa, Ee * vol, lam1, mu1, eta, ih[0], ih[1], ih[2], r);
     }
     const double coefalpha = (3. * lam + 2. * mu) * alpha;
     const double sgn = mode == 1 ? -1. : 1.;
@@ -285,7 +331,9 @@ struct URhsOp {
             gv[2] += (k ? ih[2] : -ih[2]) * px * py * xa[3];
           }
           double W = vol * gW(gx) * gW(gy) * gW(gz);
-          double tp = W * (coefalpha * th * (vv * vv + eta) + beta * pr * vv);
+          const bool comp = (cm >> ((gz * 3 + gy) * 3 + gx)) & 1u;  // VFMech.c:505-513
+          double tp = comp ? W * (coefalpha * th * (1. + eta) + beta * pr)
+                           : W * (coefalpha * th * (vv * vv + eta) + beta * pr * vv);
           double cp = crack ? W * pr : 0.;
           for (int a = 0; a < 8; a++) {
             int i = a & 1, j = (a >> 1) & 1, k = a >> 2;
@@ -361,19 +409,43 @@ static int device_slots(K kernel, int threads) {
   return per * cus;
 }
 
-template <typename T>
-static void launch_u_apply_t(const DevProblem &P, const T *x, T *y, const T *v, hipStream_t s) {
-  UApplyOp<T> op;
+template <typename T, bool UNI>
+static void launch_u_apply_u(const DevProblem &P, const T *x, T *y, const T *v, hipStream_t s) {
+  UApplyOp<T, UNI> op;
   op.x = x; op.y = y; op.v = v;
   op.E_elem = P.E_elem;
+  op.cmask = P.cmask;
   op.G = P.G; op.bc = P.bcU;
   op.E = T(P.E); op.lam1 = T(P.lam1); op.mu1 = T(P.mu1); op.eta = T(P.eta);
   op.vol = T(P.G.hx * P.G.hy * P.G.hz);
   op.ihx = T(1. / P.G.hx); op.ihy = T(1. / P.G.hy); op.ihz = T(1. / P.G.hz);
   int tiles = ((P.G.nx + 14) / 15) * ((P.G.ny + 14) / 15);
-  static int slots = device_slots(k_u_apply<T, 16, 16>, 256);
+  static int slots = device_slots(k_u_apply<T, 16, 16, UNI>, 256);
   int OZ = choose_oz(P.G, tiles, slots);
-  hipLaunchKernelGGL((k_u_apply<T, 16, 16>), march_grid(P.G, 16, 16, OZ), dim3(256), 0, s, op, OZ);
+  hipLaunchKernelGGL((k_u_apply<T, 16, 16, UNI>), march_grid(P.G, 16, 16, OZ), dim3(256), 0, s, op, OZ);
+}
+
+template <typename T>
+static void launch_u_apply_t(const DevProblem &P, const T *x, T *y, const T *v, hipStream_t s) {
+  if (P.unilateral) launch_u_apply_u<T, true>(P, x, y, v, s);
+  else launch_u_apply_u<T, false>(P, x, y, v, s);
+}
+
+// per element: the Gauss points where tr(eps(U)) < 0
+__global__ void k_u_cmask(Grid G, const double *__restrict__ U, uint32_t *__restrict__ cmask) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)G.nex * G.ney * G.nez) return;
+  const int ei = (int)(e % G.nex), ej = (int)((e / G.nex) % G.ney), ek = (int)(e / ((int64_t)G.nex * G.ney));
+  double u[8][3];
+  for (int a = 0; a < 8; a++) {
+    const int64_t n = node_id(G, ei + (a & 1), ej + ((a >> 1) & 1), ek + (a >> 2));
+    for (int c = 0; c < 3; c++) u[a][c] = U[3 * n + c];
+  }
+  cmask[e] = compression_mask(u, 1. / G.hx, 1. / G.hy, 1. / G.hz);
+}
+
+void launch_u_cmask(const DevProblem &P, const double *U, hipStream_t s) {
+  hipLaunchKernelGGL(k_u_cmask, dim3((unsigned)((P.ne + 255) / 256)), dim3(256), 0, s, P.G, U, P.cmask);
 }
 
 void launch_u_apply(const DevProblem &P, const double *x, double *y, hipStream_t s) {
@@ -386,6 +458,7 @@ void launch_u_apply_f32(const DevProblem &P, const float *x, float *y, hipStream
 void launch_u_diag(const DevProblem &P, double *d, hipStream_t s) {
   UDiagOp op;
   op.v = P.V; op.d = d; op.E_elem = P.E_elem; op.G = P.G; op.bc = P.bcU;
+  op.cmask = P.unilateral ? P.cmask : nullptr;
   op.E = P.E; op.lam1 = P.lam1; op.mu1 = P.mu1; op.eta = P.eta;
   op.vol = P.G.hx * P.G.hy * P.G.hz;
   int tiles = ((P.G.nx + 14) / 15) * ((P.G.ny + 14) / 15);
@@ -405,7 +478,7 @@ void launch_u_rhs(const DevProblem &P, const double *U, const double *BCU, doubl
   }
   op.E = P.E; op.lam1 = P.lam1; op.mu1 = P.mu1; op.eta = P.eta;
   op.vol = P.G.hx * P.G.hy * P.G.hz; op.alpha = P.alpha; op.beta = P.beta;
-  op.crack = P.hasCrackPressure; op.mode = mode;
+  op.crack = P.hasCrackPressure; op.mode = mode; op.uni = P.unilateral;
   int tiles = ((P.G.nx + 14) / 15) * ((P.G.ny + 14) / 15);
   static int slots = device_slots(k_u_rhs, 256);
   int OZ = choose_oz(P.G, tiles, slots);

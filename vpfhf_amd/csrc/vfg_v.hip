@@ -88,7 +88,17 @@ __global__ __launch_bounds__(256) void k_v_setup(DevProblem P, const double *__r
     double s11 = (lam + 2. * mu) * e11 + lam * e22 + lam * e33;
     double s22 = lam * e11 + (lam + 2. * mu) * e22 + lam * e33;
     double s33 = lam * e11 + lam * e22 + (lam + 2. * mu) * e33;
-    double W = (s11 * e11 + s22 * e22 + s33 * e33) * .5 + 2. * mu * (e12 * e12 + e23 * e23 + e13 * e13);
+    double W;
+    if (P.unilateral) {
+      // ElasticEnergyDensitySphericalDeviatoricNoCompression3D_local
+      // (VFMech.c:183-208): 1/2 sigma(eps - e0) : eps, less kappa tr^2 / 2
+      // where tr(eps) < 0 (S + D of VFMech.c:2125-2153)
+      const double tr = (H[0][0] + H[1][1]) + H[2][2];
+      W = (s11 * H[0][0] + s22 * H[1][1] + s33 * H[2][2]) * .5 + 2. * mu * (e12 * e12 + e23 * e23 + e13 * e13);
+      if (tr < 0.) W -= (lam + 2. * mu / 3.) * tr * tr * .5;
+    } else {
+      W = (s11 * e11 + s22 * e22 + s33 * e33) * .5 + 2. * mu * (e12 * e12 + e23 * e23 + e13 * e13);
+    }
     double Wg = vol * gW(gx) * gW(gy) * gW(gz);
     q[g] = Wg * (cm + 2. * W);
   }

@@ -161,6 +161,14 @@ int vfh_options_parse(vfh_options *o, const char *text) {
     else if (ieq(k, "-eta") && v) o->props.eta = atof(v);
     else if (ieq(k, "-irrevtol") && v) o->props.irrevtol = atof(v);
     else if (ieq(k, "-atnum") && v) o->props.atnum = atoi(v);
+    else if (ieq(k, "-unilateral") && v) { /* PetscOptionsEnum, VFCommon.c:176-177 */
+      if (ieq(v, "NOCOMPRESSION") || ieq(v, "UNILATERAL_NOCOMPRESSION") || ieq(v, "1"))
+        o->props.unilateral = VFG_UNILATERAL_NOCOMPRESSION;
+      else if (ieq(v, "NONE") || ieq(v, "UNILATERAL_NONE") || ieq(v, "0"))
+        o->props.unilateral = VFG_UNILATERAL_NONE;
+      else
+        rc = -1;
+    }
     else if (ieq(k, "-pressurize")) o->props.hasCrackPressure = !(v && (ieq(v, "0") || ieq(v, "false")));
     else if (ieq(k, "-altmintol") && v) o->altmintol = atof(v);
     else if (ieq(k, "-altminmaxit") && v) o->altminmaxit = atoi(v);
