@@ -144,8 +144,9 @@ def ensemble_bench(L, n_members=64, seed=0):
     Ba = (C.c_void_p * n_members)(*[p[1] for p in ptr])
     infos = (_lib.SolveInfo * n_members)()
     res = {}
-    for mixed in (False, True):
-        om = _lib.SolveOpts(1e-10, 0., 100000, _lib.PC_JACOBI, _lib.PREC_MIXED if mixed else _lib.PREC_FP64, 4)
+    for key, mixed, pc in (("fp64", False, _lib.PC_JACOBI), ("mixed", True, _lib.PC_JACOBI),
+                           ("mixed_mg", True, _lib.PC_MG)):
+        om = _lib.SolveOpts(1e-10, 0., 100000, pc, _lib.PREC_MIXED if mixed else _lib.PREC_FP64, 4)
 
         def batched():
             rc = L.vfg_batch_u_solve(bt.b, Ua, Ba, C.byref(om), infos, _lib.DEVICE)
@@ -153,7 +154,7 @@ def ensemble_bench(L, n_members=64, seed=0):
         reset(); batched(); reset()
         t0 = time.time(); batched(); tb = time.time() - t0
         Ub = download()
-        res[mixed] = (tb, max(_relerr(a, b) for a, b in zip(Ub, U64)))
+        res[key] = (tb, max(_relerr(a, b) for a, b in zip(Ub, U64)))
     bt.close()
     for H in hosts:
         H.close()
@@ -163,10 +164,14 @@ def ensemble_bench(L, n_members=64, seed=0):
             "fp64_single_solves_per_s": round(n_members / t64, 2),
             "fp64_single_multigrid_solves_per_s": round(n_members / tmg, 2),
             "max_rel_err_multigrid_vs_jacobi": max(_relerr(a, b) for a, b in zip(Umg, U64)),
-            "batched_fp64_solves_per_s": round(n_members / res[False][0], 2),
-            "batched_mixed_solves_per_s": round(n_members / res[True][0], 2),
-            "speedup_mixed_batched_vs_fp64_single": round(t64 / res[True][0], 2),
-            "max_rel_err_mixed_vs_fp64": res[True][1], "max_rel_err_batched_fp64_vs_single": res[False][1],
+            "batched_fp64_solves_per_s": round(n_members / res["fp64"][0], 2),
+            "batched_mixed_solves_per_s": round(n_members / res["mixed"][0], 2),
+            "batched_mixed_multigrid_solves_per_s": round(n_members / res["mixed_mg"][0], 2),
+            "speedup_mixed_batched_vs_fp64_single": round(t64 / res["mixed"][0], 2),
+            "speedup_mixed_multigrid_batched_vs_fp64_single": round(t64 / res["mixed_mg"][0], 2),
+            "speedup_mixed_multigrid_batched_vs_multigrid_single": round(tmg / res["mixed_mg"][0], 2),
+            "max_rel_err_mixed_vs_fp64": res["mixed"][1], "max_rel_err_batched_fp64_vs_single": res["fp64"][1],
+            "max_rel_err_mixed_multigrid_vs_fp64": res["mixed_mg"][1],
             "persistent_fp32_apply_ms": round(apply_ms, 4)}
 
 

@@ -62,12 +62,13 @@ class Batch:
         self._c((self.L.vfg_batch_u_apply_f32 if fp32 else self.L.vfg_batch_u_apply)(self.b, xa, ya))
         return [d.download(dt, len(x)) for d, x in zip(dy, xs)]
 
-    def u_solve(self, Us, BCUs, rtol=1e-10, maxit=100000, mixed=False):
+    def u_solve(self, Us, BCUs, rtol=1e-10, maxit=100000, mixed=False, pc=_lib.PC_JACOBI):
+        """Batched u-solve; pc = PC_MG (with mixed) runs the members' V-cycles batched."""
         Us = [np.array(u, np.float64, copy=True) for u in Us]
         Bs = [np.ascontiguousarray(b, np.float64) for b in BCUs]
         ua = (C.c_void_p * len(Us))(*[u.ctypes.data for u in Us])
         ba = (C.c_void_p * len(Bs))(*[b.ctypes.data for b in Bs])
-        o = _lib.SolveOpts(rtol, 0.0, maxit, _lib.PC_JACOBI, _lib.PREC_MIXED if mixed else _lib.PREC_FP64, 4)
+        o = _lib.SolveOpts(rtol, 0.0, maxit, pc, _lib.PREC_MIXED if mixed else _lib.PREC_FP64, 4)
         infos = (_lib.SolveInfo * len(Us))()
         rc = self.L.vfg_batch_u_solve(self.b, ua, ba, C.byref(o), infos, _lib.HOST)
         self._c(rc)

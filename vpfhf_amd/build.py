@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libvfg.so")
 SOURCES = ["vfg_u.hip", "vfg_v.hip", "vfg_misc.hip", "vfg_api.hip", "vfg_solve.hip",
-           "vfg_batch.hip", "vfg_engine.hip", "vfg_mg.hip"]
+           "vfg_batch.hip", "vfg_engine.hip", "vfg_mg.hip", "vfg_bmg.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "--offload-arch=gfx950", "-fPIC", "-std=c++17",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -148,6 +148,41 @@ __global__ __launch_bounds__(BRB) void k_bxr(const BVec *vt, int o, const BChunk
   if (threadIdx.x == 0) { part[2 * blockIdx.x] = rz; part[2 * blockIdx.x + 1] = rr; }
 }
 
+// multigrid CG: x += alpha p, r -= alpha q (z comes from the V-cycle)
+template <typename T>
+__global__ __launch_bounds__(BRB) void k_bxr_mg(const BVec *vt, int o, const BChunk *__restrict__ ch,
+                                                const double *sc) {
+  const BChunk c = ch[blockIdx.x];
+  if (sc[(int64_t)c.p * B_NS + B_ACTIVE] == 0.) return;  // frozen member
+  T *x = vec<T>(vt, c, o + VX), *r = vec<T>(vt, c, o + VR);
+  const T *p = vec<T>(vt, c, o + VP), *q = vec<T>(vt, c, o + VQ);
+  const double al = sc[(int64_t)c.p * B_NS + B_ALPHA];
+  for (int k = threadIdx.x; k < c.len; k += BRB) {
+    x[k] = (T)((double)x[k] + al * (double)p[k]);
+    r[k] = (T)((double)r[k] - al * (double)q[k]);
+  }
+}
+// multigrid CG: partial sums of r.z and r.r (and p = z at the start); sc
+// (optional) skips frozen members
+template <typename T>
+__global__ __launch_bounds__(BRB) void k_brz(const BVec *vt, int o, const BChunk *__restrict__ ch, const double *sc,
+                                             int setp, double *part) {
+  const BChunk c = ch[blockIdx.x];
+  if (sc && sc[(int64_t)c.p * B_NS + B_ACTIVE] == 0.) return;
+  const T *r = vec<T>(vt, c, o + VR), *z = vec<T>(vt, c, o + VZ);
+  T *p = vec<T>(vt, c, o + VP);
+  double rz = 0., rr = 0.;
+  for (int k = threadIdx.x; k < c.len; k += BRB) {
+    const double ri = r[k];
+    rz += ri * (double)z[k];
+    rr += ri * ri;
+    if (setp) p[k] = z[k];
+  }
+  rz = bbsum(rz);
+  rr = bbsum(rr);
+  if (threadIdx.x == 0) { part[2 * blockIdx.x] = rz; part[2 * blockIdx.x + 1] = rr; }
+}
+
 template <typename T>
 __global__ __launch_bounds__(BRB) void k_bp(const BVec *vt, int o, const BChunk *__restrict__ ch, const double *sc) {
   const BChunk c = ch[blockIdx.x];
@@ -194,6 +229,8 @@ __global__ __launch_bounds__(BRB) void k_bneg(const BVec *vt, const BChunk *__re
   double *R = vec<double>(vt, c, VR);
   for (int k = threadIdx.x; k < c.len; k += BRB) R[k] = -R[k];
 }
+__global__ void k_bset_active(double *sc) { sc[(int64_t)blockIdx.x * B_NS + B_ACTIVE] = 1.; }
+
 __global__ __launch_bounds__(BRB) void k_bmember_sum(const double *part, const int *first, const int *nch,
                                                      double *out) {
   const int p = blockIdx.x;
@@ -210,6 +247,7 @@ void batch_member_sums(const double *part, const int *first, const int *nch, int
 
 // ------------------------------------------------------------------ host side
 #include "vfg_batch_apply.h"
+#include "vfg_bmg.h"
 
 using namespace vfg;
 
@@ -233,8 +271,9 @@ struct vfg_batch {
   int grid = 0;  // persistent grid size (0: one block per item)
   int graph_on = 1, graph_chunk = 10;
   hipGraphExec_t gexec = nullptr, gexec32 = nullptr;
-  double g_rtol[2] = {-1., -1.};  // rtol / maxit baked into each captured graph
-  int g_maxit[2] = {-1, -1};
+  double g_rtol[2] = {-1., -1.};  // rtol / maxit / preconditioner baked into each captured graph
+  int g_maxit[2] = {-1, -1}, g_mg[2] = {-1, -1};
+  BatchMG *bmg = nullptr;  // members' multigrid (built at the first VFG_PC_MG solve)
   std::vector<void *> allocs;
 };
 
@@ -359,6 +398,7 @@ int vfg_batch_destroy(vfg_batch *b) {
   if (!b) return VFG_OK;
   if (b->stream) hipStreamSynchronize(b->stream);
   drop_graphs(b);
+  bmg_destroy(b->bmg);
   for (void *p : b->allocs)
     if (p) hipFree(p);
   if (b->stream) hipStreamDestroy(b->stream);
@@ -451,9 +491,10 @@ int vfg_batch_set_graph(vfg_batch *b, int enable, int chunk) {
 
 }  // extern "C"
 
-// one CG iteration of the whole ensemble; o = 0 (FP64 vectors) or VX32
+// one CG iteration of the whole ensemble; o = 0 (FP64 vectors) or VX32;
+// mg: the FP32 preconditioner is the members' V-cycle instead of Jacobi
 template <typename T>
-static void cg_iter(vfg_batch *b, int o, void *ops, int fp32, double rtol, int maxit) {
+static void cg_iter(vfg_batch *b, int o, void *ops, int fp32, double rtol, int maxit, bool mg) {
   hipStream_t s = b->stream;
   const BChunk *ch = b->d_ch;
   // frozen members are skipped by the apply and the vector kernels
@@ -463,8 +504,16 @@ static void cg_iter(vfg_batch *b, int o, void *ops, int fp32, double rtol, int m
                      (const double *)b->d_sc, b->d_part);
   hipLaunchKernelGGL(k_bscalar, dim3(b->np), dim3(BRB), 0, s, (const double *)b->d_part, (const int *)b->d_first,
                      (const int *)b->d_nch, 1, b->d_sc, rtol, maxit);
-  hipLaunchKernelGGL(k_bxr<T>, dim3(b->nch), dim3(BRB), 0, s, (const BVec *)b->d_vt, o, ch, (const double *)b->d_sc,
-                     b->d_part);
+  if (mg) {
+    hipLaunchKernelGGL(k_bxr_mg<T>, dim3(b->nch), dim3(BRB), 0, s, (const BVec *)b->d_vt, o, ch,
+                       (const double *)b->d_sc);
+    bmg_vcycle(b->bmg, b->d_sc + B_ACTIVE, B_NS, s);
+    hipLaunchKernelGGL(k_brz<T>, dim3(b->nch), dim3(BRB), 0, s, (const BVec *)b->d_vt, o, ch, (const double *)b->d_sc,
+                       0, b->d_part);
+  } else {
+    hipLaunchKernelGGL(k_bxr<T>, dim3(b->nch), dim3(BRB), 0, s, (const BVec *)b->d_vt, o, ch, (const double *)b->d_sc,
+                       b->d_part);
+  }
   hipLaunchKernelGGL(k_bscalar, dim3(b->np), dim3(BRB), 0, s, (const double *)b->d_part, (const int *)b->d_first,
                      (const int *)b->d_nch, 2, b->d_sc, rtol, maxit);
   hipLaunchKernelGGL(k_bp<T>, dim3(b->nch), dim3(BRB), 0, s, (const BVec *)b->d_vt, o, ch, (const double *)b->d_sc);
@@ -473,24 +522,33 @@ static void cg_iter(vfg_batch *b, int o, void *ops, int fp32, double rtol, int m
 // runs CG on all members to rtol; the iterate state (x, r) is prepared by the caller
 template <typename T>
 static int cg_run(vfg_batch *b, int o, void *ops, int fp32, double rtol, int maxit, hipGraphExec_t *gexec,
-                  std::vector<double> &S) {
+                  std::vector<double> &S, bool mg) {
   hipStream_t s = b->stream;
-  hipLaunchKernelGGL(k_binit<T>, dim3(b->nch), dim3(BRB), 0, s, (const BVec *)b->d_vt, o, (const BChunk *)b->d_ch,
-                     b->d_part);
+  if (mg) {
+    // every member starts: the V-cycle on all of them (flags = 1)
+    hipLaunchKernelGGL(k_bset_active, dim3(b->np), dim3(1), 0, s, b->d_sc);
+    bmg_vcycle(b->bmg, b->d_sc + B_ACTIVE, B_NS, s);
+    hipLaunchKernelGGL(k_brz<T>, dim3(b->nch), dim3(BRB), 0, s, (const BVec *)b->d_vt, o, (const BChunk *)b->d_ch,
+                       (const double *)nullptr, 1, b->d_part);
+  } else {
+    hipLaunchKernelGGL(k_binit<T>, dim3(b->nch), dim3(BRB), 0, s, (const BVec *)b->d_vt, o, (const BChunk *)b->d_ch,
+                       b->d_part);
+  }
   hipLaunchKernelGGL(k_bscalar, dim3(b->np), dim3(BRB), 0, s, (const double *)b->d_part, (const int *)b->d_first,
                      (const int *)b->d_nch, 0, b->d_sc, rtol, maxit);
   const int K = b->graph_chunk;
   const int gi = fp32 ? 1 : 0;
-  if (b->graph_on && *gexec && (b->g_rtol[gi] != rtol || b->g_maxit[gi] != maxit)) {
+  if (b->graph_on && *gexec && (b->g_rtol[gi] != rtol || b->g_maxit[gi] != maxit || b->g_mg[gi] != (int)mg)) {
     hipGraphExecDestroy(*gexec);
     *gexec = nullptr;
   }
   if (b->graph_on && !*gexec) {
     b->g_rtol[gi] = rtol;
     b->g_maxit[gi] = maxit;
+    b->g_mg[gi] = (int)mg;
     hipGraph_t g;
     BCHK(b, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < K; k++) cg_iter<T>(b, o, ops, fp32, rtol, maxit);
+    for (int k = 0; k < K; k++) cg_iter<T>(b, o, ops, fp32, rtol, maxit, mg);
     BCHK(b, hipStreamEndCapture(s, &g));
     hipError_t e = hipGraphInstantiate(gexec, g, nullptr, nullptr, 0);
     hipGraphDestroy(g);
@@ -503,7 +561,7 @@ static int cg_run(vfg_batch *b, int o, void *ops, int fp32, double rtol, int max
   for (int done = 0; done < maxit + K; done += K) {
     if (b->graph_on) BCHK(b, hipGraphLaunch(*gexec, s));
     else
-      for (int k = 0; k < K; k++) cg_iter<T>(b, o, ops, fp32, rtol, maxit);
+      for (int k = 0; k < K; k++) cg_iter<T>(b, o, ops, fp32, rtol, maxit, mg);
     BCHK(b, hipMemcpyAsync(S.data(), b->d_sc, 8 * S.size(), hipMemcpyDeviceToHost, s));
     BCHK(b, hipStreamSynchronize(s));
     bool any = false;
@@ -546,11 +604,12 @@ extern "C" int vfg_batch_u_solve(vfg_batch *b, double *const *U, const double *c
     ys32[p] = b->vt[p].v[VQ32];
   }
   std::vector<int> status(np, 0), iters(np, 0), outer(np, 0);
+  const bool mg = o->pc == VFG_PC_MG && o->precision == VFG_PREC_MIXED;
   std::vector<double> relres(np, 0.), S;
   int rc = VFG_OK;
   if (o->precision != VFG_PREC_MIXED) {
     fill_ops(b->m, xs.data(), ys.data(), 0, b->oz, b->d_ops, s);
-    rc = cg_run<double>(b, 0, b->d_ops, 0, o->rtol, o->maxit, &b->gexec, S);
+    rc = cg_run<double>(b, 0, b->d_ops, 0, o->rtol, o->maxit, &b->gexec, S, false);
     for (int p = 0; p < np && !rc; p++) {
       const double *Sp = &S[(size_t)p * B_NS];
       iters[p] = (int)Sp[B_ITS];
@@ -561,6 +620,21 @@ extern "C" int vfg_batch_u_solve(vfg_batch *b, double *const *U, const double *c
   } else {
     // FP64 refinement around the FP32 inner CG
     fill_ops(b->m, xs32.data(), ys32.data(), 1, b->oz, b->d_ops32, s);
+    if (mg) {
+      if (!b->bmg) {
+        std::vector<float *> r32(np), z32(np);
+        for (int p = 0; p < np; p++) {
+          r32[p] = (float *)b->vt[p].v[VR32];
+          z32[p] = (float *)b->vt[p].v[VZ32];
+        }
+        BCHK(b, hipStreamSynchronize(s));
+        if ((rc = bmg_create(b->m, r32.data(), z32.data(), &b->bmg))) return bfail(b, rc, "batched multigrid set-up failed");
+        drop_graphs(b);
+      }
+      std::vector<const double *> dinv(np);
+      for (int p = 0; p < np; p++) dinv[p] = (const double *)b->vt[p].v[VD];
+      if ((rc = bmg_setup(b->bmg, dinv.data(), s))) return bfail(b, rc, "batched multigrid set-up failed");
+    }
     std::vector<double> rn0(np, -1.), scale(np), iscale(np), nrm(np);
     std::vector<int> done(np, 0);
     const double inner_rtol = 1e-4;
@@ -592,7 +666,7 @@ extern "C" int vfg_batch_u_solve(vfg_batch *b, double *const *U, const double *c
       BCHK(b, hipMemcpyAsync(b->d_scale, scale.data(), 8 * np, hipMemcpyHostToDevice, s));
       BCHK(b, hipMemcpyAsync(b->d_iscale, iscale.data(), 8 * np, hipMemcpyHostToDevice, s));
       hipLaunchKernelGGL(k_b64to32, gch, blk, 0, s, vt, ch, (const double *)b->d_scale);
-      rc = cg_run<float>(b, VX32, b->d_ops32, 1, inner_rtol, o->maxit, &b->gexec32, S);
+      rc = cg_run<float>(b, VX32, b->d_ops32, 1, inner_rtol, o->maxit, &b->gexec32, S, mg);
       for (int p = 0; p < np && !rc; p++) {
         if (done[p]) continue;
         iters[p] += (int)S[(size_t)p * B_NS + B_ITS];

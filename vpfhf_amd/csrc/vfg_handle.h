@@ -46,6 +46,14 @@ struct vfg_handle {
   vfg::MgHier *mg = nullptr;  // u-solve multigrid (VFG_PC_MG)
 };
 
+namespace vfg {
+// multigrid (vfg_solve.hip): V-cycle smoothing degree, coarsest-level degree,
+// smallest coarse element count per direction
+constexpr int MG_SMOOTH = 2, MG_COARSE_DEG = 4, MG_MIN = 4;
+int mg_build(vfg_handle *h);                      // hierarchy shape and buffers (once)
+int mg_setup(vfg_handle *h, const double *dinv64);  // coefficients, diagonals, bounds (per solve)
+}  // namespace vfg
+
 int vfg_fail(vfg_handle *h, int code, const char *fmt, ...);
 int vfg_alloc(vfg_handle *h, void **p, size_t bytes);
 int vfg_scratch(vfg_handle *h, int64_t ndoubles);

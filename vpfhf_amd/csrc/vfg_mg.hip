@@ -16,23 +16,9 @@
 // level (their masks come from the same face/vertex table on each grid).
 #include "vfg_internal.h"
 #include "vfg_kernels.h"
+#include "vfg_mg.h"
 
 namespace vfg {
-
-// Coarse grids cover the same box with ceil(n/2) elements per direction
-// (directions already at the minimum keep theirs), so the grids are nested
-// only where n is even.  Everything between two levels is built from the 1-D
-// map xi(i) = i * nc / nf of fine node i onto the coarse node axis:
-//   prolongation: trilinear interpolation at xi, weight 1 - |xi - I| for the
-//     two coarse nodes I bracketing xi;
-//   restriction: its transpose (the same weights, gathered per coarse node);
-//   coefficients: the overlap-weighted mean of the fine elements under each
-//     coarse element.
-__device__ __forceinline__ double mg_xi(int i, int nf, int nc) { return (double)i * (double)nc / (double)nf; }
-__device__ __forceinline__ float mg_w(double xi, int I) {
-  const double d = fabs(xi - (double)I);
-  return d < 1. ? (float)(1. - d) : 0.f;
-}
 
 // ------------------------------------------------------------ coefficients
 // per fine element: E_e * sbar_e, sbar_e = sum_g w_g (v_g^2 + eta)
@@ -91,93 +77,28 @@ __global__ void k_mg_coarsen(Grid Gf, const double *__restrict__ cf, Grid Gc, do
   cc[c] = acc / wsum;
 }
 
-// ------------------------------------------------- restriction / prolongation
-// fine nodes i of one axis that coarse node I gathers from, with weights
-__device__ __forceinline__ int mg_gather(int I, int nf, int nc, int *i0, float *w) {
-  const int lo = max(0, (int)floor((double)(I - 1) * nf / nc));
-  const int hi = min(nf, (int)ceil((double)(I + 1) * nf / nc));
-  int m = 0;
-  for (int i = lo; i <= hi && m < 6; i++) {
-    const float wi = mg_w(mg_xi(i, nf, nc), I);
-    if (wi > 0.f) {
-      if (m == 0) *i0 = i;
-      w[m++] = wi;
-    }
-  }
-  return m;
-}
-
-// coarse rhs = P^T (b - y) over the free fine rows; zero on coarse Dirichlet
-// rows.  One thread per coarse (node, component).
+// one thread per coarse (node, component)
 __global__ __launch_bounds__(256) void k_mg_restrict(Grid Gf, Grid Gc, BcMask bc, const float *__restrict__ b,
                                                      const float *__restrict__ y, float *__restrict__ bcoarse) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 3 * (int64_t)Gc.nx * Gc.ny * Gc.nz) return;
   const int64_t N = t / 3;
-  const int c = (int)(t - 3 * N);
-  const int I = (int)(N % Gc.nx), J = (int)((N / Gc.nx) % Gc.ny), K = (int)(N / ((int64_t)Gc.nx * Gc.ny));
-  if (on_dirichlet(Gc, bc.face[c], bc.vert[c], I, J, K)) {
-    bcoarse[t] = 0.f;
-    return;
-  }
-  int i0 = 0, j0 = 0, k0 = 0;
-  float wx[6], wy[6], wz[6];
-  const int mx = mg_gather(I, Gf.nex, Gc.nex, &i0, wx);
-  const int my = mg_gather(J, Gf.ney, Gc.ney, &j0, wy);
-  const int mz = mg_gather(K, Gf.nez, Gc.nez, &k0, wz);
-  float acc = 0.f;
-  for (int kk = 0; kk < mz; kk++)
-    for (int jj = 0; jj < my; jj++) {
-      float row = 0.f;
-      for (int ii = 0; ii < mx; ii++) {
-        const int i = i0 + ii, j = j0 + jj, k = k0 + kk;
-        if (on_dirichlet(Gf, bc.face[c], bc.vert[c], i, j, k)) continue;
-        const int64_t n = 3 * node_id(Gf, i, j, k) + c;
-        row += wx[ii] * (b[n] - y[n]);
-      }
-      acc += wy[jj] * wz[kk] * row;
-    }
-  bcoarse[t] = acc;
+  bcoarse[t] = mg_restrict_dof(Gf, Gc, bc, b, y, N, (int)(t - 3 * N));
 }
 
-// x_fine += P x_coarse on the free fine rows
 __global__ __launch_bounds__(256) void k_mg_prolong(Grid Gf, Grid Gc, BcMask bc, const float *__restrict__ xc,
                                                     float *__restrict__ xf) {
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= (int64_t)Gf.nx * Gf.ny * Gf.nz) return;
-  const int i = (int)(n % Gf.nx), j = (int)((n / Gf.nx) % Gf.ny), k = (int)(n / ((int64_t)Gf.nx * Gf.ny));
-  const double xi = mg_xi(i, Gf.nex, Gc.nex), yj = mg_xi(j, Gf.ney, Gc.ney), zk = mg_xi(k, Gf.nez, Gc.nez);
-  const int I0 = min((int)floor(xi), Gc.nex - 1), J0 = min((int)floor(yj), Gc.ney - 1),
-            K0 = min((int)floor(zk), Gc.nez - 1);
-  const float wx[2] = {mg_w(xi, I0), mg_w(xi, I0 + 1)}, wy[2] = {mg_w(yj, J0), mg_w(yj, J0 + 1)},
-              wz[2] = {mg_w(zk, K0), mg_w(zk, K0 + 1)};
-  float acc[3] = {0.f, 0.f, 0.f};
-  for (int c3 = 0; c3 < 2; c3++)
-    for (int c2 = 0; c2 < 2; c2++)
-      for (int c1 = 0; c1 < 2; c1++) {
-        const float w = wx[c1] * wy[c2] * wz[c3];
-        if (w == 0.f) continue;
-        const int64_t N = node_id(Gc, I0 + c1, J0 + c2, K0 + c3);
-        for (int c = 0; c < 3; c++) acc[c] += w * xc[3 * N + c];
-      }
-  for (int c = 0; c < 3; c++)
-    if (!on_dirichlet(Gf, bc.face[c], bc.vert[c], i, j, k)) xf[3 * n + c] += acc[c];
+  mg_prolong_node(Gf, Gc, bc, xc, xf, n);
 }
 
 // ----------------------------------------------------------------- smoother
-// one Chebyshev step on the Jacobi-scaled system:
-//   res = b - y (y = A x; y == nullptr: x = 0, res = b)
-//   d = c1 d + c2 D^-1 res   (dfirst: d = c2 D^-1 res)
-//   x = x + d                (xzero: x = d)
 __global__ void k_mg_cheb(const float *__restrict__ b, const float *__restrict__ y, const float *__restrict__ dinv,
                           float *__restrict__ d, float *__restrict__ x, float c1, float c2, int dfirst, int xzero,
                           int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float res = y ? b[i] - y[i] : b[i];
-  const float di = (dfirst ? 0.f : c1 * d[i]) + c2 * dinv[i] * res;
-  d[i] = di;
-  x[i] = xzero ? di : x[i] + di;
+  if (i < n) mg_cheb_entry(b, y, dinv, d, x, c1, c2, dfirst, xzero, i);
 }
 
 __global__ void k_mg_recip32(const double *__restrict__ d, float *__restrict__ dinv, int64_t n) {

@@ -515,9 +515,7 @@ static int pcg32(vfg_handle *h, int64_t n, const std::function<void(const float 
 // operators of vfg_mg.hip.  Each level has ceil(n/2) elements per direction
 // where that leaves at least MG_MIN, else n; coarsening stops when no
 // direction changes.
-constexpr int MG_SMOOTH = 2, MG_COARSE_DEG = 8, MG_MIN = 4;
-
-static int mg_build(vfg_handle *h) {
+int mg_build(vfg_handle *h) {
   if (h->mg) return VFG_OK;
   MgHier *m = new MgHier();
   int rc = VFG_OK;
@@ -568,7 +566,7 @@ static int mg_build(vfg_handle *h) {
 // Refreshes the hierarchy from the current problem (V, E, properties, BCs):
 // coarse coefficients, FP32 inverse diagonals and the smoother intervals.
 // dinv64: the problem's FP64 inverse diagonal.
-static int mg_setup(vfg_handle *h, const double *dinv64) {
+int mg_setup(vfg_handle *h, const double *dinv64) {
   int rc = mg_build(h);
   if (rc) return rc;
   MgHier &m = *h->mg;

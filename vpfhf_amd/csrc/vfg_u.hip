@@ -279,6 +279,7 @@ __global__ __launch_bounds__(256) void k_u_diag(UDiagOp op, int OZ) { march<doub
 // b_{a,c} = sum_g W d_c phi_a [(3lam+2mu) alpha dth s + beta p v]          (thermo-poro)
 //         + sum_g W p (d_c v) phi_a                       (crack pressure, optional)
 // mode 0: out = b (no BC); mode 1: out = K U - b with ResidualApplyDirichletBC
+template <bool UNI>
 struct URhsOp {
   static constexpr int NF = 6, NC = 3;  // u0,u1,u2,v,dth,p
   const double *U, *v, *dth, *p, *BCU;
@@ -289,7 +290,7 @@ struct URhsOp {
   uint8_t ftype[3][6];   // face types for the residual BC values
   uint8_t vtype[3][8];
   double E, lam1, mu1, eta, vol, alpha, beta;
-  int crack, mode, uni;
+  int crack, mode;
   __device__ double load(int f, int64_t n) const {
     return f < 3 ? U[3 * n + f] : f == 3 ? v[n] : f == 4 ? dth[n] : p[n];
   }
@@ -300,7 +301,7 @@ struct URhsOp {
     // UNILATERAL_NOCOMPRESSION: the branch of every Gauss point follows U itself
     // (VFMech.c:1566-1615: VF_UResidual evaluates both terms at the iterate)
     uint32_t cm = 0u;
-    if (uni) {
+    if constexpr (UNI) {
       double u[8][3];
       for (int a = 0; a < 8; a++)
         for (int c = 0; c < 3; c++) u[a][c] = X[a >> 2][a & 3][c];
@@ -311,8 +312,7 @@ struct URhsOp {
       for (int k = 0; k < 2; k++)
         for (int q = 0; q < 4; q++)
           for (int f = 0; f < 4; f++) Xa[k][q][f] = X[k][q][f];
-      if (uni) u_elem_apply<double, true>(Xa, Ee * vol, lam1, mu1, eta, ih[0], ih[1], ih[2], r, cm);
-      else u_elem_apply<double>(Xa, Ee * vol, lam1, mu1, eta, ih[0], ih[1], ih[2], r);
+      u_elem_apply<double, UNI>(Xa, Ee * vol, lam1, mu1, eta, ih[0], ih[1], ih[2], r, cm);
     }
     const double coefalpha = (3. * lam + 2. * mu) * alpha;
     const double sgn = mode == 1 ? -1. : 1.;
@@ -368,7 +368,10 @@ struct URhsOp {
   }
 };
 
-__global__ __launch_bounds__(256) void k_u_rhs(URhsOp op, int OZ) { march<double, 16, 16>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z); }
+template <bool UNI>
+__global__ __launch_bounds__(256) void k_u_rhs(URhsOp<UNI> op, int OZ) {
+  march<double, 16, 16>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z);
+}
 
 // ------------------------------------------------------------ host launchers
 static dim3 march_grid(const Grid &G, int TX, int TY, int OZ) {
@@ -467,9 +470,10 @@ void launch_u_diag(const DevProblem &P, double *d, hipStream_t s) {
   hipLaunchKernelGGL(k_u_diag, march_grid(P.G, 16, 16, OZ), dim3(256), 0, s, op, OZ);
 }
 
-void launch_u_rhs(const DevProblem &P, const double *U, const double *BCU, double *out, int mode,
-                  hipStream_t s) {
-  URhsOp op;
+template <bool UNI>
+static void launch_u_rhs_t(const DevProblem &P, const double *U, const double *BCU, double *out, int mode,
+                           hipStream_t s) {
+  URhsOp<UNI> op;
   op.U = U ? U : P.zero3; op.v = P.V; op.dth = P.dth; op.p = P.p; op.BCU = BCU;
   op.out = out; op.E_elem = P.E_elem; op.G = P.G; op.bc = P.bcU;
   for (int c = 0; c < 3; c++) {
@@ -478,11 +482,17 @@ void launch_u_rhs(const DevProblem &P, const double *U, const double *BCU, doubl
   }
   op.E = P.E; op.lam1 = P.lam1; op.mu1 = P.mu1; op.eta = P.eta;
   op.vol = P.G.hx * P.G.hy * P.G.hz; op.alpha = P.alpha; op.beta = P.beta;
-  op.crack = P.hasCrackPressure; op.mode = mode; op.uni = P.unilateral;
+  op.crack = P.hasCrackPressure; op.mode = mode;
   int tiles = ((P.G.nx + 14) / 15) * ((P.G.ny + 14) / 15);
-  static int slots = device_slots(k_u_rhs, 256);
+  static int slots = device_slots(k_u_rhs<UNI>, 256);
   int OZ = choose_oz(P.G, tiles, slots);
-  hipLaunchKernelGGL(k_u_rhs, march_grid(P.G, 16, 16, OZ), dim3(256), 0, s, op, OZ);
+  hipLaunchKernelGGL(k_u_rhs<UNI>, march_grid(P.G, 16, 16, OZ), dim3(256), 0, s, op, OZ);
+}
+
+void launch_u_rhs(const DevProblem &P, const double *U, const double *BCU, double *out, int mode,
+                  hipStream_t s) {
+  if (P.unilateral) launch_u_rhs_t<true>(P, U, BCU, out, mode, s);
+  else launch_u_rhs_t<false>(P, U, BCU, out, mode, s);
 }
 
 }  // namespace vfg
