@@ -91,6 +91,33 @@ int  vfh_crack_opening_field(vfh_ctx *ctx, double *node);
  * 3 N_e interleaved (element terms of VFMech.c:1111-1427, 2474-2526) */
 int  vfh_energy_fields(vfh_ctx *ctx, double *cell3);
 int  vfh_get_state(vfh_ctx *ctx, double *U, double *V, double *VIrrev);
+/* the options the context was created with */
+const vfh_options *vfh_options_of(const vfh_ctx *ctx);
+
+/* ------------------------------------------------------------ field output
+ * PETSc binary Vec records as VecView writes them (int32 1211214, int32
+ * length, doubles; big-endian), nvec vectors of len[v] values (vec[v] = NULL:
+ * zeros); append != 0 adds to an existing file */
+int  vfh_petsc_vec_write(const char *file, int nvec, const double *const *vec, const int64_t *len, int append);
+/* FieldsBinaryWrite (VFCommon.c:1353-1375): <prefix>.<step, 5 digits>.bin
+ * with U, velocity, V, pmult, theta, pressure, VolCrackOpening,
+ * VolLeakOffRate; the flow fields as the reference initialises them (0),
+ * theta and pressure from the caller (NULL: 0) */
+int  vfh_fields_binary_write(vfh_ctx *ctx, const char *prefix, int step, const double *theta,
+                             const double *pressure);
+/* the Coordinates Vec of <prefix>.bin (VFCommon.c:383-424); the DMView record
+ * the reference writes before it depends on the PETSc version and is omitted */
+int  vfh_coordinates_write(vfh_ctx *ctx, const char *file);
+/* VTK XML structured grid (.vts, ASCII) with named nodal and cell arrays */
+int  vfh_vts_write(const char *file, int nx, int ny, int nz, double lx, double ly, double lz, int npt,
+                   const char *const *pnames, const double *const *pdata, int ncell,
+                   const char *const *cnames, const double *const *cdata);
+/* FieldsVTKWrite (VFCommon.c:1415-1460) for this path's fields:
+ * <prefix>_nodal.<step>.vts (Displacement_0..2, Fracture, Temperature,
+ * Pressure, Volumetric Crack Opening) and <prefix>_cell.<step>.vts (elastic
+ * energy, pressure work, surface energy of each cell) */
+int  vfh_fields_vts_write(vfh_ctx *ctx, const char *prefix, int step, const double *theta,
+                          const double *pressure);
 int  vfh_set_state(vfh_ctx *ctx, const double *U, const double *V, const double *VIrrev);
 
 /* ---------------------------------------------------------------- serving

@@ -144,3 +144,44 @@ def test_serve_field_outputs():
         assert abs(R.energy[i][:, 0].sum() - e[0]) <= 1e-12 * abs(e[0])
         assert abs(R.energy[i][:, 2].sum() - e[3]) <= 1e-12 * abs(e[3])
         assert np.all(np.isfinite(R.opening[i])) and R.opening[i].shape == R.V[i].shape
+
+
+def test_context_field_writers(tmp_path):
+    """FieldsBinaryWrite / FieldsVTKWrite of a solved context: the records in
+    the reference's order (U, velocity, V, pmult, theta, pressure,
+    VolCrackOpening, VolLeakOffRate; VFCommon.c:1363-1370) hold the context's
+    state bit for bit, and the .vts arrays hold the same values by component."""
+    from vpfhf_amd import host, serve, workloads
+    R = serve.make_requests(1, seed=4, outputs=True, lo=9, hi=12, am_range=(2, 2))
+    s = R.specs[0]
+    H, f = workloads.make_host(s)
+    H.altmin(2)
+    U, V, _ = H.state()
+    op = H.crack_opening_field()
+    E = H.energy_fields()
+    nn, ne = H.nn, H.ne
+    name = H.fields_binary_write(str(tmp_path / "run"), 3, theta=f["theta"], pressure=f["pressure"])
+    assert name.endswith("run.00003.bin")
+    r = host.petsc_vec_read(name)
+    assert [x.size for x in r] == [3 * nn, 3 * nn, nn, ne, nn, nn, nn, nn]
+    assert np.array_equal(r[0], U) and np.array_equal(r[2], V) and np.array_equal(r[6], op)
+    assert np.array_equal(r[4], f["theta"]) and np.array_equal(r[5], f["pressure"])
+    assert not r[1].any() and not r[3].any() and not r[7].any()
+    H.coordinates_write(str(tmp_path / "run.bin"))
+    X = host.petsc_vec_read(str(tmp_path / "run.bin"))[0].reshape(-1, 3)
+    g = H.opts.grid
+    assert X.shape == (nn, 3) and X[-1].tolist() == [g.lx, g.ly, g.lz] and X[1, 0] == g.lx / (g.nx - 1)
+    nod, cel = H.fields_vts_write(str(tmp_path / "run"), 3, theta=f["theta"])
+    from tests.test_fieldio import _vts
+    _, _, pts, a = _vts(nod)
+    assert np.array_equal(pts, X)
+    for c in range(3):
+        assert np.array_equal(a[("PointData", "Displacement_%d" % c)], U[c::3])
+    assert np.array_equal(a[("PointData", "Fracture")], V)
+    assert np.array_equal(a[("PointData", "Temperature")], f["theta"])
+    assert not a[("PointData", "Pressure")].any()  # pressure not given: zeros
+    assert np.array_equal(a[("PointData", "Volumetric Crack Opening")], op)
+    _, _, _, b = _vts(cel)
+    for c, k in enumerate(("Elastic Energy", "Pressure Work", "Surface Energy")):
+        assert np.array_equal(b[("CellData", k)], E[:, c])
+    H.close()

@@ -46,7 +46,8 @@ def build(verbose=False):
     # C host layer (plain C, gcc) over the C-ABI
     host = os.path.join(HERE, "libvfhost.so")
     cmd = ["gcc", "-O2", "-std=c11", "-fPIC", "-shared", "-Wall", "-o", host + ".tmp",
-           os.path.join(HERE, "host", "vfhost.c"), "-L" + HERE, "-lvfg", "-lm", "-Wl,-rpath,$ORIGIN"]
+           os.path.join(HERE, "host", "vfhost.c"), os.path.join(HERE, "host", "vfio.c"), "-L" + HERE, "-lvfg",
+           "-lm", "-Wl,-rpath,$ORIGIN"]
     p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:
         raise RuntimeError("gcc failed for vfhost.c:\n%s" % p.stderr[-4000:])

@@ -87,6 +87,13 @@ def load():
             "vfh_crack_opening_field": (C.c_int, [vp, vp]),
             "vfh_energy_fields": (C.c_int, [vp, vp]),
             "vfh_set_state": (C.c_int, [vp, vp, vp, vp]),
+            "vfh_petsc_vec_write": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(vp), C.POINTER(C.c_int64), C.c_int]),
+            "vfh_fields_binary_write": (C.c_int, [vp, C.c_char_p, C.c_int, vp, vp]),
+            "vfh_coordinates_write": (C.c_int, [vp, C.c_char_p]),
+            "vfh_vts_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                        C.c_int, C.POINTER(C.c_char_p), C.POINTER(vp), C.c_int,
+                                        C.POINTER(C.c_char_p), C.POINTER(vp)]),
+            "vfh_fields_vts_write": (C.c_int, [vp, C.c_char_p, C.c_int, vp, vp]),
             "vfh_serve_opts_default": (None, [C.POINTER(ServeOpts)]),
             "vfh_request_bytes": (C.c_int64, [C.c_int, C.c_int, C.c_int]),
             "vfh_serve": (C.c_int, [C.POINTER(ServeOpts), C.POINTER(Request), C.c_int, C.POINTER(RequestResult),
@@ -98,6 +105,60 @@ def load():
             f.argtypes = args
         _hl = L
     return _hl
+
+
+def _f64(a):
+    return None if a is None else np.ascontiguousarray(a, np.float64)
+
+
+def petsc_vec_write(file, vecs, append=False):
+    """Writes the arrays of ``vecs`` as PETSc binary Vec records (VecView with
+    a binary viewer: int32 1211214, int32 length, big-endian doubles)."""
+    keep = [_f64(v).ravel() for v in vecs]
+    n = len(keep)
+    ptrs = (C.c_void_p * max(n, 1))(*[v.ctypes.data for v in keep])
+    lens = (C.c_int64 * max(n, 1))(*[v.size for v in keep])
+    rc = load().vfh_petsc_vec_write(os.fsencode(file), n, ptrs, lens, int(bool(append)))
+    if rc:
+        raise _lib.VfgError(rc, "vfh_petsc_vec_write(%s)" % file)
+
+
+def petsc_vec_read(file):
+    """The Vec records of a PETSc binary file, in order (VecLoad,
+    Utils/vtkexport.c:94-110)."""
+    raw = open(file, "rb").read()
+    out, q = [], 0
+    while q < len(raw):
+        cid, n = np.frombuffer(raw, ">i4", 2, q)
+        if cid != 1211214:
+            raise ValueError("%s: record at byte %d is not a Vec (class id %d)" % (file, q, cid))
+        out.append(np.frombuffer(raw, ">f8", int(n), q + 8).astype(np.float64))
+        q += 8 + 8 * int(n)
+    return out
+
+
+def vts_write(file, grid_n, grid_l, point_data=None, cell_data=None):
+    """A VTK XML structured grid (.vts, ASCII) over the reference's node
+    coordinates with the named nodal and cell arrays of the two dicts."""
+    pd, cd = list((point_data or {}).items()), list((cell_data or {}).items())
+    keep = [_f64(a).ravel() for _, a in pd + cd]
+    nx, ny, nz = map(int, grid_n)
+    nn, ne = nx * ny * nz, (nx - 1) * (ny - 1) * (nz - 1)
+    for i, (k, _) in enumerate(pd + cd):
+        want = nn if i < len(pd) else ne
+        if keep[i].size != want:
+            raise ValueError("vts_write: %r has %d values, the grid needs %d" % (k, keep[i].size, want))
+
+    def arrs(items, off):
+        names = (C.c_char_p * max(len(items), 1))(*[k.encode() for k, _ in items])
+        ptrs = (C.c_void_p * max(len(items), 1))(*[keep[off + i].ctypes.data for i in range(len(items))])
+        return names, ptrs
+    pn, pp = arrs(pd, 0)
+    cn, cp = arrs(cd, len(pd))
+    rc = load().vfh_vts_write(os.fsencode(file), *map(int, grid_n), *map(float, grid_l), len(pd), pn, pp, len(cd),
+                              cn, cp)
+    if rc:
+        raise _lib.VfgError(rc, "vfh_vts_write(%s)" % file)
 
 
 def options(text=""):
@@ -181,6 +242,30 @@ class Host:
         out = np.empty(3 * self.ne)
         self._c(self.L.vfh_energy_fields(self.ctx, out.ctypes.data))
         return out.reshape(-1, 3)
+
+    def fields_binary_write(self, prefix, step, theta=None, pressure=None):
+        """FieldsBinaryWrite (VFCommon.c:1352-1373): ``<prefix>.<step>.bin``."""
+        keep = [_f64(theta), _f64(pressure)]
+        for a in keep:
+            if a is not None and a.size != self.nn:
+                raise ValueError("a nodal field needs %d values, got %d" % (self.nn, a.size))
+        self._c(self.L.vfh_fields_binary_write(self.ctx, os.fsencode(prefix), step,
+                                               *[None if a is None else a.ctypes.data for a in keep]))
+        return "%s.%05d.bin" % (prefix, step)
+
+    def coordinates_write(self, file):
+        """The Coordinates Vec of the reference's <prefix>.bin (VFCommon.c:383-424)."""
+        self._c(self.L.vfh_coordinates_write(self.ctx, os.fsencode(file)))
+
+    def fields_vts_write(self, prefix, step, theta=None, pressure=None):
+        """FieldsVTKWrite (VFCommon.c:1415-1460): the nodal and cell .vts files."""
+        keep = [_f64(theta), _f64(pressure)]
+        for a in keep:
+            if a is not None and a.size != self.nn:
+                raise ValueError("a nodal field needs %d values, got %d" % (self.nn, a.size))
+        self._c(self.L.vfh_fields_vts_write(self.ctx, os.fsencode(prefix), step,
+                                            *[None if a is None else a.ctypes.data for a in keep]))
+        return "%s_nodal.%05d.vts" % (prefix, step), "%s_cell.%05d.vts" % (prefix, step)
 
     def state(self):
         U = np.empty(3 * self.nn)

@@ -371,6 +371,8 @@ int vfh_energy_fields(vfh_ctx *c, double *cell3) {
   return vchk(c, rc);
 }
 
+const vfh_options *vfh_options_of(const vfh_ctx *c) { return c ? &c->o : NULL; }
+
 int vfh_get_state(vfh_ctx *c, double *U, double *V, double *VIrrev) {
   if (!c) return fail(NULL, VFG_E_ARG, "NULL context");
   CHK(c, vfg_synchronize(c->h));
