@@ -25,9 +25,13 @@ def test_u_solve_matches_oracle(oracle):
     Ug, info = P.VF_StepU(U0, BCU, rtol=1e-12)
     assert info.status == 0
     assert relerr(Ug, Uo) < 1e-8
-    # residual of the GPU solution through the oracle's VF_UResidual
+    # residual of the GPU solution through the oracle's VF_UResidual, relative
+    # to the initial residual F(U0) (crack pressure and Dirichlet load), the
+    # quantity rtol refers to; the slack over 1e-12 covers the difference
+    # between the solver's recursive residual and the recomputed one
     r = oracle.u_residual(op, F, Ug, BCU)
-    assert np.linalg.norm(r) <= 1e-9 * np.linalg.norm(oracle.u_rhs(op, F)) + 1e-14
+    r0 = oracle.u_residual(op, F, U0, BCU)
+    assert np.linalg.norm(r) <= 1e-10 * np.linalg.norm(r0)
     P.close()
 
 
