@@ -306,6 +306,142 @@ __device__ __forceinline__ void march(const Grid &G, const Op &op, int OZ, int b
   }
 }
 
+// march with the node planes kept in LDS.  Three planes rotate through LDS
+// (bottom and top of the current element layer, and the one being committed
+// for the next layer), so the element computation can re-read its corners
+// instead of keeping them in registers:
+//   __device__ void element_lds(int ex, int ey, int ek, const XS &xs, A aux, T (&r)[8][NC])
+// with xs(k, q, f) the value of staged field f at corner (k, q = j*2+i).  The
+// next plane is prefetched through registers as in march; the combination of
+// the element results (x by lane shuffles, y through LDS, z carried) and the
+// stores are those of march.  Two barriers per layer.  No tiled or reducing
+// ops.  PF: the next plane is loaded one layer ahead into registers (held
+// during the element computation); without PF it is loaded at the start of
+// its layer and its latency is left to the other resident workgroup.
+template <typename T, int TX, int TY, bool PF, class Op>
+__device__ __forceinline__ void march_lds(const Grid &G, const Op &op, int OZ, int bx, int by, int bz) {
+  constexpr int NF = Op::NF, NC = Op::NC;
+  constexpr int NT = TX * TY, OX = TX - 1, OY = TY - 1, SX = TX + 1, SY = TY + 1, PL = SX * SY;
+  constexpr int NPF = (PL + NT - 1) / NT;
+  static_assert(!is_tiled<Op>::value && !has_red<Op>::value, "march_lds: plain ops only");
+  __shared__ T s_in[3][NF][PL];
+  __shared__ T s_red[2 * NC][NT];
+  const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
+  const int ox0 = bx * OX, oy0 = by * OY, oz0 = bz * OZ;
+  const int ex = ox0 - 1 + tx, ey = oy0 - 1 + ty;
+  const bool exy_ok = ex >= 0 && ex < G.nex && ey >= 0 && ey < G.ney;
+  const bool owner = tx < OX && ty < OY && (ox0 + tx) < G.nx && (oy0 + ty) < G.ny;
+  const int64_t plane = (int64_t)G.nx * G.ny;
+  int64_t soff[NPF];
+  bool sok[NPF];
+#pragma unroll
+  for (int q = 0; q < NPF; q++) {
+    const int n = tid + q * NT;
+    const int lx = n % SX, ly = n / SX;
+    const int gx = ox0 - 1 + lx, gy = oy0 - 1 + ly;
+    sok[q] = n < PL && gx >= 0 && gx < G.nx && gy >= 0 && gy < G.ny;
+    soff[q] = sok[q] ? (int64_t)gx + (int64_t)G.nx * gy : 0;
+  }
+  T pf[NPF][NF];
+  auto fetch = [&](int iz) {
+    const bool zok = iz >= 0 && iz < G.nz;  // uniform
+#pragma unroll
+    for (int q = 0; q < NPF; q++) {
+      const bool ok = zok && sok[q];
+      const int64_t nd = soff[q] + plane * iz;
+#pragma unroll
+      for (int f = 0; f < NF; f++) pf[q][f] = ok ? op.load(f, nd) : T(0);
+    }
+  };
+  auto commit = [&](int b) {
+#pragma unroll
+    for (int q = 0; q < NPF; q++) {
+      const int n = tid + q * NT;
+      if (n < PL)
+#pragma unroll
+        for (int f = 0; f < NF; f++) s_in[b][f][n] = pf[q][f];
+    }
+  };
+  using AuxT = typename aux_of<Op>::type;
+  auto aux_at = [&](int ek) -> AuxT {
+    AuxT a{};
+    if constexpr (has_aux<Op>::value)
+      if (exy_ok && ek >= 0 && ek < G.nez) a = op.aux(ex, ey, ek);
+    return a;
+  };
+  const int c0 = ty * SX + tx;  // corner (j, i) = (0, 0) of this thread's element in a plane
+  T carry[NC];
+#pragma unroll
+  for (int c = 0; c < NC; c++) carry[c] = T(0);
+
+  // planes rotate through the buffers: bottom bb, top bt, next bn
+  int bb = 0, bt = 1, bn = 2;
+  fetch(oz0 - 1);
+  commit(bb);
+  if constexpr (PF) fetch(oz0);
+  AuxT aux_next = aux_at(oz0 - 1);
+  const int ek_end = min(oz0 + OZ - 1, G.nz - 1);
+  for (int ek = oz0 - 1; ek <= ek_end; ek++) {
+    const AuxT aux_cur = aux_next;
+    // buffer bt last held a plane read two layers ago; the barriers since
+    // then order this write after those reads
+    if constexpr (PF) {
+      commit(bt);     // plane ek + 1
+      fetch(ek + 2);  // in flight during this layer
+    } else {
+      fetch(ek + 1);
+      commit(bt);
+    }
+    aux_next = aux_at(ek + 1);
+    __syncthreads();
+    T r[8][NC];
+#pragma unroll
+    for (int a = 0; a < 8; a++)
+#pragma unroll
+      for (int c = 0; c < NC; c++) r[a][c] = T(0);
+    if (exy_ok && ek >= 0 && ek < G.nez) {
+      const T *pb = &s_in[bb][0][c0], *pt = &s_in[bt][0][c0];
+      auto xs = [pb, pt](int k, int q, int f) -> T {
+        return (k ? pt : pb)[f * PL + (q >> 1) * SX + (q & 1)];
+      };
+      if constexpr (has_aux<Op>::value) op.element_lds(ex, ey, ek, xs, aux_cur, r);
+      else op.element_lds(ex, ey, ek, xs, r);
+    }
+    T cx[2][2][NC];
+#pragma unroll
+    for (int kz = 0; kz < 2; kz++)
+#pragma unroll
+      for (int jy = 0; jy < 2; jy++)
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+          T right = __shfl_down(r[(kz * 2 + jy) * 2 + 0][c], 1, TX);
+          cx[kz][jy][c] = r[(kz * 2 + jy) * 2 + 1][c] + right;
+        }
+#pragma unroll
+    for (int kz = 0; kz < 2; kz++)
+#pragma unroll
+      for (int c = 0; c < NC; c++) s_red[kz * NC + c][tid] = cx[kz][0][c];
+    __syncthreads();
+    if (owner) {
+      T bot[NC], own[NF];
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        bot[c] = carry[c] + (cx[0][1][c] + s_red[c][tid + TX]);
+        carry[c] = cx[1][1][c] + s_red[NC + c][tid + TX];
+      }
+      if (ek >= oz0 && ek >= 0) {
+#pragma unroll
+        for (int f = 0; f < NF; f++) own[f] = s_in[bb][f][c0 + SX + 1];
+        op.store(ox0 + tx, oy0 + ty, ek, bot, own);
+      }
+    }
+    const int b = bb;
+    bb = bt;
+    bt = bn;
+    bn = b;
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ T interp(int g, T f0, T f1) {
   return T(gP0(g)) * f0 + T(gP1(g)) * f1;

@@ -184,6 +184,153 @@ __device__ __forceinline__ void u_elem_apply(const T (&X)[2][4][4], T cE, T lam1
   }
 }
 
+// The same element apply for march_lds: the corners are re-read from LDS in
+// every Gauss layer gz (xs(k, q, f)), so that only one layer's
+// interpolants are live and the FP64 kernel fits 256 registers (two waves
+// per SIMD).  Per layer: the corners interpolated to the layer (Uz, Vz) and
+// their z-differences (Dz); x-derivatives Gx[c][gy] for the layer; then
+// gx-outer, gy-inner over the 9 points, with interpolation written as
+// a + p1 (b - a) (one FMA per point and value).  Accumulation and the final
+// projections are those of u_elem_apply.  UNILATERAL_NONE only: the
+// NOCOMPRESSION apply keeps the register march.
+template <typename T, typename XS>
+__device__ __forceinline__ void u_elem_apply_lds(const XS &xs, T cE, T lam1, T mu1, T eta, T ihx, T ihy, T ihz,
+                                                 T (&r)[8][3]) {
+  const T lr = lam1 / mu1;
+  const T mx = mu1 * ihx, my = mu1 * ihy, mz = mu1 * ihz;
+  T Rx[3][2][2], Ry[3][2][2], Sz[3][3][3];  // Rx[c][k][j], Ry[c][k][i], Sz[c][gy][gx]
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+#pragma unroll
+    for (int p = 0; p < 2; p++)
+#pragma unroll
+      for (int q = 0; q < 2; q++) Rx[c][p][q] = Ry[c][p][q] = T(0);
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+      for (int q = 0; q < 3; q++) Sz[c][p][q] = T(0);
+  }
+#pragma unroll
+  for (int gz = 0; gz < 3; gz++) {
+    const T pz0 = T(gP0(gz)), pz1 = T(gP1(gz));
+    // compiler barrier: the corners are read again from LDS for this layer
+    // rather than kept live across layers
+    asm volatile("" ::: "memory");
+    T Dz[4][3], Vz[4], Gx[3][3], by0[3], byd[3];
+    {
+      T Uz[4][3];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          const T a = xs(0, q, c), d = xs(1, q, c) - a;
+          Uz[q][c] = a + pz1 * d;
+          Dz[q][c] = d * mz;
+        }
+        const T a = xs(0, q, 3);
+        Vz[q] = a + pz1 * (xs(1, q, 3) - a);
+      }
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const T bx0 = (Uz[1][c] - Uz[0][c]) * mx, bxd = (Uz[3][c] - Uz[2][c]) * mx - bx0;
+#pragma unroll
+        for (int gy = 0; gy < 3; gy++) Gx[c][gy] = bx0 + T(gP1(gy)) * bxd;
+        by0[c] = (Uz[2][c] - Uz[0][c]) * my;
+        byd[c] = (Uz[3][c] - Uz[1][c]) * my - by0[c];
+      }
+    }
+    T Dzx0[3], Dzx2[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      Dzx0[c] = Dz[1][c] - Dz[0][c];
+      Dzx2[c] = Dz[3][c] - Dz[2][c];
+    }
+    const T vdx0 = Vz[1] - Vz[0], vdx2 = Vz[3] - Vz[2];
+    T Sx[3][3], ay[3][2];  // Sx[c][gy] = sum_gx sigma_c0; ay[c][i] = y-projection of sum_gy sigma_c1
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+#pragma unroll
+      for (int g = 0; g < 3; g++) Sx[c][g] = T(0);
+      ay[c][0] = ay[c][1] = T(0);
+    }
+    const T wz = cE * T(gW(gz));
+#pragma unroll
+    for (int gx = 0; gx < 3; gx++) {
+      const T px = T(gP1(gx));
+      T Gy[3], az0[3], azd[3], Sy[3];
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        Gy[c] = by0[c] + px * byd[c];
+        az0[c] = Dz[0][c] + px * Dzx0[c];
+        azd[c] = Dz[2][c] + px * Dzx2[c] - az0[c];
+        Sy[c] = T(0);
+      }
+      const T av0 = Vz[0] + px * vdx0, avd = Vz[2] + px * vdx2 - av0;
+      const T wzx = wz * T(gW(gx));
+      const T w0 = wzx * T(gW(0)), w1 = wzx * T(gW(1));
+#pragma unroll
+      for (int gy = 0; gy < 3; gy++) {
+        const T v = av0 + T(gP1(gy)) * avd;
+        const T cq = (gy == 1 ? w1 : w0) * (v * v + eta);
+        const T H00 = Gx[0][gy], H01 = Gy[0], H02 = az0[0] + T(gP1(gy)) * azd[0];
+        const T H10 = Gx[1][gy], H11 = Gy[1], H12 = az0[1] + T(gP1(gy)) * azd[1];
+        const T H20 = Gx[2][gy], H21 = Gy[2], H22 = az0[2] + T(gP1(gy)) * azd[2];
+        const T t01 = H01 + H10, t02 = H02 + H20, t12 = H12 + H21;
+        const T lt = lr * (H00 + H11 + H22);
+        const T t00 = lt + T(2) * H00, t11 = lt + T(2) * H11, t22 = lt + T(2) * H22;
+        Sx[0][gy] += cq * t00; Sx[1][gy] += cq * t01; Sx[2][gy] += cq * t02;
+        Sy[0] += cq * t01; Sy[1] += cq * t11; Sy[2] += cq * t12;
+        Sz[0][gy][gx] += cq * t02; Sz[1][gy][gx] += cq * t12; Sz[2][gy][gx] += cq * t22;
+      }
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        ay[c][0] += T(gP0(gx)) * Sy[c];
+        ay[c][1] += T(gP1(gx)) * Sy[c];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      T ax0 = T(0), ax1 = T(0);
+#pragma unroll
+      for (int g = 0; g < 3; g++) { ax0 += T(gP0(g)) * Sx[c][g]; ax1 += T(gP1(g)) * Sx[c][g]; }
+      Rx[c][0][0] += pz0 * ax0; Rx[c][0][1] += pz0 * ax1;
+      Rx[c][1][0] += pz1 * ax0; Rx[c][1][1] += pz1 * ax1;
+      Ry[c][0][0] += pz0 * ay[c][0]; Ry[c][0][1] += pz0 * ay[c][1];
+      Ry[c][1][0] += pz1 * ay[c][0]; Ry[c][1][1] += pz1 * ay[c][1];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    T b[3][2];  // [gy][i]
+#pragma unroll
+    for (int gy = 0; gy < 3; gy++) {
+      T a0 = T(0), a1 = T(0);
+#pragma unroll
+      for (int gx = 0; gx < 3; gx++) { a0 += T(gP0(gx)) * Sz[c][gy][gx]; a1 += T(gP1(gx)) * Sz[c][gy][gx]; }
+      b[gy][0] = a0; b[gy][1] = a1;
+    }
+    T Rz[2][2];  // [j][i]
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      T a0 = T(0), a1 = T(0);
+#pragma unroll
+      for (int gy = 0; gy < 3; gy++) { a0 += T(gP0(gy)) * b[gy][i]; a1 += T(gP1(gy)) * b[gy][i]; }
+      Rz[0][i] = a0 * ihz; Rz[1][i] = a1 * ihz;
+    }
+#pragma unroll
+    for (int k = 0; k < 2; k++)
+#pragma unroll
+      for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+          const T vx = (i ? Rx[c][k][j] : -Rx[c][k][j]) * ihx;
+          const T vy = (j ? Ry[c][k][i] : -Ry[c][k][i]) * ihy;
+          const T vz = k ? Rz[j][i] : -Rz[j][i];
+          r[(k * 2 + j) * 2 + i][c] = vx + vy + vz;
+        }
+  }
+}
+
 template <typename T>
 struct UAuxM {
   T E;
@@ -213,6 +360,11 @@ struct UApplyOp {
     if constexpr (UNI) u_elem_apply<T, true>(xin, a.E * vol, lam1, mu1, eta, ihx, ihy, ihz, r, a.m);
     else u_elem_apply<T>(xin, a * vol, lam1, mu1, eta, ihx, ihy, ihz, r);
   }
+  template <typename XS, typename A>
+  __device__ void element_lds(int, int, int, const XS &xs, A a, T (&r)[8][3]) const {
+    static_assert(!UNI, "the NOCOMPRESSION apply uses the register march");
+    u_elem_apply_lds<T>(xs, a * vol, lam1, mu1, eta, ihx, ihy, ihz, r);
+  }
   __device__ void store(int i, int j, int k, const T (&acc)[3], const T (&own)[4]) const {
     int64_t n = node_id(G, i, j, k);
 #pragma unroll
@@ -225,6 +377,13 @@ struct UApplyOp {
 template <typename T, int TX, int TY, bool UNI = false>
 __global__ __launch_bounds__(TX *TY, sizeof(T) == 4 ? 2 : 1) void k_u_apply(UApplyOp<T, UNI> op, int OZ) {
   march<T, TX, TY>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// FP64 apply with the corners in LDS: at most 256 registers, two waves per
+// SIMD; the next plane is loaded at the start of its layer (the other
+// resident workgroup covers the latency; a register prefetch measured slower)
+__global__ __launch_bounds__(256, 2) void k_u_apply_lds(UApplyOp<double, false> op, int OZ) {
+  march_lds<double, 16, 16, false>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 // ---------------------------------------------------------------- diagonal
@@ -423,9 +582,15 @@ static void launch_u_apply_u(const DevProblem &P, const T *x, T *y, const T *v, 
   op.vol = T(P.G.hx * P.G.hy * P.G.hz);
   op.ihx = T(1. / P.G.hx); op.ihy = T(1. / P.G.hy); op.ihz = T(1. / P.G.hz);
   int tiles = ((P.G.nx + 14) / 15) * ((P.G.ny + 14) / 15);
-  static int slots = device_slots(k_u_apply<T, 16, 16, UNI>, 256);
-  int OZ = choose_oz(P.G, tiles, slots);
-  hipLaunchKernelGGL((k_u_apply<T, 16, 16, UNI>), march_grid(P.G, 16, 16, OZ), dim3(256), 0, s, op, OZ);
+  if constexpr (sizeof(T) == 8 && !UNI) {
+    static int slots = device_slots(k_u_apply_lds, 256);
+    int OZ = choose_oz(P.G, tiles, slots);
+    hipLaunchKernelGGL(k_u_apply_lds, march_grid(P.G, 16, 16, OZ), dim3(256), 0, s, op, OZ);
+  } else {
+    static int slots = device_slots(k_u_apply<T, 16, 16, UNI>, 256);
+    int OZ = choose_oz(P.G, tiles, slots);
+    hipLaunchKernelGGL((k_u_apply<T, 16, 16, UNI>), march_grid(P.G, 16, 16, OZ), dim3(256), 0, s, op, OZ);
+  }
 }
 
 template <typename T>
