@@ -24,6 +24,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
+PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector (dense)
+# FP64 operations of the apply's element routine (u_elem_apply_lds), counted
+# in the AMD assembly of this build: 978 FMA + 123 MUL + 261 ADD per element
+FP64_FLOPS_PER_ELEMENT = 2 * 978 + 123 + 261
 
 
 def apply_bytes(nn, ne, fp32=False, hetero=True):
@@ -347,6 +351,11 @@ def main():
                    "elements": ne, "dof": 3 * nn},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic},
+        # the compute roofline that bounds this kernel: FP64 vector issue
+        "valu": {"bound": "fp64-valu", "flops_per_element": FP64_FLOPS_PER_ELEMENT,
+                 "achieved": round(FP64_FLOPS_PER_ELEMENT * ne / t_step / 1e12, 2), "peak": PEAK_FP64_TFLOPS,
+                 "unit": "TFLOP/s",
+                 "frac": round(FP64_FLOPS_PER_ELEMENT * ne / t_step / 1e12 / PEAK_FP64_TFLOPS, 4)},
         "fp32_apply": {"ms": round(t32 * 1e3, 4), "gdofs": round(3 * nn / t32 / 1e9, 3),
                        "achieved_GBs": round(apply_bytes(nn, ne, fp32=True) / t32 / 1e9, 1)},
     }

@@ -1,7 +1,9 @@
 """Batching form: an ensemble of uneven-size problems solved together.
 
 Bar: the persistent multi-problem apply equals each member's own apply
-bitwise; the batched FP64 u-solve agrees with each member's single-problem
+bitwise in FP32 and within 1e-14 relative in FP64 (the single-problem FP64
+apply runs another element routine with another operation order); the batched
+FP64 u-solve agrees with each member's single-problem
 solve within 1e-8 relative; the mixed batched solve (FP32 applies in FP64
 refinement) within 1e-7 of the FP64 one; a member's batched result is
 bitwise identical alone and in the mix.
@@ -36,7 +38,7 @@ def test_batch_apply_equals_single():
     for h, x, y in zip(hosts, xs, ys):
         yy = np.empty_like(x)
         V.check(L.vfg_u_apply(h.h, x.ctypes.data, yy.ctypes.data, V.HOST), h.h)
-        assert np.array_equal(y, yy)
+        assert relerr(y, yy) < 1e-14, relerr(y, yy)
     y32 = B.u_apply([x.astype(np.float32) for x in xs], fp32=True)
     for h, x, y in zip(hosts, xs, y32):
         yy = np.empty(3 * h.nn, np.float32)

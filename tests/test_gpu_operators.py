@@ -186,14 +186,17 @@ def test_larger_grid_apply(oracle):
     P.close()
 
 
-EDGE_GRIDS = [(2, 2, 2), (3, 2, 2), (2, 40, 3), (64, 2, 2), (16, 16, 16), (31, 31, 5), (46, 17, 4), (9, 9, 97)]
+EDGE_GRIDS = [(2, 2, 2), (3, 2, 2), (2, 40, 3), (64, 2, 2), (16, 16, 16), (31, 31, 5), (46, 17, 4), (9, 9, 97),
+              (33, 33, 4), (49, 17, 3), (17, 50, 5)]
 
 
 @pytest.mark.parametrize("n", EDGE_GRIDS, ids=["x".join(map(str, g)) for g in EDGE_GRIDS])
 def test_apply_edge_grids(oracle, n):
     """One element, slabs one element thick, exact multiples of the 15-node
-    tile, and a tall grid crossing many z-chunks: u- and V-apply against the
-    assembled oracle operators (FP64 1e-12, FP32 1e-5), device pointers too."""
+    and of the 16-element tiles (full last tiles in x and y, corners shared by
+    four tiles), and a tall grid crossing many z-chunks: u- and V-apply
+    against the assembled oracle operators (FP64 1e-12, FP32 1e-5), device
+    pointers too."""
     import vpfhf_amd as V
     s = S.make(n=n, l=(1.0, 0.7, 1.3), seed=sum(n), bcU=S.MIXED_U, bcV=S.MIXED_V, atnum=2)
     op, F = S.to_oracle(oracle, s)

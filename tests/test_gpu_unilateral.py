@@ -77,24 +77,21 @@ def _oracle_newton(oracle, scale):
     return _NEWTON[scale]
 
 
-_SKIP_MG_COMPRESSED = pytest.mark.skip(
-    reason="ran past its 120 s limit inside vfg_u_solve (multigrid PCG within the Newton steps) in the build "
-           "where the NOCOMPRESSION FP64 apply ran on the LDS march; that apply is back on the register march, "
-           "and the case stays off until the cause is shown")
-
-
 @pytest.mark.parametrize("pc", ["mg", "jacobi"])
 @pytest.mark.parametrize("scale", [-0.02, 0.02])
 def test_nocomp_newton_u_solve(oracle, pc, scale):
     """Crack pressure only (no thermal or Biot load, whose branches jump at
-    tr eps = 0), compressed or stretched along x."""
-    if pc == "mg" and scale < 0:
-        pytest.skip(_SKIP_MG_COMPRESSED.kwargs["reason"])
+    tr eps = 0), compressed or stretched along x.  The multigrid solve of the
+    compressed case needs its smoother intervals re-estimated at every Newton
+    step (the compression state changes the operator); with intervals from
+    an earlier state it took 3e5 CG iterations instead of about 130, which
+    the bound on the iteration count below guards against."""
     from vpfhf_amd import _lib
     s, op, F, BCU, U0, Uo = _oracle_newton(oracle, scale)
     P = S.to_gpu(s)
     Ug, info = P.VF_StepU(U0, BCU, rtol=1e-11, pc=_lib.PC_MG if pc == "mg" else _lib.PC_JACOBI)
     assert info.status == 0 and info.outer >= 1
+    assert info.iterations < (1000 if pc == "mg" else 5000), info.iterations
     assert relerr(Ug, Uo) < 1e-8, relerr(Ug, Uo)
     r = oracle.u_residual(op, F, Ug, BCU)
     assert np.linalg.norm(r) <= 1e-10 * np.linalg.norm(oracle.u_residual(op, F, U0, BCU))

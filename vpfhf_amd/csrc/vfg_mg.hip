@@ -21,14 +21,21 @@
 namespace vfg {
 
 // ------------------------------------------------------------ coefficients
-// per fine element: E_e * sbar_e, sbar_e = sum_g w_g (v_g^2 + eta)
+// per fine element: E_e * sbar_e, sbar_e = sum_g w_g c_g with c_g = v_g^2 + eta,
+// or, at the compressed points of a UNILATERAL_NOCOMPRESSION Jacobian (cmask),
+// the factor that gives the isotropic coarse operator the point's P-wave
+// modulus: c_g = (kappa (1 + eta) + 4/3 mu s_g) / (lam + 2 mu), kappa =
+// lam + 2/3 mu (the spherical part there is not degraded)
 __global__ void k_mg_elem_coef(Grid G, const double *__restrict__ V, const double *__restrict__ E_elem, double E,
-                               double eta, double *__restrict__ out) {
+                               double eta, const uint32_t *__restrict__ cmask, double lam1, double mu1,
+                               double *__restrict__ out) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (int64_t)G.nex * G.ney * G.nez) return;
   const int ei = (int)(e % G.nex), ej = (int)((e / G.nex) % G.ney), ek = (int)(e / ((int64_t)G.nex * G.ney));
   double v[8];
   for (int a = 0; a < 8; a++) v[a] = V[node_id(G, ei + (a & 1), ej + ((a >> 1) & 1), ek + (a >> 2))];
+  const uint32_t cm = cmask ? cmask[e] : 0u;
+  const double kap1 = lam1 + 2. * mu1 / 3., pw = 1. / (lam1 + 2. * mu1);
   double s = 0.;
   for (int gz = 0; gz < 3; gz++)
     for (int gy = 0; gy < 3; gy++)
@@ -37,7 +44,9 @@ __global__ void k_mg_elem_coef(Grid G, const double *__restrict__ V, const doubl
         for (int a = 0; a < 8; a++)
           vg += ((a & 1) ? gP1(gx) : gP0(gx)) * (((a >> 1) & 1) ? gP1(gy) : gP0(gy)) * ((a >> 2) ? gP1(gz) : gP0(gz)) *
                 v[a];
-        s += gW(gx) * gW(gy) * gW(gz) * (vg * vg + eta);
+        const double sg = vg * vg + eta;
+        const bool comp = (cm >> ((gz * 3 + gy) * 3 + gx)) & 1u;
+        s += gW(gx) * gW(gy) * gW(gz) * (comp ? (kap1 * (1. + eta) + 4. * mu1 * sg / 3.) * pw : sg);
       }
   out[e] = (E_elem ? E_elem[e] : E) * s;
 }
@@ -126,7 +135,8 @@ static unsigned nbm(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 void launch_mg_elem_coef(const DevProblem &P, double *out, hipStream_t s) {
   hipLaunchKernelGGL(k_mg_elem_coef, dim3(nbm(P.ne)), dim3(256), 0, s, P.G, (const double *)P.V,
-                     (const double *)P.E_elem, P.E, P.eta, out);
+                     (const double *)P.E_elem, P.E, P.eta, (const uint32_t *)(P.unilateral ? P.cmask : nullptr),
+                     P.lam1, P.mu1, out);
 }
 void launch_mg_coarsen(const Grid &Gf, const double *cf, const Grid &Gc, double *cc, hipStream_t s) {
   const int64_t ne = (int64_t)Gc.nex * Gc.ney * Gc.nez;

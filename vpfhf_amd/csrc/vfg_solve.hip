@@ -778,6 +778,10 @@ static int u_solve_newton(vfg_handle *h, double *x, double *r, double *z, double
     vfg_solve_info li = {0, 0, 0., 0};
     int prc;
     if (o->pc == VFG_PC_MG) {
+      // the compression state, and with it the operator, changes from step to
+      // step: fresh smoother intervals every step (a stale lambda_max lets the
+      // Chebyshev smoother amplify modes above its interval)
+      if (h->mg) h->mg->setups = 0;
       if ((rc = mg_setup(h, dinv))) return rc;
       prc = pcg_mg(h, n, A, d, r, z, p, q, h->wf[0], h->wf[1], inner, 0., o->maxit, 4, &li);
     } else {
