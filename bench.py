@@ -92,6 +92,64 @@ def cpu_baseline(n_sample=48, seconds=3.0):
                       "after a %.1f s assembly; oracle C restatement of VF_UIJacobian" % (n, reps, t_asm)}
 
 
+def _config1_problem():
+    """The 32^3-element penny case with crack pressure only (beta = 0; with
+    beta = 1 the pore and crack loads of a uniform p cancel and U = 0)."""
+    from oracle import vforacle as O
+    O.lib()
+    n, h = 32, 1.0 / 32
+    bc = [O.make_bc({f: O.ZERO for f in range(6)})] * 3
+    op = O.OracleProblem((n + 1,) * 3, E=1., nu=.2, beta=0., atnum=1, Gc=1., epsilon=2 * h, pressurize=True,
+                         bcU=bc)
+    V = O.penny_v(op, (.5, .5, .5), .2, 0., 0., 3 * h)
+    F = op.fields(V=V, VIrrev=V, theta=np.zeros(op.nn), thetaRef=np.zeros(op.nn), pressure=np.full(op.nn, 1e-3))
+    return O, op, F
+
+
+def cpu_solves():
+    """The reference's u-solve on the host at the 32^3-element penny case:
+    the oracle's C restatement assembles the u-Jacobian (VF_UIJacobian) and
+    solves it with Jacobi-CG to rtol 1e-10 (one Newton step of
+    SNESNEWTONLS on a linear problem), OpenMP over rows."""
+    O, op, F = _config1_problem()
+    U = np.zeros(3 * op.nn)
+    t0 = time.time()
+    its, _ = O.u_solve(op, F, U, np.zeros(3 * op.nn), rtol=1e-10)
+    tu = time.time() - t0
+    return {"u_solve_s": round(tu, 4), "u_cg_iterations": int(its), "u_solves_per_s": round(1.0 / tu, 3),
+            "cores": int(O.lib().vfo_num_threads()), "kind": "port",
+            "sample": "32^3 elements, one penny crack, p = 1e-3 on the crack faces; assembly included, as the "
+                      "reference assembles the Jacobian at every Newton step"}, U
+
+
+def gpu_solves_config1(U_cpu=None):
+    """The same u-solve on the GPU (vfg_u_solve, multigrid-PCG to 1e-10, the
+    hierarchy set up inside the timed call), after one untimed solve that
+    loads the kernels; and one whole alternate-minimisation iteration of the
+    case through the C host."""
+    from vpfhf_amd import workloads
+    spec = workloads.spec_penny32(beta=0.0)
+    P, _ = workloads.make_problem(spec)
+    Z = np.zeros(3 * P.nn)
+    P.VF_StepU(Z, Z, rtol=1e-10)
+    P.close()
+    P, _ = workloads.make_problem(spec)
+    t0 = time.time()
+    Ug, info = P.VF_StepU(Z, Z, rtol=1e-10)
+    tu = time.time() - t0
+    P.close()
+    out = {"u_solve_s": round(tu, 4), "u_cg_iterations": info.iterations, "u_solves_per_s": round(1.0 / tu, 2)}
+    if U_cpu is not None:
+        out["rel_diff_U_vs_cpu"] = _relerr(Ug, U_cpu)
+    H, _ = workloads.make_host(spec, u_rtol=1e-10)
+    t0 = time.time()
+    ia = H.altmin(maxit=1)
+    out["altmin_iteration_s"] = round(time.time() - t0, 4)
+    out["altmin_v_newton"] = ia.v_outer
+    H.close()
+    return out
+
+
 def _relerr(a, b):
     d = float(np.linalg.norm(b))
     return float(np.linalg.norm(a - b)) / (d if d > 0 else 1.0)
@@ -307,13 +365,20 @@ def main():
                            "v_cg": info.v_its, "seconds_per_altmin_iteration": round(dt, 3),
                            "solves_per_s": round(2.0 / dt, 3),
                            "surface_energy": en["surface"], "crack_volume": en["crack_volume"]}
+        Um, _, _ = H.state()
+        # the rest of the workload's 10 alternate-minimisation iterations
+        t0 = time.time()
+        info9 = H.altmin(maxit=9)
+        dt9 = time.time() - t0
+        extra["solves"]["ten_iterations"] = {
+            "seconds": round(dt + dt9, 3), "solves_per_s": round(20.0 / (dt + dt9), 3),
+            "u_cg_iterations": info.u_its + info9.u_its, "v_cg": info.v_its + info9.v_its, "errV": info9.errV}
         # the same iteration with the Jacobi-preconditioned u-CG
         Hj, _ = workloads.make_host(spec, u_rtol=1e-10, extra="-vfg_pc jacobi")
         t0 = time.time()
         ij = Hj.altmin(maxit=1)
         dtj = time.time() - t0
         Uj, _, _ = Hj.state()
-        Um, _, _ = H.state()
         Hj.close()
         extra["solves"].update({"jacobi_u_cg_iterations": ij.u_its,
                                 "jacobi_seconds_per_altmin_iteration": round(dtj, 3),
@@ -371,6 +436,17 @@ def main():
             print("bench: %s form done in %.1f s" % (key, time.time() - t0), file=sys.stderr, flush=True)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
+    if not a.no_solves:
+        # u-solves of the 32^3-element penny case: GPU drop-in against the
+        # reference's assemble-and-CG path on the host
+        c1 = {}
+        U_cpu = None
+        if not a.no_cpu:
+            c1["cpu"], U_cpu = cpu_solves()
+        c1["gpu"] = gpu_solves_config1(U_cpu)
+        if "cpu" in c1:
+            c1["speedup_u_solve"] = round(c1["cpu"]["u_solve_s"] / c1["gpu"]["u_solve_s"], 1)
+        out["config1_solves"] = c1
     print(json.dumps(out), flush=True)
 
 
