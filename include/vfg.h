@@ -263,6 +263,9 @@ typedef struct {
   int    maxit;       /* CG iteration cap of an inner solve (100000) */
   int    graph_chunk; /* CG iterations per replay (4) */
   int    use_graph;   /* 1: captured hipGraph replays; 0: plain launches */
+  int    pc;          /* u preconditioner of the mixed engine: VFG_PC_MG (default; every
+                         resident request's FP32 V-cycle, batched level by level inside the
+                         replays) or VFG_PC_JACOBI; the FP64 engine uses Jacobi */
 } vfg_engine_opts;
 typedef struct {
   int64_t tag;

@@ -20,5 +20,5 @@ def test_request_bytes_estimate():
     L = load()
     nn, ne = 97 ** 3, 96 ** 3
     vq = 7 * 7 * 96 * 27 * 256 * 8  # tile-blocked V coupling table: 7 x 7 tiles of 16 x 16 slots
-    assert L.vfh_request_bytes(97, 97, 97) == 530 * nn + 8 * ne + vq + (5 << 20)
+    assert L.vfh_request_bytes(97, 97, 97) == 600 * nn + 8 * ne + vq + (5 << 20)
     assert L.vfh_request_bytes(25, 25, 25) < L.vfh_request_bytes(26, 25, 25)

@@ -48,7 +48,7 @@ class SolveInfo(C.Structure):
 
 class EngineOpts(C.Structure):
     _fields_ = [("max_slots", C.c_int), ("precision", C.c_int), ("u_rtol", C.c_double), ("v_rtol", C.c_double),
-                ("maxit", C.c_int), ("graph_chunk", C.c_int), ("use_graph", C.c_int)]
+                ("maxit", C.c_int), ("graph_chunk", C.c_int), ("use_graph", C.c_int), ("pc", C.c_int)]
 
 
 class EngineDone(C.Structure):

@@ -21,6 +21,7 @@
 #include <string>
 #include <vector>
 #include "vfg_batch_apply.h"
+#include "vfg_bmg.h"
 #include "vfg_handle.h"
 
 namespace vfg {
@@ -28,7 +29,10 @@ namespace vfg {
 // per-slot device scalars
 enum {
   E_RZ = 0, E_PQ, E_ALPHA, E_BETA, E_RR, E_RR0, E_ACTIVE, E_STATUS, E_ITS, E_RTOL2, E_MAXIT,
-  E_RN, E_PG2, E_GD, E_DKD, E_ERRV, E_NS
+  E_RN, E_PG2, E_GD, E_DKD, E_ERRV,
+  E_MG,    // 1: this CG is preconditioned by the slot's multigrid V-cycle
+  E_MGON,  // the V-cycle runs for the slot in this iteration (E_MG and E_ACTIVE)
+  E_NS
 };
 constexpr int ECH = 4096;  // vector entries per chunk
 constexpr int ERB = 256;
@@ -138,6 +142,21 @@ __global__ __launch_bounds__(ERB) void k_e_p(const EVec *__restrict__ vt, const 
   }
 }
 
+// part[2 pi] = r.z over the FP32 chunks of the slots whose z came from the
+// V-cycle this iteration (k_e_xr wrote the Jacobi value there)
+__global__ __launch_bounds__(ERB) void k_e_rz(const EVec *__restrict__ vt, const EChunk *__restrict__ c32,
+                                              const int *__restrict__ cnt, const double *__restrict__ sc,
+                                              double *part) {
+  const int n32 = cnt[C_F32];
+  for (int c = blockIdx.x; c < n32; c += gridDim.x) {
+    const EChunk k = c32[c];
+    if (sc[(int64_t)k.p * E_NS + E_MGON] == 0.) continue;
+    double rz = chunk_dot<float>(vt[k.p], QR32, QZ32, k);
+    rz = esum(rz);
+    if (threadIdx.x == 0) part[2 * k.pi] = rz;
+  }
+}
+
 // per-slot CG scalars from the chunk partials [first, first + nch), fixed order.
 // op 1: alpha = rz / p.q (breakdown if p.q <= 0); op 2: beta, convergence
 __global__ __launch_bounds__(ERB) void k_e_scalar(const double *part, const int *first, const int *nch, int op,
@@ -155,7 +174,10 @@ __global__ __launch_bounds__(ERB) void k_e_scalar(const double *part, const int 
   b = esum(b);
   if (threadIdx.x != 0) return;
   double *S = sc + (int64_t)p * E_NS;
-  if (S[E_ACTIVE] == 0.) return;
+  if (S[E_ACTIVE] == 0.) {
+    S[E_MGON] = 0.;
+    return;
+  }
   if (op == 1) {
     S[E_PQ] = a;
     if (!(a > 0.) || !isfinite(a)) {
@@ -165,6 +187,7 @@ __global__ __launch_bounds__(ERB) void k_e_scalar(const double *part, const int 
     } else {
       S[E_ALPHA] = S[E_RZ] / a;
     }
+    S[E_MGON] = S[E_ACTIVE] != 0. && S[E_MG] != 0. ? 1. : 0.;
   } else {
     const double rz = S[E_RZ];
     S[E_BETA] = rz != 0. ? a / rz : 0.;
@@ -184,13 +207,14 @@ __global__ __launch_bounds__(ERB) void k_e_scalar(const double *part, const int 
 }
 
 // ------------------------------------------------------- per-request kernels
+// z = dinv r (dinv == nullptr: z is given, the V-cycle's), p = z, partials of r.z and r.r
 template <typename T>
 __global__ __launch_bounds__(ERB) void k_e_init(const T *r, const T *dinv, T *z, T *p, int64_t n, double *part) {
   double rz = 0., rr = 0.;
   for (int64_t i = (int64_t)blockIdx.x * ERB + threadIdx.x; i < n; i += (int64_t)gridDim.x * ERB) {
     double ri = r[i];
-    T zi = (T)(dinv[i] * r[i]);
-    z[i] = zi;
+    T zi = dinv ? (T)(dinv[i] * r[i]) : z[i];
+    if (dinv) z[i] = zi;
     p[i] = zi;
     rz += ri * (double)zi;
     rr += ri * ri;
@@ -203,7 +227,7 @@ __global__ __launch_bounds__(ERB) void k_e_init(const T *r, const T *dinv, T *z,
   }
 }
 __global__ __launch_bounds__(ERB) void k_e_init_fin(const double *part, int nb, double *S, double rtol,
-                                                    double maxit) {
+                                                    double maxit, double mg) {
   double a = 0., b = 0.;
   for (int i = threadIdx.x; i < nb; i += ERB) {
     a += part[2 * i];
@@ -222,7 +246,15 @@ __global__ __launch_bounds__(ERB) void k_e_init_fin(const double *part, int nb, 
   S[E_MAXIT] = maxit;
   S[E_STATUS] = isfinite(b) ? 0. : (double)VFG_E_NAN;
   S[E_ACTIVE] = (b > 0. && isfinite(b)) ? 1. : 0.;
+  S[E_MG] = mg;
+  S[E_MGON] = mg != 0. && S[E_ACTIVE] != 0. ? 1. : 0.;
 }
+// a slot leaves its CG: no V-cycle for it until its next multigrid CG
+__global__ void k_e_mgoff(double *S) {
+  S[E_MG] = 0.;
+  S[E_MGON] = 0.;
+}
+__global__ void k_e_flag(double *f, int p, double v) { f[p] = v; }
 __global__ void k_e_neg(double *r, int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) r[i] = -r[i];
@@ -304,9 +336,14 @@ struct vfg_engine {
   double *d_part = nullptr;
   int cap_part = 0;
   bool dirty_lists = true, dirty_vt = true, dirty_uop = true, dirty_vop = true;
-  hipGraphExec_t g = nullptr;
+  hipGraphExec_t g = nullptr, g_mg = nullptr;  // replay graphs without / with the batched V-cycle
   int grid_apply = 0, grid_vapply = 0, grid_vec = 0;
   int64_t ticks = 0, replays = 0;
+  // multigrid of the mixed engine: one member per slot; d_one: flags of an
+  // eager V-cycle of a single slot
+  bool mg = false, dirty_mg = false;
+  BatchMG *bmg = nullptr;
+  double *d_one = nullptr;
 };
 
 static int efail(vfg_engine *e, int code, const std::string &msg) {
@@ -318,6 +355,12 @@ static int efail(vfg_engine *e, int code, const std::string &msg) {
     hipError_t er_ = (call);                                                   \
     if (er_ != hipSuccess) return efail(e, VFG_E_HIP, hipGetErrorString(er_)); \
   } while (0)
+
+static void drop_graphs(vfg_engine *e) {
+  if (e->g) hipGraphExecDestroy(e->g);
+  if (e->g_mg) hipGraphExecDestroy(e->g_mg);
+  e->g = e->g_mg = nullptr;
+}
 
 static int egrow(vfg_engine *e, void **p, int *cap, int need, size_t elem) {
   if (need <= *cap) return VFG_OK;
@@ -331,24 +374,34 @@ static int egrow(vfg_engine *e, void **p, int *cap, int need, size_t elem) {
     return efail(e, VFG_E_NOMEM, "vfg_engine: device allocation failed");
   }
   *cap = nc;
-  if (e->g) {  // the captured graph holds the old address
-    hipGraphExecDestroy(e->g);
-    e->g = nullptr;
-  }
+  drop_graphs(e);  // the captured graphs hold the old address
   return VFG_OK;
 }
 
 static double *scal(vfg_engine *e, int p, int f) { return e->d_sc + (size_t)p * E_NS + f; }
 static double hs(const vfg_engine *e, int p, int f) { return e->h_sc[(size_t)p * E_NS + f]; }
 
-// CG init of one slot: z = dinv r, p = z, scalars (ACTIVE = 1 unless r = 0)
+// CG init of one slot: z = dinv r (dinv == nullptr: z from the V-cycle), p = z,
+// scalars (ACTIVE = 1 unless r = 0)
 template <typename T>
 static void cg_init(vfg_engine *e, int p, const T *r, const T *dinv, T *z, T *pv, int64_t n, double rtol, int maxit) {
   vfg_handle *h = e->slot[p].h;
   const int nb = red_blocks(n);
   hipLaunchKernelGGL(k_e_init<T>, dim3(nb), dim3(ERB), 0, e->s, r, dinv, z, pv, n, h->part);
   hipLaunchKernelGGL(k_e_init_fin, dim3(1), dim3(ERB), 0, e->s, (const double *)h->part, nb, scal(e, p, 0), rtol,
-                     (double)maxit);
+                     (double)maxit, dinv ? 0. : 1.);
+}
+
+// the multigrid work lists follow the resident requests; a list that moved
+// invalidates the captured replay graph
+static int mg_lists(vfg_engine *e) {
+  if (!e->mg || !e->dirty_mg) return VFG_OK;
+  bool grew = false;
+  int rc = bmg_rebuild(e->bmg, e->s, &grew);
+  if (rc) return efail(e, rc, "vfg_engine: multigrid work lists failed");
+  if (grew) drop_graphs(e);
+  e->dirty_mg = false;
+  return VFG_OK;
 }
 
 static void set_vt_u(vfg_engine *e, int p) {
@@ -373,6 +426,7 @@ static void enter_cg(vfg_engine *e, int p, int kind) {
 static void leave_cg(vfg_engine *e, int p) {
   e->slot[p].kind = K_NONE;
   e->dirty_lists = true;
+  hipLaunchKernelGGL(k_e_mgoff, dim3(1), dim3(1), 0, e->s, scal(e, p, 0));
 }
 
 // line-search candidate: Vn = P(V + lam x), D = Vn - V, KD = K D, g.D, D.KD
@@ -399,6 +453,10 @@ static void finish(vfg_engine *e, int p) {
   d.errV = q.errV;
   e->done.push_back(d);
   if (q.kind != K_NONE) leave_cg(e, p);
+  if (e->mg) {
+    bmg_set_member(e->bmg, p, nullptr, nullptr, nullptr);
+    e->dirty_mg = true;
+  }
   q = ESlot();
   e->dirty_lists = true;
 }
@@ -422,6 +480,15 @@ static int advance(vfg_engine *e, int p) {
         launch_u_diag(P, h->w[QD], s);
         hipLaunchKernelGGL(k_e_prep, dim3(enb(n3)), dim3(256), 0, s, h->w[QD], mixed ? h->wf[QD32 - QX32] : nullptr,
                            n3);
+        if (e->mg) {
+          // coefficients, diagonals and smoother bounds of the slot's hierarchy
+          // from the current V; the set-up's eigenvalue estimate borrows the
+          // FP32 work vectors, so the FP32 inverse diagonal is written again
+          int rc = mg_lists(e);  // a request admitted since the last step: its levels' records
+          if (rc) return rc;
+          if ((rc = bmg_setup_member(e->bmg, p, h->w[QD], s))) return efail(e, rc, "vfg_engine: multigrid set-up failed");
+          launch_copy_d2f(h->w[QD], h->wf[QD32 - QX32], n3, s);
+        }
         q.outer = 0;
         q.rn0 = -1.;
         if (mixed) {
@@ -451,7 +518,16 @@ static int advance(vfg_engine *e, int p) {
         q.rs = rn;
         hipLaunchKernelGGL(k_e_r32, dim3(enb(n3)), dim3(256), 0, s, (const double *)h->w[QR], 1. / rn, h->wf[1],
                            h->wf[0], n3);
-        cg_init<float>(e, p, h->wf[1], h->wf[5], h->wf[2], h->wf[3], n3, 1e-4, e->o.maxit);
+        if (e->mg) {  // z0 = V-cycle(r0) of this slot alone
+          int rc = mg_lists(e);
+          if (rc) return rc;
+          hipLaunchKernelGGL(k_e_flag, dim3(1), dim3(1), 0, s, e->d_one, p, 1.);
+          bmg_vcycle(e->bmg, e->d_one, 1, s);
+          hipLaunchKernelGGL(k_e_flag, dim3(1), dim3(1), 0, s, e->d_one, p, 0.);
+          cg_init<float>(e, p, h->wf[1], (const float *)nullptr, h->wf[2], h->wf[3], n3, 1e-4, e->o.maxit);
+        } else {
+          cg_init<float>(e, p, h->wf[1], h->wf[5], h->wf[2], h->wf[3], n3, 1e-4, e->o.maxit);
+        }
         enter_cg(e, p, K_U32);
         q.st = ST_U_CG;
         return VFG_OK;
@@ -640,8 +716,9 @@ static int rebuild_lists(vfg_engine *e) {
   return VFG_OK;
 }
 
-// one CG iteration of every request in a CG phase
-static void e_iter(vfg_engine *e) {
+// one CG iteration of every request in a CG phase; with_mg: some request is in
+// a multigrid u-CG, so the batched V-cycle runs (for the flagged slots)
+static void e_iter(vfg_engine *e, bool with_mg) {
   hipStream_t s = e->s;
   const double *fl = e->d_sc + E_ACTIVE;
   if (e->o.precision == VFG_PREC_MIXED)
@@ -655,6 +732,11 @@ static void e_iter(vfg_engine *e) {
                      (const int *)e->d_nch, 1, e->d_sc);
   hipLaunchKernelGGL(k_e_xr, dim3(e->grid_vec), dim3(ERB), 0, s, (const EVec *)e->d_vt, (const EChunk *)e->d_ch[0],
                      (const EChunk *)e->d_ch[1], (const int *)e->d_cnt, (const double *)e->d_sc, e->d_part);
+  if (with_mg) {  // z = V-cycle(r) of the multigrid slots, then their r.z
+    bmg_vcycle(e->bmg, e->d_sc + E_MGON, E_NS, s);
+    hipLaunchKernelGGL(k_e_rz, dim3(e->grid_vec), dim3(ERB), 0, s, (const EVec *)e->d_vt, (const EChunk *)e->d_ch[0],
+                       (const int *)e->d_cnt, (const double *)e->d_sc, e->d_part);
+  }
   hipLaunchKernelGGL(k_e_scalar, dim3(e->ns), dim3(ERB), 0, s, (const double *)e->d_part, (const int *)e->d_first,
                      (const int *)e->d_nch, 2, e->d_sc);
   hipLaunchKernelGGL(k_e_p, dim3(e->grid_vec), dim3(ERB), 0, s, (const EVec *)e->d_vt, (const EChunk *)e->d_ch[0],
@@ -663,25 +745,29 @@ static void e_iter(vfg_engine *e) {
 
 static int replay(vfg_engine *e) {
   const int K = e->o.graph_chunk;
+  bool with_mg = false;
+  if (e->mg)
+    for (const ESlot &q : e->slot) with_mg |= q.st != ST_FREE && q.kind == K_U32;
   if (!e->o.use_graph) {
-    for (int k = 0; k < K; k++) e_iter(e);
+    for (int k = 0; k < K; k++) e_iter(e, with_mg);
     ECHK(e, hipGetLastError());
     e->replays++;
     return VFG_OK;
   }
-  if (!e->g) {
+  hipGraphExec_t &gx = with_mg ? e->g_mg : e->g;
+  if (!gx) {
     hipGraph_t gr;
     ECHK(e, hipStreamBeginCapture(e->s, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < K; k++) e_iter(e);
+    for (int k = 0; k < K; k++) e_iter(e, with_mg);
     ECHK(e, hipStreamEndCapture(e->s, &gr));
-    hipError_t er = hipGraphInstantiate(&e->g, gr, nullptr, nullptr, 0);
+    hipError_t er = hipGraphInstantiate(&gx, gr, nullptr, nullptr, 0);
     hipGraphDestroy(gr);
     if (er != hipSuccess) {
-      e->g = nullptr;
+      gx = nullptr;
       return efail(e, VFG_E_HIP, "vfg_engine: hipGraphInstantiate failed");
     }
   }
-  ECHK(e, hipGraphLaunch(e->g, e->s));
+  ECHK(e, hipGraphLaunch(gx, e->s));
   e->replays++;
   return VFG_OK;
 }
@@ -697,6 +783,7 @@ void vfg_engine_opts_default(vfg_engine_opts *o) {
   o->maxit = 100000;
   o->graph_chunk = 4;
   o->use_graph = 1;
+  o->pc = VFG_PC_MG;
 }
 
 const char *vfg_engine_last_error(const vfg_engine *e) { return e ? e->err.c_str() : "NULL engine"; }
@@ -704,7 +791,9 @@ const char *vfg_engine_last_error(const vfg_engine *e) { return e ? e->err.c_str
 int vfg_engine_destroy(vfg_engine *e) {
   if (!e) return VFG_OK;
   if (e->s) hipStreamSynchronize(e->s);
-  if (e->g) hipGraphExecDestroy(e->g);
+  drop_graphs(e);
+  bmg_destroy(e->bmg);
+  if (e->d_one) hipFree(e->d_one);
   void *ps[] = {e->d_sc, e->d_vt, e->d_uop, e->d_vop, e->d_it[0], e->d_it[1], e->d_it[2], e->d_ch[0], e->d_ch[1],
                 e->d_first, e->d_nch, e->d_cnt, e->d_part};
   for (void *p : ps)
@@ -722,7 +811,8 @@ int vfg_engine_create(const vfg_engine_opts *o, vfg_engine **out) {
   vfg_engine_opts_default(&d);
   if (!o) o = &d;
   if (o->max_slots < 1 || o->max_slots > 4096 || o->graph_chunk < 1 || !(o->u_rtol > 0.) || !(o->v_rtol > 0.) ||
-      o->maxit < 1 || (o->precision != VFG_PREC_MIXED && o->precision != VFG_PREC_FP64))
+      o->maxit < 1 || (o->precision != VFG_PREC_MIXED && o->precision != VFG_PREC_FP64) ||
+      (o->pc != VFG_PC_MG && o->pc != VFG_PC_JACOBI))
     return VFG_E_ARG;
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return VFG_E_NODEV;
@@ -752,6 +842,10 @@ int vfg_engine_create(const vfg_engine_opts *o, vfg_engine **out) {
             hipMalloc((void **)&e->d_first, sizeof(int) * e->ns) == hipSuccess &&
             hipMalloc((void **)&e->d_nch, sizeof(int) * e->ns) == hipSuccess &&
             hipMalloc((void **)&e->d_cnt, sizeof(int) * C_N) == hipSuccess;
+  e->mg = o->precision == VFG_PREC_MIXED && o->pc == VFG_PC_MG;
+  if (ok && e->mg)
+    ok = bmg_create_slots(e->ns, &e->bmg) == VFG_OK && hipMalloc((void **)&e->d_one, 8 * (size_t)e->ns) == hipSuccess &&
+         hipMemset(e->d_one, 0, 8 * (size_t)e->ns) == hipSuccess;
   if (!ok) {
     vfg_engine_destroy(e);
     return VFG_E_NOMEM;
@@ -813,6 +907,13 @@ int vfg_engine_submit(vfg_engine *e, vfg_handle *h, double *U, const double *BCU
            mixed ? (void *)h->wf[QQ32 - QX32] : (void *)h->w[QQ], mixed, oz, &e->h_uop[(size_t)p * e->uop_sz]);
   e->dirty_uop = true;
   set_vt_u(e, p);
+  if (e->mg) {  // the request's hierarchy (built once per handle) joins the batched V-cycle
+    if ((rc = bmg_set_member(e->bmg, p, h, h->wf[QR32 - QX32], h->wf[QZ32 - QX32]))) {
+      q = ESlot();
+      return efail(e, rc, "vfg_engine_submit: multigrid hierarchy failed");
+    }
+    e->dirty_mg = true;
+  }
   return VFG_OK;
 }
 
@@ -842,6 +943,7 @@ int vfg_engine_step(vfg_engine *e) {
   }
   int rc;
   if (e->dirty_lists && (rc = rebuild_lists(e))) return rc;
+  if ((rc = mg_lists(e))) return rc;
   bool any_cg = false;
   used = 0;
   for (const ESlot &q : e->slot) {

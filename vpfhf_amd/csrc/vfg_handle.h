@@ -38,6 +38,7 @@ struct vfg_handle {
   double *part = nullptr;      // reduction partials (4 x 65536)
   double *part_big = nullptr;  // partials for very large grids
   double *scal = nullptr;      // device scalars (64)
+  double *abh = nullptr;       // CG coefficient history of an eigenvalue estimate (3 x 64)
   bool v_ready = false;
   // solver work space (allocated on first use)
   double *w[16] = {nullptr};

@@ -456,6 +456,13 @@ static double lanczos_lmax(const double *ab, int m) {
 // Inner PCG in FP32 on A32 d = r (d0 = 0); returns iterations; lanczos
 // estimates of the extreme eigenvalues of M^-1 A are accumulated from the CG
 // coefficients when est != nullptr (used to size the Chebyshev interval).
+// alpha, beta and the breakdown flag of CG iteration k, kept for the host
+__global__ void k_ab_save(const double *sc, double *hist, int k) {
+  hist[3 * k] = sc[S_ALPHA];
+  hist[3 * k + 1] = sc[S_BETA];
+  hist[3 * k + 2] = sc[S_FLAG];
+}
+
 static int pcg32(vfg_handle *h, int64_t n, const std::function<void(const float *, float *)> &A, const float *dinv,
                  float *x, float *r, float *z, float *p, float *q, float *w1, float *w2, const Cheb &cb,
                  double rtol, int maxit, double *lmax_est, int *status) {
@@ -474,7 +481,43 @@ static int pcg32(vfg_handle *h, int64_t n, const std::function<void(const float 
   double rn = rn0, ab[128];
   int it = 0, n_ab = 0;
   *status = VFG_OK;
-  const int check = lmax_est ? 1 : 10;
+  if (lmax_est) {
+    // eigenvalue estimate: a fixed number of steps (at most 64), the
+    // coefficients recorded on the device and read once
+    if (!h->abh && vfg_alloc(h, (void **)&h->abh, 8 * 3 * 64)) {
+      *status = VFG_E_NOMEM;
+      return 0;
+    }
+    const int m = maxit < 64 ? maxit : 64;
+    for (int k = 0; k < m; k++) {
+      A(p, q);
+      hipLaunchKernelGGL(k_dotp32, dim3(nb), dim3(RB), 0, s, (const float *)p, (const float *)q, n, part);
+      hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(RB), 0, s, (const double *)part, nb, 1, sc);
+      hipLaunchKernelGGL(k_pcg32_xr, dim3(nb), dim3(RB), 0, s, x, r, (const float *)p, (const float *)q, n,
+                         (const double *)sc);
+      precond32(h, A, dinv, r, z, w1, w2, n, cb);
+      hipLaunchKernelGGL(k_rz32, dim3(nb), dim3(RB), 0, s, (const float *)r, (const float *)z, n, part);
+      hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(RB), 0, s, (const double *)part, nb, 2, sc);
+      hipLaunchKernelGGL(k_pcg32_p, dim3(nb), dim3(RB), 0, s, p, (const float *)z, n, (const double *)sc);
+      hipLaunchKernelGGL(k_ab_save, dim3(1), dim3(1), 0, s, (const double *)sc, h->abh, k);
+    }
+    double hist[3 * 64];
+    if (hipMemcpyAsync(hist, h->abh, 8 * 3 * m, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      *status = VFG_E_HIP;
+      return 0;
+    }
+    for (int k = 0; k < m; k++) {
+      if (hist[3 * k + 2] == 2.) { *status = VFG_E_NAN; break; }
+      if (hist[3 * k + 2] != 0.) break;  // breakdown: the steps before it stand
+      ab[2 * n_ab] = hist[3 * k];
+      ab[2 * n_ab + 1] = hist[3 * k + 1];
+      n_ab++;
+    }
+    *lmax_est = lanczos_lmax(ab, n_ab);
+    return m;
+  }
+  const int check = 10;
   while (rn > rtol * rn0 && it < maxit) {
     int todo = check < maxit - it ? check : maxit - it;
     for (int k = 0; k < todo; k++) {
@@ -494,15 +537,9 @@ static int pcg32(vfg_handle *h, int64_t n, const std::function<void(const float 
       return it;
     }
     rn = sqrt(hs[S_RR]);
-    if (lmax_est && n_ab < 64) {
-      ab[2 * n_ab] = hs[S_ALPHA];
-      ab[2 * n_ab + 1] = hs[S_BETA];
-      n_ab++;
-    }
     if (hs[S_FLAG] == 2. || !isfinite(rn)) { *status = VFG_E_NAN; break; }
     if (hs[S_FLAG] == 1.) { *status = rn <= rtol * rn0 ? VFG_OK : VFG_E_BREAKDOWN; break; }
   }
-  if (lmax_est) *lmax_est = lanczos_lmax(ab, n_ab);
   return it;
 }
 

@@ -407,12 +407,14 @@ void vfh_serve_opts_default(vfh_serve_opts *o) {
 }
 
 /* context arrays 80 B/node; handle fields, staging and solver vectors
- * 446 B/node; E 8 B/cell; the V coupling table, 27 doubles per element slot
- * of every 16 x 16 workgroup tile and element layer; reduction scratch */
+ * 446 B/node; the multigrid hierarchy of the u-solve 70 B/node (FP32
+ * level-0 vectors, coarse levels, transfer and set-up work); E 8 B/cell; the
+ * V coupling table, 27 doubles per element slot of every 16 x 16 workgroup
+ * tile and element layer; reduction scratch */
 int64_t vfh_request_bytes(int nx, int ny, int nz) {
   const int64_t nn = (int64_t)nx * ny * nz, ne = (int64_t)(nx - 1) * (ny - 1) * (nz - 1);
   const int64_t vq = (int64_t)((nx + 14) / 15) * ((ny + 14) / 15) * (nz - 1) * 27 * 256 * 8;
-  return 530 * nn + 8 * ne + vq + ((int64_t)5 << 20);
+  return 600 * nn + 8 * ne + vq + ((int64_t)5 << 20);
 }
 
 static int request_setup(const vfh_request *r, const vfh_options *o, vfh_ctx **out) {

@@ -89,7 +89,7 @@ def make_requests(n, rate=None, seed=0, outputs=False, fields=False, **kw):
 
 
 def serve(reqs, max_slots=64, hbm_budget=200e9, mixed=True, realtime=True, u_rtol=1e-10, v_rtol=1e-8,
-          graph_chunk=4, use_graph=True):
+          graph_chunk=4, use_graph=True, pc=None):
     """Runs the stream; returns (per-request result dicts, stats dict)."""
     L = load_host()
     o = ServeOpts()
@@ -100,6 +100,8 @@ def serve(reqs, max_slots=64, hbm_budget=200e9, mixed=True, realtime=True, u_rto
     o.engine.v_rtol = v_rtol
     o.engine.graph_chunk = graph_chunk
     o.engine.use_graph = int(use_graph)
+    if pc is not None:
+        o.engine.pc = pc
     o.hbm_budget = hbm_budget
     o.realtime = int(realtime)
     n = len(reqs)
