@@ -185,3 +185,22 @@ def test_context_field_writers(tmp_path):
     for c, k in enumerate(("Elastic Energy", "Pressure Work", "Surface Energy")):
         assert np.array_equal(b[("CellData", k)], E[:, c])
     H.close()
+
+
+def test_serve_multigrid_and_jacobi_engines_agree():
+    """The mixed engine's u-CG under each request's batched FP32 V-cycle (the
+    default) and under Jacobi reach the same results, within the solvers'
+    tolerances, in fewer u-iterations; requests of several sizes share the
+    batched V-cycle while they join and leave."""
+    from vpfhf_amd import serve, _lib
+    Rm = serve.make_requests(6, seed=13, outputs=True, lo=10, hi=28, am_range=(2, 3))
+    Rj = serve.make_requests(6, seed=13, outputs=True, lo=10, hi=28, am_range=(2, 3))
+    rm, stm = serve.serve(Rm, max_slots=4, realtime=False, pc=_lib.PC_MG)
+    rj, _ = serve.serve(Rj, max_slots=4, realtime=False, pc=_lib.PC_JACOBI)
+    for a, b in zip(rm, rj):
+        assert a["status"] == 0 and b["status"] == 0
+        assert a["u_its"] < b["u_its"], (a["u_its"], b["u_its"])
+    for i in range(6):
+        assert relerr(Rm.U[i], Rj.U[i]) < 1e-6, (i, relerr(Rm.U[i], Rj.U[i]))
+        assert relerr(Rm.V[i], Rj.V[i]) < 1e-6, (i, relerr(Rm.V[i], Rj.V[i]))
+    assert stm["completed"] == 6 and stm["max_resident"] == 4
