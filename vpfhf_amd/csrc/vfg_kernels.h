@@ -54,6 +54,14 @@ void launch_v_fixmask(const DevProblem &P, hipStream_t s);                 // vf
 void launch_v_apply(const DevProblem &P, const double *x, double *y, int mode, hipStream_t s,
                     const uint8_t *mask = nullptr);
 void launch_v_diag(const DevProblem &P, double *d, hipStream_t s, const uint8_t *mask = nullptr);
+// workgroups of the V kernels (tile x z-chunk); the ordered list of those
+// owning a node with mask == 0 into list[0, count), count into list[nt]
+int v_tiles(const Grid &G);
+void launch_v_tile_list(const DevProblem &P, const uint8_t *mask, uint32_t *flag, int *list, hipStream_t s);
+// launch_v_apply_cg over the listed workgroups only (p, q must be zero on the
+// others); returns the number of p.q partials (= count)
+int launch_v_apply_cg_list(const DevProblem &P, const double *pold, double *p, const double *z, const double *beta,
+                           double *q, double *part, const uint8_t *mask, const int *list, int count, hipStream_t s);
 // CG-fused masked V apply (mask: fixed or active nodes): p = z + beta p_old
 // staged, q = K p, p and q written, one partial of p.q per workgroup; returns
 // the number of partials

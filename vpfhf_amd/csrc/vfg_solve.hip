@@ -990,6 +990,10 @@ int vfg_v_solve(vfg_handle *h, const double *U, double *V, const double *ub, con
   int rc;
   if ((rc = work(h, 13, 3 * n))) return rc;
   if ((rc = vfg_alloc(h, (void **)&h->act, n))) return rc;
+  const int ntiles = v_tiles(P.G);
+  if ((rc = vfg_alloc(h, (void **)&h->vtflag, 4 * (size_t)ntiles)) ||
+      (rc = vfg_alloc(h, (void **)&h->vtlist, 4 * (size_t)(ntiles + 1))))
+    return rc;
   double *x = h->w[0], *r = h->w[1], *z = h->w[2], *p = h->w[3], *q = h->w[4], *dinv = h->w[5];
   double *Vd = h->w[6], *g = h->w[7], *Vn = h->w[8], *D = h->w[9], *KD = h->w[10], *ubd = nullptr;
   double *Ud = h->w[11];
@@ -1018,8 +1022,14 @@ int vfg_v_solve(vfg_handle *h, const double *U, double *V, const double *ub, con
     hipLaunchKernelGGL(k_v_active, dim3(nb), dim3(RB), 0, s, (const double *)Vd, (const double *)g,
                        (const uint8_t *)P.vfix, (const double *)ubd, act, r, n, h->part);
     launch_dot(r, r, n, h->part + 4096, h->scal + 16, s);  // ||g_I||^2 (same value, fixed order)
+    // With irreversibility most nodes sit at their upper bound: the reduced
+    // system lives on a band around the cracks.  The workgroups of the fused
+    // apply that own a free node are listed, and CG runs over those only.
+    launch_v_tile_list(P, act, h->vtflag, h->vtlist, s);
     double pg2 = 0.;
+    int nlist = 0;
     if (hipMemcpyAsync(&pg2, h->scal + 16, 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(&nlist, h->vtlist + ntiles, 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
       return vfg_fail(h, VFG_E_HIP, "vfg_v_solve: reduction failed");
     double pg = sqrt(pg2);
@@ -1029,9 +1039,14 @@ int vfg_v_solve(vfg_handle *h, const double *U, double *V, const double *ub, con
     // reduced Newton system on the inactive set
     launch_v_diag(P, q, s, act);
     hipLaunchKernelGGL(k_recip, dim3(nb_(n)), dim3(256), 0, s, (const double *)q, dinv, n);
+    // the unlisted workgroups are never written: p, q are zero there (p and
+    // z vanish on active nodes, and only those are owned by such workgroups)
     launch_fill(x, 0., n, s);
+    launch_fill(q, 0., n, s);
+    launch_fill(h->w[12], 0., n, s);
+    const int *vlist = h->vtlist;
     FusedFn A = [&](const double *pold, double *pn, double *qq) {
-      return launch_v_apply_cg(P, pold, pn, z, h->scal + S_BETA, qq, h->part, act, s);
+      return launch_v_apply_cg_list(P, pold, pn, z, h->scal + S_BETA, qq, h->part, act, vlist, nlist, s);
     };
     const double inner_rtol = fmax(inner_floor, fmin(0.1, 0.1 * pg / pg0));
     int prc = pcg_fused(h, n, A, dinv, x, r, z, p, h->w[12], q, inner_rtol, 0., 20000, 20, &inf);

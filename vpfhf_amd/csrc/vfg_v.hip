@@ -292,6 +292,53 @@ __global__ __launch_bounds__(256, 3) void k_v_apply_cg(VApplyCgOp op, int OZ) {
 
 __global__ __launch_bounds__(256, 3) void k_v_apply(VApplyOp op, int OZ) { march<double, 16, 16>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z); }
 
+// The same over a list of workgroups (tile + z-chunk, linear id bx + gx (by +
+// gy bz)): the reduced Newton systems of the V-solve live on the free nodes,
+// a thin band around the cracks, and a workgroup none of whose owned nodes is
+// free has p = q = 0 throughout.  Partials are written compactly per list entry.
+__global__ __launch_bounds__(256, 3) void k_v_apply_cg_list(VApplyCgOp op, int OZ, const int *__restrict__ list,
+                                                           int gx, int gy) {
+  const int t = list[blockIdx.x];
+  march<double, 16, 16>(op.G, op, OZ, t % gx, (t / gx) % gy, t / (gx * gy), blockIdx.x);
+}
+
+__global__ void k_fill_u32(uint32_t *a, uint32_t v, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+// flag[w] = 1 for the workgroups owning a node with mask == 0
+__global__ void k_v_mark_tiles(Grid G, int OZ, const uint8_t *__restrict__ mask, uint32_t *__restrict__ flag) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= (int64_t)G.nx * G.ny * G.nz || mask[n]) return;
+  const int i = (int)(n % G.nx), j = (int)((n / G.nx) % G.ny), k = (int)(n / ((int64_t)G.nx * G.ny));
+  const int gx = (G.nx + 14) / 15, gy = (G.ny + 14) / 15;
+  flag[i / 15 + gx * (j / 15 + gy * (k / OZ))] = 1u;  // same value from every writer
+}
+
+// ordered list of the flagged workgroups (one workgroup of 256 threads);
+// list[nt] = their count
+__global__ __launch_bounds__(256) void k_v_list_tiles(const uint32_t *__restrict__ flag, int nt, int *__restrict__ list) {
+  __shared__ int sc[256];
+  int base = 0;
+  for (int b0 = 0; b0 < nt; b0 += 256) {
+    const int t = b0 + (int)threadIdx.x;
+    const int f = t < nt && flag[t] ? 1 : 0;
+    sc[threadIdx.x] = f;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {  // inclusive scan
+      const int v = threadIdx.x >= (unsigned)o ? sc[threadIdx.x - o] : 0;
+      __syncthreads();
+      sc[threadIdx.x] += v;
+      __syncthreads();
+    }
+    if (f) list[base + sc[threadIdx.x] - 1] = t;
+    base += sc[255];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) list[nt] = base;
+}
+
 // ---------------------------------------------------------------- diagonal
 struct VDiagOp {
   static constexpr int NF = 1, NC = 1;
@@ -472,8 +519,8 @@ void launch_engine_v_apply(const void *d_ops, const BItem *items, const int *dco
                      fstride);
 }
 
-int launch_v_apply_cg(const DevProblem &P, const double *pold, double *p, const double *z, const double *beta,
-                      double *q, double *part, const uint8_t *mask, hipStream_t s) {
+static VApplyCgOp make_vcg_op(const DevProblem &P, const double *pold, double *p, const double *z,
+                              const double *beta, double *q, double *part, const uint8_t *mask) {
   VApplyCgOp op;
   op.x = pold; op.y = q; op.q = P.Vq; op.bV = P.bV; op.VIrrev = P.VIrrev; op.vfix = mask ? mask : P.vfix;
   op.G = P.G; op.ne = P.ne; op.C = make_vconst(P); op.bc = P.bcV;
@@ -481,10 +528,38 @@ int launch_v_apply_cg(const DevProblem &P, const double *pold, double *p, const 
   for (int v = 0; v < 8; v++) op.vtype[v] = (uint8_t)P.hbcV.vertex[v];
   op.mode = 0; op.atnum = P.atnum; op.irrevtol = P.irrevtol;
   op.z = z; op.beta = beta; op.pw = p; op.part = part;
+  return op;
+}
+
+int launch_v_apply_cg(const DevProblem &P, const double *pold, double *p, const double *z, const double *beta,
+                      double *q, double *part, const uint8_t *mask, hipStream_t s) {
+  const VApplyCgOp op = make_vcg_op(P, pold, p, z, beta, q, part, mask);
   const int OZ = oz_for(P.G);
   const dim3 g = mgrid(P.G, OZ);
   hipLaunchKernelGGL(k_v_apply_cg, g, dim3(256), 0, s, op, OZ);
   return (int)(g.x * g.y * g.z);
+}
+
+int v_tiles(const Grid &G) {
+  const dim3 g = mgrid(G, oz_for(G));
+  return (int)(g.x * g.y * g.z);
+}
+
+void launch_v_tile_list(const DevProblem &P, const uint8_t *mask, uint32_t *flag, int *list, hipStream_t s) {
+  const int OZ = oz_for(P.G), nt = v_tiles(P.G);
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, flag, 0u, (int64_t)nt);
+  hipLaunchKernelGGL(k_v_mark_tiles, dim3((unsigned)((P.nn + 255) / 256)), dim3(256), 0, s, P.G, OZ, mask, flag);
+  hipLaunchKernelGGL(k_v_list_tiles, dim3(1), dim3(256), 0, s, (const uint32_t *)flag, nt, list);
+}
+
+int launch_v_apply_cg_list(const DevProblem &P, const double *pold, double *p, const double *z, const double *beta,
+                           double *q, double *part, const uint8_t *mask, const int *list, int count, hipStream_t s) {
+  if (count <= 0) return 0;
+  const VApplyCgOp op = make_vcg_op(P, pold, p, z, beta, q, part, mask);
+  const int OZ = oz_for(P.G);
+  const dim3 g = mgrid(P.G, OZ);
+  hipLaunchKernelGGL(k_v_apply_cg_list, dim3((unsigned)count), dim3(256), 0, s, op, OZ, list, (int)g.x, (int)g.y);
+  return count;
 }
 
 void launch_v_diag(const DevProblem &P, double *d, hipStream_t s, const uint8_t *mask) {
