@@ -39,8 +39,8 @@ struct vfg_handle {
   double *part_big = nullptr;  // partials for very large grids
   double *scal = nullptr;      // device scalars (64)
   double *abh = nullptr;       // CG coefficient history of an eigenvalue estimate (3 x 64)
-  uint32_t *vtflag = nullptr;  // V-solve: workgroups owning free nodes (flags, v_tiles)
-  int *vtlist = nullptr;       //          and their ordered list + count (v_tiles + 1)
+  uint32_t *vflag = nullptr;   // V-solve: node-plane chunks holding free nodes (v_chunk_cap flags)
+  int *vlist = nullptr;        //          and their list, count and chunk length (v_chunk_cap + 2)
   bool v_ready = false;
   // solver work space (allocated on first use)
   double *w[16] = {nullptr};

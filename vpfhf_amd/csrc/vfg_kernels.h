@@ -54,14 +54,20 @@ void launch_v_fixmask(const DevProblem &P, hipStream_t s);                 // vf
 void launch_v_apply(const DevProblem &P, const double *x, double *y, int mode, hipStream_t s,
                     const uint8_t *mask = nullptr);
 void launch_v_diag(const DevProblem &P, double *d, hipStream_t s, const uint8_t *mask = nullptr);
-// workgroups of the V kernels (tile x z-chunk); the ordered list of those
-// owning a node with mask == 0 into list[0, count), count into list[nt]
-int v_tiles(const Grid &G);
-void launch_v_tile_list(const DevProblem &P, const uint8_t *mask, uint32_t *flag, int *list, hipStream_t s);
-// launch_v_apply_cg over the listed workgroups only (p, q must be zero on the
+// Chunks of the V kernels: node planes [zc L, zc L + L) of a 15 x 15 node
+// tile, id tile + txy zc.  launch_v_chunk_list lists, in increasing id, the
+// chunks holding a node with mask == 0 into list[0, count), with count in
+// list[cap] and L in list[cap + 1] (cap = v_chunk_cap, flag holds cap words).
+int v_chunk_cap(const Grid &G);
+void launch_v_chunk_list(const DevProblem &P, const uint8_t *mask, uint32_t *flag, int *list, hipStream_t s);
+// launch_v_apply_cg over the listed chunks only (p, q must be zero on the
 // others); returns the number of p.q partials (= count)
 int launch_v_apply_cg_list(const DevProblem &P, const double *pold, double *p, const double *z, const double *beta,
                            double *q, double *part, const uint8_t *mask, const int *list, int count, hipStream_t s);
+// the CG update x += a p, r -= a q, z = dinv r over the listed chunks, with
+// (r.z, r.r) partial pairs; returns their number (= count)
+int launch_v_xr_list(const Grid &G, const int *list, int count, double *x, double *r, double *z, const double *p,
+                     const double *q, const double *dinv, const double *alpha, double *part, hipStream_t s);
 // CG-fused masked V apply (mask: fixed or active nodes): p = z + beta p_old
 // staged, q = K p, p and q written, one partial of p.q per workgroup; returns
 // the number of partials

@@ -122,6 +122,9 @@ static unsigned nb_(int64_t n) { return (unsigned)((n + 255) / 256); }
 typedef std::function<void(const double *, double *)> ApplyFn;
 // fused apply: (p_old, p, q) -> number of p.q partials written to h->part
 typedef std::function<int(const double *, double *, double *)> FusedFn;
+// CG update over a subset of the nodes: (p, q) -> number of (r.z, r.r)
+// partial pairs written to h->part
+typedef std::function<int(const double *, const double *)> XrFn;
 
 // Preconditioned CG with the direction update and the p.q product fused into
 // the operator apply (launch_v_apply_cg): per iteration
@@ -129,7 +132,7 @@ typedef std::function<int(const double *, double *, double *)> FusedFn;
 // between two arrays.  Same convergence test and reporting as pcg().
 static int pcg_fused(vfg_handle *h, int64_t n, const FusedFn &A, const double *dinv, double *x, double *r,
                      double *z, double *p, double *p2, double *q, double rtol, double atol, int maxit, int check,
-                     vfg_solve_info *info);
+                     vfg_solve_info *info, const XrFn *xr = nullptr);
 
 // Preconditioned CG on the device.  On entry x is the initial guess and r the
 // initial residual b - A x.  Converged when ||r|| <= max(rtol*||r0||, atol).
@@ -180,7 +183,7 @@ __global__ void k_set_scalar(double *sc, int i, double v) { sc[i] = v; }
 
 static int pcg_fused(vfg_handle *h, int64_t n, const FusedFn &A, const double *dinv, double *x, double *r,
                      double *z, double *p, double *p2, double *q, double rtol, double atol, int maxit, int check,
-                     vfg_solve_info *info) {
+                     vfg_solve_info *info, const XrFn *xr) {
   hipStream_t s = h->stream;
   double *sc = h->scal, *part = h->part;
   const int nb = red_blocks(n);
@@ -201,9 +204,12 @@ static int pcg_fused(vfg_handle *h, int64_t n, const FusedFn &A, const double *d
     for (int k = 0; k < todo; k++) {
       const int na = A(pa, pb, q);  // pb = z + beta pa, q = A pb, part = p.q per workgroup
       hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(RB), 0, s, (const double *)part, na, 1, sc);
-      hipLaunchKernelGGL(k_pcg_xr, dim3(nb), dim3(RB), 0, s, x, r, z, (const double *)pb, (const double *)q, dinv,
-                         n, (const double *)sc, part);
-      hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(RB), 0, s, (const double *)part, nb, 2, sc);
+      int nx = nb;
+      if (xr) nx = (*xr)(pb, q);
+      else
+        hipLaunchKernelGGL(k_pcg_xr, dim3(nb), dim3(RB), 0, s, x, r, z, (const double *)pb, (const double *)q, dinv,
+                           n, (const double *)sc, part);
+      hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(RB), 0, s, (const double *)part, nx, 2, sc);
       double *t = pa;
       pa = pb;
       pb = t;
@@ -990,9 +996,9 @@ int vfg_v_solve(vfg_handle *h, const double *U, double *V, const double *ub, con
   int rc;
   if ((rc = work(h, 13, 3 * n))) return rc;
   if ((rc = vfg_alloc(h, (void **)&h->act, n))) return rc;
-  const int ntiles = v_tiles(P.G);
-  if ((rc = vfg_alloc(h, (void **)&h->vtflag, 4 * (size_t)ntiles)) ||
-      (rc = vfg_alloc(h, (void **)&h->vtlist, 4 * (size_t)(ntiles + 1))))
+  const int vcap = v_chunk_cap(P.G);
+  if ((rc = vfg_alloc(h, (void **)&h->vflag, 4 * (size_t)vcap)) ||
+      (rc = vfg_alloc(h, (void **)&h->vlist, 4 * (size_t)(vcap + 2))))
     return rc;
   double *x = h->w[0], *r = h->w[1], *z = h->w[2], *p = h->w[3], *q = h->w[4], *dinv = h->w[5];
   double *Vd = h->w[6], *g = h->w[7], *Vn = h->w[8], *D = h->w[9], *KD = h->w[10], *ubd = nullptr;
@@ -1023,13 +1029,13 @@ int vfg_v_solve(vfg_handle *h, const double *U, double *V, const double *ub, con
                        (const uint8_t *)P.vfix, (const double *)ubd, act, r, n, h->part);
     launch_dot(r, r, n, h->part + 4096, h->scal + 16, s);  // ||g_I||^2 (same value, fixed order)
     // With irreversibility most nodes sit at their upper bound: the reduced
-    // system lives on a band around the cracks.  The workgroups of the fused
-    // apply that own a free node are listed, and CG runs over those only.
-    launch_v_tile_list(P, act, h->vtflag, h->vtlist, s);
+    // system lives on a band around the cracks.  The chunks of node planes
+    // holding a free node are listed, and CG runs over those only.
+    launch_v_chunk_list(P, act, h->vflag, h->vlist, s);
     double pg2 = 0.;
     int nlist = 0;
     if (hipMemcpyAsync(&pg2, h->scal + 16, 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipMemcpyAsync(&nlist, h->vtlist + ntiles, 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(&nlist, h->vlist + vcap, 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
       return vfg_fail(h, VFG_E_HIP, "vfg_v_solve: reduction failed");
     double pg = sqrt(pg2);
@@ -1039,17 +1045,20 @@ int vfg_v_solve(vfg_handle *h, const double *U, double *V, const double *ub, con
     // reduced Newton system on the inactive set
     launch_v_diag(P, q, s, act);
     hipLaunchKernelGGL(k_recip, dim3(nb_(n)), dim3(256), 0, s, (const double *)q, dinv, n);
-    // the unlisted workgroups are never written: p, q are zero there (p and
-    // z vanish on active nodes, and only those are owned by such workgroups)
+    // the unlisted chunks hold active nodes only, where r = z = 0; they are
+    // never written, so p, q start at zero everywhere and stay zero there
     launch_fill(x, 0., n, s);
     launch_fill(q, 0., n, s);
     launch_fill(h->w[12], 0., n, s);
-    const int *vlist = h->vtlist;
+    const int *vlist = h->vlist;
     FusedFn A = [&](const double *pold, double *pn, double *qq) {
       return launch_v_apply_cg_list(P, pold, pn, z, h->scal + S_BETA, qq, h->part, act, vlist, nlist, s);
     };
+    XrFn XR = [&](const double *pc, const double *qc) {
+      return launch_v_xr_list(P.G, vlist, nlist, x, r, z, pc, qc, dinv, h->scal + S_ALPHA, h->part, s);
+    };
     const double inner_rtol = fmax(inner_floor, fmin(0.1, 0.1 * pg / pg0));
-    int prc = pcg_fused(h, n, A, dinv, x, r, z, p, h->w[12], q, inner_rtol, 0., 20000, 20, &inf);
+    int prc = pcg_fused(h, n, A, dinv, x, r, z, p, h->w[12], q, inner_rtol, 0., 20000, 20, &inf, &XR);
     if (prc == VFG_E_HIP || prc == VFG_E_NAN) { status = prc; break; }
     // projected step with Armijo backtracking on f(V) = 1/2 V.KV - b.V
     hipLaunchKernelGGL(k_zero_fixed, dim3(nb_(n)), dim3(256), 0, s, g, (const uint8_t *)P.vfix, n);

@@ -292,14 +292,16 @@ __global__ __launch_bounds__(256, 3) void k_v_apply_cg(VApplyCgOp op, int OZ) {
 
 __global__ __launch_bounds__(256, 3) void k_v_apply(VApplyOp op, int OZ) { march<double, 16, 16>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z); }
 
-// The same over a list of workgroups (tile + z-chunk, linear id bx + gx (by +
-// gy bz)): the reduced Newton systems of the V-solve live on the free nodes,
-// a thin band around the cracks, and a workgroup none of whose owned nodes is
-// free has p = q = 0 throughout.  Partials are written compactly per list entry.
-__global__ __launch_bounds__(256, 3) void k_v_apply_cg_list(VApplyCgOp op, int OZ, const int *__restrict__ list,
-                                                           int gx, int gy) {
-  const int t = list[blockIdx.x];
-  march<double, 16, 16>(op.G, op, OZ, t % gx, (t / gx) % gy, t / (gx * gy), blockIdx.x);
+// The same over a list of workgroups.  The reduced Newton systems of the
+// V-solve live on the free nodes, a thin band around the cracks: a workgroup
+// owning no free node has p = q = 0 throughout and is left out.  The list
+// (k_v_list_chunks) holds chunk ids tile + txy * zc, a chunk being the node
+// planes [zc L, zc L + L) of a 15 x 15 node tile; L sits in list[cap + 1].
+// Partials are written compactly, one per list entry.
+__global__ __launch_bounds__(256, 3) void k_v_apply_cg_list(VApplyCgOp op, const int *__restrict__ list, int cap) {
+  const int gx = (op.G.nx + 14) / 15, txy = gx * ((op.G.ny + 14) / 15);
+  const int c = list[blockIdx.x], L = list[cap + 1], t = c % txy;
+  march<double, 16, 16>(op.G, op, L, t % gx, t / gx, c / txy, blockIdx.x);
 }
 
 __global__ void k_fill_u32(uint32_t *a, uint32_t v, int64_t n) {
@@ -307,36 +309,102 @@ __global__ void k_fill_u32(uint32_t *a, uint32_t v, int64_t n) {
   if (i < n) a[i] = v;
 }
 
-// flag[w] = 1 for the workgroups owning a node with mask == 0
-__global__ void k_v_mark_tiles(Grid G, int OZ, const uint8_t *__restrict__ mask, uint32_t *__restrict__ flag) {
+// flag[tile + txy k] = 1 where node plane k of a 15 x 15 node tile holds a
+// node with mask == 0 (flag cleared beforehand)
+__global__ void k_v_mark_planes(Grid G, const uint8_t *__restrict__ mask, uint32_t *__restrict__ flag) {
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= (int64_t)G.nx * G.ny * G.nz || mask[n]) return;
   const int i = (int)(n % G.nx), j = (int)((n / G.nx) % G.ny), k = (int)(n / ((int64_t)G.nx * G.ny));
-  const int gx = (G.nx + 14) / 15, gy = (G.ny + 14) / 15;
-  flag[i / 15 + gx * (j / 15 + gy * (k / OZ))] = 1u;  // same value from every writer
+  const int gx = (G.nx + 14) / 15, txy = gx * ((G.ny + 14) / 15);
+  flag[i / 15 + gx * (j / 15) + (int64_t)txy * k] = 1u;  // same value from every writer
 }
 
-// ordered list of the flagged workgroups (one workgroup of 256 threads);
-// list[nt] = their count
-__global__ __launch_bounds__(256) void k_v_list_tiles(const uint32_t *__restrict__ flag, int nt, int *__restrict__ list) {
-  __shared__ int sc[256];
-  int base = 0;
-  for (int b0 = 0; b0 < nt; b0 += 256) {
-    const int t = b0 + (int)threadIdx.x;
-    const int f = t < nt && flag[t] ? 1 : 0;
-    sc[threadIdx.x] = f;
-    __syncthreads();
-    for (int o = 1; o < 256; o <<= 1) {  // inclusive scan
-      const int v = threadIdx.x >= (unsigned)o ? sc[threadIdx.x - o] : 0;
-      __syncthreads();
-      sc[threadIdx.x] += v;
-      __syncthreads();
-    }
-    if (f) list[base + sc[threadIdx.x] - 1] = t;
-    base += sc[255];
+__device__ __forceinline__ int block_isum(int v, int *sh) {
+  __syncthreads();
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
     __syncthreads();
   }
-  if (threadIdx.x == 0) list[nt] = base;
+  return sh[0];
+}
+
+// One workgroup of 256 threads.  Chunk length L = ceil(flagged planes /
+// target), clamped to [lmin, lmax]: a thin band gives many one-plane chunks
+// (latency, not work, bounds the march of a few chunks), a wide one long
+// chunks (each chunk re-reads one halo plane).  The flagged chunks are listed
+// in increasing id; list[cap] = their count, list[cap + 1] = L.
+__global__ __launch_bounds__(256) void k_v_list_chunks(const uint32_t *__restrict__ flag, int txy, int nz, int lmin,
+                                                       int lmax, int target, int *__restrict__ list, int cap) {
+  __shared__ int sh[256];
+  const int tid = threadIdx.x;
+  const int64_t nf = (int64_t)txy * nz;
+  int f = 0;
+  for (int64_t i = tid; i < nf; i += 256) f += flag[i] != 0u;
+  const int F = block_isum(f, sh);
+  int L = (F + target - 1) / target;
+  L = L < lmin ? lmin : L > lmax ? lmax : L;
+  if (L < 1) L = 1;
+  const int nzc = (nz + L - 1) / L;
+  const int64_t nch = (int64_t)txy * nzc, per = (nch + 255) / 256;
+  const int64_t c0 = tid * per < nch ? tid * per : nch, c1 = c0 + per < nch ? c0 + per : nch;
+  auto hit = [&](int64_t c) {
+    const int t = (int)(c % txy), k0 = (int)(c / txy) * L, k1 = k0 + L < nz ? k0 + L : nz;
+    for (int k = k0; k < k1; k++)
+      if (flag[t + (int64_t)txy * k]) return true;
+    return false;
+  };
+  int cnt = 0;
+  for (int64_t c = c0; c < c1; c++) cnt += hit(c);
+  // exclusive scan of the per-thread counts
+  __syncthreads();
+  sh[tid] = cnt;
+  __syncthreads();
+  for (int o = 1; o < 256; o <<= 1) {
+    const int v = tid >= o ? sh[tid - o] : 0;
+    __syncthreads();
+    sh[tid] += v;
+    __syncthreads();
+  }
+  int off = sh[tid] - cnt;
+  for (int64_t c = c0; c < c1; c++)
+    if (hit(c)) list[off++] = (int)c;
+  if (tid == 255) {
+    list[cap] = sh[255];
+    list[cap + 1] = L;
+  }
+}
+
+// x += alpha p, r -= alpha q, z = dinv r and the partials of r.z, r.r over
+// the owned nodes of the listed chunks (p = q = r = 0 elsewhere)
+__global__ __launch_bounds__(256) void k_v_xr_list(Grid G, const int *__restrict__ list, int cap,
+                                                   double *__restrict__ x, double *__restrict__ r,
+                                                   double *__restrict__ z, const double *__restrict__ p,
+                                                   const double *__restrict__ q, const double *__restrict__ dinv,
+                                                   const double *__restrict__ alpha, double *__restrict__ part) {
+  __shared__ double sh[4];
+  const int gx = (G.nx + 14) / 15, txy = gx * ((G.ny + 14) / 15);
+  const int c = list[blockIdx.x], L = list[cap + 1], t = c % txy;
+  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+  const int i = (t % gx) * 15 + tx, j = (t / gx) * 15 + ty, k0 = (c / txy) * L;
+  const int k1 = k0 + L < G.nz ? k0 + L : G.nz;
+  const double al = *alpha;
+  double rz = 0., rr = 0.;
+  if (tx < 15 && ty < 15 && i < G.nx && j < G.ny)
+    for (int k = k0; k < k1; k++) {
+      const int64_t n = node_id(G, i, j, k);
+      x[n] += al * p[n];
+      const double ri = r[n] - al * q[n];
+      r[n] = ri;
+      const double zi = dinv[n] * ri;
+      z[n] = zi;
+      rz += ri * zi;
+      rr += ri * ri;
+    }
+  rz = block_sum<256>(rz, sh);
+  rr = block_sum<256>(rr, sh);
+  if (threadIdx.x == 0) { part[2 * blockIdx.x] = rz; part[2 * blockIdx.x + 1] = rr; }
 }
 
 // ---------------------------------------------------------------- diagonal
@@ -540,25 +608,31 @@ int launch_v_apply_cg(const DevProblem &P, const double *pold, double *p, const 
   return (int)(g.x * g.y * g.z);
 }
 
-int v_tiles(const Grid &G) {
-  const dim3 g = mgrid(G, oz_for(G));
-  return (int)(g.x * g.y * g.z);
-}
+int v_chunk_cap(const Grid &G) { return ((G.nx + 14) / 15) * ((G.ny + 14) / 15) * G.nz; }
 
-void launch_v_tile_list(const DevProblem &P, const uint8_t *mask, uint32_t *flag, int *list, hipStream_t s) {
-  const int OZ = oz_for(P.G), nt = v_tiles(P.G);
-  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, flag, 0u, (int64_t)nt);
-  hipLaunchKernelGGL(k_v_mark_tiles, dim3((unsigned)((P.nn + 255) / 256)), dim3(256), 0, s, P.G, OZ, mask, flag);
-  hipLaunchKernelGGL(k_v_list_tiles, dim3(1), dim3(256), 0, s, (const uint32_t *)flag, nt, list);
+void launch_v_chunk_list(const DevProblem &P, const uint8_t *mask, uint32_t *flag, int *list, hipStream_t s) {
+  const int cap = v_chunk_cap(P.G), txy = cap / P.G.nz;
+  // at most 32768 + txy chunks, whatever the band: partials fit the reduction buffers
+  const int lmin = (cap + 32767) / 32768;
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, s, flag, 0u, (int64_t)cap);
+  hipLaunchKernelGGL(k_v_mark_planes, dim3((unsigned)((P.nn + 255) / 256)), dim3(256), 0, s, P.G, mask, flag);
+  hipLaunchKernelGGL(k_v_list_chunks, dim3(1), dim3(256), 0, s, (const uint32_t *)flag, txy, P.G.nz, lmin, P.G.nz,
+                     1024, list, cap);
 }
 
 int launch_v_apply_cg_list(const DevProblem &P, const double *pold, double *p, const double *z, const double *beta,
                            double *q, double *part, const uint8_t *mask, const int *list, int count, hipStream_t s) {
   if (count <= 0) return 0;
   const VApplyCgOp op = make_vcg_op(P, pold, p, z, beta, q, part, mask);
-  const int OZ = oz_for(P.G);
-  const dim3 g = mgrid(P.G, OZ);
-  hipLaunchKernelGGL(k_v_apply_cg_list, dim3((unsigned)count), dim3(256), 0, s, op, OZ, list, (int)g.x, (int)g.y);
+  hipLaunchKernelGGL(k_v_apply_cg_list, dim3((unsigned)count), dim3(256), 0, s, op, list, v_chunk_cap(P.G));
+  return count;
+}
+
+int launch_v_xr_list(const Grid &G, const int *list, int count, double *x, double *r, double *z, const double *p,
+                     const double *q, const double *dinv, const double *alpha, double *part, hipStream_t s) {
+  if (count <= 0) return 0;
+  hipLaunchKernelGGL(k_v_xr_list, dim3((unsigned)count), dim3(256), 0, s, G, list, v_chunk_cap(G), x, r, z, p, q,
+                     dinv, alpha, part);
   return count;
 }
 
