@@ -64,6 +64,12 @@ void launch_v_chunk_list(const DevProblem &P, const uint8_t *mask, uint32_t *fla
 // others); returns the number of p.q partials (= count)
 int launch_v_apply_cg_list(const DevProblem &P, const double *pold, double *p, const double *z, const double *beta,
                            double *q, double *part, const uint8_t *mask, const int *list, int count, hipStream_t s);
+// launch_v_apply and launch_v_diag over the listed chunks' owned nodes (the
+// other entries of y, d are left as they are)
+void launch_v_apply_list(const DevProblem &P, const double *x, double *y, int mode, const int *list, int count,
+                         hipStream_t s, const uint8_t *mask = nullptr);
+void launch_v_diag_list(const DevProblem &P, double *d, const uint8_t *mask, const int *list, int count,
+                        hipStream_t s);
 // the CG update x += a p, r -= a q, z = dinv r over the listed chunks, with
 // (r.z, r.r) partial pairs; returns their number (= count)
 int launch_v_xr_list(const Grid &G, const int *list, int count, double *x, double *r, double *z, const double *p,

@@ -1013,6 +1013,7 @@ int vfg_v_solve(vfg_handle *h, const double *U, double *V, const double *ub, con
   if ((rc = vfg_alloc(h, (void **)&P.Vq, 8 * (size_t)vq_doubles(P.G)))) return rc;
   launch_v_setup(P, Ud, s);
   h->v_ready = true;
+  launch_fill(KD, 0., n, s);
   launch_v_fixmask(P, s);
   launch_vec_apply_dirichlet(P, 1, Vd, Vd, s);
   hipLaunchKernelGGL(k_v_clamp, dim3(nb_(n)), dim3(256), 0, s, Vd, (const uint8_t *)P.vfix, (const double *)ubd, n);
@@ -1042,15 +1043,17 @@ int vfg_v_solve(vfg_handle *h, const double *U, double *V, const double *ub, con
     if (!isfinite(pg)) { status = VFG_E_NAN; break; }
     if (pg0 < 0.) pg0 = pg > 0. ? pg : 1.;
     if (pg <= o->rtol * pg0 || pg <= 1e-14) break;
-    // reduced Newton system on the inactive set
-    launch_v_diag(P, q, s, act);
-    hipLaunchKernelGGL(k_recip, dim3(nb_(n)), dim3(256), 0, s, (const double *)q, dinv, n);
-    // the unlisted chunks hold active nodes only, where r = z = 0; they are
-    // never written, so p, q start at zero everywhere and stay zero there
-    launch_fill(x, 0., n, s);
-    launch_fill(q, 0., n, s);
-    launch_fill(h->w[12], 0., n, s);
+    // reduced Newton system on the inactive set.  The unlisted chunks hold
+    // active nodes only, where r = z = 0; they are never written, so p, q
+    // start at zero everywhere and stay zero there.  The diagonal is formed
+    // in q on the listed chunks (1 / 0 -> 1 elsewhere); the first CG apply
+    // overwrites it there.
     const int *vlist = h->vlist;
+    launch_fill(q, 0., n, s);
+    launch_v_diag_list(P, q, act, vlist, nlist, s);
+    hipLaunchKernelGGL(k_recip, dim3(nb_(n)), dim3(256), 0, s, (const double *)q, dinv, n);
+    launch_fill(x, 0., n, s);
+    launch_fill(h->w[12], 0., n, s);
     FusedFn A = [&](const double *pold, double *pn, double *qq) {
       return launch_v_apply_cg_list(P, pold, pn, z, h->scal + S_BETA, qq, h->part, act, vlist, nlist, s);
     };
@@ -1066,7 +1069,9 @@ int vfg_v_solve(vfg_handle *h, const double *U, double *V, const double *ub, con
     for (int ls = 0; ls < 40; ls++) {
       hipLaunchKernelGGL(k_v_step, dim3(nb_(n)), dim3(256), 0, s, (const double *)Vd, (const double *)x,
                          (const uint8_t *)act, (const uint8_t *)P.vfix, (const double *)ubd, lam, Vn, D, n);
-      launch_v_apply(P, D, KD, 2, s);
+      // D vanishes off the free nodes: K D is needed on the listed chunks only
+      // (KD is zeroed when the solve starts, so it is finite everywhere)
+      launch_v_apply_list(P, D, KD, 2, vlist, nlist, s);
       int e2 = 0;
       double gd = dot_host(h, g, D, n, &e2);
       double dkd = dot_host(h, D, KD, n, &e2);

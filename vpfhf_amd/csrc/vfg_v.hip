@@ -298,10 +298,22 @@ __global__ __launch_bounds__(256, 3) void k_v_apply(VApplyOp op, int OZ) { march
 // (k_v_list_chunks) holds chunk ids tile + txy * zc, a chunk being the node
 // planes [zc L, zc L + L) of a 15 x 15 node tile; L sits in list[cap + 1].
 // Partials are written compactly, one per list entry.
+struct VChunk {
+  int bx, by, bz, L;
+};
+__device__ __forceinline__ VChunk chunk_at(const Grid &G, const int *__restrict__ list, int cap, int w) {
+  const int gx = (G.nx + 14) / 15, txy = gx * ((G.ny + 14) / 15);
+  const int c = list[w], t = c % txy;
+  return VChunk{t % gx, t / gx, c / txy, list[cap + 1]};
+}
 __global__ __launch_bounds__(256, 3) void k_v_apply_cg_list(VApplyCgOp op, const int *__restrict__ list, int cap) {
-  const int gx = (op.G.nx + 14) / 15, txy = gx * ((op.G.ny + 14) / 15);
-  const int c = list[blockIdx.x], L = list[cap + 1], t = c % txy;
-  march<double, 16, 16>(op.G, op, L, t % gx, t / gx, c / txy, blockIdx.x);
+  const VChunk q = chunk_at(op.G, list, cap, blockIdx.x);
+  march<double, 16, 16>(op.G, op, q.L, q.bx, q.by, q.bz, blockIdx.x);
+}
+// the plain apply (any mode) over the listed chunks
+__global__ __launch_bounds__(256, 3) void k_v_apply_list(VApplyOp op, const int *__restrict__ list, int cap) {
+  const VChunk q = chunk_at(op.G, list, cap, blockIdx.x);
+  march<double, 16, 16>(op.G, op, q.L, q.bx, q.by, q.bz);
 }
 
 __global__ void k_fill_u32(uint32_t *a, uint32_t v, int64_t n) {
@@ -467,6 +479,10 @@ struct VDiagOp {
 };
 
 __global__ __launch_bounds__(256) void k_v_diag(VDiagOp op, int OZ) { march<double, 16, 16>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z); }
+__global__ __launch_bounds__(256) void k_v_diag_list(VDiagOp op, const int *__restrict__ list, int cap) {
+  const VChunk q = chunk_at(op.G, list, cap, blockIdx.x);
+  march<double, 16, 16>(op.G, op, q.L, q.bx, q.by, q.bz);
+}
 
 // ------------------------------------------------------------ load vector
 // b_a = crhs sum_g W phi_a + [pressurize] sum_g W p (u . grad phi_a)
@@ -535,16 +551,28 @@ void launch_v_fixmask(const DevProblem &P, hipStream_t s) {
   hipLaunchKernelGGL(k_v_fixmask, dim3((unsigned)((P.nn + 255) / 256)), dim3(256), 0, s, P);
 }
 
-void launch_v_apply(const DevProblem &P, const double *x, double *y, int mode, hipStream_t s,
-                    const uint8_t *mask) {
+static VApplyOp make_vapply_op(const DevProblem &P, const double *x, double *y, int mode, const uint8_t *mask) {
   VApplyOp op;
   op.x = x; op.y = y; op.q = P.Vq; op.bV = P.bV; op.VIrrev = P.VIrrev; op.vfix = mask ? mask : P.vfix;
   op.G = P.G; op.ne = P.ne; op.C = make_vconst(P); op.bc = P.bcV;
   for (int f = 0; f < 6; f++) op.ftype[f] = (uint8_t)P.hbcV.face[f];
   for (int v = 0; v < 8; v++) op.vtype[v] = (uint8_t)P.hbcV.vertex[v];
   op.mode = mode; op.atnum = P.atnum; op.irrevtol = P.irrevtol;
+  return op;
+}
+
+void launch_v_apply(const DevProblem &P, const double *x, double *y, int mode, hipStream_t s,
+                    const uint8_t *mask) {
+  const VApplyOp op = make_vapply_op(P, x, y, mode, mask);
   int OZ = oz_for(P.G);
   hipLaunchKernelGGL(k_v_apply, mgrid(P.G, OZ), dim3(256), 0, s, op, OZ);
+}
+
+void launch_v_apply_list(const DevProblem &P, const double *x, double *y, int mode, const int *list, int count,
+                         hipStream_t s, const uint8_t *mask) {
+  if (count <= 0) return;
+  const VApplyOp op = make_vapply_op(P, x, y, mode, mask);
+  hipLaunchKernelGGL(k_v_apply_list, dim3((unsigned)count), dim3(256), 0, s, op, list, v_chunk_cap(P.G));
 }
 
 // ------------------------------------------------ persistent multi-problem apply
@@ -641,6 +669,14 @@ void launch_v_diag(const DevProblem &P, double *d, hipStream_t s, const uint8_t 
   op.d = d; op.q = P.Vq; op.vfix = mask ? mask : P.vfix; op.G = P.G; op.ne = P.ne; op.C = make_vconst(P);
   int OZ = oz_for(P.G);
   hipLaunchKernelGGL(k_v_diag, mgrid(P.G, OZ), dim3(256), 0, s, op, OZ);
+}
+
+void launch_v_diag_list(const DevProblem &P, double *d, const uint8_t *mask, const int *list, int count,
+                        hipStream_t s) {
+  if (count <= 0) return;
+  VDiagOp op;
+  op.d = d; op.q = P.Vq; op.vfix = mask ? mask : P.vfix; op.G = P.G; op.ne = P.ne; op.C = make_vconst(P);
+  hipLaunchKernelGGL(k_v_diag_list, dim3((unsigned)count), dim3(256), 0, s, op, list, v_chunk_cap(P.G));
 }
 
 }  // namespace vfg
