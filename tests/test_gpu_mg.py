@@ -18,12 +18,12 @@ from tests.scenarios import relerr
 pytestmark = pytest.mark.gpu
 
 GRIDS = [
-    ((17, 17, 17), "clamped"),   # 16^3 -> 8^3 -> 4^3 (nested)
-    ((17, 9, 13), "mixed"),      # 16x8x12 -> 8x4x6 -> 4x4x3
-    ((11, 10, 9), "mixed"),      # 10x9x8 -> 5x5x4 (non-nested)
-    ((26, 26, 26), "clamped"),   # 25^3 -> 13^3 -> 7^3 -> 4^3 (non-nested)
+    ((17, 17, 17), "clamped"),   # 16^3 -> 8^3 (nested)
+    ((17, 9, 13), "mixed"),      # 16x8x12 -> 8x8x6 (y kept)
+    ((11, 10, 9), "mixed"),      # 10x9x8 -> 5x5x8 (non-nested, z kept)
+    ((26, 26, 26), "clamped"),   # 25^3 -> 13^3 -> 7^3 (non-nested)
     ((41, 21, 6), "test1"),      # 40x20x5 -> 20x10x5 -> 10x5x5 -> 5x5x5 (flat: z not coarsened)
-    ((33, 33, 33), "test1"),     # 32^3 -> 16^3 -> 8^3 -> 4^3
+    ((33, 33, 33), "test1"),     # 32^3 -> 16^3 -> 8^3
 ]
 BCS = {"clamped": S.CLAMPED, "mixed": S.MIXED_U, "test1": S.TEST1_U}
 

@@ -52,7 +52,7 @@ struct vfg_handle {
 namespace vfg {
 // multigrid (vfg_solve.hip): V-cycle smoothing degree, coarsest-level degree,
 // smallest coarse element count per direction
-constexpr int MG_SMOOTH = 1, MG_COARSE_DEG = 4, MG_MIN = 4;
+constexpr int MG_SMOOTH = 1, MG_COARSE_DEG = 6, MG_COARSEST = 8;
 int mg_build(vfg_handle *h);                      // hierarchy shape and buffers (once)
 int mg_setup(vfg_handle *h, const double *dinv64);  // coefficients, diagonals, bounds (per solve)
 }  // namespace vfg

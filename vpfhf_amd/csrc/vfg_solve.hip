@@ -556,8 +556,9 @@ static int pcg32(vfg_handle *h, int64_t n, const std::function<void(const float 
 // of the residual, recursion, prolongation; the coarsest level gets
 // Chebyshev(MG_COARSE_DEG) on [lmax/30, lmax].  All in FP32 with the level
 // operators of vfg_mg.hip.  Each level has ceil(n/2) elements per direction
-// where that leaves at least MG_MIN, else n; coarsening stops when no
-// direction changes.
+// where n > MG_COARSEST, else n; coarsening stops when no direction changes
+// (the coarsest level has at most 8 elements per direction, or the fine
+// level's count: at 128^3 five levels down to 8^3).
 int mg_build(vfg_handle *h) {
   if (h->mg) return VFG_OK;
   MgHier *m = new MgHier();
@@ -570,7 +571,7 @@ int mg_build(vfg_handle *h) {
   if (!rc) rc = vfg_alloc(h, (void **)&l0.d, 4 * (size_t)l0.n);
   int L = 1;
   Grid G = h->P.G;
-  auto halve = [](int n) { return (n + 1) / 2 >= MG_MIN ? (n + 1) / 2 : n; };
+  auto halve = [](int n) { return n > MG_COARSEST ? (n + 1) / 2 : n; };
   while (!rc && L < MgHier::MAXL) {
     Grid Gc;
     Gc.nex = halve(G.nex); Gc.ney = halve(G.ney); Gc.nez = halve(G.nez);
