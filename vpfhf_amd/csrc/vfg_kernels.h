@@ -37,6 +37,16 @@ struct DevProblem {
 // u operator (vfg_u.hip)
 void launch_u_apply(const DevProblem &P, const double *x, double *y, hipStream_t s);
 void launch_u_apply_f32(const DevProblem &P, const float *x, float *y, hipStream_t s);
+// Chebyshev(k)-Jacobi from x = 0 on a small FP32 level in one launch (k <= 8;
+// step i: coefficients c1[i], c2[i] of mg_cheb_entry); mg_coarse_fits tells
+// whether the level qualifies (at most 8^3 elements, no NOCOMPRESSION)
+struct MgChebCoefs {
+  float c1[8], c2[8];
+  int k;
+};
+bool mg_coarse_fits(const DevProblem &P);
+void launch_mg_coarse_cheb(const DevProblem &P, const float *b, const float *dinv, float *x, const MgChebCoefs &cc,
+                           hipStream_t s);
 void launch_u_diag(const DevProblem &P, double *d, hipStream_t s);
 // cmask from the state U (UNILATERAL_NOCOMPRESSION)
 void launch_u_cmask(const DevProblem &P, const double *U, hipStream_t s);

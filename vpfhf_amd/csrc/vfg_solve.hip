@@ -660,24 +660,33 @@ int mg_setup(vfg_handle *h, const double *dinv64) {
   return VFG_OK;
 }
 
-static void mg_smooth(vfg_handle *h, MgLevel &L, int k, double lo, double hi, bool xzero) {
-  hipStream_t s = h->stream;
+// Chebyshev coefficients of mg_cheb_entry's step i on [lo, hi]
+static void cheb_step_coefs(int k, double lo, double hi, float *c1, float *c2) {
   const double th = .5 * (hi + lo), de = .5 * (hi - lo), sg = th / de;
   double rho = 1. / sg;
+  for (int i = 0; i < k; i++) {
+    c1[i] = 0.f;
+    c2[i] = (float)(1. / th);
+    if (i > 0) {
+      const double rn = 1. / (2. * sg - rho);
+      c1[i] = (float)(rn * rho);
+      c2[i] = (float)(2. * rn / de);
+      rho = rn;
+    }
+  }
+}
+
+static void mg_smooth(vfg_handle *h, MgLevel &L, int k, double lo, double hi, bool xzero) {
+  hipStream_t s = h->stream;
+  float c1[16], c2[16];
+  cheb_step_coefs(k, lo, hi, c1, c2);
   for (int i = 0; i < k; i++) {
     const float *y = nullptr;
     if (i > 0 || !xzero) {
       launch_u_apply_f32(L.P, L.x, L.y, s);
       y = L.y;
     }
-    float c1 = 0.f, c2 = (float)(1. / th);
-    if (i > 0) {
-      const double rn = 1. / (2. * sg - rho);
-      c1 = (float)(rn * rho);
-      c2 = (float)(2. * rn / de);
-      rho = rn;
-    }
-    launch_mg_cheb(L.b, y, L.dinv, L.d, L.x, c1, c2, i == 0, i == 0 && xzero, L.n, s);
+    launch_mg_cheb(L.b, y, L.dinv, L.d, L.x, c1[i], c2[i], i == 0, i == 0 && xzero, L.n, s);
   }
 }
 
@@ -686,6 +695,15 @@ static void mg_vcycle(vfg_handle *h, int l) {
   MgLevel &L = m.lv[l];
   hipStream_t s = h->stream;
   if (l == m.L - 1) {
+    // a coarse level of at most 8^3 elements: the whole solve in one launch
+    if (l > 0 && mg_coarse_fits(L.P)) {
+      MgChebCoefs cc;
+      static_assert(MG_COARSE_DEG <= 8, "MgChebCoefs holds 8 steps");
+      cc.k = MG_COARSE_DEG;
+      cheb_step_coefs(MG_COARSE_DEG, L.lmax / 30., L.lmax, cc.c1, cc.c2);
+      launch_mg_coarse_cheb(L.P, L.b, L.dinv, L.x, cc, s);
+      return;
+    }
     mg_smooth(h, L, MG_COARSE_DEG, L.lmax / 30., L.lmax, true);
     return;
   }

@@ -14,6 +14,7 @@
 #include "vfg_kernels.h"
 #include "vfg_batch_apply.h"
 #include "vfg_handle.h"
+#include "vfg_mg.h"
 
 namespace vfg {
 
@@ -614,6 +615,93 @@ __global__ void k_u_cmask(Grid G, const double *__restrict__ U, uint32_t *__rest
 
 void launch_u_cmask(const DevProblem &P, const double *U, hipStream_t s) {
   hipLaunchKernelGGL(k_u_cmask, dim3((unsigned)((P.ne + 255) / 256)), dim3(256), 0, s, P.G, U, P.cmask);
+}
+
+// Chebyshev-Jacobi solve of a small FP32 level from x = 0 in one workgroup
+// (the coarsest multigrid level, at most MGC_NE elements): x, d and the
+// operator output stay in LDS.  An apply is an element pass (one element per
+// thread, its 24 outputs to LDS) and a node pass that adds each node's eight
+// element contributions in the order of the marching apply (x-pairs, then
+// y-pairs, the lower element layer before the upper), so it gives the values
+// of launch_u_apply_f32; the step is mg_cheb_entry, as in the multi-launch
+// smoother.
+constexpr int MGC_T = 512, MGC_NE = 512, MGC_NN = 729;
+
+__global__ __launch_bounds__(MGC_T) void k_mg_coarse_cheb(UApplyOp<float, false> op, const float *__restrict__ b,
+                                                          const float *__restrict__ dinv, float *__restrict__ xout,
+                                                          MgChebCoefs cc) {
+  __shared__ float sx[3 * MGC_NN], sd[3 * MGC_NN], sy[3 * MGC_NN];
+  __shared__ float sr[MGC_NE * 24];
+  const Grid &G = op.G;
+  const int tid = threadIdx.x;
+  const int nn = G.nx * G.ny * G.nz, ne = G.nex * G.ney * G.nez;
+  auto R = [&](int ex, int ey, int ez, int corner, int c) -> float {
+    if (ex < 0 || ex >= G.nex || ey < 0 || ey >= G.ney || ez < 0 || ez >= G.nez) return 0.f;
+    return sr[(ex + G.nex * (ey + G.ney * ez)) * 24 + corner * 3 + c];
+  };
+  // corner (i, J, K) of element (ex, ey, ez) plus corner (0, J, K) of its +x neighbour
+  auto cx = [&](int ex, int ey, int ez, int J, int K, int c) {
+    return R(ex, ey, ez, (K * 2 + J) * 2 + 1, c) + R(ex + 1, ey, ez, (K * 2 + J) * 2, c);
+  };
+  for (int i = 0; i < cc.k; i++) {
+    if (i > 0) {
+      for (int e = tid; e < ne; e += MGC_T) {
+        const int ex = e % G.nex, ey = (e / G.nex) % G.ney, ez = e / (G.nex * G.ney);
+        float xin[2][4][4];
+#pragma unroll
+        for (int kz = 0; kz < 2; kz++)
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            const int64_t n = node_id(G, ex + (q & 1), ey + (q >> 1), ez + kz);
+#pragma unroll
+            for (int f = 0; f < 3; f++) xin[kz][q][f] = sx[3 * n + f];
+            xin[kz][q][3] = op.v[n];
+          }
+        float r[8][3];
+        op.element(ex, ey, ez, xin, op.aux(ex, ey, ez), r);
+#pragma unroll
+        for (int a = 0; a < 8; a++)
+#pragma unroll
+          for (int c = 0; c < 3; c++) sr[e * 24 + a * 3 + c] = r[a][c];
+      }
+      __syncthreads();
+      for (int n = tid; n < nn; n += MGC_T) {
+        const int ni = n % G.nx, nj = (n / G.nx) % G.ny, nk = n / (G.nx * G.ny);
+        const int ex = ni - 1, ey = nj - 1;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          // element layer nk - 1 (corners K = 1), then layer nk (K = 0); in
+          // each, element row ey (corners J = 1) before row ey + 1 (J = 0)
+          const float carry = cx(ex, ey, nk - 1, 1, 1, c) + cx(ex, ey + 1, nk - 1, 0, 1, c);
+          const float acc = carry + (cx(ex, ey, nk, 1, 0, c) + cx(ex, ey + 1, nk, 0, 0, c));
+          sy[3 * n + c] = on_dirichlet(G, op.bc.face[c], op.bc.vert[c], ni, nj, nk) ? sx[3 * n + c] : acc;
+        }
+      }
+      __syncthreads();
+    }
+    const float c1 = cc.c1[i], c2 = cc.c2[i];
+    for (int t = tid; t < 3 * nn; t += MGC_T)
+      mg_cheb_entry(b, i > 0 ? sy : nullptr, dinv, sd, sx, c1, c2, i == 0, i == 0, t);
+    __syncthreads();
+  }
+  for (int t = tid; t < 3 * nn; t += MGC_T) xout[t] = sx[t];
+}
+
+bool mg_coarse_fits(const DevProblem &P) {
+  return !P.unilateral && P.ne <= MGC_NE && P.nn <= MGC_NN;
+}
+
+void launch_mg_coarse_cheb(const DevProblem &P, const float *b, const float *dinv, float *x, const MgChebCoefs &cc,
+                           hipStream_t s) {
+  UApplyOp<float, false> op;
+  op.x = nullptr; op.y = nullptr; op.v = P.V32;
+  op.E_elem = P.E_elem;
+  op.cmask = nullptr;
+  op.G = P.G; op.bc = P.bcU;
+  op.E = float(P.E); op.lam1 = float(P.lam1); op.mu1 = float(P.mu1); op.eta = float(P.eta);
+  op.vol = float(P.G.hx * P.G.hy * P.G.hz);
+  op.ihx = float(1. / P.G.hx); op.ihy = float(1. / P.G.hy); op.ihz = float(1. / P.G.hz);
+  hipLaunchKernelGGL(k_mg_coarse_cheb, dim3(1), dim3(MGC_T), 0, s, op, b, dinv, x, cc);
 }
 
 void launch_u_apply(const DevProblem &P, const double *x, double *y, hipStream_t s) {
