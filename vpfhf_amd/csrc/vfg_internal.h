@@ -319,11 +319,14 @@ __device__ __forceinline__ void march(const Grid &G, const Op &op, int OZ, int b
 // during the element computation); without PF it is loaded at the start of
 // its layer and its latency is left to the other resident workgroup.
 template <typename T, int TX, int TY, bool PF, class Op>
-__device__ __forceinline__ void march_lds(const Grid &G, const Op &op, int OZ, int bx, int by, int bz) {
+__device__ __forceinline__ void march_lds(const Grid &G, const Op &op, int OZ, int bx, int by, int bz,
+                                          int rslot = 0) {
   constexpr int NF = Op::NF, NC = Op::NC;
   constexpr int NT = TX * TY, OX = TX - 1, OY = TY - 1, SX = TX + 1, SY = TY + 1, PL = SX * SY;
   constexpr int NPF = (PL + NT - 1) / NT;
-  static_assert(!is_tiled<Op>::value && !has_red<Op>::value, "march_lds: plain ops only");
+  constexpr bool RED = has_red<Op>::value;
+  static_assert(!is_tiled<Op>::value, "march_lds: untiled ops only");
+  double racc = 0.;  // owned-node contributions (reducing ops)
   __shared__ T s_in[3][NF][PL];
   __shared__ T s_red[2 * NC][NT];
   const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
@@ -432,13 +435,19 @@ __device__ __forceinline__ void march_lds(const Grid &G, const Op &op, int OZ, i
       if (ek >= oz0 && ek >= 0) {
 #pragma unroll
         for (int f = 0; f < NF; f++) own[f] = s_in[bb][f][c0 + SX + 1];
-        op.store(ox0 + tx, oy0 + ty, ek, bot, own);
+        if constexpr (RED) racc += op.store(ox0 + tx, oy0 + ty, ek, bot, own);
+        else op.store(ox0 + tx, oy0 + ty, ek, bot, own);
       }
     }
     const int b = bb;
     bb = bt;
     bt = bn;
     bn = b;
+  }
+  if constexpr (RED) {
+    __shared__ double s_rs[NT / 64];
+    const double v = block_sum<NT>(racc, s_rs);
+    if (tid == 0) op.partial(rslot, v);
   }
 }
 

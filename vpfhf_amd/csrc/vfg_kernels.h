@@ -37,6 +37,9 @@ struct DevProblem {
 // u operator (vfg_u.hip)
 void launch_u_apply(const DevProblem &P, const double *x, double *y, hipStream_t s);
 void launch_u_apply_f32(const DevProblem &P, const float *x, float *y, hipStream_t s);
+// FP64 y = A x (no NOCOMPRESSION) with one x.y partial per workgroup in part;
+// returns the number of partials (0, and nothing launched, for NOCOMPRESSION)
+int launch_u_apply_dot(const DevProblem &P, const double *x, double *y, double *part, hipStream_t s);
 // Chebyshev(k)-Jacobi from x = 0 on a small FP32 level in one launch (k <= 8;
 // step i: coefficients c1[i], c2[i] of mg_cheb_entry); mg_coarse_fits tells
 // whether the level qualifies (at most 8^3 elements, no NOCOMPRESSION)

@@ -108,6 +108,14 @@ __global__ __launch_bounds__(RB) void k_pcg_p(double *__restrict__ p, const doub
     p[i] = z[i] + be * p[i];
 }
 
+// p = z + beta p with z the FP32 preconditioner output (exact in FP64)
+__global__ __launch_bounds__(RB) void k_pcg_p32(double *__restrict__ p, const float *__restrict__ z32, int64_t n,
+                                                const double *sc) {
+  const double be = sc[S_BETA];
+  for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB)
+    p[i] = (double)z32[i] + be * p[i];
+}
+
 __global__ void k_recip(const double *d, double *dinv, int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dinv[i] = d[i] != 0. ? 1. / d[i] : 1.;
@@ -125,6 +133,9 @@ typedef std::function<int(const double *, double *, double *)> FusedFn;
 // CG update over a subset of the nodes: (p, q) -> number of (r.z, r.r)
 // partial pairs written to h->part
 typedef std::function<int(const double *, const double *)> XrFn;
+// apply with the x.y partials written to h->part: (x, y) -> their number, 0
+// when the operator has no such kernel (nothing launched then)
+typedef std::function<int(const double *, double *)> DotApplyFn;
 
 // Preconditioned CG with the direction update and the p.q product fused into
 // the operator apply (launch_v_apply_cg): per iteration
@@ -736,14 +747,12 @@ __global__ __launch_bounds__(RB) void k_mg_xr(double *__restrict__ x, double *__
     r32[i] = (float)ri;
   }
 }
-// z = z32 (p = z too when p != nullptr); partial sums of r.z and r.r
+// partial sums of r.z and r.r with z = z32 (and p = z when p != nullptr)
 __global__ __launch_bounds__(RB) void k_mg_rz(const double *__restrict__ r, const float *__restrict__ z32,
-                                              double *__restrict__ z, double *__restrict__ p, int64_t n,
-                                              double *part) {
+                                              double *__restrict__ p, int64_t n, double *part) {
   double rz = 0., rr = 0.;
   for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB) {
     const double ri = r[i], zi = (double)z32[i];
-    z[i] = zi;
     if (p) p[i] = zi;
     rz += ri * zi;
     rr += ri * ri;
@@ -754,14 +763,15 @@ __global__ __launch_bounds__(RB) void k_mg_rz(const double *__restrict__ r, cons
 }
 
 // FP64 PCG with the FP32 V-cycle as preconditioner; same contract as pcg().
-static int pcg_mg(vfg_handle *h, int64_t n, const ApplyFn &A, double *x, double *r, double *z, double *p, double *q,
-                  float *r32, float *z32, double rtol, double atol, int maxit, int check, vfg_solve_info *info) {
+static int pcg_mg(vfg_handle *h, int64_t n, const ApplyFn &A, double *x, double *r, double *p, double *q,
+                  float *r32, float *z32, double rtol, double atol, int maxit, int check, vfg_solve_info *info,
+                  const DotApplyFn *Ad = nullptr) {
   hipStream_t s = h->stream;
   double *sc = h->scal, *part = h->part;
   const int nb = red_blocks(n);
   launch_copy_d2f(r, r32, n, s);
   mg_apply(h, r32, z32);
-  hipLaunchKernelGGL(k_mg_rz, dim3(nb), dim3(RB), 0, s, (const double *)r, (const float *)z32, z, p, n, part);
+  hipLaunchKernelGGL(k_mg_rz, dim3(nb), dim3(RB), 0, s, (const double *)r, (const float *)z32, p, n, part);
   hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(RB), 0, s, (const double *)part, nb, 0, sc);
   double hs[8];
   if (hipMemcpyAsync(hs, sc, 64, hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
@@ -774,16 +784,20 @@ static int pcg_mg(vfg_handle *h, int64_t n, const ApplyFn &A, double *x, double 
   while (status == VFG_OK && rn > tol && it < maxit) {
     int todo = check < maxit - it ? check : maxit - it;
     for (int k = 0; k < todo; k++) {
-      A(p, q);
-      hipLaunchKernelGGL(k_dotp, dim3(nb), dim3(RB), 0, s, (const double *)p, (const double *)q, n, part);
-      hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(RB), 0, s, (const double *)part, nb, 1, sc);
+      int na = Ad ? (*Ad)(p, q) : 0;  // q = A p with the p.q partials
+      if (!na) {
+        A(p, q);
+        hipLaunchKernelGGL(k_dotp, dim3(nb), dim3(RB), 0, s, (const double *)p, (const double *)q, n, part);
+        na = nb;
+      }
+      hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(RB), 0, s, (const double *)part, na, 1, sc);
       hipLaunchKernelGGL(k_mg_xr, dim3(nb), dim3(RB), 0, s, x, r, (const double *)p, (const double *)q, r32, n,
                          (const double *)sc);
       mg_apply(h, r32, z32);
-      hipLaunchKernelGGL(k_mg_rz, dim3(nb), dim3(RB), 0, s, (const double *)r, (const float *)z32, z,
+      hipLaunchKernelGGL(k_mg_rz, dim3(nb), dim3(RB), 0, s, (const double *)r, (const float *)z32,
                          (double *)nullptr, n, part);
       hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(RB), 0, s, (const double *)part, nb, 2, sc);
-      hipLaunchKernelGGL(k_pcg_p, dim3(nb), dim3(RB), 0, s, p, (const double *)z, n, (const double *)sc);
+      hipLaunchKernelGGL(k_pcg_p32, dim3(nb), dim3(RB), 0, s, p, (const float *)z32, n, (const double *)sc);
     }
     it += todo;
     if (hipMemcpyAsync(hs, sc, 64, hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
@@ -845,7 +859,7 @@ static int u_solve_newton(vfg_handle *h, double *x, double *r, double *z, double
       // Chebyshev smoother amplify modes above its interval)
       if (h->mg) h->mg->setups = 0;
       if ((rc = mg_setup(h, dinv))) return rc;
-      prc = pcg_mg(h, n, A, d, r, z, p, q, h->wf[0], h->wf[1], inner, 0., o->maxit, 4, &li);
+      prc = pcg_mg(h, n, A, d, r, p, q, h->wf[0], h->wf[1], inner, 0., o->maxit, 4, &li);
     } else {
       prc = pcg(h, n, A, dinv, d, r, z, p, q, inner, 0., o->maxit, 20, &li);
     }
@@ -981,8 +995,9 @@ int vfg_u_solve(vfg_handle *h, double *U, const double *BCU, const vfg_solve_opt
     rc = vfg_fail(h, VFG_E_ARG, "vfg_u_solve: the Chebyshev preconditioner is provided for the mixed-precision solve");
   } else if (o->pc == VFG_PC_MG) {
     ApplyFn A = [&](const double *in, double *out) { launch_u_apply(P, in, out, s); };
+    DotApplyFn Ad = [&](const double *in, double *out) { return launch_u_apply_dot(P, in, out, h->part, s); };
     rc = mg_setup(h, dinv);
-    if (!rc) rc = pcg_mg(h, n, A, x, r, z, p, q, h->wf[0], h->wf[1], o->rtol, o->atol, o->maxit, 4, &inf);
+    if (!rc) rc = pcg_mg(h, n, A, x, r, p, q, h->wf[0], h->wf[1], o->rtol, o->atol, o->maxit, 4, &inf, &Ad);
     inf.outer = 1;
   } else {
     // the u-operator is issue-bound at one wave per SIMD: fusing the direction

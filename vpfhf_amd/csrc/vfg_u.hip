@@ -387,6 +387,28 @@ __global__ __launch_bounds__(256, 2) void k_u_apply_lds(UApplyOp<double, false> 
   march_lds<double, 16, 16, false>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
+// the same apply, also reducing p.q (p = the input, q = the output, every
+// row) to one partial per workgroup: the CG's p.q without another pass
+struct UApplyDotOp : UApplyOp<double, false> {
+  double *part;
+  __device__ double store(int i, int j, int k, const double (&acc)[3], const double (&own)[4]) const {
+    const int64_t n = node_id(G, i, j, k);
+    double d = 0.;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const double yv = on_dirichlet(G, bc.face[c], bc.vert[c], i, j, k) ? own[c] : acc[c];
+      y[3 * n + c] = yv;
+      d += own[c] * yv;
+    }
+    return d;
+  }
+  __device__ void partial(int slot, double v) const { part[slot] = v; }
+};
+__global__ __launch_bounds__(256, 2) void k_u_apply_lds_dot(UApplyDotOp op, int OZ) {
+  march_lds<double, 16, 16, false>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z,
+                                   blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
+}
+
 // ---------------------------------------------------------------- diagonal
 // Ke[(a,c),(a,c)] = sum_g W s [(lam+mu) (d_c phi_a)^2 + mu |grad phi_a|^2]
 struct UDiagOp {
@@ -719,6 +741,25 @@ void launch_mg_coarse_cheb(const DevProblem &P, const float *b, const float *din
 
 void launch_u_apply(const DevProblem &P, const double *x, double *y, hipStream_t s) {
   launch_u_apply_t<double>(P, x, y, P.V, s);
+}
+
+int launch_u_apply_dot(const DevProblem &P, const double *x, double *y, double *part, hipStream_t s) {
+  if (P.unilateral) return 0;
+  UApplyDotOp op;
+  op.x = x; op.y = y; op.v = P.V;
+  op.E_elem = P.E_elem;
+  op.cmask = nullptr;
+  op.G = P.G; op.bc = P.bcU;
+  op.E = P.E; op.lam1 = P.lam1; op.mu1 = P.mu1; op.eta = P.eta;
+  op.vol = P.G.hx * P.G.hy * P.G.hz;
+  op.ihx = 1. / P.G.hx; op.ihy = 1. / P.G.hy; op.ihz = 1. / P.G.hz;
+  op.part = part;
+  static int slots = device_slots(k_u_apply_lds_dot, 256);
+  const int tiles = ((P.G.nx + 14) / 15) * ((P.G.ny + 14) / 15);
+  const int OZ = choose_oz(P.G, tiles, slots);
+  const dim3 g = march_grid(P.G, 16, 16, OZ);
+  hipLaunchKernelGGL(k_u_apply_lds_dot, g, dim3(256), 0, s, op, OZ);
+  return (int)(g.x * g.y * g.z);
 }
 void launch_u_apply_f32(const DevProblem &P, const float *x, float *y, hipStream_t s) {
   launch_u_apply_t<float>(P, x, y, P.V32, s);
