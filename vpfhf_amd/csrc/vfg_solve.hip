@@ -128,14 +128,15 @@ __global__ void k_neg(const double *a, double *b, int64_t n) {
 static unsigned nb_(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 typedef std::function<void(const double *, double *)> ApplyFn;
+// apply with the x.y partials written to h->part: (x, y) -> their number, 0
+// when the operator has no such kernel (nothing launched then)
+typedef std::function<int(const double *, double *)> DotApplyFn;
 // fused apply: (p_old, p, q) -> number of p.q partials written to h->part
 typedef std::function<int(const double *, double *, double *)> FusedFn;
 // CG update over a subset of the nodes: (p, q) -> number of (r.z, r.r)
 // partial pairs written to h->part
 typedef std::function<int(const double *, const double *)> XrFn;
-// apply with the x.y partials written to h->part: (x, y) -> their number, 0
-// when the operator has no such kernel (nothing launched then)
-typedef std::function<int(const double *, double *)> DotApplyFn;
+
 
 // Preconditioned CG with the direction update and the p.q product fused into
 // the operator apply (launch_v_apply_cg): per iteration
@@ -148,7 +149,8 @@ static int pcg_fused(vfg_handle *h, int64_t n, const FusedFn &A, const double *d
 // Preconditioned CG on the device.  On entry x is the initial guess and r the
 // initial residual b - A x.  Converged when ||r|| <= max(rtol*||r0||, atol).
 static int pcg(vfg_handle *h, int64_t n, const ApplyFn &A, const double *dinv, double *x, double *r, double *z,
-               double *p, double *q, double rtol, double atol, int maxit, int check, vfg_solve_info *info) {
+               double *p, double *q, double rtol, double atol, int maxit, int check, vfg_solve_info *info,
+               const DotApplyFn *Ad = nullptr) {
   hipStream_t s = h->stream;
   double *sc = h->scal, *part = h->part;
   const int nb = red_blocks(n);
@@ -165,9 +167,13 @@ static int pcg(vfg_handle *h, int64_t n, const ApplyFn &A, const double *dinv, d
   while (status == VFG_OK && rn > tol && it < maxit) {
     int todo = check < maxit - it ? check : maxit - it;
     for (int k = 0; k < todo; k++) {
-      A(p, q);
-      hipLaunchKernelGGL(k_dotp, dim3(nb), dim3(RB), 0, s, (const double *)p, (const double *)q, n, part);
-      hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(RB), 0, s, (const double *)part, nb, 1, sc);
+      int na = Ad ? (*Ad)(p, q) : 0;  // q = A p with the p.q partials
+      if (!na) {
+        A(p, q);
+        hipLaunchKernelGGL(k_dotp, dim3(nb), dim3(RB), 0, s, (const double *)p, (const double *)q, n, part);
+        na = nb;
+      }
+      hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(RB), 0, s, (const double *)part, na, 1, sc);
       hipLaunchKernelGGL(k_pcg_xr, dim3(nb), dim3(RB), 0, s, x, r, z, (const double *)p, (const double *)q, dinv, n,
                          (const double *)sc, part);
       hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(RB), 0, s, (const double *)part, nb, 2, sc);
@@ -1004,7 +1010,8 @@ int vfg_u_solve(vfg_handle *h, double *U, const double *BCU, const vfg_solve_opt
     // update into its staging, as the V-solve does, measured slower than the
     // separate vector kernels (282 us against 215 + 33 us at 128^3)
     ApplyFn A = [&](const double *in, double *out) { launch_u_apply(P, in, out, s); };
-    rc = pcg(h, n, A, dinv, x, r, z, p, q, o->rtol, o->atol, o->maxit, 20, &inf);
+    DotApplyFn Ad = [&](const double *in, double *out) { return launch_u_apply_dot(P, in, out, h->part, s); };
+    rc = pcg(h, n, A, dinv, x, r, z, p, q, o->rtol, o->atol, o->maxit, 20, &inf, &Ad);
     inf.outer = 1;
   }
   if (info) *info = inf;
