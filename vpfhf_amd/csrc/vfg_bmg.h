@@ -2,10 +2,37 @@
 #pragma once
 #include <vector>
 #include <hip/hip_runtime.h>
-
-struct vfg_handle;
+#include "vfg_handle.h"
 
 namespace vfg {
+
+constexpr int BML = MgHier::MAXL;
+constexpr int BMK = MG_COARSE_DEG > MG_SMOOTH ? MG_COARSE_DEG : MG_SMOOTH;
+
+// device tables of the batched V-cycle: a member's level vectors, work-list
+// chunks (member, first entry, count) and Chebyshev step coefficients
+struct BmgVec {
+  float *b, *x, *y, *d;
+  const float *dinv;
+  Grid G;
+  BcMask bc;
+  int coarsest;  // 1 on the member's coarsest level
+};
+struct BmgChunk {
+  int p;
+  int64_t start;
+  int len;
+};
+struct BmgCoef {
+  float c1, c2;
+};
+
+// the coarsest-level Chebyshev(MG_COARSE_DEG) solve, from x = 0, of every
+// member listed in ch[0, *dn) whose chunk starts its vector, one workgroup
+// each with the level in LDS (levels that satisfy mg_coarse_fits); ops: the
+// level's FP32 apply records, one per member
+void launch_bmg_coarse_cheb(const void *ops, const BmgVec *lvt, int level, const BmgChunk *ch, const int *dn,
+                            int grid, const BmgCoef *coef, const double *flags, int fstride, hipStream_t s);
 
 struct BatchMG;
 // level plans of the members' hierarchies; r32[p] / z32[p]: the member's FP32

@@ -21,26 +21,9 @@
 
 namespace vfg {
 
-constexpr int BML = MgHier::MAXL;
-constexpr int BMK = MG_COARSE_DEG > MG_SMOOTH ? MG_COARSE_DEG : MG_SMOOTH;
 constexpr int BCH = 4096;  // entries per Chebyshev chunk
 constexpr int BTR = 256;   // entries per transfer chunk: one gather per thread
 
-struct BmgVec {
-  float *b, *x, *y, *d;
-  const float *dinv;
-  Grid G;
-  BcMask bc;
-  int coarsest;  // 1 on the member's coarsest level
-};
-struct BmgChunk {
-  int p;
-  int64_t start;
-  int len;
-};
-struct BmgCoef {
-  float c1, c2;
-};
 
 // Launch grids are list capacities; dn holds the live chunk count, so a
 // captured graph stays valid while lists are rebuilt in place.
@@ -105,6 +88,7 @@ struct BmgPlan {
   // live counts: M items, M chunks, D items, D chunks, C items, C chunks, rch, pch
   int *dcnt = nullptr;
   int hcnt[8] = {0};
+  int cfit = 0;  // every member whose coarsest level this is fits the one-launch solve
 };
 struct BatchMG {
   std::vector<vfg_handle *> m;        // per member (nullptr: empty slot)
@@ -230,6 +214,11 @@ int bmg_rebuild(BatchMG *g, hipStream_t s, bool *grew) {
     }
     P.hcnt[6] = (int)P.rch.h.size();
     P.hcnt[7] = (int)P.pch.h.size();
+    int cfit = P.C.ch.h.empty() ? 0 : 1;
+    for (int p = 0; p < g->np && cfit; p++)
+      if (g->m[p] && g->L[p] - 1 == l && !mg_coarse_fits(g->m[p]->mg->lv[l].P)) cfit = 0;
+    if (cfit != P.cfit) *grew = true;  // the V-cycle's launches change
+    P.cfit = cfit;
     for (int k = 0; k < 3 && !rc; k++)
       if ((rc = push(ls[k]->items, s)) || (rc = push(ls[k]->ch, s))) break;
     if (rc || (rc = push(P.rch, s)) || (rc = push(P.pch, s))) break;
@@ -385,7 +374,12 @@ void bmg_vcycle(BatchMG *g, const double *flags, int fstride, hipStream_t s) {
   for (int l = 0; l < g->Lcap; l++) {
     BmgPlan &P = g->lv[l];
     // pre-smoothing (descending members) and the coarsest-level solve, from x = 0
-    for (int i = 0; i < BMK; i++) {
+    if (P.cfit) {
+      cheb(g, l, P.D, 1, 0, 0, 1, flags, fstride, s);
+      launch_bmg_coarse_cheb(P.ops, (const BmgVec *)g->d_lvt, l, (const BmgChunk *)P.C.ch.d,
+                             (const int *)(P.dcnt + 5), P.C.ch.cap, (const BmgCoef *)g->d_coef, flags, fstride, s);
+    }
+    for (int i = 0; i < BMK && !P.cfit; i++) {
       const int which = i < MG_SMOOTH ? 0 : 2;
       const BmgList &L = which == 0 ? P.M : P.C;
       if (!L.ch.cap) break;

@@ -15,6 +15,7 @@
 #include "vfg_batch_apply.h"
 #include "vfg_handle.h"
 #include "vfg_mg.h"
+#include "vfg_bmg.h"
 
 namespace vfg {
 
@@ -660,11 +661,18 @@ void launch_u_cmask(const DevProblem &P, const double *U, hipStream_t s) {
 // y-pairs, the lower element layer before the upper), so it gives the values
 // of launch_u_apply_f32; the step is mg_cheb_entry, as in the multi-launch
 // smoother.
+// one apply record of the batched / engine applies
+template <typename T>
+struct BApply {
+  UApplyOp<T> op;
+  int OZ;
+};
+
 constexpr int MGC_T = 512, MGC_NE = 512, MGC_NN = 729;
 
-__global__ __launch_bounds__(MGC_T) void k_mg_coarse_cheb(UApplyOp<float, false> op, const float *__restrict__ b,
-                                                          const float *__restrict__ dinv, float *__restrict__ xout,
-                                                          MgChebCoefs cc) {
+__device__ __forceinline__ void mg_coarse_cheb_body(const UApplyOp<float, false> &op, const float *__restrict__ b,
+                                                    const float *__restrict__ dinv, float *__restrict__ xout,
+                                                    const MgChebCoefs &cc) {
   __shared__ float sx[3 * MGC_NN], sd[3 * MGC_NN], sy[3 * MGC_NN];
   __shared__ float sr[MGC_NE * 24];
   const Grid &G = op.G;
@@ -722,6 +730,35 @@ __global__ __launch_bounds__(MGC_T) void k_mg_coarse_cheb(UApplyOp<float, false>
   for (int t = tid; t < 3 * nn; t += MGC_T) xout[t] = sx[t];
 }
 
+__global__ __launch_bounds__(MGC_T) void k_mg_coarse_cheb(UApplyOp<float, false> op, const float *__restrict__ b,
+                                                          const float *__restrict__ dinv, float *__restrict__ xout,
+                                                          MgChebCoefs cc) {
+  mg_coarse_cheb_body(op, b, dinv, xout, cc);
+}
+
+// the batched form: one workgroup per listed member (its first chunk), the
+// member's level operator, vectors and coarsest-level coefficients from the
+// engine's tables
+__global__ __launch_bounds__(MGC_T) void k_bmg_coarse_cheb(const BApply<float> *__restrict__ ops,
+                                                           const BmgVec *__restrict__ lvt, int level,
+                                                           const BmgChunk *__restrict__ ch, const int *__restrict__ dn,
+                                                           const BmgCoef *__restrict__ coef,
+                                                           const double *__restrict__ flags, int fstride) {
+  if ((int)blockIdx.x >= *dn) return;
+  const BmgChunk c = ch[blockIdx.x];
+  if (c.start != 0 || flags[(int64_t)c.p * fstride] == 0.) return;  // uniform in the workgroup
+  const BmgVec &v = lvt[c.p * BML + level];
+  MgChebCoefs cc;
+  cc.k = MG_COARSE_DEG;
+#pragma unroll
+  for (int i = 0; i < MG_COARSE_DEG; i++) {
+    const BmgCoef k = coef[((c.p * BML + level) * 2 + 1) * BMK + i];
+    cc.c1[i] = k.c1;
+    cc.c2[i] = k.c2;
+  }
+  mg_coarse_cheb_body(ops[c.p].op, v.b, v.dinv, v.x, cc);
+}
+
 bool mg_coarse_fits(const DevProblem &P) {
   return !P.unilateral && P.ne <= MGC_NE && P.nn <= MGC_NN;
 }
@@ -737,6 +774,13 @@ void launch_mg_coarse_cheb(const DevProblem &P, const float *b, const float *din
   op.vol = float(P.G.hx * P.G.hy * P.G.hz);
   op.ihx = float(1. / P.G.hx); op.ihy = float(1. / P.G.hy); op.ihz = float(1. / P.G.hz);
   hipLaunchKernelGGL(k_mg_coarse_cheb, dim3(1), dim3(MGC_T), 0, s, op, b, dinv, x, cc);
+}
+
+void launch_bmg_coarse_cheb(const void *ops, const BmgVec *lvt, int level, const BmgChunk *ch, const int *dn,
+                            int grid, const BmgCoef *coef, const double *flags, int fstride, hipStream_t s) {
+  if (grid <= 0) return;
+  hipLaunchKernelGGL(k_bmg_coarse_cheb, dim3(grid), dim3(MGC_T), 0, s, (const BApply<float> *)ops, lvt, level, ch,
+                     dn, coef, flags, fstride);
 }
 
 void launch_u_apply(const DevProblem &P, const double *x, double *y, hipStream_t s) {
@@ -807,11 +851,6 @@ void launch_u_rhs(const DevProblem &P, const double *U, const double *BCU, doubl
 // ------------------------------------------------- persistent multi-problem apply
 namespace vfg {
 
-template <typename T>
-struct BApply {
-  UApplyOp<T> op;
-  int OZ;
-};
 
 // Persistent walk over a work list.  dcount (optional) holds the item count on
 // the device, so a captured graph follows a list that changes between replays;
