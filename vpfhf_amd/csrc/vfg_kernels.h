@@ -73,8 +73,10 @@ void launch_v_diag(const DevProblem &P, double *d, hipStream_t s, const uint8_t 
 // list[cap] and L in list[cap + 1] (cap = v_chunk_cap, flag holds cap words).
 int v_chunk_cap(const Grid &G);
 void launch_v_chunk_list(const DevProblem &P, const uint8_t *mask, uint32_t *flag, int *list, hipStream_t s);
-// launch_v_apply_cg over the listed chunks only (p, q must be zero on the
-// others); returns the number of p.q partials (= count)
+// CG-fused masked V apply (mask: fixed or active nodes) over the listed
+// chunks: p = z + beta p_old staged, q = K p, p and q written on the chunks'
+// owned nodes (p, q must be zero on the other chunks), one partial of p.q per
+// chunk; returns the number of partials (= count)
 int launch_v_apply_cg_list(const DevProblem &P, const double *pold, double *p, const double *z, const double *beta,
                            double *q, double *part, const uint8_t *mask, const int *list, int count, hipStream_t s);
 // launch_v_apply and launch_v_diag over the listed chunks' owned nodes (the
@@ -87,11 +89,6 @@ void launch_v_diag_list(const DevProblem &P, double *d, const uint8_t *mask, con
 // (r.z, r.r) partial pairs; returns their number (= count)
 int launch_v_xr_list(const Grid &G, const int *list, int count, double *x, double *r, double *z, const double *p,
                      const double *q, const double *dinv, const double *alpha, double *part, hipStream_t s);
-// CG-fused masked V apply (mask: fixed or active nodes): p = z + beta p_old
-// staged, q = K p, p and q written, one partial of p.q per workgroup; returns
-// the number of partials
-int launch_v_apply_cg(const DevProblem &P, const double *pold, double *p, const double *z, const double *beta,
-                      double *q, double *part, const uint8_t *mask, hipStream_t s);
 
 // misc (vfg_misc.hip)
 void launch_bc_mask(const DevProblem &P, int which, uint8_t *mask, hipStream_t s);

@@ -139,7 +139,7 @@ typedef std::function<int(const double *, const double *)> XrFn;
 
 
 // Preconditioned CG with the direction update and the p.q product fused into
-// the operator apply (launch_v_apply_cg): per iteration
+// the operator apply (launch_v_apply_cg_list): per iteration
 // one apply, one vector update of x, r, z and two scalar steps.  p alternates
 // between two arrays.  Same convergence test and reporting as pcg().
 static int pcg_fused(vfg_handle *h, int64_t n, const FusedFn &A, const double *dinv, double *x, double *r,

@@ -285,10 +285,6 @@ struct VApplyCgOp : VApplyOp {
   }
   __device__ void partial(int slot, double v) const { part[slot] = v; }
 };
-__global__ __launch_bounds__(256, 3) void k_v_apply_cg(VApplyCgOp op, int OZ) {
-  march<double, 16, 16>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z,
-                        blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
-}
 
 __global__ __launch_bounds__(256, 3) void k_v_apply(VApplyOp op, int OZ) { march<double, 16, 16>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z); }
 
@@ -625,15 +621,6 @@ static VApplyCgOp make_vcg_op(const DevProblem &P, const double *pold, double *p
   op.mode = 0; op.atnum = P.atnum; op.irrevtol = P.irrevtol;
   op.z = z; op.beta = beta; op.pw = p; op.part = part;
   return op;
-}
-
-int launch_v_apply_cg(const DevProblem &P, const double *pold, double *p, const double *z, const double *beta,
-                      double *q, double *part, const uint8_t *mask, hipStream_t s) {
-  const VApplyCgOp op = make_vcg_op(P, pold, p, z, beta, q, part, mask);
-  const int OZ = oz_for(P.G);
-  const dim3 g = mgrid(P.G, OZ);
-  hipLaunchKernelGGL(k_v_apply_cg, g, dim3(256), 0, s, op, OZ);
-  return (int)(g.x * g.y * g.z);
 }
 
 int v_chunk_cap(const Grid &G) { return ((G.nx + 14) / 15) * ((G.ny + 14) / 15) * G.nz; }
