@@ -78,7 +78,8 @@ void launch_v_chunk_list(const DevProblem &P, const uint8_t *mask, uint32_t *fla
 // owned nodes (p, q must be zero on the other chunks), one partial of p.q per
 // chunk; returns the number of partials (= count)
 int launch_v_apply_cg_list(const DevProblem &P, const double *pold, double *p, const double *z, const double *beta,
-                           double *q, double *part, const uint8_t *mask, const int *list, int count, hipStream_t s);
+                           double *q, double *part, const uint8_t *mask, const int *list, int count, hipStream_t s,
+                           const double *done = nullptr);
 // launch_v_apply and launch_v_diag over the listed chunks' owned nodes (the
 // other entries of y, d are left as they are)
 void launch_v_apply_list(const DevProblem &P, const double *x, double *y, int mode, const int *list, int count,
@@ -88,7 +89,9 @@ void launch_v_diag_list(const DevProblem &P, double *d, const uint8_t *mask, con
 // the CG update x += a p, r -= a q, z = dinv r over the listed chunks, with
 // (r.z, r.r) partial pairs; returns their number (= count)
 int launch_v_xr_list(const Grid &G, const int *list, int count, double *x, double *r, double *z, const double *p,
-                     const double *q, const double *dinv, const double *alpha, double *part, hipStream_t s);
+                     const double *q, const double *dinv, const double *alpha, double *part, hipStream_t s,
+                     const double *done = nullptr);
+// (both: the launch does nothing once *done != 0, a device-side stop of the CG)
 
 // misc (vfg_misc.hip)
 void launch_bc_mask(const DevProblem &P, int which, uint8_t *mask, hipStream_t s);

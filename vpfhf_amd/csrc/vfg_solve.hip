@@ -22,7 +22,7 @@
 
 namespace vfg {
 
-enum { S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_RR0, S_FLAG, S_TMP0, S_TMP1, S_TMP2 };
+enum { S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_RR0, S_FLAG, S_TMP0, S_TMP1, S_TMP2, S_TOL2, S_DONE, S_ITS };
 constexpr int RB = 256;
 
 __device__ __forceinline__ double bsum(double v) {
@@ -60,6 +60,9 @@ __global__ __launch_bounds__(RB) void k_pcg_scalar(const double *part, int nb, i
   if (threadIdx.x != 0) return;
   if (op == 0) {
     sc[S_RZ] = a; sc[S_RR] = b; sc[S_RR0] = b; sc[S_FLAG] = 0.;
+    sc[S_TOL2] = 0.; sc[S_DONE] = 0.; sc[S_ITS] = 0.;
+  } else if (sc[S_DONE] != 0.) {
+    return;  // converged: the rest of the iteration batch does nothing
   } else if (op == 1) {
     sc[S_PQ] = a;
     if (!(a > 0.) || !isfinite(a)) { sc[S_ALPHA] = 0.; sc[S_FLAG] = isfinite(a) ? 1. : 2.; }
@@ -69,7 +72,9 @@ __global__ __launch_bounds__(RB) void k_pcg_scalar(const double *part, int nb, i
     sc[S_BETA] = rz != 0. ? a / rz : 0.;
     sc[S_RZ] = a;
     sc[S_RR] = b;
+    sc[S_ITS] += 1.;
     if (!isfinite(a) || !isfinite(b)) sc[S_FLAG] = 2.;
+    else if (sc[S_TOL2] > 0. && b <= sc[S_TOL2]) sc[S_DONE] = 1.;  // only solvers whose kernels all check S_DONE set S_TOL2
   }
 }
 
@@ -144,7 +149,7 @@ typedef std::function<int(const double *, const double *)> XrFn;
 // between two arrays.  Same convergence test and reporting as pcg().
 static int pcg_fused(vfg_handle *h, int64_t n, const FusedFn &A, const double *dinv, double *x, double *r,
                      double *z, double *p, double *p2, double *q, double rtol, double atol, int maxit, int check,
-                     vfg_solve_info *info, const XrFn *xr = nullptr);
+                     vfg_solve_info *info, const XrFn *xr = nullptr, double rn0_known = -1.);
 
 // Preconditioned CG on the device.  On entry x is the initial guess and r the
 // initial residual b - A x.  Converged when ||r|| <= max(rtol*||r0||, atol).
@@ -200,21 +205,26 @@ __global__ void k_set_scalar(double *sc, int i, double v) { sc[i] = v; }
 
 static int pcg_fused(vfg_handle *h, int64_t n, const FusedFn &A, const double *dinv, double *x, double *r,
                      double *z, double *p, double *p2, double *q, double rtol, double atol, int maxit, int check,
-                     vfg_solve_info *info, const XrFn *xr) {
+                     vfg_solve_info *info, const XrFn *xr, double rn0_known) {
   hipStream_t s = h->stream;
   double *sc = h->scal, *part = h->part;
   const int nb = red_blocks(n);
   hipLaunchKernelGGL(k_pcg_init, dim3(nb), dim3(RB), 0, s, r, z, p, dinv, n, part);
   hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(RB), 0, s, (const double *)part, nb, 0, sc);
   hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, sc, (int)S_BETA, 0.);  // first direction p = z
-  double hs[8];
-  if (hipMemcpyAsync(hs, sc, 64, hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+  double hs[13];
+  // ||r0|| from the caller when it has just read it back; else read here
+  if (rn0_known < 0. &&
+      (hipMemcpyAsync(hs, sc, 64, hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess))
     return vfg_fail(h, VFG_E_HIP, "pcg: initial reduction failed");
-  const double rn0 = sqrt(hs[S_RR0]);
+  const double rn0 = rn0_known < 0. ? sqrt(hs[S_RR0]) : rn0_known;
   double rn = rn0;
   int it = 0, status = VFG_OK;
   const double tol = fmax(rtol * rn0, atol);
   if (!isfinite(rn0)) status = VFG_E_NAN;
+  // with xr (the listed kernels), the iteration that reaches tol marks the
+  // solve done on the device and the rest of its batch exits at once
+  if (xr) hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, sc, (int)S_TOL2, tol * tol);
   double *pa = p, *pb = p2;  // previous / current direction
   while (status == VFG_OK && rn > tol && it < maxit) {
     int todo = check < maxit - it ? check : maxit - it;
@@ -231,13 +241,14 @@ static int pcg_fused(vfg_handle *h, int64_t n, const FusedFn &A, const double *d
       pa = pb;
       pb = t;
     }
-    it += todo;
-    if (hipMemcpyAsync(hs, sc, 64, hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+    if (hipMemcpyAsync(hs, sc, 8 * 13, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
       return vfg_fail(h, VFG_E_HIP, "pcg: iteration failed");
+    it = (int)hs[S_ITS];  // iterations that ran (with xr the batch stops at convergence)
     rn = sqrt(hs[S_RR]);
     if (hs[S_FLAG] == 2. || !isfinite(rn)) status = VFG_E_NAN;
     else if (hs[S_FLAG] == 1.) status = rn <= tol ? VFG_OK : VFG_E_BREAKDOWN;
-    if (hs[S_FLAG] != 0.) break;
+    if (hs[S_FLAG] != 0. || hs[S_DONE] != 0.) break;
   }
   if (status == VFG_OK && rn > tol) status = VFG_E_MAXIT;
   if (info) {
@@ -1096,13 +1107,16 @@ int vfg_v_solve(vfg_handle *h, const double *U, double *V, const double *ub, con
     launch_fill(x, 0., n, s);
     launch_fill(h->w[12], 0., n, s);
     FusedFn A = [&](const double *pold, double *pn, double *qq) {
-      return launch_v_apply_cg_list(P, pold, pn, z, h->scal + S_BETA, qq, h->part, act, vlist, nlist, s);
+      return launch_v_apply_cg_list(P, pold, pn, z, h->scal + S_BETA, qq, h->part, act, vlist, nlist, s,
+                                    h->scal + S_DONE);
     };
     XrFn XR = [&](const double *pc, const double *qc) {
-      return launch_v_xr_list(P.G, vlist, nlist, x, r, z, pc, qc, dinv, h->scal + S_ALPHA, h->part, s);
+      return launch_v_xr_list(P.G, vlist, nlist, x, r, z, pc, qc, dinv, h->scal + S_ALPHA, h->part, s,
+                              h->scal + S_DONE);
     };
     const double inner_rtol = fmax(inner_floor, fmin(0.1, 0.1 * pg / pg0));
-    int prc = pcg_fused(h, n, A, dinv, x, r, z, p, h->w[12], q, inner_rtol, 0., 20000, 20, &inf, &XR);
+    // r = -g on the free nodes, so ||r0|| = pg
+    int prc = pcg_fused(h, n, A, dinv, x, r, z, p, h->w[12], q, inner_rtol, 0., 20000, 20, &inf, &XR, pg);
     if (prc == VFG_E_HIP || prc == VFG_E_NAN) { status = prc; break; }
     // projected step with Armijo backtracking on f(V) = 1/2 V.KV - b.V
     hipLaunchKernelGGL(k_zero_fixed, dim3(nb_(n)), dim3(256), 0, s, g, (const uint8_t *)P.vfix, n);
@@ -1113,10 +1127,14 @@ int vfg_v_solve(vfg_handle *h, const double *U, double *V, const double *ub, con
       // D vanishes off the free nodes: K D is needed on the listed chunks only
       // (KD is zeroed when the solve starts, so it is finite everywhere)
       launch_v_apply_list(P, D, KD, 2, vlist, nlist, s);
-      int e2 = 0;
-      double gd = dot_host(h, g, D, n, &e2);
-      double dkd = dot_host(h, D, KD, n, &e2);
-      if (e2) return e2;
+      // g.D and D.KD, read back together
+      launch_dot(g, D, n, h->part, h->scal + 16, s);
+      launch_dot(D, KD, n, h->part, h->scal + 17, s);
+      double dots[2];
+      if (hipMemcpyAsync(dots, h->scal + 16, 16, hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess)
+        return vfg_fail(h, VFG_E_HIP, "vfg_v_solve: line-search reduction failed");
+      const double gd = dots[0], dkd = dots[1];
       if (gd + .5 * dkd <= 1e-4 * gd || ls == 39) break;
       lam *= .5;
     }

@@ -302,7 +302,9 @@ __device__ __forceinline__ VChunk chunk_at(const Grid &G, const int *__restrict_
   const int c = list[w], t = c % txy;
   return VChunk{t % gx, t / gx, c / txy, list[cap + 1]};
 }
-__global__ __launch_bounds__(256, 3) void k_v_apply_cg_list(VApplyCgOp op, const int *__restrict__ list, int cap) {
+__global__ __launch_bounds__(256, 3) void k_v_apply_cg_list(VApplyCgOp op, const int *__restrict__ list, int cap,
+                                                           const double *__restrict__ done) {
+  if (done && *done != 0.) return;  // the CG has converged
   const VChunk q = chunk_at(op.G, list, cap, blockIdx.x);
   march<double, 16, 16>(op.G, op, q.L, q.bx, q.by, q.bz, blockIdx.x);
 }
@@ -390,7 +392,9 @@ __global__ __launch_bounds__(256) void k_v_xr_list(Grid G, const int *__restrict
                                                    double *__restrict__ x, double *__restrict__ r,
                                                    double *__restrict__ z, const double *__restrict__ p,
                                                    const double *__restrict__ q, const double *__restrict__ dinv,
-                                                   const double *__restrict__ alpha, double *__restrict__ part) {
+                                                   const double *__restrict__ alpha, double *__restrict__ part,
+                                                   const double *__restrict__ done) {
+  if (done && *done != 0.) return;  // the CG has converged
   __shared__ double sh[4];
   const int gx = (G.nx + 14) / 15, txy = gx * ((G.ny + 14) / 15);
   const int c = list[blockIdx.x], L = list[cap + 1], t = c % txy;
@@ -636,18 +640,20 @@ void launch_v_chunk_list(const DevProblem &P, const uint8_t *mask, uint32_t *fla
 }
 
 int launch_v_apply_cg_list(const DevProblem &P, const double *pold, double *p, const double *z, const double *beta,
-                           double *q, double *part, const uint8_t *mask, const int *list, int count, hipStream_t s) {
+                           double *q, double *part, const uint8_t *mask, const int *list, int count, hipStream_t s,
+                           const double *done) {
   if (count <= 0) return 0;
   const VApplyCgOp op = make_vcg_op(P, pold, p, z, beta, q, part, mask);
-  hipLaunchKernelGGL(k_v_apply_cg_list, dim3((unsigned)count), dim3(256), 0, s, op, list, v_chunk_cap(P.G));
+  hipLaunchKernelGGL(k_v_apply_cg_list, dim3((unsigned)count), dim3(256), 0, s, op, list, v_chunk_cap(P.G), done);
   return count;
 }
 
 int launch_v_xr_list(const Grid &G, const int *list, int count, double *x, double *r, double *z, const double *p,
-                     const double *q, const double *dinv, const double *alpha, double *part, hipStream_t s) {
+                     const double *q, const double *dinv, const double *alpha, double *part, hipStream_t s,
+                     const double *done) {
   if (count <= 0) return 0;
   hipLaunchKernelGGL(k_v_xr_list, dim3((unsigned)count), dim3(256), 0, s, G, list, v_chunk_cap(G), x, r, z, p, q,
-                     dinv, alpha, part);
+                     dinv, alpha, part, done);
   return count;
 }
 
