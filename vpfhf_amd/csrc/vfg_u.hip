@@ -462,6 +462,106 @@ __global__ __launch_bounds__(256) void k_u_diag(UDiagOp op, int OZ) { march<doub
 // b_{a,c} = sum_g W d_c phi_a [(3lam+2mu) alpha dth s + beta p v]          (thermo-poro)
 //         + sum_g W p (d_c v) phi_a                       (crack pressure, optional)
 // mode 0: out = b (no BC); mode 1: out = K U - b with ResidualApplyDirichletBC
+// Load terms of the residual for one element: at every Gauss point the
+// damaged thermal / poroelastic stress tp (times grad N_a) and the crack
+// pressure term cp grad(v) (times N_a), both scaled by sgn, added into r.
+// The fields are interpolated along x first, and the projection is contracted
+// over gx, then gy, then added per gz.
+template <bool UNI>
+__device__ __forceinline__ void rhs_loads(const double (&X)[2][4][6], double coefalpha, double beta, double eta,
+                                          double vol, bool crack, const double (&ih)[3], uint32_t cm, double sgn,
+                                          double (&r)[8][3]) {
+  double f1[3][2][2][3], dvx[2][2];  // [v, th, p][k][j][gx] interpolated along x; dv/dx on the (j, k) lines
+#pragma unroll
+  for (int k = 0; k < 2; k++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      const double *a0 = X[k][j * 2], *a1 = X[k][j * 2 + 1];
+#pragma unroll
+      for (int f = 0; f < 3; f++)
+#pragma unroll
+        for (int gx = 0; gx < 3; gx++) f1[f][k][j][gx] = gP0(gx) * a0[3 + f] + gP1(gx) * a1[3 + f];
+      dvx[k][j] = (a1[3] - a0[3]) * ih[0];
+    }
+#pragma unroll
+  for (int gz = 0; gz < 3; gz++) {
+    double SA[3], SAi[2][3], SB[3][2][3];  // sums over gx: of tp; of tp P_i; of B_c P_i  ([..][gy])
+#pragma unroll
+    for (int gy = 0; gy < 3; gy++) {
+      const double wjk[2][2] = {{gP0(gy) * gP0(gz), gP1(gy) * gP0(gz)}, {gP0(gy) * gP1(gz), gP1(gy) * gP1(gz)}};
+      const double gvx = wjk[0][0] * dvx[0][0] + wjk[0][1] * dvx[0][1] + wjk[1][0] * dvx[1][0] + wjk[1][1] * dvx[1][1];
+      SA[gy] = 0.;
+#pragma unroll
+      for (int i = 0; i < 2; i++) {
+        SAi[i][gy] = 0.;
+#pragma unroll
+        for (int c = 0; c < 3; c++) SB[c][i][gy] = 0.;
+      }
+#pragma unroll
+      for (int gx = 0; gx < 3; gx++) {
+        double vv = 0., th = 0., pr = 0.;
+#pragma unroll
+        for (int k = 0; k < 2; k++)
+#pragma unroll
+          for (int j = 0; j < 2; j++) {
+            vv += wjk[k][j] * f1[0][k][j][gx];
+            th += wjk[k][j] * f1[1][k][j][gx];
+            pr += wjk[k][j] * f1[2][k][j][gx];
+          }
+        const double gvy = ih[1] * (gP0(gz) * (f1[0][0][1][gx] - f1[0][0][0][gx]) +
+                                    gP1(gz) * (f1[0][1][1][gx] - f1[0][1][0][gx]));
+        const double gvz = ih[2] * (gP0(gy) * (f1[0][1][0][gx] - f1[0][0][0][gx]) +
+                                    gP1(gy) * (f1[0][1][1][gx] - f1[0][0][1][gx]));
+        const double W = vol * gW(gx) * gW(gy) * gW(gz);
+        const bool comp = UNI && ((cm >> ((gz * 3 + gy) * 3 + gx)) & 1u);  // VFMech.c:505-513
+        const double tp = sgn * (comp ? W * (coefalpha * th * (1. + eta) + beta * pr)
+                                      : W * (coefalpha * th * (vv * vv + eta) + beta * pr * vv));
+        const double cp = crack ? sgn * W * pr : 0.;
+        const double B[3] = {cp * gvx, cp * gvy, cp * gvz};
+        SA[gy] += tp;
+        SAi[0][gy] += tp * gP0(gx);
+        SAi[1][gy] += tp * gP1(gx);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          SB[c][0][gy] += B[c] * gP0(gx);
+          SB[c][1][gy] += B[c] * gP1(gx);
+        }
+      }
+    }
+    // corner a = (i, j, k): x-term sx P_j P_k, y-term P_i sy P_k, z-term
+    // P_i P_j sz (s = -+1/h), N-term P_i P_j P_k
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      const double pz = k ? gP1(gz) : gP0(gz), sz = k ? ih[2] : -ih[2];
+#pragma unroll
+      for (int j = 0; j < 2; j++) {
+        const double sy = j ? ih[1] : -ih[1];
+        double TAx = 0., TAy[2] = {0., 0.}, TAz[2] = {0., 0.}, TB[3][2] = {{0., 0.}, {0., 0.}, {0., 0.}};
+#pragma unroll
+        for (int gy = 0; gy < 3; gy++) {
+          const double py = j ? gP1(gy) : gP0(gy);
+          TAx += SA[gy] * py;
+#pragma unroll
+          for (int i = 0; i < 2; i++) {
+            TAy[i] += SAi[i][gy];
+            TAz[i] += SAi[i][gy] * py;
+#pragma unroll
+            for (int c = 0; c < 3; c++) TB[c][i] += SB[c][i][gy] * py;
+          }
+        }
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+          const double sx = i ? ih[0] : -ih[0];
+          const int a = (k * 2 + j) * 2 + i;
+          r[a][0] += pz * (sx * TAx + TB[0][i]);
+          r[a][1] += pz * (sy * TAy[i] + TB[1][i]);
+          r[a][2] += sz * TAz[i] + pz * TB[2][i];
+        }
+      }
+    }
+  }
+}
+
 template <bool UNI>
 struct URhsOp {
   static constexpr int NF = 6, NC = 3;  // u0,u1,u2,v,dth,p
@@ -498,35 +598,7 @@ struct URhsOp {
       u_elem_apply<double, UNI>(Xa, Ee * vol, lam1, mu1, eta, ih[0], ih[1], ih[2], r, cm);
     }
     const double coefalpha = (3. * lam + 2. * mu) * alpha;
-    const double sgn = mode == 1 ? -1. : 1.;
-    for (int gz = 0; gz < 3; gz++)
-      for (int gy = 0; gy < 3; gy++)
-        for (int gx = 0; gx < 3; gx++) {
-          double vv = 0., th = 0., pr = 0., gv[3] = {0., 0., 0.};
-          for (int a = 0; a < 8; a++) {
-            int i = a & 1, j = (a >> 1) & 1, k = a >> 2;
-            double px = i ? gP1(gx) : gP0(gx), py = j ? gP1(gy) : gP0(gy), pz = k ? gP1(gz) : gP0(gz);
-            double ph = px * py * pz;
-            const double *xa = X[k][j * 2 + i];
-            vv += ph * xa[3]; th += ph * xa[4]; pr += ph * xa[5];
-            gv[0] += (i ? ih[0] : -ih[0]) * py * pz * xa[3];
-            gv[1] += (j ? ih[1] : -ih[1]) * px * pz * xa[3];
-            gv[2] += (k ? ih[2] : -ih[2]) * px * py * xa[3];
-          }
-          double W = vol * gW(gx) * gW(gy) * gW(gz);
-          const bool comp = (cm >> ((gz * 3 + gy) * 3 + gx)) & 1u;  // VFMech.c:505-513
-          double tp = comp ? W * (coefalpha * th * (1. + eta) + beta * pr)
-                           : W * (coefalpha * th * (vv * vv + eta) + beta * pr * vv);
-          double cp = crack ? W * pr : 0.;
-          for (int a = 0; a < 8; a++) {
-            int i = a & 1, j = (a >> 1) & 1, k = a >> 2;
-            double px = i ? gP1(gx) : gP0(gx), py = j ? gP1(gy) : gP0(gy), pz = k ? gP1(gz) : gP0(gz);
-            double dg[3] = {(i ? ih[0] : -ih[0]) * py * pz, (j ? ih[1] : -ih[1]) * px * pz,
-                            (k ? ih[2] : -ih[2]) * px * py};
-            double ph = px * py * pz;
-            for (int c = 0; c < 3; c++) r[a][c] += sgn * (tp * dg[c] + cp * gv[c] * ph);
-          }
-        }
+    rhs_loads<UNI>(X, coefalpha, beta, eta, vol, crack != 0, ih, cm, mode == 1 ? -1. : 1., r);
   }
   // value written by ResidualApplyDirichletBC: traversal order X0..Z1 then
   // vertices, later writers win (VFCartFE.c:747-999)
