@@ -329,35 +329,37 @@ __global__ void k_v_mark_planes(Grid G, const uint8_t *__restrict__ mask, uint32
   flag[i / 15 + gx * (j / 15) + (int64_t)txy * k] = 1u;  // same value from every writer
 }
 
+constexpr int VLT = 1024;  // threads of the chunk-list kernel
+
 __device__ __forceinline__ int block_isum(int v, int *sh) {
   __syncthreads();
   sh[threadIdx.x] = v;
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
+  for (int o = VLT / 2; o > 0; o >>= 1) {
     if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
     __syncthreads();
   }
   return sh[0];
 }
 
-// One workgroup of 256 threads.  Chunk length L = ceil(flagged planes /
+// One workgroup of VLT threads.  Chunk length L = ceil(flagged planes /
 // target), clamped to [lmin, lmax]: a thin band gives many one-plane chunks
 // (latency, not work, bounds the march of a few chunks), a wide one long
 // chunks (each chunk re-reads one halo plane).  The flagged chunks are listed
 // in increasing id; list[cap] = their count, list[cap + 1] = L.
-__global__ __launch_bounds__(256) void k_v_list_chunks(const uint32_t *__restrict__ flag, int txy, int nz, int lmin,
+__global__ __launch_bounds__(VLT) void k_v_list_chunks(const uint32_t *__restrict__ flag, int txy, int nz, int lmin,
                                                        int lmax, int target, int *__restrict__ list, int cap) {
-  __shared__ int sh[256];
+  __shared__ int sh[VLT];
   const int tid = threadIdx.x;
   const int64_t nf = (int64_t)txy * nz;
   int f = 0;
-  for (int64_t i = tid; i < nf; i += 256) f += flag[i] != 0u;
+  for (int64_t i = tid; i < nf; i += VLT) f += flag[i] != 0u;
   const int F = block_isum(f, sh);
   int L = (F + target - 1) / target;
   L = L < lmin ? lmin : L > lmax ? lmax : L;
   if (L < 1) L = 1;
   const int nzc = (nz + L - 1) / L;
-  const int64_t nch = (int64_t)txy * nzc, per = (nch + 255) / 256;
+  const int64_t nch = (int64_t)txy * nzc, per = (nch + VLT - 1) / VLT;
   const int64_t c0 = tid * per < nch ? tid * per : nch, c1 = c0 + per < nch ? c0 + per : nch;
   auto hit = [&](int64_t c) {
     const int t = (int)(c % txy), k0 = (int)(c / txy) * L, k1 = k0 + L < nz ? k0 + L : nz;
@@ -371,7 +373,7 @@ __global__ __launch_bounds__(256) void k_v_list_chunks(const uint32_t *__restric
   __syncthreads();
   sh[tid] = cnt;
   __syncthreads();
-  for (int o = 1; o < 256; o <<= 1) {
+  for (int o = 1; o < VLT; o <<= 1) {
     const int v = tid >= o ? sh[tid - o] : 0;
     __syncthreads();
     sh[tid] += v;
@@ -380,8 +382,8 @@ __global__ __launch_bounds__(256) void k_v_list_chunks(const uint32_t *__restric
   int off = sh[tid] - cnt;
   for (int64_t c = c0; c < c1; c++)
     if (hit(c)) list[off++] = (int)c;
-  if (tid == 255) {
-    list[cap] = sh[255];
+  if (tid == VLT - 1) {
+    list[cap] = sh[VLT - 1];
     list[cap + 1] = L;
   }
 }
@@ -635,7 +637,7 @@ void launch_v_chunk_list(const DevProblem &P, const uint8_t *mask, uint32_t *fla
   const int lmin = (cap + 32767) / 32768;
   hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, s, flag, 0u, (int64_t)cap);
   hipLaunchKernelGGL(k_v_mark_planes, dim3((unsigned)((P.nn + 255) / 256)), dim3(256), 0, s, P.G, mask, flag);
-  hipLaunchKernelGGL(k_v_list_chunks, dim3(1), dim3(256), 0, s, (const uint32_t *)flag, txy, P.G.nz, lmin, P.G.nz,
+  hipLaunchKernelGGL(k_v_list_chunks, dim3(1), dim3(VLT), 0, s, (const uint32_t *)flag, txy, P.G.nz, lmin, P.G.nz,
                      1024, list, cap);
 }
 
