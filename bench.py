@@ -26,7 +26,7 @@ sys.path.insert(0, ROOT)
 PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector (dense)
 # FP64 operations of the apply's element routine (u_elem_apply_lds), counted
-# in the AMD assembly of this build: 978 FMA + 123 MUL + 261 ADD per element
+# in the compiled kernel's AMD assembly: 978 FMA + 123 MUL + 261 ADD per element
 FP64_FLOPS_PER_ELEMENT = 2 * 978 + 123 + 261
 
 
@@ -312,8 +312,9 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=16)
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py measures one GPU: every box here has one MI355X and this path has no multi-GPU "
-              "decomposition to run (see DESIGN.md for how it would shard). Use --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: this path runs on one MI355X and has no multi-GPU decomposition "
+              "to run (a large problem would split into z-slabs, ensembles and request streams by problem). "
+              "Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import vpfhf_amd as V
