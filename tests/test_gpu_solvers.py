@@ -50,6 +50,27 @@ def test_v_solve_matches_oracle(oracle, atnum):
     P.close()
 
 
+def test_v_solve_long_chunks_matches_oracle(oracle):
+    """The V-solve's CG runs over chunks of node planes that hold free nodes;
+    once more than 1024 (tile, plane) pairs hold one, a chunk spans several
+    planes.  A tall grid of one tile per plane whose free set covers every
+    plane (AT2, strong elastic load) takes that path in its last Newton
+    steps: the converged V against the oracle's solve."""
+    s = S.make(n=(9, 8, 1100), l=(1.0, 0.9, 120.0), seed=11, atnum=2, bcV=S.MIXED_V)
+    s["U"] = s["U"] * 30.0
+    op, F = S.to_oracle(oracle, s)
+    P = S.to_gpu(s)
+    Vo = s["V"].copy()
+    oracle.v_solve(op, F, s["U"], Vo, ub=None, tol=1e-12)
+    free = (Vo > 1e-12) & (Vo < 1. - 1e-12)
+    planes = np.unique(np.arange(Vo.size)[free] // (9 * 8))
+    assert planes.size > 1024  # more flagged (tile, plane) pairs than one-plane chunks allow
+    Vg, info = P.VF_StepV(s["U"], s["V"], ub=None, rtol=1e-12)
+    assert info.status == 0
+    assert np.max(np.abs(Vg - Vo)) < 1e-7
+    P.close()
+
+
 def test_reference_test2_goldens(oracle):
     """ValidationTests/test2.c with the runtest2 options (makefile:207):
     -n 41,81,6 -l 1,2,.25 -length .4 -orientation 3 -nu 0 -epsilon 0.05 -eta 1e-8."""
