@@ -63,6 +63,33 @@ __device__ __forceinline__ float mg_restrict_dof(const Grid &Gf, const Grid &Gc,
   return acc;
 }
 
+// The same entry split over the fine planes: lane kk of four sums planes
+// k0 + kk, k0 + kk + 4, ...; the four lanes' values add up to the entry.
+__device__ __forceinline__ float mg_restrict_dof_plane(const Grid &Gf, const Grid &Gc, const BcMask &bc,
+                                                       const float *__restrict__ b, const float *__restrict__ y,
+                                                       int64_t N, int c, int kk) {
+  const int I = (int)(N % Gc.nx), J = (int)((N / Gc.nx) % Gc.ny), K = (int)(N / ((int64_t)Gc.nx * Gc.ny));
+  if (on_dirichlet(Gc, bc.face[c], bc.vert[c], I, J, K)) return 0.f;
+  int i0 = 0, j0 = 0, k0 = 0;
+  float wx[6], wy[6], wz[6];
+  const int mx = mg_gather(I, Gf.nex, Gc.nex, &i0, wx);
+  const int my = mg_gather(J, Gf.ney, Gc.ney, &j0, wy);
+  const int mz = mg_gather(K, Gf.nez, Gc.nez, &k0, wz);
+  float acc = 0.f;
+  for (int kp = kk; kp < mz; kp += 4)
+    for (int jj = 0; jj < my; jj++) {
+      float row = 0.f;
+      for (int ii = 0; ii < mx; ii++) {
+        const int i = i0 + ii, j = j0 + jj, k = k0 + kp;
+        if (on_dirichlet(Gf, bc.face[c], bc.vert[c], i, j, k)) continue;
+        const int64_t n = 3 * node_id(Gf, i, j, k) + c;
+        row += wx[ii] * (b[n] - y[n]);
+      }
+      acc += wy[jj] * wz[kp] * row;
+    }
+  return acc;
+}
+
 // x_fine(node n) += P x_coarse on the free fine rows
 __device__ __forceinline__ void mg_prolong_node(const Grid &Gf, const Grid &Gc, const BcMask &bc,
                                                 const float *__restrict__ xc, float *__restrict__ xf, int64_t n) {

@@ -95,6 +95,20 @@ __global__ __launch_bounds__(256) void k_mg_restrict(Grid Gf, Grid Gc, BcMask bc
   bcoarse[t] = mg_restrict_dof(Gf, Gc, bc, b, y, N, (int)(t - 3 * N));
 }
 
+// small coarse grids: four lanes per (node, component), each over every
+// fourth fine plane (one plane each under ceil(n/2) coarsening), summed in
+// lane order
+__global__ __launch_bounds__(256) void k_mg_restrict4(Grid Gf, Grid Gc, BcMask bc, const float *__restrict__ b,
+                                                      const float *__restrict__ y, float *__restrict__ bcoarse) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, t = g >> 2;
+  const int kk = (int)(g & 3);
+  const bool ok = t < 3 * (int64_t)Gc.nx * Gc.ny * Gc.nz;
+  const int64_t N = ok ? t / 3 : 0;
+  float v = ok ? mg_restrict_dof_plane(Gf, Gc, bc, b, y, N, (int)(t - 3 * N), kk) : 0.f;
+  const float v1 = __shfl_down(v, 1, 4), v2 = __shfl_down(v, 2, 4), v3 = __shfl_down(v, 3, 4);
+  if (ok && kk == 0) bcoarse[t] = ((v + v1) + v2) + v3;
+}
+
 __global__ __launch_bounds__(256) void k_mg_prolong(Grid Gf, Grid Gc, BcMask bc, const float *__restrict__ xc,
                                                     float *__restrict__ xf) {
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -145,7 +159,9 @@ void launch_mg_coarsen(const Grid &Gf, const double *cf, const Grid &Gc, double 
 void launch_mg_restrict(const Grid &Gf, const Grid &Gc, const BcMask &bc, const float *b, const float *y,
                         float *bcoarse, hipStream_t s) {
   const int64_t n3 = 3 * (int64_t)Gc.nx * Gc.ny * Gc.nz;
-  hipLaunchKernelGGL(k_mg_restrict, dim3(nbm(n3)), dim3(256), 0, s, Gf, Gc, bc, b, y, bcoarse);
+  // few coarse entries: latency-bound, split each over its fine planes
+  if (n3 <= (1 << 16)) hipLaunchKernelGGL(k_mg_restrict4, dim3(nbm(4 * n3)), dim3(256), 0, s, Gf, Gc, bc, b, y, bcoarse);
+  else hipLaunchKernelGGL(k_mg_restrict, dim3(nbm(n3)), dim3(256), 0, s, Gf, Gc, bc, b, y, bcoarse);
 }
 void launch_mg_prolong(const Grid &Gf, const Grid &Gc, const BcMask &bc, const float *xc, float *xf,
                        hipStream_t s) {
