@@ -552,8 +552,15 @@ static int pcg32(vfg_handle *h, int64_t n, const std::function<void(const float 
     return m;
   }
   const int check = 10;
+  double rate = -1.;  // observed residual reduction per iteration
   while (rn > rtol * rn0 && it < maxit) {
     int todo = check < maxit - it ? check : maxit - it;
+    // run only the iterations the observed rate says are left (at most check)
+    if (rate > 0. && rate < 1.) {
+      const double need = ceil(log(rtol * rn0 / rn) / log(rate));
+      if (need >= 1. && need < todo) todo = (int)need;
+    }
+    const double rn_before = rn;
     for (int k = 0; k < todo; k++) {
       A(p, q);
       hipLaunchKernelGGL(k_dotp32, dim3(nb), dim3(RB), 0, s, (const float *)p, (const float *)q, n, part);
@@ -571,6 +578,7 @@ static int pcg32(vfg_handle *h, int64_t n, const std::function<void(const float 
       return it;
     }
     rn = sqrt(hs[S_RR]);
+    if (rn_before > 0. && rn > 0.) rate = pow(rn / rn_before, 1. / todo);
     if (hs[S_FLAG] == 2. || !isfinite(rn)) { *status = VFG_E_NAN; break; }
     if (hs[S_FLAG] == 1.) { *status = rn <= rtol * rn0 ? VFG_OK : VFG_E_BREAKDOWN; break; }
   }
