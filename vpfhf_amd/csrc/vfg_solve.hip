@@ -169,8 +169,15 @@ static int pcg(vfg_handle *h, int64_t n, const ApplyFn &A, const double *dinv, d
   int it = 0, status = VFG_OK;
   const double tol = fmax(rtol * rn0, atol);
   if (!isfinite(rn0)) status = VFG_E_NAN;
+  double rate = -1.;  // observed residual reduction per iteration
   while (status == VFG_OK && rn > tol && it < maxit) {
     int todo = check < maxit - it ? check : maxit - it;
+    // run only the iterations the observed rate says are left (at most check)
+    if (rate > 0. && rate < 1.) {
+      const double need = ceil(log(tol / rn) / log(rate));
+      if (need >= 1. && need < todo) todo = (int)need;
+    }
+    const double rn_before = rn;
     for (int k = 0; k < todo; k++) {
       int na = Ad ? (*Ad)(p, q) : 0;  // q = A p with the p.q partials
       if (!na) {
@@ -188,6 +195,7 @@ static int pcg(vfg_handle *h, int64_t n, const ApplyFn &A, const double *dinv, d
     if (hipMemcpyAsync(hs, sc, 64, hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
       return vfg_fail(h, VFG_E_HIP, "pcg: iteration failed");
     rn = sqrt(hs[S_RR]);
+    if (rn_before > 0. && rn > 0.) rate = pow(rn / rn_before, 1. / todo);
     if (hs[S_FLAG] == 2. || !isfinite(rn)) status = VFG_E_NAN;
     else if (hs[S_FLAG] == 1.) status = rn <= tol ? VFG_OK : VFG_E_BREAKDOWN;
     if (hs[S_FLAG] != 0.) break;
