@@ -143,6 +143,183 @@ def test_rhs_linearity_and_pressure_work_identity(oracle):
     assert abs(pw - 1.7e-3 * cv) < 1e-12 * abs(pw) + 1e-20
 
 
+def _node_coords(n, l):
+    """Nodal x, y, z in DMDA natural order (VFCommon.c:391-406)."""
+    ax = [np.arange(n[d]) * l[d] / (n[d] - 1.) for d in range(3)]
+    Z, Y, X = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+    return X.ravel(), Y.ravel(), Z.ravel()
+
+
+def test_thermal_load_free_expansion(oracle):
+    """Free thermal expansion (VFMech.c:416-419, 434-435).  With a uniform
+    theta - thetaRef = dth, p = 0 and no Dirichlet rows, u = alpha*dth*x has the
+    strain alpha*dth*I at every Gauss point, whose stress under the degraded
+    stiffness is s_g (3 lambda + 2 mu) alpha dth I with s_g = v_g^2 + eta.  The
+    stiffness applied to u is then sum_g W_g s_g (3 lambda + 2 mu) alpha dth
+    dphi_a(g), which is the thermal load itself, on every row, for any V and any
+    per-element E.  A different modulus in the load ((lambda + 2 mu) alpha), a
+    missing eta or a load weighted by v instead of v^2 + eta fails."""
+    n, l = (6, 5, 7), (1.0, 0.8, 1.2)
+    s = S.make(n=n, l=l, seed=21, bcU={}, alpha=3e-3, eta=0.05, nu=0.27)
+    s["thetaRef"] = np.random.default_rng(3).uniform(0, 5, len(s["V"]))
+    dth = 7.0
+    s["theta"] = s["thetaRef"] + dth
+    s["pressure"] = np.zeros_like(s["V"])
+    op, F = S.to_oracle(oracle, s)
+    X, Y, Z = _node_coords(n, l)
+    u = np.zeros(3 * op.nn)
+    u[0::3], u[1::3], u[2::3] = s["alpha"] * dth * X, s["alpha"] * dth * Y, s["alpha"] * dth * Z
+    A = oracle.u_jacobian(op, F)
+    b = oracle.u_rhs(op, F)
+    assert np.abs(b).max() > 1e-3
+    assert np.abs(A @ u - b).max() < 1e-12 * np.abs(b).max()
+    # the same holds with V = 1 (s = 1 + eta everywhere)
+    s1 = dict(s); s1["V"] = np.ones_like(s["V"])
+    op1, F1 = S.to_oracle(oracle, s1)
+    b1 = oracle.u_rhs(op1, F1)
+    assert np.abs(oracle.u_jacobian(op1, F1) @ u - b1).max() < 1e-12 * np.abs(b1).max()
+
+
+def test_biot_and_crack_pressure_loads_cancel(oracle):
+    """With beta = 1 and a uniform p the poroelastic load p v dphi_a
+    (VFMech.c:434-435) and the crack-pressure load p (grad v) phi_a
+    (VFMech.c:575-577) add up to p times the integral of grad(v phi_a), which
+    vanishes on every node off the boundary (phi_a is zero there) for any V.
+    The crack-pressure load is pinned by runtest2; this pins the Biot term's
+    linear dependence on v (v^2, or a sign slip, fails)."""
+    n = (7, 6, 5)
+    s = S.make(n=n, seed=22, bcU={}, beta=1.0, thermal=False)
+    s["pressure"] = np.full(len(s["V"]), 2.5e-3)
+    op, F = S.to_oracle(oracle, s)
+    b = oracle.u_rhs(op, F).reshape(-1, 3)
+    i = np.arange(op.nn) % n[0]; j = (np.arange(op.nn) // n[0]) % n[1]; k = np.arange(op.nn) // (n[0] * n[1])
+    inner = (i > 0) & (i < n[0] - 1) & (j > 0) & (j < n[1] - 1) & (k > 0) & (k < n[2] - 1)
+    s0 = dict(s); s0["beta"] = 0.0
+    op0, F0 = S.to_oracle(oracle, s0)
+    crack_only = oracle.u_rhs(op0, F0).reshape(-1, 3)
+    assert np.abs(crack_only[inner]).max() > 1e-5
+    assert np.abs(b[inner]).max() < 1e-14 * np.abs(crack_only[inner]).max()
+
+
+def _q1_mass_1d(h):
+    return h / 6. * np.array([[2., 1.], [1., 2.]])
+
+
+def test_elastic_energy_affine_state_closed_form(oracle):
+    """VF_ElasticEnergy3D_local (VFMech.c:725-809) for an affine displacement
+    u = G x with uniform theta - thetaRef and uniform p over a random trilinear
+    V and per-element E.  The strain e = sym(G) is constant, so per element the
+    energy is mu (1+eta) |dev e|^2 int v^2 + (3(3lambda+2mu)/2) int WD^2 +
+    ((3lambda+2mu)/6) tr^2 eta vol, with WD = (tr/3 - alpha dth) v - beta p/(3lambda+2mu),
+    and int v^2, int v come from the Q1 mass matrix (exact).  Every coefficient
+    of the thermal and Biot terms enters: 2 alpha dth, a missing beta term or
+    a different bulk modulus fails.  Two special states are checked as well:
+    free expansion (only the eta term remains) and a dilatation that balances
+    the pore pressure at v = 1 (the spherical term vanishes)."""
+    n, l = (5, 6, 4), (1.0, 1.3, 0.7)
+    rng = np.random.default_rng(23)
+    s = S.make(n=n, l=l, seed=23, bcU={}, alpha=2e-3, beta=0.8, eta=0.03, nu=0.3)
+    dth, p = 4.0, 1.5e-3
+    s["theta"] = s["thetaRef"] + dth
+    s["pressure"] = np.full(len(s["V"]), p)
+    X, Y, Z = _node_coords(n, l)
+    h = [l[d] / (n[d] - 1) for d in range(3)]
+    vol = h[0] * h[1] * h[2]
+    M = np.kron(_q1_mass_1d(h[2]), np.kron(_q1_mass_1d(h[1]), _q1_mass_1d(h[0])))
+    lam1 = s["nu"] / (1 + s["nu"]) / (1 - 2 * s["nu"])
+    mu1 = 1 / (1 + s["nu"]) * .5
+
+    def closed_form(G, V, Ee):
+        e = (G + G.T) / 2
+        tr = np.trace(e)
+        dev = e - tr / 3 * np.eye(3)
+        dev2 = (dev * dev).sum()
+        tot = 0.
+        for ek in range(n[2] - 1):
+            for ej in range(n[1] - 1):
+                for ei in range(n[0] - 1):
+                    cell = ei + (n[0] - 1) * (ej + (n[1] - 1) * ek)
+                    nodes = [(ei + (a & 1)) + n[0] * ((ej + ((a >> 1) & 1)) + n[1] * (ek + (a >> 2)))
+                             for a in range(8)]
+                    v8 = V[nodes]
+                    iv2, iv = v8 @ M @ v8, vol * v8.mean()
+                    lam, mu = Ee[cell] * lam1, Ee[cell] * mu1
+                    tk = 3 * lam + 2 * mu
+                    a_, b_ = tr / 3 - s["alpha"] * dth, s["beta"] * p / tk
+                    tot += mu * (1 + s["eta"]) * dev2 * iv2
+                    tot += 1.5 * tk * (a_ * a_ * iv2 - 2 * a_ * b_ * iv + b_ * b_ * vol)
+                    tot += tk / 6 * tr * tr * s["eta"] * vol
+        return tot
+
+    op, F = S.to_oracle(oracle, s)
+    for _ in range(2):
+        G = 1e-3 * rng.standard_normal((3, 3))
+        U = np.stack([G[c, 0] * X + G[c, 1] * Y + G[c, 2] * Z for c in range(3)], 1).ravel()
+        el, _ = oracle.u_energy(op, F, U)
+        ref = closed_form(G, s["V"], s["E_elem"])
+        assert abs(el - ref) < 1e-12 * abs(ref)
+    # free expansion u = alpha dth x: only ((3 lambda + 2 mu)/6)(3 alpha dth)^2 eta vol remains
+    s0 = dict(s); s0["beta"] = 0.0
+    op0, F0 = S.to_oracle(oracle, s0)
+    ad = s["alpha"] * dth
+    U = np.stack([ad * X, ad * Y, ad * Z], 1).ravel()
+    el, _ = oracle.u_energy(op0, F0, U)
+    ref = sum((3 * lam1 + 2 * mu1) * Ee / 6 * (3 * ad) ** 2 * s["eta"] * vol for Ee in s["E_elem"])
+    assert abs(el - ref) < 1e-12 * ref
+    # homogeneous E, V = 1, no thermal strain, tr e / 3 = beta p / (3 lambda + 2 mu):
+    # the spherical term vanishes and the eta term alone remains
+    s2 = dict(s); s2["E_elem"] = None; s2["V"] = np.ones_like(s["V"]); s2["alpha"] = 0.0
+    op2, F2 = S.to_oracle(oracle, s2)
+    tk = 3 * lam1 + 2 * mu1
+    e0 = s["beta"] * p / tk
+    U = np.stack([e0 * X, e0 * Y, e0 * Z], 1).ravel()
+    el, _ = oracle.u_energy(op2, F2, U)
+    ref = tk / 6 * (3 * e0) ** 2 * s["eta"] * vol * op.ne
+    assert abs(el - ref) < 1e-12 * ref
+
+
+def test_v_crack_pressure_rhs(oracle):
+    """VF_ResidualVCrackPressure3D_local (VFMech.c:2294-2341): b_a = sum_g W p_g
+    (u_g . grad phi_a).  Contracted with V it is sum_g W p (u . grad v), the
+    pressure work of VF_PressureWork3D_local (VFMech.c:963-966; pinned by
+    runtest2 and by p x crack volume).  For a constant u0 under a uniform p,
+    b_a = p u0 . (boundary integral of phi_a n): zero off the boundary; on the
+    face x = 0 it is -p u0_x times the integral of phi_a over that face
+    (h_y h_z, halved at each edge of the face), +p u0_x on x = 1, and likewise
+    for y and z (edge and corner nodes sum their faces)."""
+    n = (6, 7, 5)
+    for atnum in (1, 2):
+        # Gc = 0 removes the AT part of the rhs, which leaves the crack-pressure term alone
+        s = S.make(n=n, seed=24 + atnum, bcU={}, atnum=atnum, Gc=0.0)
+        op, F = S.to_oracle(oracle, s)
+        sn = dict(s); sn["pressurize"] = False
+        opn, Fn = S.to_oracle(oracle, sn)
+        assert np.all(oracle.v_rhs(opn, Fn, s["U"]) == 0)
+        bp = oracle.v_rhs(op, F, s["U"])
+        _, pw = oracle.u_energy(op, F, s["U"])
+        assert abs(pw) > 1e-9
+        assert abs(s["V"] @ bp - pw) < 1e-12 * np.abs(s["V"] * bp).sum()
+    # constant u0, uniform p
+    p, u0 = 1.3e-3, np.array([0.4, -0.7, 0.25])
+    s["pressure"] = np.full(op.nn, p)
+    U = np.tile(u0, op.nn)
+    op, F = S.to_oracle(oracle, s)
+    bp = oracle.v_rhs(op, F, U)
+    h = [s["l"][d] / (n[d] - 1) for d in range(3)]
+    ref = np.zeros(op.nn)
+    idx = np.arange(op.nn)
+    ijk = [idx % n[0], (idx // n[0]) % n[1], idx // (n[0] * n[1])]
+    for d in range(3):
+        t = [q for q in range(3) if q != d]
+        # integral of phi_a over the face it lies on: product of the 1-D hat integrals
+        w = np.ones(op.nn)
+        for q in t:
+            w *= np.where((ijk[q] == 0) | (ijk[q] == n[q] - 1), h[q] / 2, h[q])
+        ref -= p * u0[d] * w * (ijk[d] == 0)
+        ref += p * u0[d] * w * (ijk[d] == n[d] - 1)
+    assert np.abs(bp - ref).max() < 1e-13 * np.abs(ref).max()
+
+
 def test_energy_density_uniaxial(oracle):
     """W = 1/2 E eps^2 for uniaxial strain at nu = 0 (VFMech.c:32-106)."""
     h = (0.5, 0.5, 0.5)
