@@ -1,0 +1,392 @@
+"""The benchmark workloads at their full sizes, and the reference's multi-step
+golden (ValidationTests/results/runtest1.out), through the HIP path.
+
+* runtest1 (ValidationTests/makefile:196-199, test1.c:40-95): AT1 bar under
+  imposed end displacements, 11 load steps, nucleation at step 4, run by the C
+  host (vfh_altmin) on the GPU; the bars of the oracle's own run of it.
+* 128^3 elements (the plain FP64 drop-in workload: lognormal E, 3 penny
+  cracks, pore pressure field, uniform temperature rise, clamped faces):
+  every operator on sub-boxes of the full grid against the oracle evaluated
+  on the same sub-box (a node's row depends only on its 8 elements, so the
+  rows away from the sub-box's cut faces are the full grid's rows; per-cell
+  energies are element-local); symmetry and rigid-mode annihilation of the
+  masked apply over the whole grid; bitwise determinism; the u-solve's FP64
+  residual through VF_UResidual; the V-solve's projected gradient through
+  VF_VResidual; ten alternate-minimisation iterations.
+* 256^3 elements (the precision form): FP32 CG inside FP64 refinement against
+  the FP64 solve in U, V, crack volume and surface energy (1e-6).
+* 64-member ensemble, sides 16-64: the batched mixed-precision multigrid solve
+  against each member's FP64 drop-in solve (1e-6) and each member's FP64
+  residual; the smallest members against the oracle's u-solve.
+* serving stream, sides 24-96: the mixed-precision engine against the FP64
+  engine (1e-6); the smallest request's first alternate-minimisation
+  iteration against the oracle.
+
+Tolerances: operators 1e-12 relative (the drop-in contract); solves as stated
+per test.
+"""
+import numpy as np
+import pytest
+
+from tests.scenarios import relerr
+
+pytestmark = pytest.mark.gpu
+
+
+# ------------------------------------------------------------------ runtest1
+RUNTEST1 = ("-n 101,5,5 -l 10,1,1 -nu 0 -epsilon 0.5 -eta 1e-8 -gc .1 -altmintol 1.e-3 -atnum 1 "
+            "-V_X0_BC ONE -U_X0_BC_0 FIXED -U_X0_0 -1. -U_X0_BC_1 ZERO -U_X0_BC_2 ZERO "
+            "-V_X1_BC ONE -U_X1_BC_0 FIXED -U_X1_0 1. -U_X1_BC_1 ZERO -U_X1_BC_2 ZERO "
+            "-alpha 0 -beta 0 -U_ksp_rtol 1e-12 -V_snes_rtol 1e-11")
+
+
+def test_runtest1_through_gpu():
+    """ValidationTests/test1.c with the runtest1 options: at step ts the end
+    displacement is t = 0.4 ts (BCU scaled by t, test1.c:49-50), VIrrev = V
+    (VFTimeStepPrepare), then alternate minimisation until ||V - Vold||_inf
+    < 1e-3.  Steps 0-5 agree with the printed goldens to their 7 digits, the
+    iteration counts of every step up to 5 match, and steps 6-10 follow the same
+    crack within the loose alternate-minimisation tolerance (the bars of the
+    oracle's run, tests/test_oracle.py::test_runtest1_goldens)."""
+    import json
+    import os
+    from vpfhf_amd.host import Host, options
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "runtest_goldens.json")))
+    gold = gold["runtest1"]["steps"]
+    H = Host(options(RUNTEST1))
+    try:
+        H.fields_initialize()
+        for g in gold:
+            H.set_bcu(g["t"])
+            H.timestep_prepare()
+            info = H.altmin(maxit=0)
+            assert info.status == 0
+            en = H.energies()
+            r = dict(elastic=en["elastic"], surface=en["surface"], crack_opening=en["crack_volume"])
+            print("runtest1 step %d: altmin %d elastic %.7e surface %.7e opening %.7e" % (
+                g["step"], info.iterations, r["elastic"], r["surface"], r["crack_opening"]), flush=True)
+            if g["step"] <= 5:
+                assert info.iterations == g["altmin"], g["step"]
+                for k in ("elastic", "surface", "crack_opening"):
+                    assert abs(r[k] - g[k]) <= 6e-7 * abs(g[k]) + 1e-6 * (k == "crack_opening"), (g["step"], k)
+            else:
+                assert abs(r["surface"] - g["surface"]) < 1e-4 * g["surface"]
+                assert abs(r["crack_opening"] - g["crack_opening"]) < 2e-4 * g["crack_opening"]
+    finally:
+        H.close()
+
+
+# --------------------------------------------------------- full-size helpers
+def _cube(a, N, dof=1):
+    return a.reshape((N, N, N, dof)) if dof > 1 else a.reshape((N, N, N))
+
+
+class Box:
+    """Nodes lo..lo+m-1 per direction of an N^3-node unit-cube workload as an
+    oracle problem; ``rows`` selects the nodes whose full-grid rows the box
+    reproduces (off the box's cut faces; global faces count as complete)."""
+
+    def __init__(self, O, spec, fields, lo, m, N):
+        n = N - 1
+        h = 1.0 / n
+        self.sl = tuple(slice(lo[d], lo[d] + m) for d in (2, 1, 0))        # (k, j, i)
+        self.cl = tuple(slice(lo[d], lo[d] + m - 1) for d in (2, 1, 0))
+        self.N, self.m = N, m
+        faces = {}
+        for d in range(3):
+            if lo[d] == 0:
+                faces[2 * d] = O.ZERO
+            if lo[d] + m - 1 == N - 1:
+                faces[2 * d + 1] = O.ZERO
+        bc = [O.make_bc(dict(faces))] * 3                                     # BCfiles/3DClamped.txt
+        E = fields.get("E_elem")
+        self.op = O.OracleProblem((m, m, m), ((m - 1) * h,) * 3, E=spec["E"], nu=spec["nu"], alpha=spec["alpha"],
+                                  beta=spec["beta"], Gc=spec["Gc"], eta=spec["eta"], epsilon=spec["eps"],
+                                  atnum=spec["atnum"], pressurize=spec["pressurize"],
+                                  E_elem=None if E is None else _cube(E, n)[self.cl].ravel(), bcU=bc,
+                                  bcV=O.make_bc())
+        ok = []
+        for d in range(3):
+            t = np.arange(m)
+            ok.append(((t > 0) | (lo[d] == 0)) & ((t < m - 1) | (lo[d] + m - 1 == N - 1)))
+        self.rows = (ok[2][:, None, None] & ok[1][None, :, None] & ok[0][None, None, :]).ravel()
+
+    def nodal(self, a, dof=1):
+        return np.ascontiguousarray(_cube(a, self.N, dof)[self.sl]).ravel()
+
+    def cells(self, a, dof=1):
+        n = self.N - 1
+        return np.ascontiguousarray(a.reshape((n, n, n, dof))[self.cl]).reshape(-1, dof)
+
+
+def _boxes(N, cracks, m=10):
+    """corner, edge, face and interior boxes plus one at each crack centre"""
+    out = [(0, 0, 0), (N - m, 0, N - m), (N // 2, N - m, N // 3), (N // 3, N // 2, 2 * N // 3)]
+    for c in cracks:
+        out.append(tuple(round(x * (N - 1)) - m // 2 for x in c["center"]))
+    return [tuple(int(min(max(q, 0), N - m)) for q in b) for b in out]
+
+
+@pytest.fixture(scope="module")
+def hetero128():
+    from vpfhf_amd import Problem, workloads
+    spec = workloads.spec_hetero(128, seed=0)
+    H, f = workloads.make_host(spec, u_rtol=1e-10)
+    _, V, VI = H.state()
+    f["V"], f["VIrrev"] = V, VI
+    N = spec["n"] + 1
+    P = Problem.borrow(H.h, (N, N, N))
+    yield spec, H, P, f, N
+    P.close()
+    H.close()
+
+
+def test_hetero128_operators_on_subboxes(oracle, hetero128):
+    """Every operator of the u/V path at the full 128^3 grid against the oracle
+    on seven sub-boxes of 10^3 nodes (corner, edge, face, interior and one at
+    each crack): u-apply (MatShell MatMult with the Dirichlet rows), u load
+    vector, u residual, V-apply, V load vector, V residual, per-cell energy
+    terms and crack opening, all within 1e-12 relative."""
+    spec, H, P, f, N = hetero128
+    rng = np.random.default_rng(5)
+    nn = N ** 3
+    x = rng.standard_normal(3 * nn)
+    U = 1e-3 * rng.standard_normal(3 * nn)
+    xv = rng.standard_normal(nn)
+    V = f["V"]
+    yu = P.UIJacobian_mult(x)
+    bu = P.u_rhs()
+    Fu = P.VF_UResidual(U, np.zeros(3 * nn))
+    P.VIJacobian_setup(U)
+    yv = P.VIJacobian_mult(xv)
+    bv = P.v_rhs()
+    Fv = P.VF_VResidual(V)
+    cells = P.energy_fields(U, V)
+    _, cf = P.VolumetricCrackOpening(U, V, cellfield=True)
+    assert np.abs(bu).max() > 0 and np.abs(bv).max() > 0
+    damaged = 0
+    for lo in _boxes(N, spec["cracks"]):
+        B = Box(oracle, spec, f, lo, 10, N)
+        F = B.op.fields(V=B.nodal(V), VIrrev=B.nodal(f["VIrrev"]), theta=B.nodal(f["theta"]),
+                        thetaRef=B.nodal(f["thetaRef"]), pressure=B.nodal(f["pressure"]))
+        Ub = B.nodal(U, 3)
+        r3 = np.repeat(B.rows, 3)
+        damaged += int((B.nodal(V) < 0.5).sum())
+        checks = [
+            ("u-apply", B.nodal(yu, 3), oracle.u_jacobian(B.op, F) @ B.nodal(x, 3), r3),
+            ("u-rhs", B.nodal(bu, 3), oracle.u_rhs(B.op, F), r3),
+            ("u-residual", B.nodal(Fu, 3), oracle.u_residual(B.op, F, Ub, np.zeros(3 * B.op.nn)), r3),
+            ("V-apply", B.nodal(yv), oracle.v_jacobian(B.op, F, Ub) @ B.nodal(xv), B.rows),
+            ("V-rhs", B.nodal(bv), oracle.v_rhs(B.op, F, Ub), B.rows),
+            ("V-residual", B.nodal(Fv), oracle.v_residual(B.op, F, Ub, B.nodal(V)), B.rows),
+            ("energy cells", B.cells(cells, 3), oracle.energy_fields(B.op, F, Ub, B.nodal(V)), None),
+            ("opening cells", B.cells(cf), oracle.crack_volume(B.op, Ub, B.nodal(V), cellfield=True)[1], None),
+        ]
+        for name, g, o, sel in checks:
+            g, o = np.asarray(g).reshape(-1), np.asarray(o).reshape(-1)
+            if sel is not None:
+                g, o = g[sel], o[sel]
+            assert relerr(g, o) < 1e-12, (lo, name, relerr(g, o))
+    assert damaged > 0  # the crack boxes hold damaged nodes
+
+
+def _free_mask(P):
+    return P.bc_mask("U") == 0
+
+
+def test_hetero128_apply_symmetry_rigid_modes_determinism(hetero128):
+    """Over the whole 128^3 grid: x.Ay = y.Ax for x, y supported on the free
+    rows (K is symmetric; the Dirichlet rows are identity with columns kept);
+    translations and infinitesimal rotations give A r = 0 on every free row
+    (the element stiffness annihilates rigid motions, s_g >= eta > 0); two
+    applies of the same input are bitwise equal."""
+    spec, H, P, f, N = hetero128
+    rng = np.random.default_rng(7)
+    free = _free_mask(P)
+    x = rng.standard_normal(free.size) * free
+    y = rng.standard_normal(free.size) * free
+    Ax, Ay = P.UIJacobian_mult(x), P.UIJacobian_mult(y)
+    assert abs(x @ Ay - y @ Ax) < 1e-12 * np.linalg.norm(x) * np.linalg.norm(Ay)
+    assert np.array_equal(Ax, P.UIJacobian_mult(x))
+    scale = np.linalg.norm(Ax) / np.linalg.norm(x)
+    t = np.arange(N) / (N - 1.)
+    Z, Y, X = np.meshgrid(t, t, t, indexing="ij")
+    X, Y, Z = X.ravel(), Y.ravel(), Z.ravel()
+    zero = np.zeros_like(X)
+    one = np.ones_like(X)
+    modes = [(one, zero, zero), (zero, one, zero), (zero, zero, one),
+             (-Y, X, zero), (zero, -Z, Y), (Z, zero, -X)]
+    for m in modes:
+        r = np.stack(m, 1).ravel()
+        Ar = P.UIJacobian_mult(r)
+        assert np.linalg.norm(Ar[free]) < 1e-12 * scale * np.linalg.norm(r), np.linalg.norm(Ar[free])
+        assert np.array_equal(Ar[~free], r[~free])
+
+
+def test_hetero128_solves_residual_and_kkt(hetero128):
+    """The u-solve (multigrid PCG, rtol 1e-10) leaves an FP64 residual
+    VF_UResidual below 2e-10 of the initial one (the slack covers the
+    recursive-vs-true residual difference); the bound-constrained V-solve from
+    the crack state leaves a projected gradient of VF_VResidual below 2x its
+    relative tolerance of the initial one, with 0 <= V <= VIrrev."""
+    spec, H, P, f, N = hetero128
+    nn = N ** 3
+    Z = np.zeros(3 * nn)
+    r0 = np.linalg.norm(P.VF_UResidual(Z, Z))
+    U, info = P.VF_StepU(Z, Z, rtol=1e-10)
+    assert info.status == 0
+    r = np.linalg.norm(P.VF_UResidual(U, Z))
+    assert r <= 2e-10 * r0, (r, r0)
+    V0, ub = f["V"].copy(), f["VIrrev"].copy()
+    U *= 300.0  # a load that moves V (the crack grows under it)
+    P.VIJacobian_setup(U)
+
+    def pgrad(V):
+        g = P.VF_VResidual(V)
+        g = np.where((V <= 0.) & (g > 0.), 0., g)
+        return np.where((V >= ub) & (g < 0.), 0., g)
+    pg0 = np.linalg.norm(pgrad(V0))
+    V, vinfo = P.VF_StepV(U, V0, ub=ub, rtol=1e-8)
+    assert vinfo.status == 0 and vinfo.outer >= 1
+    assert np.all(V >= 0.) and np.all(V <= ub)
+    assert np.abs(V - V0).max() > 1e-3
+    assert np.linalg.norm(pgrad(V)) <= 2e-8 * pg0
+    P.set_fields(V=V0)
+
+
+def test_hetero128_ten_altmin_iterations():
+    """The plain FP64 drop-in workload as run by the bench: ten alternate-
+    minimisation iterations through the C host; every solve succeeds, V stays
+    in [0, VIrrev], and the final state's u-residual is at the solver tolerance."""
+    from vpfhf_amd import Problem, workloads
+    spec = workloads.spec_hetero(128, seed=0)
+    H, f = workloads.make_host(spec, u_rtol=1e-10)
+    try:
+        info = H.altmin(maxit=10)
+        assert info.status == 0 and info.iterations >= 1
+        U, V, VI = H.state()
+        assert np.all(V >= 0.) and np.all(V <= VI)
+        N = spec["n"] + 1
+        P = Problem.borrow(H.h, (N, N, N))
+        Z = np.zeros(3 * N ** 3)
+        assert np.linalg.norm(P.VF_UResidual(U, Z)) <= 1e-8 * np.linalg.norm(P.VF_UResidual(Z, Z))
+        en = H.energies()
+        assert en["surface"] > 0 and en["crack_volume"] > 0 and en["elastic"] > 0
+    finally:
+        H.close()
+
+
+# --------------------------------------------------------------- precision
+def test_hetero256_mixed_vs_fp64():
+    """The precision form at 256^3 elements: one alternate-minimisation
+    iteration with FP32 CG inside FP64 refinement (rtol 1e-10) against the FP64
+    solve; U, V, crack volume and surface energy within 1e-6 relative; both u
+    residuals at the tolerance."""
+    from vpfhf_amd import Problem, workloads
+    spec = workloads.spec_hetero(256, seed=0)
+    out = {}
+    for name, extra in (("fp64", ""), ("mixed", "-vfg_precision mixed")):
+        H, _ = workloads.make_host(spec, u_rtol=1e-10, extra=extra)
+        try:
+            info = H.altmin(maxit=1)
+            assert info.status == 0
+            U, V, _ = H.state()
+            N = spec["n"] + 1
+            P = Problem.borrow(H.h, (N, N, N))
+            Z = np.zeros(3 * N ** 3)
+            # V changed after the u-solve: its residual is checked on the
+            # u-problem of the V it was solved with, i.e. before the V-solve
+            out[name] = (U, V, H.energies(), np.linalg.norm(P.VF_UResidual(U, Z)))
+        finally:
+            H.close()
+    U64, V64, e64, _ = out["fp64"]
+    Um, Vm, em, _ = out["mixed"]
+    assert relerr(Um, U64) < 1e-6
+    assert relerr(Vm, V64) < 1e-6
+    assert abs(em["crack_volume"] - e64["crack_volume"]) <= 1e-6 * abs(e64["crack_volume"])
+    assert abs(em["surface"] - e64["surface"]) <= 1e-6 * abs(e64["surface"])
+    assert np.linalg.norm(U64) > 0
+
+
+# ----------------------------------------------------------------- ensemble
+def test_ensemble64_batched_mixed_vs_single_fp64(oracle):
+    """The 64-member ensemble (sides log-uniform in [16, 64], own lognormal E,
+    1-4 cracks, injection pressure): the batched mixed-precision multigrid
+    u-solve against each member's FP64 drop-in solve (1e-6 relative), each
+    member's FP64 residual against its initial one (2e-10), and the three
+    smallest members against the oracle's u-solve (1e-7)."""
+    import ctypes as C
+    from vpfhf_amd import Problem, _lib, workloads
+    from vpfhf_amd.batch import Batch
+    specs = [workloads.spec_ensemble_member(i, 0) for i in range(64)]
+    hosts = [workloads.make_host(s, u_rtol=1e-10) for s in specs]
+    try:
+        probs = [Problem.borrow(H.h, (s["n"] + 1,) * 3) for s, (H, _) in zip(specs, hosts)]
+        Z = [np.zeros(3 * P.nn) for P in probs]
+        single = [P.VF_StepU(z, z, rtol=1e-10, pc=_lib.PC_MG)[0] for P, z in zip(probs, Z)]
+        bt = Batch([H for H, _ in hosts])
+        bt.autotune(fp32=True)
+        bt.set_graph(True, 10)
+        Ub, infos = bt.u_solve(Z, Z, rtol=1e-10, mixed=True, pc=_lib.PC_MG)
+        bt.close()
+        assert all(i.status == 0 for i in infos)
+        for P, z, ub, us in zip(probs, Z, Ub, single):
+            assert relerr(ub, us) < 1e-6
+            assert np.linalg.norm(P.VF_UResidual(ub, z)) <= 2e-10 * np.linalg.norm(P.VF_UResidual(z, z))
+        order = np.argsort([s["n"] for s in specs])[:3]
+        for i in order:
+            s, (H, f) = specs[i], hosts[i]
+            n = s["n"] + 1
+            _, V, VI = H.state()
+            bc = [oracle.make_bc({q: oracle.ZERO for q in range(6)})] * 3
+            op = oracle.OracleProblem((n, n, n), E=s["E"], nu=s["nu"], alpha=s["alpha"], beta=s["beta"], Gc=s["Gc"],
+                                      eta=s["eta"], epsilon=s["eps"], atnum=s["atnum"], pressurize=s["pressurize"],
+                                      E_elem=f.get("E_elem"), bcU=bc)
+            F = op.fields(V=V, VIrrev=VI, theta=f["theta"], thetaRef=f["thetaRef"], pressure=f["pressure"])
+            Uo = np.zeros(3 * op.nn)
+            oracle.u_solve(op, F, Uo, np.zeros(3 * op.nn), rtol=1e-12)
+            assert relerr(Ub[i], Uo) < 1e-7, (s["n"], relerr(Ub[i], Uo))
+    finally:
+        for H, _ in hosts:
+            H.close()
+
+
+# ------------------------------------------------------------------ serving
+def test_serving_sides_24_96(oracle):
+    """A 24-request stream at sides 24-96 with 1-20 alternate-minimisation
+    iterations: the mixed-precision engine against the FP64 engine within 1e-6
+    in U and V, and the smallest request's first iteration (served alone)
+    against the oracle's u- and V-solves within 1e-6."""
+    from vpfhf_amd import serve, workloads
+    R = serve.make_requests(24, rate=None, seed=11, outputs=True)
+    sides = [s["n"] for s in R.specs]
+    assert min(sides) < 40 and max(sides) > 70
+    res, st = serve.serve(R, max_slots=64, realtime=False)
+    assert all(r["status"] == 0 for r in res) and st["completed"] == 24
+    Um, Vm = [u.copy() for u in R.U], [v.copy() for v in R.V]
+    res64, _ = serve.serve(R, max_slots=64, realtime=False, mixed=False)
+    for i in range(24):
+        assert res64[i]["am_iterations"] == res[i]["am_iterations"], i
+        assert relerr(Um[i], R.U[i]) < 1e-6, (i, relerr(Um[i], R.U[i]))
+        assert relerr(Vm[i], R.V[i]) < 1e-6, (i, relerr(Vm[i], R.V[i]))
+    i = int(np.argmin(sides))
+    s = R.specs[i]
+    A = serve.RequestSet([s], [1], [0.], outputs=True)
+    ra, _ = serve.serve(A, max_slots=1, realtime=False)
+    assert ra[0]["status"] == 0 and ra[0]["am_iterations"] == 1
+    H, f = workloads.make_host(s, u_rtol=1e-10)
+    U0, V0, VI0 = H.state()
+    H.close()
+    n = s["n"] + 1
+    bc = [oracle.make_bc({q: oracle.ZERO for q in range(6)})] * 3
+    op = oracle.OracleProblem((n, n, n), E=s["E"], nu=s["nu"], alpha=s["alpha"], beta=s["beta"], Gc=s["Gc"],
+                              eta=s["eta"], epsilon=s["eps"], atnum=s["atnum"], pressurize=s["pressurize"],
+                              E_elem=f.get("E_elem"), bcU=bc)
+    U, V = U0.copy(), V0.copy()
+    F = op.fields(V=V, VIrrev=VI0, theta=f["theta"], thetaRef=f["thetaRef"], pressure=f["pressure"])
+    oracle.u_solve(op, F, U, np.zeros(3 * op.nn), rtol=1e-12)
+    F = op.fields(V=V, VIrrev=VI0, theta=f["theta"], thetaRef=f["thetaRef"], pressure=f["pressure"])
+    oracle.v_solve(op, F, U, V, ub=VI0.copy(), tol=1e-12)
+    assert np.linalg.norm(U) > 0
+    assert relerr(A.U[0], U) < 1e-6
+    assert relerr(A.V[0], V) < 1e-6

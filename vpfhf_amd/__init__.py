@@ -92,14 +92,29 @@ class Problem:
         h = C.c_void_p()
         check(self.L.vfg_create(C.byref(g), C.byref(self.props), C.byref(h)))
         self.h = h.value
+        self._owned = True
         self.n = tuple(n)
         self.nn = n[0] * n[1] * n[2]
         self.ne = (n[0] - 1) * (n[1] - 1) * (n[2] - 1)
 
+    @classmethod
+    def borrow(cls, handle, n):
+        """A Problem view of a handle owned elsewhere (e.g. by a C-host
+        context): every operator call works, close() leaves the handle alone."""
+        P = cls.__new__(cls)
+        P.L = load()
+        P.h = handle
+        P._owned = False
+        P.n = tuple(n)
+        P.nn = n[0] * n[1] * n[2]
+        P.ne = (n[0] - 1) * (n[1] - 1) * (n[2] - 1)
+        return P
+
     # -- lifecycle ----------------------------------------------------------
     def close(self):
         if getattr(self, "h", None):
-            self.L.vfg_destroy(self.h)
+            if getattr(self, "_owned", True):
+                self.L.vfg_destroy(self.h)
             self.h = None
 
     def __del__(self):

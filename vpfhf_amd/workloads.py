@@ -62,9 +62,27 @@ def spec_ensemble_member(i, seed=0, lo=16, hi=64, uniform_p=True):
     return s
 
 
+_FIELD_CACHE = {}
+
+
 def build_fields(spec, problem=None):
     """Host arrays for a spec.  V is built by the GPU penny kernel when a
-    Problem is given (pointwise min over cracks, VFCommon.c:827-857)."""
+    Problem is given (pointwise min over cracks, VFCommon.c:827-857).  The
+    inputs of grids of 128^3 elements and more (seconds of host work each) are
+    kept for the last two specs, so a process that sets the same workload up
+    several times generates it once; callers get a fresh dict."""
+    key = repr(sorted((k, v) for k, v in spec.items() if k != "cracks")) if spec["n"] >= 128 else None
+    if problem is None and key in _FIELD_CACHE:
+        return dict(_FIELD_CACHE[key])
+    out = _build_fields(spec, problem)
+    if problem is None and key is not None:
+        while len(_FIELD_CACHE) >= 2:
+            _FIELD_CACHE.pop(next(iter(_FIELD_CACHE)))
+        _FIELD_CACHE[key] = dict(out)
+    return out
+
+
+def _build_fields(spec, problem=None):
     n = spec["n"]
     nn = (n + 1) ** 3
     ne = n ** 3
