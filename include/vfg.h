@@ -155,6 +155,15 @@ int vfg_vec_apply_dirichlet(vfg_handle *h, int which, double *vec, const double 
  * out[2] pressure work (VF_UEnergy3D, VFMech.c:1111-1427), out[3] surface
  * energy (VF_VEnergy3D, VFMech.c:2474-2526). */
 int vfg_energies(vfg_handle *h, const double *U, const double *V, double out[4], int loc);
+/* vfg_energies without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * U and V are device pointers, out is a host array; the reduction partials
+ * live in a static 8 MiB device array per GPU, so grids are limited to
+ * 67,108,864 cells (VFG_E_ARG beyond).  Calls that overlap on one GPU share
+ * that array and race; the call synchronizes the handle's stream.  Results
+ * are bitwise those of vfg_energies. */
+int vfg_energies_static(vfg_handle *h, const double *U, const double *V, double out[4]);
 /* VolumetricCrackOpening (VFPermfield.c:75-150); cellfield (N_e) optional */
 int vfg_crack_volume(vfg_handle *h, const double *U, const double *V, double *vol,
                      double *cellfield, int loc);

@@ -93,6 +93,7 @@ PROTOS = {
     "vfg_vec_set_from_bc": (C.c_int, [H, C.c_int, vp, C.c_int]),
     "vfg_vec_apply_dirichlet": (C.c_int, [H, C.c_int, vp, vp, C.c_int]),
     "vfg_energies": (C.c_int, [H, vp, vp, dp, C.c_int]),
+    "vfg_energies_static": (C.c_int, [H, vp, vp, dp]),
     "vfg_crack_volume": (C.c_int, [H, vp, vp, dp, vp, C.c_int]),
     "vfg_cell_to_node": (C.c_int, [H, vp, vp, C.c_int, C.c_int]),
     "vfg_u_set_state": (C.c_int, [H, vp, C.c_int]),

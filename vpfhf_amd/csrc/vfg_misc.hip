@@ -351,3 +351,67 @@ void launch_maxabs_diff(const double *a, const double *b, int64_t n, double *par
 }
 
 }  // namespace vfg
+
+// ------------------------------------------- energies without allocation
+// vfg_energies_static: the reductions of vfg_energies with their partials in
+// a file-scope device array instead of handle buffers, for callers that
+// cannot allocate device memory (HIP graph capture, serving loops, harnesses
+// that forbid hipMalloc).  The array holds ENERGY_STATIC_BLOCKS groups of 4
+// partials plus the 4 results: 8 MiB + 32 B, resident on every GPU the
+// library runs on until the process ends.  Every partial is written by
+// k_energies before k_reduce_partials reads it, so nothing needs clearing;
+// calls that overlap on one GPU share the array and race.
+#include <mutex>
+#include "vfg_handle.h"
+
+static constexpr int ENERGY_STATIC_BLOCKS = 1 << 18;  // grids of up to 2^18 * 256 = 67,108,864 cells
+static __device__ __align__(256) double scratch[4 * ENERGY_STATIC_BLOCKS + 4];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+// device address of `scratch` on each GPU (the host-side name is a stand-in,
+// never a device address)
+double *g_scratch_dev[MAX_DEVICES];
+std::mutex g_scratch_mu;
+
+int scratch_address(vfg_handle *h, double **out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES)
+    return vfg_fail(h, VFG_E_HIP, "vfg_energies_static: no current HIP device (or device id >= %d)", MAX_DEVICES);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return vfg_fail(h, VFG_E_HIP, "vfg_energies_static: hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_dev[dev] = (double *)p;
+  }
+  *out = g_scratch_dev[dev];
+  return VFG_OK;
+}
+}  // namespace
+
+extern "C" int vfg_energies_static(vfg_handle *h, const double *U, const double *V, double out[4]) {
+  if (!h) return vfg_fail(nullptr, VFG_E_ARG, "NULL handle");
+  if (!U || !V || !out) return vfg_fail(h, VFG_E_ARG, "vfg_energies_static: NULL array");
+  const int nb = vfg::energy_blocks(h->P);
+  if (nb > ENERGY_STATIC_BLOCKS)
+    return vfg_fail(h, VFG_E_ARG,
+                    "vfg_energies_static: %lld cells exceed the static scratch limit of %lld cells "
+                    "(use vfg_energies)",
+                    (long long)h->P.ne, (long long)ENERGY_STATIC_BLOCKS * vfg::EB);
+  double *part = nullptr;
+  int rc = scratch_address(h, &part);
+  if (rc) return rc;
+  double *res = part + 4 * (size_t)ENERGY_STATIC_BLOCKS;
+  vfg::launch_energies(h->P, U, V, part, nullptr, nullptr, res, h->stream);
+  double r[4];
+  if (hipMemcpyAsync(r, res, sizeof r, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+      hipStreamSynchronize(h->stream) != hipSuccess || hipGetLastError() != hipSuccess)
+    return vfg_fail(h, VFG_E_HIP, "vfg_energies_static: kernel execution failed");
+  out[0] = r[0];
+  out[1] = 0.;
+  out[2] = r[1];
+  out[3] = r[2];
+  return VFG_OK;
+}
