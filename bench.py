@@ -15,6 +15,7 @@ import argparse
 import ctypes as C
 import json
 import os
+import platform
 import sys
 import time
 
@@ -61,13 +62,25 @@ def copy_calibration(L, h, n=1 << 26, reps=4):
         b.free()
 
 
-def cpu_baseline(n_sample=48, seconds=3.0):
+def cpu_model():
+    """Model name of the host CPU (first 'model name' of /proc/cpuinfo)."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(n_sample=48, seconds=6.0):
     """The reference's per-iteration kernel on the host: the assembled AIJ
     u-Jacobian (VF_UIJacobian + MatApplyDirichletBC, 81 nnz/row) times a
-    vector, from the oracle's C restatement, OpenMP over rows."""
+    vector, from the oracle's C restatement, OpenMP over rows; timed on every
+    core the process may use and on one core."""
     from oracle import vforacle as O
     from vpfhf_amd import inputs
-    O.lib()
+    L = O.lib()
     n = n_sample
     E = inputs.lognormal_E((n, n, n), .5, 8., 0)
     bc = [O.make_bc({f: O.ZERO for f in range(6)})] * 3
@@ -75,21 +88,69 @@ def cpu_baseline(n_sample=48, seconds=3.0):
                          E_elem=E, bcU=bc)
     V = O.penny_v(op, (.5, .5, .5), .2, 0., 0., 3. / n)
     F = op.fields(V=V, VIrrev=V, theta=np.zeros(op.nn), thetaRef=np.zeros(op.nn), pressure=np.full(op.nn, 1e-3))
+    nt = int(L.vfo_num_threads())
     t0 = time.time()
     A = O.u_jacobian(op, F, csr=False)
     t_asm = time.time() - t0
     x = np.random.default_rng(0).standard_normal(3 * op.nn)
-    O.ell_matvec(A, x)
-    reps, t0 = 0, time.time()
-    while time.time() - t0 < seconds:
+
+    def rate(threads):
+        L.vfo_set_num_threads(threads)
         O.ell_matvec(A, x)
-        reps += 1
-    t = (time.time() - t0) / reps
-    O.lib().vfo_ell_free(A)
-    return {"value": round(3 * op.nn / t / 1e9, 4), "unit": "GDoF/s", "cores": int(O.lib().vfo_num_threads()),
-            "kind": "port",
-            "sample": "assembled AIJ MatMult (81 nnz/row, FP64) of the %d^3-element u-Jacobian, %d products "
-                      "after a %.1f s assembly; oracle C restatement of VF_UIJacobian" % (n, reps, t_asm)}
+        reps, t0 = 0, time.time()
+        while time.time() - t0 < seconds:
+            O.ell_matvec(A, x)
+            reps += 1
+        return 3 * op.nn / ((time.time() - t0) / reps) / 1e9, reps
+    v_all, r_all = rate(nt)
+    v_one, r_one = rate(1)
+    L.vfo_set_num_threads(nt)
+    L.vfo_ell_free(A)
+    return {"value": round(v_all, 4), "unit": "GDoF/s", "cores": nt, "kind": "port",
+            "sample": "assembled AIJ MatMult (81 nnz/row, FP64) of the %d^3-element u-Jacobian, %d products on %d "
+                      "threads and %d on one, after a %.1f s assembly; oracle C restatement of VF_UIJacobian"
+                      % (n, r_all, nt, r_one, t_asm),
+            "one_core": {"value": round(v_one, 4), "unit": "GDoF/s", "cores": 1},
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count()}
+
+
+def time_apply(L, h, nn, steps, warmup, fp32=False, nbuf=4):
+    """Seconds per u-apply from HIP events on the handle's stream over `steps`
+    back-to-back launches after `warmup` untimed ones; the (x, y) pairs rotate
+    over nbuf buffers (>= 4 x 103 MB at 128^3) so the 256 MB last-level cache
+    cannot hold the working set between launches."""
+    import vpfhf_amd as V
+    from vpfhf_amd import _lib
+    w = 4 if fp32 else 8
+    rng = np.random.default_rng(1)
+    dt = np.float32 if fp32 else np.float64
+    xs = [V.DeviceBuffer(3 * w * nn).upload(rng.standard_normal(3 * nn).astype(dt)) for _ in range(nbuf)]
+    ys = [V.DeviceBuffer(3 * w * nn) for _ in range(nbuf)]
+    xa = (C.c_void_p * nbuf)(*[b.ptr for b in xs])
+    ya = (C.c_void_p * nbuf)(*[b.ptr for b in ys])
+    ms = C.c_double()
+    _lib.check(L.vfg_synchronize(h), h)
+    _lib.check(L.vfg_time_u_apply(h, xa, ya, nbuf, max(warmup, 1), int(fp32), C.byref(ms)), h)
+    _lib.check(L.vfg_time_u_apply(h, xa, ya, nbuf, steps, int(fp32), C.byref(ms)), h)
+    for b in xs + ys:
+        b.free()
+    return ms.value * 1e-3 / steps
+
+
+def apply_at(n, steps, warmup, seed=0):
+    """The FP64 u-apply of the same workload generators at n^3 elements
+    (256^3: the precision form's grid, whose working set is 4 x the last-level
+    cache), with its algorithmic HBM fraction."""
+    from vpfhf_amd import _lib, workloads
+    H, _ = workloads.make_host(workloads.spec_hetero(n, seed=seed), u_rtol=1e-10)
+    try:
+        t = time_apply(_lib.load(), H.h, H.nn, steps, warmup)
+        byt = apply_bytes(H.nn, H.ne)
+        return {"elements": H.ne, "dof": 3 * H.nn, "ms": round(t * 1e3, 4), "gdofs": round(3 * H.nn / t / 1e9, 3),
+                "algorithmic_bytes": byt, "achieved_GBs": round(byt / t / 1e9, 1),
+                "frac": round(byt / t / 1e9 / PEAK_HBM_GBS, 4)}
+    finally:
+        H.close()
 
 
 def _config1_problem():
@@ -303,6 +364,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--n3", type=int, default=256, help="also time the FP64 apply at this grid (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solves", action="store_true")
     ap.add_argument("--seed", type=int, default=0)
@@ -325,34 +387,15 @@ def main():
     L = _lib.load()
     h = H.h
     nn, ne = H.nn, H.ne
-    nbuf = 4
-    rng = np.random.default_rng(1)
-    xs = [V.DeviceBuffer(24 * nn).upload(rng.standard_normal(3 * nn)) for _ in range(nbuf)]
-    ys = [V.DeviceBuffer(24 * nn) for _ in range(nbuf)]
-    xa = (C.c_void_p * nbuf)(*[b.ptr for b in xs])
-    ya = (C.c_void_p * nbuf)(*[b.ptr for b in ys])
-    ms = C.c_double()
-    _lib.check(L.vfg_synchronize(h), h)
-    _lib.check(L.vfg_time_u_apply(h, xa, ya, nbuf, max(a.warmup, 1), 0, C.byref(ms)), h)
-    _lib.check(L.vfg_time_u_apply(h, xa, ya, nbuf, a.steps, 0, C.byref(ms)), h)
-    t_step = ms.value * 1e-3 / a.steps
+    t_step = time_apply(L, h, nn, a.steps, a.warmup)
     gdofs = 3 * nn / t_step / 1e9
     byt = apply_bytes(nn, ne)
     achieved = byt / t_step / 1e9
     traffic = read_traffic("u_apply_hetero%d_counters.txt" % a.n)
     if a.calib:
         copy_calibration(L, h)
-
     # FP32 apply (precision form) on the same operator
-    xs32 = [V.DeviceBuffer(12 * nn).upload(rng.standard_normal(3 * nn).astype(np.float32)) for _ in range(nbuf)]
-    ys32 = [V.DeviceBuffer(12 * nn) for _ in range(nbuf)]
-    xa32 = (C.c_void_p * nbuf)(*[b.ptr for b in xs32])
-    ya32 = (C.c_void_p * nbuf)(*[b.ptr for b in ys32])
-    _lib.check(L.vfg_time_u_apply(h, xa32, ya32, nbuf, max(a.warmup, 1), 1, C.byref(ms)), h)
-    _lib.check(L.vfg_time_u_apply(h, xa32, ya32, nbuf, a.steps, 1, C.byref(ms)), h)
-    t32 = ms.value * 1e-3 / a.steps
-    for b in xs + ys + xs32 + ys32:
-        b.free()
+    t32 = time_apply(L, h, nn, a.steps, a.warmup, fp32=True)
 
     extra = {}
     if not a.no_solves:
@@ -385,6 +428,7 @@ def main():
                                 "jacobi_seconds_per_altmin_iteration": round(dtj, 3),
                                 "rel_diff_U_multigrid_vs_jacobi": _relerr(Um, Uj)})
         # the V-Jacobian apply of the same problem (coupling set by the V-solve)
+        nbuf, rng, ms = 4, np.random.default_rng(2), C.c_double()
         vx = [V.DeviceBuffer(8 * nn).upload(rng.standard_normal(nn)) for _ in range(nbuf)]
         vy = [V.DeviceBuffer(8 * nn) for _ in range(nbuf)]
         va = (C.c_void_p * nbuf)(*[b.ptr for b in vx])
@@ -425,6 +469,8 @@ def main():
         "fp32_apply": {"ms": round(t32 * 1e3, 4), "gdofs": round(3 * nn / t32 / 1e9, 3),
                        "achieved_GBs": round(apply_bytes(nn, ne, fp32=True) / t32 / 1e9, 1)},
     }
+    if a.n3 > 0:
+        out["apply_%d" % a.n3] = apply_at(a.n3, a.steps, a.warmup, a.seed)
     out.update(extra)
     H.close()
     if not (a.no_solves or a.no_forms):

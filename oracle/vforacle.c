@@ -1456,3 +1456,4 @@ int vfo_v_solve(const vfo_problem *P, const vfo_fields *F, const double *U, doub
 /* threads the OpenMP loops of the oracle use (for the CPU-baseline report) */
 #include <omp.h>
 int vfo_num_threads(void) { return omp_get_max_threads(); }
+void vfo_set_num_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }

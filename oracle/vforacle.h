@@ -226,6 +226,7 @@ int vfo_v_solve(const vfo_problem *P, const vfo_fields *F, const double *U,
                 double *V, const double *ub, double tol, int maxit);
 
 int vfo_num_threads(void);
+void vfo_set_num_threads(int n);
 
 #ifdef __cplusplus
 }

@@ -119,6 +119,7 @@ def lib():
         L.vfo_pcg.argtypes = [Ep, dp, dp, C.c_double, C.c_double, C.c_int, dp]
         L.vfo_pcg.restype = C.c_int
         L.vfo_num_threads.restype = C.c_int
+        L.vfo_set_num_threads.argtypes = [C.c_int]
         L.vfo_elastic_energy_density.argtypes = [dp, dp, dp, C.c_double, C.c_double, C.c_double, C.POINTER(Elem)]
         L.vfo_bilinear_v_at.argtypes = [dp, C.c_int, C.c_double, C.c_double, C.c_double, C.POINTER(Elem)]
         L.vfo_bilinear_v_coupling.argtypes = [dp, dp, dp, C.c_double, C.c_double, C.c_double, C.POINTER(Elem)]
