@@ -31,10 +31,13 @@ PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector (dense)
 FP64_FLOPS_PER_ELEMENT = 2 * 978 + 123 + 261
 
 
-def apply_bytes(nn, ne, fp32=False, hetero=True):
-    """Algorithmic HBM bytes of one apply: read x and V, write y (per node),
-    read E (per element)."""
+def apply_bytes(nn, ne, fp32=False, hetero=True, stored=False):
+    """Algorithmic HBM bytes of one apply.  Recomputing form: read x and V,
+    write y (per node), read E (per element).  Stored form: read x, write y
+    (per node), read the 27 stored weights W_g s_g E_e (per element)."""
     w = 4 if fp32 else 8
+    if stored:
+        return 6 * w * nn + 27 * w * ne
     return (3 * w + 3 * w + w) * nn + (8 if hetero else 0) * ne
 
 
@@ -143,14 +146,28 @@ def apply_at(n, steps, warmup, seed=0):
     cache), with its algorithmic HBM fraction."""
     from vpfhf_amd import _lib, workloads
     H, _ = workloads.make_host(workloads.spec_hetero(n, seed=seed), u_rtol=1e-10)
+    L = _lib.load()
     try:
-        t = time_apply(_lib.load(), H.h, H.nn, steps, warmup)
-        byt = apply_bytes(H.nn, H.ne)
-        return {"elements": H.ne, "dof": 3 * H.nn, "ms": round(t * 1e3, 4), "gdofs": round(3 * H.nn / t / 1e9, 3),
-                "algorithmic_bytes": byt, "achieved_GBs": round(byt / t / 1e9, 1),
-                "frac": round(byt / t / 1e9 / PEAK_HBM_GBS, 4)}
+        out = {"elements": H.ne, "dof": 3 * H.nn, "forms": time_forms(L, H.h, H.nn, H.ne, steps, warmup)}
+        out.update(out["forms"]["recompute"])
+        return out
     finally:
         H.close()
+
+
+def time_forms(L, h, nn, ne, steps, warmup):
+    """Both forms of the FP64 apply on one handle (the recomputing form, the
+    library default, last, so the handle is left in it)."""
+    from vpfhf_amd import _lib
+    forms = {}
+    for name, form in (("stored", _lib.U_FORM_STORED), ("recompute", _lib.U_FORM_RECOMPUTE)):
+        _lib.check(L.vfg_u_set_form(h, form), h)
+        t = time_apply(L, h, nn, steps, warmup)
+        b = apply_bytes(nn, ne, stored=form == _lib.U_FORM_STORED)
+        forms[name] = {"form": name, "ms": round(t * 1e3, 4), "gdofs": round(3 * nn / t / 1e9, 3),
+                       "algorithmic_bytes": b, "achieved_GBs": round(b / t / 1e9, 1),
+                       "frac": round(b / t / 1e9 / PEAK_HBM_GBS, 4)}
+    return forms
 
 
 def _config1_problem():
@@ -262,6 +279,11 @@ def ensemble_bench(L, n_members=64, seed=0):
 
     bt = Batch(hosts)
     apply_ms = bt.autotune(fp32=True)
+    pick, points = bt.tuning(), bt.tune_points()
+    tuned = {"schedules": Batch.SCHEDULES, "points": len(points), "pick": pick,
+             "best_per_schedule_ms": {str(sc): round(min(p["ms"] for p in points if p["schedule"] == sc), 4)
+                                      for sc in sorted({p["schedule"] for p in points})},
+             "slowest_ms": round(max(p["ms"] for p in points), 4)}
     bt.set_graph(True, 10)
     Ua = (C.c_void_p * n_members)(*[p[0] for p in ptr])
     Ba = (C.c_void_p * n_members)(*[p[1] for p in ptr])
@@ -295,7 +317,7 @@ def ensemble_bench(L, n_members=64, seed=0):
             "speedup_mixed_multigrid_batched_vs_multigrid_single": round(tmg / res["mixed_mg"][0], 2),
             "max_rel_err_mixed_vs_fp64": res["mixed"][1], "max_rel_err_batched_fp64_vs_single": res["fp64"][1],
             "max_rel_err_mixed_multigrid_vs_fp64": res["mixed_mg"][1],
-            "persistent_fp32_apply_ms": round(apply_ms, 4)}
+            "persistent_fp32_apply_ms": round(apply_ms, 4), "autotune": tuned}
 
 
 def precision_bench(n=256, seed=0):
@@ -332,29 +354,48 @@ def precision_bench(n=256, seed=0):
     return res
 
 
-def serving_bench(rates=(50., 100., 200., 500.), n_req=16, seed=0):
+def serving_bench(rates=(50., 100., 200., 500.), n_req=200, seed=0, n_check=32):
     """Serving form: a Poisson stream of n_req alternate-minimisation requests
-    (sides 24-96 elements, 1-20 iterations, 1-3 cracks) at each rate, 64
-    slots, 200 GB HBM budget, FP32 u-CG inside FP64 refinement.  The same
-    requests re-run on the FP64 engine give the error of every result."""
+    (sides 24-96 elements, 1-3 cracks, 1-20 iterations each, every drawn
+    iteration run: altmintol 0) at each rate, 64 slots, 200 GB HBM budget,
+    FP32 u-CG inside FP64 refinement.  The first n_check requests re-run on the
+    FP64 engine give the error of the mixed-precision results."""
     from vpfhf_amd import serve
-    R = serve.make_requests(n_req, rate=None, seed=seed, outputs=True)
+    R = serve.make_requests(n_req, rate=None, seed=seed, outputs=True, altmintol=0.0)
     base_t = serve.arrival_times(n_req, 1.0, seed)  # unit-rate arrivals, scaled per rate
     rows = []
     for k, rate in enumerate(rates):
         for i in range(n_req):
             R.arr[i].arrival = float(base_t[i] / rate)
+        t0 = time.time()
         res, st = serve.serve(R, max_slots=64, hbm_budget=200e9, realtime=True)
         if k == 0:
-            Um = [u.copy() for u in R.U]
-            Vm = [v.copy() for v in R.V]
+            Um = [u.copy() for u in R.U[:n_check]]
+            Vm = [v.copy() for v in R.V[:n_check]]
+            ams = [r["am_iterations"] for r in res]
+            bad = sum(r["status"] != 0 for r in res)
+            statuses = {}
+            for r in res:
+                statuses[str(r["status"])] = statuses.get(str(r["status"]), 0) + 1
+            per_req = {"u_its_mean": round(float(np.mean([r["u_its"] for r in res])), 1),
+                       "v_outer_mean": round(float(np.mean([r["v_outer"] for r in res])), 1),
+                       "v_its_mean": round(float(np.mean([r["v_its"] for r in res])), 1)}
         rows.append({"rate_per_s": rate, "p50_s": round(st["p50"], 4), "p99_s": round(st["p99"], 4),
-                     "solves_per_s": round(st["solves_per_s"], 2), "wall_s": round(st["wall"], 3),
-                     "max_resident": st["max_resident"], "solves": st["solves"]})
-    serve.serve(R, max_slots=64, mixed=False, realtime=False)
-    err_u = max(_relerr(a, b) for a, b in zip(Um, R.U))
-    err_v = max(_relerr(a, b) for a, b in zip(Vm, R.V))
-    return {"requests": n_req, "sides": [s["n"] for s in R.specs], "am_iterations": R.ams, "rates": rows,
+                     "mean_s": round(st["mean_latency"], 4), "solves_per_s": round(st["solves_per_s"], 2),
+                     "wall_s": round(st["wall"], 3), "max_resident": st["max_resident"], "solves": st["solves"],
+                     "completed": st["completed"]})
+        print("bench: serving at %g req/s done in %.1f s" % (rate, time.time() - t0), file=sys.stderr, flush=True)
+    C = serve.RequestSet(R.specs[:n_check], R.ams[:n_check], [0.] * n_check, outputs=True)
+    serve.serve(C, max_slots=64, mixed=False, realtime=False)
+    err_u = max(_relerr(a, b) for a, b in zip(Um, C.U))
+    err_v = max(_relerr(a, b) for a, b in zip(Vm, C.V))
+    sides = [s["n"] for s in R.specs]
+    return {"requests": n_req, "sides": {"min": min(sides), "max": max(sides), "mean": round(float(np.mean(sides)), 1)},
+            "am_iterations": {"min": min(ams), "max": max(ams), "mean": round(float(np.mean(ams)), 2),
+                              "total": int(sum(ams)),
+                              "histogram": [int(sum(1 for a in ams if a == q)) for q in range(1, 21)]},
+            "failed": bad, "status_counts": statuses, "per_request": per_req, "rates": rows,
+            "checked_vs_fp64": n_check,
             "max_rel_err_U_vs_fp64": err_u, "max_rel_err_V_vs_fp64": err_v}
 
 
@@ -371,7 +412,7 @@ def main():
     ap.add_argument("--calib", action="store_true", help="also run a plain streaming kernel of known size")
     ap.add_argument("--no-forms", action="store_true", help="skip the ensemble, precision and serving forms")
     ap.add_argument("--forms", default="ensemble,precision,serving", help="comma-separated subset of the forms")
-    ap.add_argument("--serve-requests", type=int, default=16)
+    ap.add_argument("--serve-requests", type=int, default=200)
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this path runs on one MI355X and has no multi-GPU decomposition "
@@ -387,7 +428,9 @@ def main():
     L = _lib.load()
     h = H.h
     nn, ne = H.nn, H.ne
-    t_step = time_apply(L, h, nn, a.steps, a.warmup)
+    # both forms of the FP64 apply; the library default (recompute) is the headline
+    forms = time_forms(L, h, nn, ne, a.steps, a.warmup)
+    t_step = forms["recompute"]["ms"] * 1e-3
     gdofs = 3 * nn / t_step / 1e9
     byt = apply_bytes(nn, ne)
     achieved = byt / t_step / 1e9
@@ -461,6 +504,7 @@ def main():
                    "elements": ne, "dof": 3 * nn},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic},
+        "apply_forms": forms,
         # the compute roofline that bounds this kernel: FP64 vector issue
         "valu": {"bound": "fp64-valu", "flops_per_element": FP64_FLOPS_PER_ELEMENT,
                  "achieved": round(FP64_FLOPS_PER_ELEMENT * ne / t_step / 1e12, 2), "peak": PEAK_FP64_TFLOPS,

@@ -116,6 +116,16 @@ int vfg_u_apply(vfg_handle *h, const double *x, double *y, int loc);
  * (all tensile) until called.  No effect for UNILATERAL_NONE. */
 int vfg_u_set_state(vfg_handle *h, const double *U, int loc);
 int vfg_u_apply_f32(vfg_handle *h, const float *x, float *y, int loc);
+/* How the FP64 u-apply (vfg_u_apply and the FP64 u-solve) holds the integrand
+ * weights W_g s_g E_e of VF_BilinearFormU3D_local (VFMech.c:240-252): the
+ * recomputing form forms s_g = v_g^2 + eta from the nodal V in every apply;
+ * the stored form keeps the 27 products per element (216 B per element of
+ * device memory, allocated on first use), refreshed on the first apply after
+ * V, E or the properties change -- the role VF_UIJacobian's assembly plays
+ * for the reference's MatMult (VFMech.c:1897-2029).  Both give the same
+ * operator (to rounding).  Default: VFG_U_FORM_RECOMPUTE. */
+enum { VFG_U_FORM_RECOMPUTE = 0, VFG_U_FORM_STORED = 1 };
+int vfg_u_set_form(vfg_handle *h, int form);
 /* MATOP_GET_DIAGONAL of the same matrix (1 on Dirichlet rows) */
 int vfg_u_diag(vfg_handle *h, double *d, int loc);
 /* element load vector sum_e b_e of VF_GradientUThermoPoro3D_local
@@ -240,8 +250,22 @@ int vfg_batch_u_apply(vfg_batch *b, const double *const *x, double *const *y);
 int vfg_batch_u_apply_f32(vfg_batch *b, const float *const *x, float *const *y);
 /* times nrep persistent applies on internal vectors (HIP events), total ms */
 int vfg_batch_time_u_apply(vfg_batch *b, int fp32, int nrep, double *ms);
-/* picks z-chunk length and persistent grid size by timing; best ms per apply */
+/* picks the persistent apply's schedule (0: 16 x 16 element tiles on the
+ * register march, 1: 8 x 8 tiles on the register march, 2: 16 x 16 tiles on
+ * the LDS-ring march), z-chunk length (0.5 / 1 / 2 x the whole-round choice)
+ * and grid size (one workgroup per item, one or two rounds of resident
+ * workgroups) by timing every combination; best ms per apply */
 int vfg_batch_autotune(vfg_batch *b, int fp32, double *best_ms);
+typedef struct {
+  int    schedule;  /* 0, 1, 2 as above */
+  double oz_scale;  /* z-chunk length relative to the whole-round choice */
+  int    grid;      /* 0: one workgroup per item; else the persistent grid */
+  double ms;        /* measured time of one apply */
+} vfg_tune_point;
+/* the points the last vfg_batch_autotune timed (returns their number; copies
+ * at most cap) and the configuration in use */
+int vfg_batch_tune_points(const vfg_batch *b, vfg_tune_point *out, int cap);
+int vfg_batch_tuning(const vfg_batch *b, vfg_tune_point *pick);
 /* graph capture of `chunk` CG iterations per replay (enable = 0: plain launches) */
 int vfg_batch_set_graph(vfg_batch *b, int enable, int chunk);
 /* VF_StepU for every member; info has one entry per member */

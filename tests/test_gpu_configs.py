@@ -68,7 +68,9 @@ def test_runtest1_through_gpu():
             if g["step"] <= 5:
                 assert info.iterations == g["altmin"], g["step"]
                 for k in ("elastic", "surface", "crack_opening"):
-                    assert abs(r[k] - g[k]) <= 6e-7 * abs(g[k]) + 1e-6 * (k == "crack_opening"), (g["step"], k)
+                    # golden values printed as 0 or +-1e-17 are rounding noise of an exact zero
+                    floor = 1e-6 if k == "crack_opening" else 1e-12
+                    assert abs(r[k] - g[k]) <= 6e-7 * abs(g[k]) + floor, (g["step"], k)
             else:
                 assert abs(r["surface"] - g["surface"]) < 1e-4 * g["surface"]
                 assert abs(r["crack_opening"] - g["crack_opening"]) < 2e-4 * g["crack_opening"]
@@ -146,7 +148,8 @@ def test_hetero128_operators_on_subboxes(oracle, hetero128):
     on seven sub-boxes of 10^3 nodes (corner, edge, face, interior and one at
     each crack): u-apply (MatShell MatMult with the Dirichlet rows), u load
     vector, u residual, V-apply, V load vector, V residual, per-cell energy
-    terms and crack opening, all within 1e-12 relative."""
+    terms and crack opening, all within 1e-12 relative (with a floor of 1e-15
+    times the field's largest entry for boxes where the field is rounding noise)."""
     spec, H, P, f, N = hetero128
     rng = np.random.default_rng(5)
     nn = N ** 3
@@ -164,6 +167,9 @@ def test_hetero128_operators_on_subboxes(oracle, hetero128):
     cells = P.energy_fields(U, V)
     _, cf = P.VolumetricCrackOpening(U, V, cellfield=True)
     assert np.abs(bu).max() > 0 and np.abs(bv).max() > 0
+    scale = {"u-apply": yu, "u-rhs": bu, "u-residual": Fu, "V-apply": yv, "V-rhs": bv, "V-residual": Fv,
+             "energy cells": cells, "opening cells": cf}
+    scale = {k: float(np.abs(np.asarray(a)).max()) for k, a in scale.items()}
     damaged = 0
     for lo in _boxes(N, spec["cracks"]):
         B = Box(oracle, spec, f, lo, 10, N)
@@ -186,7 +192,11 @@ def test_hetero128_operators_on_subboxes(oracle, hetero128):
             g, o = np.asarray(g).reshape(-1), np.asarray(o).reshape(-1)
             if sel is not None:
                 g, o = g[sel], o[sel]
-            assert relerr(g, o) < 1e-12, (lo, name, relerr(g, o))
+            # A box where the field is zero up to rounding (an undamaged
+            # corner's crack opening) is held to the field's largest entry
+            # over the whole grid instead of its own norm.
+            floor = 1e-15 * np.sqrt(o.size) * scale[name]
+            assert np.linalg.norm(g - o) <= 1e-12 * np.linalg.norm(o) + floor, (lo, name, relerr(g, o))
     assert damaged > 0  # the crack boxes hold damaged nodes
 
 

@@ -64,6 +64,34 @@ def test_u_apply_matches_assembled(pair, oracle):
     assert np.array_equal(y[m], s["x"][m])
 
 
+@pytest.mark.parametrize("n", [(9, 8, 7), (37, 33, 41), (18, 34, 17)])
+def test_u_apply_forms_and_refresh(oracle, n):
+    """Both forms of the FP64 apply (weights recomputed from V in every apply,
+    or stored per element and refreshed after V changes) against the
+    assembled operator, on grids of one tile, several tiles with a partial
+    last tile, and tiles whose shared sides meet at 16-element boundaries;
+    after vfg_set_fields(V) the stored form follows the new V."""
+    from vpfhf_amd import _lib
+    s = S.make(n=n, seed=31, bcU=S.MIXED_U, bcV=S.MIXED_V, atnum=2)
+    op, F = S.to_oracle(oracle, s)
+    P = S.to_gpu(s)
+    try:
+        y_ref = oracle.u_jacobian(op, F) @ s["x"]
+        for form in (_lib.U_FORM_RECOMPUTE, _lib.U_FORM_STORED):
+            P._c(P.L.vfg_u_set_form(P.h, form))
+            assert relerr(P.UIJacobian_mult(s["x"]), y_ref) < 1e-12, form
+        V2 = np.clip(s["V"] + 0.3 * np.sin(np.arange(op.nn)), 0., 1.)
+        P.set_fields(V=V2)
+        F2 = op.fields(V=V2, VIrrev=s["VIrrev"], theta=s["theta"], thetaRef=s["thetaRef"], pressure=s["pressure"])
+        y2 = oracle.u_jacobian(op, F2) @ s["x"]
+        assert relerr(y2, y_ref) > 1e-3
+        assert relerr(P.UIJacobian_mult(s["x"]), y2) < 1e-12
+        P._c(P.L.vfg_u_set_form(P.h, _lib.U_FORM_RECOMPUTE))
+        assert relerr(P.UIJacobian_mult(s["x"]), y2) < 1e-12
+    finally:
+        P.close()
+
+
 def test_u_apply_f32(pair, oracle):
     s, op, F, P = pair
     A = oracle.u_jacobian(op, F)

@@ -17,6 +17,7 @@ ERRORS = {0: "VFG_OK", -1: "VFG_E_ARG", -2: "VFG_E_HIP", -3: "VFG_E_NOMEM", -4: 
 PC_JACOBI, PC_CHEBYSHEV, PC_MG = 0, 1, 2
 UNILATERAL_NONE, UNILATERAL_NOCOMPRESSION = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
+U_FORM_RECOMPUTE, U_FORM_STORED = 0, 1
 
 
 class Grid(C.Structure):
@@ -56,6 +57,10 @@ class EngineDone(C.Structure):
                 ("v_outer", C.c_int), ("v_its", C.c_int), ("errV", C.c_double)]
 
 
+class TunePoint(C.Structure):
+    _fields_ = [("schedule", C.c_int), ("oz_scale", C.c_double), ("grid", C.c_int), ("ms", C.c_double)]
+
+
 class VfgError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("%s (%d): %s" % (ERRORS.get(code, "?"), code, msg))
@@ -81,6 +86,7 @@ PROTOS = {
     "vfg_synchronize": (C.c_int, [H]),
     "vfg_u_apply": (C.c_int, [H, vp, vp, C.c_int]),
     "vfg_u_apply_f32": (C.c_int, [H, vp, vp, C.c_int]),
+    "vfg_u_set_form": (C.c_int, [H, C.c_int]),
     "vfg_u_diag": (C.c_int, [H, vp, C.c_int]),
     "vfg_u_rhs": (C.c_int, [H, vp, C.c_int]),
     "vfg_u_residual": (C.c_int, [H, vp, vp, vp, C.c_int]),
@@ -110,6 +116,8 @@ PROTOS = {
     "vfg_batch_u_apply_f32": (C.c_int, [vp, C.POINTER(vp), C.POINTER(vp)]),
     "vfg_batch_time_u_apply": (C.c_int, [vp, C.c_int, C.c_int, dp]),
     "vfg_batch_autotune": (C.c_int, [vp, C.c_int, dp]),
+    "vfg_batch_tune_points": (C.c_int, [vp, C.POINTER(TunePoint), C.c_int]),
+    "vfg_batch_tuning": (C.c_int, [vp, C.POINTER(TunePoint)]),
     "vfg_batch_set_graph": (C.c_int, [vp, C.c_int, C.c_int]),
     "vfg_batch_u_solve": (C.c_int, [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(SolveOpts),
                                     C.POINTER(SolveInfo), C.c_int]),

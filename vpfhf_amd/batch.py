@@ -42,6 +42,21 @@ class Batch:
         self._c(self.L.vfg_batch_autotune(self.b, int(fp32), C.byref(ms)))
         return ms.value
 
+    SCHEDULES = {0: "16x16 tiles, register march", 1: "8x8 tiles, register march", 2: "16x16 tiles, LDS-ring march"}
+
+    def tune_points(self):
+        """The (schedule, z-chunk scale, grid, ms) points the last autotune timed."""
+        n = self.L.vfg_batch_tune_points(self.b, None, 0)
+        arr = (_lib.TunePoint * max(n, 1))()
+        self.L.vfg_batch_tune_points(self.b, arr, n)
+        return [dict(schedule=p.schedule, oz_scale=p.oz_scale, grid=p.grid, ms=p.ms) for p in arr[:n]]
+
+    def tuning(self):
+        """The configuration in use: schedule, z-chunk scale, grid, its time."""
+        p = _lib.TunePoint()
+        self._c(self.L.vfg_batch_tuning(self.b, C.byref(p)))
+        return dict(schedule=p.schedule, oz_scale=p.oz_scale, grid=p.grid, ms=p.ms)
+
     def set_graph(self, enable=True, chunk=10):
         self._c(self.L.vfg_batch_set_graph(self.b, int(enable), chunk))
 

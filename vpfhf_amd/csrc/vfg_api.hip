@@ -103,6 +103,16 @@ int vfg_scratch(vfg_handle *h, int64_t ndoubles) {
 }
 
 
+int vfg_u_prepare(vfg_handle *h) {
+  DevProblem &P = h->P;
+  if (h->u_form != VFG_U_FORM_STORED || P.unilateral || P.uq) return VFG_OK;
+  int rc = vfg_alloc(h, (void **)&h->uq_buf, 8 * (size_t)uq_doubles(P.G));
+  if (rc) return rc;
+  launch_u_qsetup(P, h->uq_buf, h->stream);
+  P.uq = h->uq_buf;
+  return VFG_OK;
+}
+
 extern "C" {
 
 const char *vfg_last_error(const vfg_handle *h) { return h ? h->err.c_str() : g_err; }
@@ -173,6 +183,7 @@ int vfg_set_props(vfg_handle *h, const vfg_props *pr) {
   if (!h) return vfg_fail(nullptr, VFG_E_ARG, "NULL handle");
   if (!check_props(pr)) return vfg_fail(h, VFG_E_ARG, "vfg_set_props: invalid properties");
   set_props_(h, pr);
+  vfg_u_invalidate(h);
   launch_v_fixmask(h->P, h->stream);
   if (h->mg) h->mg->setups = 0;  // the multigrid smoother bounds depend on nu
   return VFG_OK;
@@ -230,6 +241,7 @@ int vfg_set_fields(vfg_handle *h, const double *V, const double *VIrrev, const d
     if ((rc = vfg_in(h, P.E_elem, E_elem, 8 * P.ne, loc))) return rc;
   }
   if (V) launch_copy_d2f(P.V, P.V32, P.nn, h->stream);
+  if (V || E_elem) vfg_u_invalidate(h);
   if (VIrrev) launch_v_fixmask(P, h->stream);
   if (loc == VFG_HOST) HIPCHK(h, hipStreamSynchronize(h->stream));
   return VFG_OK;
@@ -290,9 +302,20 @@ int vfg_u_set_state(vfg_handle *h, const double *U, int loc) {
   return io.finish();
 }
 
+int vfg_u_set_form(vfg_handle *h, int form) {
+  NEED_H(h);
+  if (form != VFG_U_FORM_RECOMPUTE && form != VFG_U_FORM_STORED)
+    return vfg_fail(h, VFG_E_ARG, "vfg_u_set_form: unknown form %d", form);
+  h->u_form = form;
+  vfg_u_invalidate(h);
+  return VFG_OK;
+}
+
 int vfg_u_apply(vfg_handle *h, const double *x, double *y, int loc) {
   NEED_H(h);
   if (!x || !y) return vfg_fail(h, VFG_E_ARG, "vfg_u_apply: NULL array");
+  int prc = vfg_u_prepare(h);
+  if (prc) return prc;
   IO io(h, loc, x, 24 * h->P.nn, y, 24 * h->P.nn);
   if (io.rc) return io.rc;
   launch_u_apply(h->P, (const double *)io.din, (double *)io.dout, h->stream);
@@ -577,6 +600,10 @@ int vfg_time_u_apply(vfg_handle *h, const double *const *x, double *const *y, in
                      double *ms) {
   NEED_H(h);
   if (!x || !y || nbuf < 1 || nrep < 1 || !ms) return vfg_fail(h, VFG_E_ARG, "vfg_time_u_apply: bad arguments");
+  if (!fp32) {
+    int rc = vfg_u_prepare(h);
+    if (rc) return rc;
+  }
   hipEvent_t e0, e1;
   HIPCHK(h, hipEventCreate(&e0));
   HIPCHK(h, hipEventCreate(&e1));

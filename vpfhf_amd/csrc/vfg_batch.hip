@@ -269,6 +269,9 @@ struct vfg_batch {
   double *d_part = nullptr, *d_sc = nullptr, *d_scale = nullptr, *d_iscale = nullptr;
   void *d_ops = nullptr, *d_ops32 = nullptr, *d_opsx = nullptr;
   int grid = 0;  // persistent grid size (0: one block per item)
+  int sched = 0;  // persistent-apply schedule (batch_tile, BATCH_SCHEDS)
+  double oz_scale = 1.;
+  std::vector<vfg_tune_point> tune;  // the last autotuning's measurements
   int graph_on = 1, graph_chunk = 10;
   hipGraphExec_t gexec = nullptr, gexec32 = nullptr;
   double g_rtol[2] = {-1., -1.};  // rtol / maxit / preconditioner baked into each captured graph
@@ -309,9 +312,11 @@ static void drop_graphs(vfg_batch *b) {
 static int build_items(vfg_batch *b, int slots, double oz_scale) {
   b->items.clear();
   b->oz.assign(b->np, 1);
+  b->oz_scale = oz_scale;
+  const int ow = batch_tile(b->sched) - 1;  // owned node columns per tile
   for (int p = 0; p < b->np; p++) {
     const Grid &G = b->m[p]->P.G;
-    int tx = (G.nx + 14) / 15, ty = (G.ny + 14) / 15;
+    int tx = (G.nx + ow - 1) / ow, ty = (G.ny + ow - 1) / ow;
     int oz = choose_oz_pub(G, tx * ty, slots);
     oz = (int)(oz * oz_scale + .5);
     if (oz < 2) oz = 2;
@@ -413,7 +418,7 @@ int vfg_batch_u_apply(vfg_batch *b, const double *const *x, double *const *y) {
   if (!b || !x || !y) return VFG_E_ARG;
   std::vector<const void *> xs(x, x + b->np), ys(y, y + b->np);
   fill_ops(b->m, xs.data(), (void *const *)ys.data(), 0, b->oz, b->d_opsx, b->stream);
-  launch_batch_apply(0, b->d_opsx, b->d_items, b->nitems, b->grid, b->stream);
+  launch_batch_apply(0, b->sched, b->d_opsx, b->d_items, b->nitems, b->grid, b->stream);
   BCHK(b, hipStreamSynchronize(b->stream));
   BCHK(b, hipGetLastError());
   return VFG_OK;
@@ -423,7 +428,7 @@ int vfg_batch_u_apply_f32(vfg_batch *b, const float *const *x, float *const *y) 
   if (!b || !x || !y) return VFG_E_ARG;
   std::vector<const void *> xs(x, x + b->np), ys(y, y + b->np);
   fill_ops(b->m, xs.data(), (void *const *)ys.data(), 1, b->oz, b->d_ops32, b->stream);
-  launch_batch_apply(1, b->d_ops32, b->d_items, b->nitems, b->grid, b->stream);
+  launch_batch_apply(1, b->sched, b->d_ops32, b->d_items, b->nitems, b->grid, b->stream);
   BCHK(b, hipStreamSynchronize(b->stream));
   BCHK(b, hipGetLastError());
   return VFG_OK;
@@ -444,7 +449,7 @@ int vfg_batch_time_u_apply(vfg_batch *b, int fp32, int nrep, double *ms) {
   BCHK(b, hipEventCreate(&e0));
   BCHK(b, hipEventCreate(&e1));
   hipEventRecord(e0, b->stream);
-  for (int i = 0; i < nrep; i++) launch_batch_apply(fp32, ops, b->d_items, b->nitems, b->grid, b->stream);
+  for (int i = 0; i < nrep; i++) launch_batch_apply(fp32, b->sched, ops, b->d_items, b->nitems, b->grid, b->stream);
   hipEventRecord(e1, b->stream);
   hipEventSynchronize(e1);
   float t = 0.f;
@@ -456,28 +461,58 @@ int vfg_batch_time_u_apply(vfg_batch *b, int fp32, int nrep, double *ms) {
   return VFG_OK;
 }
 
-// Autotuner: times the persistent apply for each (z-chunk scale, grid) pair
-// and keeps the fastest; best_ms receives the time of one apply.
+// Autotuner: times the persistent apply for every (schedule, z-chunk scale,
+// grid) point -- schedules: 16 x 16 element tiles on the register march, 8 x 8
+// tiles on the register march, 16 x 16 tiles on the LDS-ring march; z-chunk
+// lengths 0.5, 1 and 2 times the whole-round choice; one workgroup per item,
+// or a persistent grid of one or two rounds of resident workgroups -- keeps the
+// fastest and records every point (vfg_batch_tune_points); best_ms receives
+// the time of one apply.
 int vfg_batch_autotune(vfg_batch *b, int fp32, double *best_ms) {
   if (!b) return VFG_E_ARG;
-  const int slots = batch_slots();
   const double scales[3] = {0.5, 1.0, 2.0};
-  const int grids[3] = {0, slots, 2 * slots};
   double best = 1e300, bscale = 1.0;
-  int bgrid = 0, rc;
-  for (double sc : scales) {
-    if ((rc = build_items(b, slots, sc))) return rc;
-    for (int g : grids) {
-      b->grid = g;
-      double ms = 0.;
-      if ((rc = vfg_batch_time_u_apply(b, fp32, 2, &ms)) || (rc = vfg_batch_time_u_apply(b, fp32, 5, &ms))) return rc;
-      if (ms < best) { best = ms; bscale = sc; bgrid = g; }
+  int bgrid = 0, bsched = 0, rc;
+  b->tune.clear();
+  for (int sc = 0; sc < BATCH_SCHEDS; sc++) {
+    b->sched = sc;
+    const int slots = batch_slots_s(fp32, sc);
+    const int grids[3] = {0, slots, 2 * slots};
+    for (double z : scales) {
+      if ((rc = build_items(b, slots, z))) return rc;
+      for (int g : grids) {
+        b->grid = g;
+        double ms = 0.;
+        if ((rc = vfg_batch_time_u_apply(b, fp32, 2, &ms)) || (rc = vfg_batch_time_u_apply(b, fp32, 5, &ms)))
+          return rc;
+        b->tune.push_back({sc, z, g, ms / 5.});
+        if (ms < best) { best = ms; bscale = z; bgrid = g; bsched = sc; }
+      }
     }
   }
-  if ((rc = build_items(b, slots, bscale))) return rc;
+  b->sched = bsched;
+  if ((rc = build_items(b, batch_slots_s(fp32, bsched), bscale))) return rc;
   b->grid = bgrid;
   drop_graphs(b);
   if (best_ms) *best_ms = best / 5.;
+  return VFG_OK;
+}
+
+int vfg_batch_tune_points(const vfg_batch *b, vfg_tune_point *out, int cap) {
+  if (!b) return VFG_E_ARG;
+  const int n = (int)b->tune.size();
+  for (int i = 0; i < n && i < cap && out; i++) out[i] = b->tune[i];
+  return n;
+}
+
+int vfg_batch_tuning(const vfg_batch *b, vfg_tune_point *pick) {
+  if (!b || !pick) return VFG_E_ARG;
+  pick->schedule = b->sched;
+  pick->oz_scale = b->oz_scale;
+  pick->grid = b->grid;
+  pick->ms = 0.;
+  for (const vfg_tune_point &t : b->tune)
+    if (t.schedule == b->sched && t.oz_scale == b->oz_scale && t.grid == b->grid) pick->ms = t.ms;
   return VFG_OK;
 }
 
@@ -499,7 +534,7 @@ static void cg_iter(vfg_batch *b, int o, void *ops, int fp32, double rtol, int m
   const BChunk *ch = b->d_ch;
   // frozen members are skipped by the apply and the vector kernels
   const int g = b->grid > 0 && b->grid < b->nitems ? b->grid : b->nitems;
-  launch_engine_u_apply(fp32, ops, b->d_items, b->d_nitems, g, b->d_sc + B_ACTIVE, B_NS, s);
+  launch_engine_u_apply(fp32, b->sched, ops, b->d_items, 0, b->d_nitems, g, b->d_sc + B_ACTIVE, B_NS, s);
   hipLaunchKernelGGL(k_bdot<T>, dim3(b->nch), dim3(BRB), 0, s, (const BVec *)b->d_vt, o + VP, o + VQ, ch,
                      (const double *)b->d_sc, b->d_part);
   hipLaunchKernelGGL(k_bscalar, dim3(b->np), dim3(BRB), 0, s, (const double *)b->d_part, (const int *)b->d_first,

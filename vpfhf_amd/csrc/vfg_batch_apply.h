@@ -8,18 +8,24 @@ struct vfg_handle;
 namespace vfg {
 
 size_t batch_op_bytes(int fp32);
-int batch_slots();
+// persistent-apply schedules: 0: 16 x 16 tiles, register march; 1: 8 x 8
+// tiles, register march; 2: 16 x 16 tiles, LDS-ring march
+constexpr int BATCH_SCHEDS = 3;
+int batch_tile(int sched);             // element tile width (owned node columns + 1)
+int batch_slots_s(int fp32, int sched);  // resident workgroups of that apply on the device
+int batch_slots();                     // schedule 0, FP64
 int choose_oz_pub(const Grid &G, int tiles_xy, int slots);
 // writes one {UApplyOp, OZ} record per member (x[p] -> y[p]) into d_ops
 void fill_ops(const std::vector<vfg_handle *> &m, const void *const *x, void *const *y, int fp32,
               const std::vector<int> &oz, void *d_ops, hipStream_t s);
 // grid == 0: one workgroup per item; otherwise a persistent grid striding the items
-void launch_batch_apply(int fp32, const void *d_ops, const BItem *items, int nitems, int grid, hipStream_t s);
+void launch_batch_apply(int fp32, int sched, const void *d_ops, const BItem *items, int nitems, int grid,
+                        hipStream_t s);
 
-// serving engine: persistent applies over device-counted work lists; items of
-// members whose flags[p * fstride] is 0 are skipped
-void launch_engine_u_apply(int fp32, const void *d_ops, const BItem *items, const int *dcount, int grid,
-                           const double *flags, int fstride, hipStream_t s);
+// persistent applies over (device-counted, when dcount is given) work lists;
+// items of members whose flags[p * fstride] is 0 are skipped
+void launch_engine_u_apply(int fp32, int sched, const void *d_ops, const BItem *items, int nitems,
+                           const int *dcount, int grid, const double *flags, int fstride, hipStream_t s);
 void make_uop(const DevProblem &P, const void *x, void *y, int fp32, int OZ, void *out);  // host record
 size_t vop_bytes();
 int v_oz(const Grid &G);

@@ -360,7 +360,7 @@ int bmg_setup_member(BatchMG *g, int p, const double *dinv64, hipStream_t s) {
 static void apply(BatchMG *g, int l, const BmgList &L, int which, const double *flags, int fstride, hipStream_t s) {
   if (!L.items.cap) return;
   const int grid = L.items.cap < g->slots ? L.items.cap : g->slots;
-  launch_engine_u_apply(1, g->lv[l].ops, L.items.d, g->lv[l].dcnt + 2 * which, grid, flags, fstride, s);
+  launch_engine_u_apply(1, 0, g->lv[l].ops, L.items.d, 0, g->lv[l].dcnt + 2 * which, grid, flags, fstride, s);
 }
 static void cheb(BatchMG *g, int l, const BmgList &L, int which, int step, int use_y, int xzero, const double *flags,
                  int fstride, hipStream_t s) {

@@ -509,8 +509,11 @@ static int advance(vfg_engine *e, int p) {
           return VFG_OK;
         }
         if (q.rn0 < 0.) q.rn0 = rn;
-        if (rn <= e->o.u_rtol * q.rn0 || rn == 0. || q.outer >= 60) {
-          if (!(rn <= e->o.u_rtol * q.rn0 || rn == 0.) && !q.status) q.status = VFG_E_MAXIT;
+        // converged at the tolerance or at the FP64 residual's rounding floor
+        // (an outer step no longer halves it: the rule of u_solve_mixed)
+        const bool conv = rn <= e->o.u_rtol * q.rn0 || rn == 0. || (q.outer > 0 && rn > 0.5 * q.rs);
+        if (conv || q.outer >= 60) {
+          if (!conv && !q.status) q.status = VFG_E_MAXIT;
           q.st = ST_V_BEGIN;
           break;
         }
@@ -639,6 +642,7 @@ static int advance(vfg_engine *e, int p) {
       case ST_V_END: {  // the handle's V for the next u-solve; ||V - Vold||_inf
         ECHK(e, hipMemcpyAsync(P.V, q.V, 8 * nn, hipMemcpyDeviceToDevice, s));
         launch_copy_d2f(P.V, P.V32, nn, s);
+        vfg_u_invalidate(q.h);
         launch_maxabs_diff(q.V, h->w[11], nn, h->part, scal(e, p, E_ERRV), s);
         q.st = ST_AM_CHECK;
         return VFG_OK;
@@ -722,9 +726,9 @@ static void e_iter(vfg_engine *e, bool with_mg) {
   hipStream_t s = e->s;
   const double *fl = e->d_sc + E_ACTIVE;
   if (e->o.precision == VFG_PREC_MIXED)
-    launch_engine_u_apply(1, e->d_uop, e->d_it[0], e->d_cnt + C_U32, e->grid_apply, fl, E_NS, s);
+    launch_engine_u_apply(1, 0, e->d_uop, e->d_it[0], 0, e->d_cnt + C_U32, e->grid_apply, fl, E_NS, s);
   else
-    launch_engine_u_apply(0, e->d_uop, e->d_it[1], e->d_cnt + C_U64, e->grid_apply, fl, E_NS, s);
+    launch_engine_u_apply(0, 0, e->d_uop, e->d_it[1], 0, e->d_cnt + C_U64, e->grid_apply, fl, E_NS, s);
   launch_engine_v_apply(e->d_vop, e->d_it[2], e->d_cnt + C_V, e->grid_vapply, fl, E_NS, s);
   hipLaunchKernelGGL(k_e_dot, dim3(e->grid_vec), dim3(ERB), 0, s, (const EVec *)e->d_vt, (const EChunk *)e->d_ch[0],
                      (const EChunk *)e->d_ch[1], (const int *)e->d_cnt, (const double *)e->d_sc, e->d_part);

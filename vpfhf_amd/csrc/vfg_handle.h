@@ -47,6 +47,10 @@ struct vfg_handle {
   float *wf[8] = {nullptr};
   uint8_t *act = nullptr;
   vfg::MgHier *mg = nullptr;  // u-solve multigrid (VFG_PC_MG)
+  // FP64 u-apply form (VFG_U_FORM_*); the stored form keeps its integrand
+  // weights in uq_buf, valid while P.uq is set (cleared when V, E or eta change)
+  int u_form = 0;  // VFG_U_FORM_RECOMPUTE
+  double *uq_buf = nullptr;
 };
 
 namespace vfg {
@@ -58,6 +62,10 @@ int mg_setup(vfg_handle *h, const double *dinv64);  // coefficients, diagonals, 
 }  // namespace vfg
 
 int vfg_fail(vfg_handle *h, int code, const char *fmt, ...);
+// the stored form's weights for the current coefficients (no-op for the
+// recomputing form and for NOCOMPRESSION problems); u_invalidate drops them
+int vfg_u_prepare(vfg_handle *h);
+inline void vfg_u_invalidate(vfg_handle *h) { h->P.uq = nullptr; }
 int vfg_alloc(vfg_handle *h, void **p, size_t bytes);
 int vfg_scratch(vfg_handle *h, int64_t ndoubles);
 // solver work vectors of 3*nn entries: w[0..n64) FP64 and wf[0..n32) FP32

@@ -451,6 +451,67 @@ __device__ __forceinline__ void march_lds(const Grid &G, const Op &op, int OZ, i
   }
 }
 
+// ------------------------------------------------- Gauss-point walk
+// Values (and, for the first NG fields, gradients) of trilinear nodal fields
+// at the 27 Gauss points of one element, by sum factorisation one axis at a
+// time: the 8 corner values f[a], a = (k*2+j)*2+i, are interpolated along z
+// for each Gauss layer, then along y, then along x, with each derivative
+// taken at the stage of its axis (d/dz of the corner pairs, constant in z).
+// f(gx, gy, gz, val[NG+NV], grad[NG][3]) is called for every point in the
+// order gz, gy, gx.  About 11 operations per point and gradient field and 3
+// per value-only field, against 32 and 8 for the direct 8-corner sums.
+template <int NG, int NV, typename F>
+__device__ __forceinline__ void gauss_walk(const double (&fld)[NG + NV][8], double ihx, double ihy, double ihz,
+                                           const F &f) {
+  constexpr int NF = NG + NV;
+  double dz[NG > 0 ? NG : 1][4];  // d/dz at the 4 (x, y) corners
+#pragma unroll
+  for (int q = 0; q < NG; q++)
+#pragma unroll
+    for (int c = 0; c < 4; c++) dz[q][c] = (fld[q][c + 4] - fld[q][c]) * ihz;
+#pragma unroll
+  for (int gz = 0; gz < 3; gz++) {
+    const double pz = gP1(gz);
+    double z[NF][4];
+#pragma unroll
+    for (int q = 0; q < NF; q++)
+#pragma unroll
+      for (int c = 0; c < 4; c++) z[q][c] = fld[q][c] + pz * (fld[q][c + 4] - fld[q][c]);
+#pragma unroll
+    for (int gy = 0; gy < 3; gy++) {
+      const double py = gP1(gy);
+      double y[NF][2], dyv[NG > 0 ? NG : 1][2], dzy[NG > 0 ? NG : 1][2];
+#pragma unroll
+      for (int q = 0; q < NF; q++)
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+          const double d = z[q][i + 2] - z[q][i];
+          y[q][i] = z[q][i] + py * d;
+          if (q < NG) {
+            dyv[q][i] = d * ihy;
+            dzy[q][i] = dz[q][i] + py * (dz[q][i + 2] - dz[q][i]);
+          }
+        }
+#pragma unroll
+      for (int gx = 0; gx < 3; gx++) {
+        const double px = gP1(gx);
+        double val[NF], grad[NG > 0 ? NG : 1][3];
+#pragma unroll
+        for (int q = 0; q < NF; q++) {
+          const double d = y[q][1] - y[q][0];
+          val[q] = y[q][0] + px * d;
+          if (q < NG) {
+            grad[q][0] = d * ihx;
+            grad[q][1] = dyv[q][0] + px * (dyv[q][1] - dyv[q][0]);
+            grad[q][2] = dzy[q][0] + px * (dzy[q][1] - dzy[q][0]);
+          }
+        }
+        f(gx, gy, gz, val, grad);
+      }
+    }
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ T interp(int g, T f0, T f1) {
   return T(gP0(g)) * f0 + T(gP1(g)) * f1;

@@ -29,14 +29,25 @@ struct DevProblem {
   float *V32;                             // FP32 copy of V for the FP32 apply
   double *zero3;                          // 3*nn zeros
   // V-subproblem state, refreshed by vfg_v_set_u
+  double *uq;      // FP64 u-apply, stored form: W_g s_g E_e vol per element (uq_doubles); null: recomputed
   double *Vq;      // W_g * (c_AT/eps [AT2] + 2 W^el_g), tile-blocked (vq_doubles)
   double *bV;      // V load vector (nn)
   uint8_t *vfix;   // fixed V rows/cols (Dirichlet + AT2 irreversibility)
 };
 
 // u operator (vfg_u.hip)
+// size of the stored integrand weights: node-owned tiles x element layers x 27 x 256
+int64_t uq_doubles(const Grid &G);
+void launch_u_qsetup(const DevProblem &P, double *uq, hipStream_t s);
 void launch_u_apply(const DevProblem &P, const double *x, double *y, hipStream_t s);
 void launch_u_apply_f32(const DevProblem &P, const float *x, float *y, hipStream_t s);
+// FP32 multigrid level: one Chebyshev-Jacobi step from x = 0 fused with the
+// residual (x = c2 D^-1 b, r = b - A x), and one step from x fused with its
+// apply (xo = x + c2 D^-1 (b - A x)); xo must not alias x
+void launch_mg_presmooth(const DevProblem &P, const float *b, const float *dinv, float c2, float *x, float *r,
+                         hipStream_t s);
+void launch_mg_postsmooth(const DevProblem &P, const float *b, const float *dinv, float c2, const float *x, float *xo,
+                          hipStream_t s);
 // FP64 y = A x (no NOCOMPRESSION) with one x.y partial per workgroup in part;
 // returns the number of partials (0, and nothing launched, for NOCOMPRESSION)
 int launch_u_apply_dot(const DevProblem &P, const double *x, double *y, double *part, hipStream_t s);

@@ -36,8 +36,8 @@ __device__ __forceinline__ int mg_gather(int I, int nf, int nc, int *i0, float *
   return m;
 }
 
-// coarse rhs entry (node N, component c) = P^T (b - y) over the free fine rows;
-// zero on coarse Dirichlet rows
+// coarse rhs entry (node N, component c) = P^T (b - y) over the free fine rows
+// (y = nullptr: b already holds the residual); zero on coarse Dirichlet rows
 __device__ __forceinline__ float mg_restrict_dof(const Grid &Gf, const Grid &Gc, const BcMask &bc,
                                                  const float *__restrict__ b, const float *__restrict__ y,
                                                  int64_t N, int c) {
@@ -56,7 +56,7 @@ __device__ __forceinline__ float mg_restrict_dof(const Grid &Gf, const Grid &Gc,
         const int i = i0 + ii, j = j0 + jj, k = k0 + kk;
         if (on_dirichlet(Gf, bc.face[c], bc.vert[c], i, j, k)) continue;
         const int64_t n = 3 * node_id(Gf, i, j, k) + c;
-        row += wx[ii] * (b[n] - y[n]);
+        row += wx[ii] * (y ? b[n] - y[n] : b[n]);
       }
       acc += wy[jj] * wz[kk] * row;
     }
@@ -83,7 +83,7 @@ __device__ __forceinline__ float mg_restrict_dof_plane(const Grid &Gf, const Gri
         const int i = i0 + ii, j = j0 + jj, k = k0 + kp;
         if (on_dirichlet(Gf, bc.face[c], bc.vert[c], i, j, k)) continue;
         const int64_t n = 3 * node_id(Gf, i, j, k) + c;
-        row += wx[ii] * (b[n] - y[n]);
+        row += wx[ii] * (y ? b[n] - y[n] : b[n]);
       }
       acc += wy[jj] * wz[kp] * row;
     }

@@ -109,6 +109,62 @@ __global__ __launch_bounds__(256) void k_mg_restrict4(Grid Gf, Grid Gc, BcMask b
   if (ok && kk == 0) bcoarse[t] = ((v + v1) + v2) + v3;
 }
 
+// Nested levels (nf = 2 nc in every direction): the same transfers with the
+// weights of the 1-D map xi = i / 2 written out -- fine node 2I gets coarse
+// node I with weight 1, fine node 2I + 1 gets I and I + 1 with 1/2 each --
+// and one thread per node doing all three components (the general kernels
+// form the weights from the map in FP64 per entry).
+__device__ __forceinline__ float mg_w2(int d) { return d == 0 ? 1.f : .5f; }
+__global__ __launch_bounds__(256) void k_mg_restrict2(Grid Gf, Grid Gc, BcMask bc, const float *__restrict__ b,
+                                                      const float *__restrict__ y, float *__restrict__ bcoarse) {
+  const int64_t N = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (N >= (int64_t)Gc.nx * Gc.ny * Gc.nz) return;
+  const int I = (int)(N % Gc.nx), J = (int)((N / Gc.nx) % Gc.ny), K = (int)(N / ((int64_t)Gc.nx * Gc.ny));
+  float acc[3] = {0.f, 0.f, 0.f};
+  for (int dk = -1; dk <= 1; dk++) {
+    const int k = 2 * K + dk;
+    if (k < 0 || k >= Gf.nz) continue;
+    for (int dj = -1; dj <= 1; dj++) {
+      const int j = 2 * J + dj;
+      if (j < 0 || j >= Gf.ny) continue;
+      float row[3] = {0.f, 0.f, 0.f};
+      for (int di = -1; di <= 1; di++) {
+        const int i = 2 * I + di;
+        if (i < 0 || i >= Gf.nx) continue;
+        const int64_t n = 3 * node_id(Gf, i, j, k);
+        for (int c = 0; c < 3; c++)
+          if (!on_dirichlet(Gf, bc.face[c], bc.vert[c], i, j, k))
+            row[c] += mg_w2(di) * (y ? b[n + c] - y[n + c] : b[n + c]);
+      }
+      const float w = mg_w2(dj) * mg_w2(dk);
+      for (int c = 0; c < 3; c++) acc[c] += w * row[c];
+    }
+  }
+  for (int c = 0; c < 3; c++) bcoarse[3 * N + c] = on_dirichlet(Gc, bc.face[c], bc.vert[c], I, J, K) ? 0.f : acc[c];
+}
+__global__ __launch_bounds__(256) void k_mg_prolong2(Grid Gf, Grid Gc, BcMask bc, const float *__restrict__ xc,
+                                                     float *__restrict__ xf) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= (int64_t)Gf.nx * Gf.ny * Gf.nz) return;
+  const int i = (int)(n % Gf.nx), j = (int)((n / Gf.nx) % Gf.ny), k = (int)(n / ((int64_t)Gf.nx * Gf.ny));
+  const int I0 = i >> 1, J0 = j >> 1, K0 = k >> 1;
+  const int mi = i & 1, mj = j & 1, mk = k & 1;  // odd: between I0 and I0 + 1
+  const float wx = mi ? .5f : 1.f, wy = mj ? .5f : 1.f, wz = mk ? .5f : 1.f;
+  float acc[3] = {0.f, 0.f, 0.f};
+  for (int c3 = 0; c3 <= mk; c3++)
+    for (int c2 = 0; c2 <= mj; c2++)
+      for (int c1 = 0; c1 <= mi; c1++) {
+        const float w = wx * wy * wz;
+        const int64_t N = 3 * node_id(Gc, I0 + c1, J0 + c2, K0 + c3);
+        for (int c = 0; c < 3; c++) acc[c] += w * xc[N + c];
+      }
+  for (int c = 0; c < 3; c++)
+    if (!on_dirichlet(Gf, bc.face[c], bc.vert[c], i, j, k)) xf[3 * n + c] += acc[c];
+}
+static bool mg_nested(const Grid &Gf, const Grid &Gc) {
+  return Gf.nex == 2 * Gc.nex && Gf.ney == 2 * Gc.ney && Gf.nez == 2 * Gc.nez;
+}
+
 __global__ __launch_bounds__(256) void k_mg_prolong(Grid Gf, Grid Gc, BcMask bc, const float *__restrict__ xc,
                                                     float *__restrict__ xf) {
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -161,12 +217,15 @@ void launch_mg_restrict(const Grid &Gf, const Grid &Gc, const BcMask &bc, const 
   const int64_t n3 = 3 * (int64_t)Gc.nx * Gc.ny * Gc.nz;
   // few coarse entries: latency-bound, split each over its fine planes
   if (n3 <= (1 << 16)) hipLaunchKernelGGL(k_mg_restrict4, dim3(nbm(4 * n3)), dim3(256), 0, s, Gf, Gc, bc, b, y, bcoarse);
+  else if (mg_nested(Gf, Gc))
+    hipLaunchKernelGGL(k_mg_restrict2, dim3(nbm(n3 / 3)), dim3(256), 0, s, Gf, Gc, bc, b, y, bcoarse);
   else hipLaunchKernelGGL(k_mg_restrict, dim3(nbm(n3)), dim3(256), 0, s, Gf, Gc, bc, b, y, bcoarse);
 }
 void launch_mg_prolong(const Grid &Gf, const Grid &Gc, const BcMask &bc, const float *xc, float *xf,
                        hipStream_t s) {
   const int64_t nn = (int64_t)Gf.nx * Gf.ny * Gf.nz;
-  hipLaunchKernelGGL(k_mg_prolong, dim3(nbm(nn)), dim3(256), 0, s, Gf, Gc, bc, xc, xf);
+  if (mg_nested(Gf, Gc)) hipLaunchKernelGGL(k_mg_prolong2, dim3(nbm(nn)), dim3(256), 0, s, Gf, Gc, bc, xc, xf);
+  else hipLaunchKernelGGL(k_mg_prolong, dim3(nbm(nn)), dim3(256), 0, s, Gf, Gc, bc, xc, xf);
 }
 void launch_mg_cheb(const float *b, const float *y, const float *dinv, float *d, float *x, float c1, float c2,
                     int dfirst, int xzero, int64_t n, hipStream_t s) {

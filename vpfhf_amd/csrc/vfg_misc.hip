@@ -102,55 +102,43 @@ __global__ __launch_bounds__(EB) void k_energies(DevProblem P, const double *__r
   if (e < P.ne) {
     const Grid &G = P.G;
     const int ei = (int)(e % G.nex), ej = (int)((e / G.nex) % G.ney), ek = (int)(e / ((int64_t)G.nex * G.ney));
-    double u[8][3], v[8], th[8], pp[8];
+    // fields 0-2: u_c, 3: v (values and gradients); 4: p, 5: theta - thetaRef
+    double fld[6][8];
     for (int a = 0; a < 8; a++) {
       int64_t n = node_id(G, ei + (a & 1), ej + ((a >> 1) & 1), ek + (a >> 2));
-      for (int c = 0; c < 3; c++) u[a][c] = U[3 * n + c];
-      v[a] = V[n]; th[a] = P.dth[n]; pp[a] = P.p[n];
+      for (int c = 0; c < 3; c++) fld[c][a] = U[3 * n + c];
+      fld[3][a] = V[n]; fld[4][a] = P.p[n]; fld[5][a] = P.dth[n];
     }
     const double Ee = P.E_elem ? P.E_elem[e] : P.E;
     const double lam = P.lam1 * Ee, mu = P.mu1 * Ee;
     const double tk = 3. * lam + 2. * mu, cb = P.beta / tk;
-    const double ih[3] = {1. / G.hx, 1. / G.hy, 1. / G.hz};
     const double vol = G.hx * G.hy * G.hz;
-    const double csurf = P.Gc / P.atcv * .25;
+    const double csurf = P.Gc / P.atcv * .25, ieps = 1. / P.eps, third = 1. / 3.;
     double cell = 0.;
-    for (int g = 0; g < 27; g++) {
-      int gx = g % 3, gy = (g / 3) % 3, gz = g / 9;
-      double H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, ug[3] = {0, 0, 0}, gv[3] = {0, 0, 0};
-      double vg = 0., pg = 0., tg = 0.;
-      for (int a = 0; a < 8; a++) {
-        int i = a & 1, j = (a >> 1) & 1, k = a >> 2;
-        double px = i ? gP1(gx) : gP0(gx), py = j ? gP1(gy) : gP0(gy), pz = k ? gP1(gz) : gP0(gz);
-        double ph = px * py * pz;
-        double dg[3] = {(i ? ih[0] : -ih[0]) * py * pz, (j ? ih[1] : -ih[1]) * px * pz,
-                        (k ? ih[2] : -ih[2]) * px * py};
-        for (int c = 0; c < 3; c++) {
-          ug[c] += ph * u[a][c];
-          gv[c] += dg[c] * v[a];
-          for (int d = 0; d < 3; d++) H[c][d] += dg[d] * u[a][c];
-        }
-        vg += ph * v[a]; pg += ph * pp[a]; tg += ph * th[a];
-      }
+    gauss_walk<4, 2>(fld, 1. / G.hx, 1. / G.hy, 1. / G.hz,
+                     [&](int gx, int gy, int gz, const double (&val)[6], const double (&H)[4][3]) {
+      const double vg = val[3], pg = val[4], tg = val[5];
       const double W = vol * gW(gx) * gW(gy) * gW(gz);
       double tr = H[0][0] + H[1][1] + H[2][2];
-      double d11 = H[0][0] - tr / 3., d22 = H[1][1] - tr / 3., d33 = H[2][2] - tr / 3.;
+      const double tr3 = tr * third;
+      double d11 = H[0][0] - tr3, d22 = H[1][1] - tr3, d33 = H[2][2] - tr3;
       double d12 = (H[0][1] + H[1][0]) * .5, d23 = (H[1][2] + H[2][1]) * .5, d13 = (H[0][2] + H[2][0]) * .5;
       el += (d11 * d11 + d22 * d22 + d33 * d33 + (d12 * d12 + d23 * d23 + d13 * d13) * 2.) * vg * vg *
             (1. + P.eta) * mu * W;
       // UNILATERAL_NOCOMPRESSION: the spherical term is not degraded where
       // tr(eps) < 0 (VF_ElasticEnergyNoCompression3D_local, VFMech.c:871-903)
-      double WD = (tr / 3. - P.alpha * tg) * (P.unilateral && tr < 0. ? 1. : vg) - cb * pg;
+      double WD = (tr3 - P.alpha * tg) * (P.unilateral && tr < 0. ? 1. : vg) - cb * pg;
       el += 3. * tk / 2. * WD * WD * W;
       el += tk / 6. * tr * tr * P.eta * W;
-      double ugv = ug[0] * gv[0] + ug[1] * gv[1] + ug[2] * gv[2];
+      const double *gv = H[3];
+      double ugv = val[0] * gv[0] + val[1] * gv[1] + val[2] * gv[2];
       if (P.hasCrackPressure) pw += W * pg * ugv;
       double g2 = gv[0] * gv[0] + gv[1] * gv[1] + gv[2] * gv[2];
       double omv = 1. - vg;
-      se += W * ((P.atnum == 2 ? omv * omv : omv) / P.eps + g2 * P.eps) * csurf;
+      se += W * ((P.atnum == 2 ? omv * omv : omv) * ieps + g2 * P.eps) * csurf;
       cv += ugv * W;
       cell += ugv * W;
-    }
+    });
     if (cellfield) cellfield[e] = cell / vol;
     if (efield) {
       efield[3 * e] = el;

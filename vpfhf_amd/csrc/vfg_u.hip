@@ -195,9 +195,13 @@ __device__ __forceinline__ void u_elem_apply(const T (&X)[2][4][4], T cE, T lam1
 // a + p1 (b - a) (one FMA per point and value).  Accumulation and the final
 // projections are those of u_elem_apply.  UNILATERAL_NONE only: the
 // NOCOMPRESSION apply keeps the register march.
-template <typename T, typename XS>
-__device__ __forceinline__ void u_elem_apply_lds(const XS &xs, T cE, T lam1, T mu1, T eta, T ihx, T ihy, T ihz,
-                                                 T (&r)[8][3]) {
+// QS != nullptr (stored coefficients): the 27 products W_g s_g E_e vol of the
+// element are read from QS[g * QSTRIDE], g = (gz * 3 + gy) * 3 + gx, instead
+// of being formed from the nodal v (then xs needs no field 3).
+template <typename T, typename XS, typename OUT, int QSTRIDE>
+__device__ __forceinline__ void u_elem_lds_impl(const XS &xs, T cE, T lam1, T mu1, T eta, T ihx, T ihy, T ihz,
+                                                const OUT &out, const T *__restrict__ QS) {
+  constexpr bool STORED = QSTRIDE > 0;
   const T lr = lam1 / mu1;
   const T mx = mu1 * ihx, my = mu1 * ihy, mz = mu1 * ihz;
   T Rx[3][2][2], Ry[3][2][2], Sz[3][3][3];  // Rx[c][k][j], Ry[c][k][i], Sz[c][gy][gx]
@@ -219,6 +223,13 @@ __device__ __forceinline__ void u_elem_apply_lds(const XS &xs, T cE, T lam1, T m
     // rather than kept live across layers
     asm volatile("" ::: "memory");
     T Dz[4][3], Vz[4], Gx[3][3], by0[3], byd[3];
+    T Ql[3][3];  // stored coefficients of the layer [gx][gy]
+    if constexpr (STORED) {
+#pragma unroll
+      for (int gx = 0; gx < 3; gx++)
+#pragma unroll
+        for (int gy = 0; gy < 3; gy++) Ql[gx][gy] = QS[((gz * 3 + gy) * 3 + gx) * QSTRIDE];
+    }
     {
       T Uz[4][3];
 #pragma unroll
@@ -229,8 +240,10 @@ __device__ __forceinline__ void u_elem_apply_lds(const XS &xs, T cE, T lam1, T m
           Uz[q][c] = a + pz1 * d;
           Dz[q][c] = d * mz;
         }
-        const T a = xs(0, q, 3);
-        Vz[q] = a + pz1 * (xs(1, q, 3) - a);
+        if constexpr (!STORED) {
+          const T a = xs(0, q, 3);
+          Vz[q] = a + pz1 * (xs(1, q, 3) - a);
+        }
       }
 #pragma unroll
       for (int c = 0; c < 3; c++) {
@@ -247,7 +260,8 @@ __device__ __forceinline__ void u_elem_apply_lds(const XS &xs, T cE, T lam1, T m
       Dzx0[c] = Dz[1][c] - Dz[0][c];
       Dzx2[c] = Dz[3][c] - Dz[2][c];
     }
-    const T vdx0 = Vz[1] - Vz[0], vdx2 = Vz[3] - Vz[2];
+    T vdx0 = T(0), vdx2 = T(0);
+    if constexpr (!STORED) { vdx0 = Vz[1] - Vz[0]; vdx2 = Vz[3] - Vz[2]; }
     T Sx[3][3], ay[3][2];  // Sx[c][gy] = sum_gx sigma_c0; ay[c][i] = y-projection of sum_gy sigma_c1
 #pragma unroll
     for (int c = 0; c < 3; c++) {
@@ -267,13 +281,22 @@ __device__ __forceinline__ void u_elem_apply_lds(const XS &xs, T cE, T lam1, T m
         azd[c] = Dz[2][c] + px * Dzx2[c] - az0[c];
         Sy[c] = T(0);
       }
-      const T av0 = Vz[0] + px * vdx0, avd = Vz[2] + px * vdx2 - av0;
-      const T wzx = wz * T(gW(gx));
-      const T w0 = wzx * T(gW(0)), w1 = wzx * T(gW(1));
+      T av0 = T(0), avd = T(0), w0 = T(0), w1 = T(0);
+      if constexpr (!STORED) {
+        av0 = Vz[0] + px * vdx0;
+        avd = Vz[2] + px * vdx2 - av0;
+        const T wzx = wz * T(gW(gx));
+        w0 = wzx * T(gW(0));
+        w1 = wzx * T(gW(1));
+      }
 #pragma unroll
       for (int gy = 0; gy < 3; gy++) {
-        const T v = av0 + T(gP1(gy)) * avd;
-        const T cq = (gy == 1 ? w1 : w0) * (v * v + eta);
+        T cq;
+        if constexpr (STORED) cq = Ql[gx][gy];
+        else {
+          const T v = av0 + T(gP1(gy)) * avd;
+          cq = (gy == 1 ? w1 : w0) * (v * v + eta);
+        }
         const T H00 = Gx[0][gy], H01 = Gy[0], H02 = az0[0] + T(gP1(gy)) * azd[0];
         const T H10 = Gx[1][gy], H11 = Gy[1], H12 = az0[1] + T(gP1(gy)) * azd[1];
         const T H20 = Gx[2][gy], H21 = Gy[2], H22 = az0[2] + T(gP1(gy)) * azd[2];
@@ -328,9 +351,29 @@ __device__ __forceinline__ void u_elem_apply_lds(const XS &xs, T cE, T lam1, T m
           const T vx = (i ? Rx[c][k][j] : -Rx[c][k][j]) * ihx;
           const T vy = (j ? Ry[c][k][i] : -Ry[c][k][i]) * ihy;
           const T vz = k ? Rz[j][i] : -Rz[j][i];
-          r[(k * 2 + j) * 2 + i][c] = vx + vy + vz;
+          out((k * 2 + j) * 2 + i, c, vx + vy + vz);
         }
   }
+}
+
+// recomputing form: s_g = v_g^2 + eta from the staged nodal v (field 3)
+template <typename T, typename XS, typename OUT>
+__device__ __forceinline__ void u_elem_apply_lds_out(const XS &xs, T cE, T lam1, T mu1, T eta, T ihx, T ihy, T ihz,
+                                                     const OUT &out) {
+  u_elem_lds_impl<T, XS, OUT, 0>(xs, cE, lam1, mu1, eta, ihx, ihy, ihz, out, nullptr);
+}
+// stored form: the 27 weights W_g s_g E_e vol at QS[g * QSTRIDE]
+template <int QSTRIDE, typename T, typename XS, typename OUT>
+__device__ __forceinline__ void u_elem_apply_lds_stored(const XS &xs, T lam1, T mu1, T ihx, T ihy, T ihz,
+                                                        const OUT &out, const T *__restrict__ QS) {
+  static_assert(QSTRIDE > 0, "stored weights need their stride");
+  u_elem_lds_impl<T, XS, OUT, QSTRIDE>(xs, T(0), lam1, mu1, T(0), ihx, ihy, ihz, out, QS);
+}
+// the same with the 24 outputs in registers
+template <typename T, typename XS>
+__device__ __forceinline__ void u_elem_apply_lds(const XS &xs, T cE, T lam1, T mu1, T eta, T ihx, T ihy, T ihz,
+                                                 T (&r)[8][3]) {
+  u_elem_apply_lds_out<T>(xs, cE, lam1, mu1, eta, ihx, ihy, ihz, [&](int a, int c, T v) { r[a][c] = v; });
 }
 
 template <typename T>
@@ -367,6 +410,12 @@ struct UApplyOp {
     static_assert(!UNI, "the NOCOMPRESSION apply uses the register march");
     u_elem_apply_lds<T>(xs, a * vol, lam1, mu1, eta, ihx, ihy, ihz, r);
   }
+  // the element's outputs handed to out(corner, component, value) as they are formed
+  template <typename XS, typename A, typename OUT>
+  __device__ void element_out(int, int, int, const XS &xs, A a, const OUT &out) const {
+    static_assert(!UNI, "the NOCOMPRESSION apply uses the register march");
+    u_elem_apply_lds_out<T>(xs, a * vol, lam1, mu1, eta, ihx, ihy, ihz, out);
+  }
   __device__ void store(int i, int j, int k, const T (&acc)[3], const T (&own)[4]) const {
     int64_t n = node_id(G, i, j, k);
 #pragma unroll
@@ -379,6 +428,44 @@ struct UApplyOp {
 template <typename T, int TX, int TY, bool UNI = false>
 __global__ __launch_bounds__(TX *TY, sizeof(T) == 4 ? 2 : 1) void k_u_apply(UApplyOp<T, UNI> op, int OZ) {
   march<T, TX, TY>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// FP32 multigrid apply fused with its Chebyshev(1)-Jacobi step (mg_cheb_entry
+// with one step, vfg_solve.hip's V-cycle):
+//   PRE  (from x = 0): x = c2 D^-1 b staged on the fly, r = b - A x stored
+//        (0 on Dirichlet rows, which the restriction skips) with x;
+//   POST: xo = x + c2 D^-1 (b - A x), A x = x on Dirichlet rows, into a
+//        buffer other than x (neighbouring tiles still stage x).
+enum { MG_PRE = 0, MG_POST = 1 };
+template <int MODE>
+struct UApplySmoothOp : UApplyOp<float, false> {
+  const float *b, *dinv;
+  float *xo, *ro;
+  float c2;
+  __device__ float load(int f, int64_t n) const {
+    if (f == 3) return v[n];
+    if constexpr (MODE == MG_PRE) return c2 * dinv[3 * n + f] * b[3 * n + f];
+    else return x[3 * n + f];
+  }
+  __device__ void store(int i, int j, int k, const float (&acc)[3], const float (&own)[4]) const {
+    const int64_t n = node_id(G, i, j, k);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const bool dir = on_dirichlet(G, bc.face[c], bc.vert[c], i, j, k);
+      const int64_t q = 3 * n + c;
+      if constexpr (MODE == MG_PRE) {
+        xo[q] = own[c];
+        ro[q] = dir ? 0.f : b[q] - acc[c];
+      } else {
+        const float yv = dir ? own[c] : acc[c];
+        xo[q] = own[c] + c2 * dinv[q] * (b[q] - yv);
+      }
+    }
+  }
+};
+template <int MODE, int TX, int TY>
+__global__ __launch_bounds__(TX *TY, 2) void k_u_apply_smooth(UApplySmoothOp<MODE> op, int OZ) {
+  march<float, TX, TY>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 // FP64 apply with the corners in LDS: at most 256 registers, two waves per
@@ -408,6 +495,87 @@ struct UApplyDotOp : UApplyOp<double, false> {
 __global__ __launch_bounds__(256, 2) void k_u_apply_lds_dot(UApplyDotOp op, int OZ) {
   march_lds<double, 16, 16, false>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z,
                                    blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
+}
+
+// ------------------------------------------------ stored-form FP64 apply
+// The stored form (variant B): q_g = W_g (v_g^2 + eta) E_e vol for the 27
+// Gauss points of every element, the integrand weights of
+// VF_BilinearFormU3D_local (VFMech.c:240-252), computed once per coefficient
+// state (k_u_qsetup_lds) so that the apply stages only x.
+// The stored form on the node-owned march (march_lds): 16 x 16 element slots
+// per tile, (15 bx - 1 + tx, 15 by - 1 + ty), halo elements stored in every
+// tile that computes them: [tile][element layer][27][256], the layout of the
+// V operator's weights.
+template <bool DOT>
+struct UApplyLdsqOp {
+  static constexpr int NF = 3, NC = 3;
+  const double *x;
+  double *y, *part;
+  const double *uq;
+  Grid G;
+  BcMask bc;
+  double lam1, mu1, ihx, ihy, ihz;
+  __device__ double load(int f, int64_t n) const { return x[3 * n + f]; }
+  template <typename XS>
+  __device__ void element_lds(int, int, int ek, const XS &xs, double (&r)[8][3]) const {
+    const int tile = blockIdx.x + gridDim.x * blockIdx.y;
+    const double *q = uq + ((int64_t)tile * G.nez + ek) * 27 * 256 + threadIdx.x;
+    u_elem_apply_lds_stored<256, double>(xs, lam1, mu1, ihx, ihy, ihz, [&](int a, int c, double v) { r[a][c] = v; },
+                                         q);
+  }
+  __device__ auto store(int i, int j, int k, const double (&acc)[3], const double (&own)[3]) const {
+    const int64_t n = node_id(G, i, j, k);
+    double d = 0.;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const double yv = on_dirichlet(G, bc.face[c], bc.vert[c], i, j, k) ? own[c] : acc[c];
+      y[3 * n + c] = yv;
+      d += own[c] * yv;
+    }
+    if constexpr (DOT) return d;
+  }
+};
+struct UApplyLdsqDotOp : UApplyLdsqOp<true> {
+  __device__ void partial(int slot, double v) const { part[slot] = v; }
+};
+__global__ __launch_bounds__(256, 2) void k_u_apply_ldsq(UApplyLdsqOp<false> op, int OZ) {
+  march_lds<double, 16, 16, false>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+__global__ __launch_bounds__(256, 2) void k_u_apply_ldsq_dot(UApplyLdsqDotOp op, int OZ) {
+  march_lds<double, 16, 16, false>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z,
+                                   blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
+}
+// q of the node-owned tiles: block (bx, by, ek), thread = slot
+__global__ __launch_bounds__(256) void k_u_qsetup_lds(Grid G, const double *__restrict__ V,
+                                                      const double *__restrict__ E_elem, double E, double eta,
+                                                      double *__restrict__ uq) {
+  const int tid = threadIdx.x;
+  const int ex = blockIdx.x * 15 - 1 + tid % 16, ey = blockIdx.y * 15 - 1 + tid / 16, ek = blockIdx.z;
+  double *q = uq + ((int64_t)(blockIdx.x + gridDim.x * blockIdx.y) * G.nez + ek) * 27 * 256 + tid;
+  if (ex < 0 || ex >= G.nex || ey < 0 || ey >= G.ney) {
+    for (int g = 0; g < 27; g++) q[g * 256] = 0.;
+    return;
+  }
+  double v[8];
+  for (int a = 0; a < 8; a++) v[a] = V[node_id(G, ex + (a & 1), ey + ((a >> 1) & 1), ek + (a >> 2))];
+  const double cE = (E_elem ? E_elem[cell_id(G, ex, ey, ek)] : E) * (G.hx * G.hy * G.hz);
+  for (int gz = 0; gz < 3; gz++)
+    for (int gy = 0; gy < 3; gy++)
+      for (int gx = 0; gx < 3; gx++) {
+        double vg = 0.;
+        for (int a = 0; a < 8; a++)
+          vg += ((a & 1) ? gP1(gx) : gP0(gx)) * (((a >> 1) & 1) ? gP1(gy) : gP0(gy)) * ((a >> 2) ? gP1(gz) : gP0(gz)) *
+                v[a];
+        q[((gz * 3 + gy) * 3 + gx) * 256] = cE * gW(gz) * gW(gx) * gW(gy) * (vg * vg + eta);
+      }
+}
+
+
+int64_t uq_doubles(const Grid &G) { return (int64_t)((G.nx + 14) / 15) * ((G.ny + 14) / 15) * G.nez * 27 * 256; }
+void launch_u_qsetup(const DevProblem &P, double *uq, hipStream_t s) {
+  const dim3 g((P.G.nx + 14) / 15, (P.G.ny + 14) / 15, P.G.nez);
+  hipLaunchKernelGGL(k_u_qsetup_lds, g, dim3(256), 0, s, P.G, (const double *)P.V, (const double *)P.E_elem, P.E,
+                     P.eta, uq);
 }
 
 // ---------------------------------------------------------------- diagonal
@@ -623,6 +791,156 @@ struct URhsOp {
   }
 };
 
+// Load vector of one element under UNILATERAL_NONE, sum-factorised over the
+// Gauss-point walk (v with its gradient, dth, p):
+//   thermo-poro (VFMech.c:386-447): b[a][c] += sum_g W d_c phi_a T,
+//     T = (3 lam + 2 mu) alpha dth (v^2 + eta) + beta p v  -- the x-, y- and
+//     z-derivative projections of one scalar: T summed over gx per (gy, gz),
+//     over gy per (gx, gz), over gz per (gx, gy);
+//   crack pressure (VFMech.c:533-589): b[a][c] += sum_g W p (d_c v) phi_a --
+//     a value projection, summed over gx, then gy, then gz.
+// Every contribution is multiplied by sgn (-1 for the residual).
+__device__ __forceinline__ void u_loads_elem(const double (&fld)[3][8], double coefalpha, double beta, double eta,
+                                             double vol, bool crack, double ihx, double ihy, double ihz, double sgn,
+                                             double (&r)[8][3]) {
+  double Tx = 0., Ty[3] = {0., 0., 0.}, Tz[3][3], Rx[2][2] = {{0., 0.}, {0., 0.}}, Ry[2][2] = {{0., 0.}, {0., 0.}};
+  double Vx[3][2], Vy[3][2][2];
+#pragma unroll
+  for (int q = 0; q < 3; q++)
+#pragma unroll
+    for (int w = 0; w < 3; w++) Tz[q][w] = 0.;
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+      Vx[c][i] = 0.;
+#pragma unroll
+      for (int j = 0; j < 2; j++) Vy[c][j][i] = 0.;
+    }
+  gauss_walk<1, 2>(fld, ihx, ihy, ihz, [&](int gx, int gy, int gz, const double (&val)[3], const double (&gv)[1][3]) {
+    const double W = vol * gW(gx) * gW(gy) * gW(gz);
+    const double v = val[0];
+    const double T = sgn * (W * (coefalpha * val[1] * (v * v + eta) + beta * val[2] * v));
+    Tx += T;
+    Ty[gx] += T;
+    Tz[gy][gx] += T;
+    if (crack) {
+      const double cp = sgn * W * val[2];
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const double Gc = cp * gv[0][c];
+        Vx[c][0] += gP0(gx) * Gc;
+        Vx[c][1] += gP1(gx) * Gc;
+      }
+    }
+    if (gx == 2) {  // end of a row (gy, gz)
+#pragma unroll
+      for (int k = 0; k < 2; k++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) Rx[k][j] += (k ? gP1(gz) : gP0(gz)) * (j ? gP1(gy) : gP0(gy)) * Tx;
+      Tx = 0.;
+      if (crack) {
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+          for (int i = 0; i < 2; i++) {
+            Vy[c][0][i] += gP0(gy) * Vx[c][i];
+            Vy[c][1][i] += gP1(gy) * Vx[c][i];
+            Vx[c][i] = 0.;
+          }
+      }
+    }
+    if (gx == 2 && gy == 2) {  // end of a layer gz
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+#pragma unroll
+        for (int k = 0; k < 2; k++)
+#pragma unroll
+          for (int i = 0; i < 2; i++) Ry[k][i] += (k ? gP1(gz) : gP0(gz)) * (i ? gP1(q) : gP0(q)) * Ty[q];
+        Ty[q] = 0.;
+      }
+      if (crack) {
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+          for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int i = 0; i < 2; i++) {  // straight into the corners (j, i) of both node planes
+              r[(0 * 2 + j) * 2 + i][c] += gP0(gz) * Vy[c][j][i];
+              r[(1 * 2 + j) * 2 + i][c] += gP1(gz) * Vy[c][j][i];
+              Vy[c][j][i] = 0.;
+            }
+      }
+    }
+  });
+#pragma unroll
+  for (int a = 0; a < 8; a++) {
+    const int i = a & 1, j = (a >> 1) & 1, k = a >> 2;
+    double rz = 0.;
+#pragma unroll
+    for (int gy = 0; gy < 3; gy++)
+#pragma unroll
+      for (int gx = 0; gx < 3; gx++) rz += (j ? gP1(gy) : gP0(gy)) * (i ? gP1(gx) : gP0(gx)) * Tz[gy][gx];
+    r[a][0] += (i ? ihx : -ihx) * Rx[k][j];
+    r[a][1] += (j ? ihy : -ihy) * Ry[k][i];
+    r[a][2] += (k ? ihz : -ihz) * rz;
+  }
+}
+
+// The load vector alone (UNILATERAL_NONE: the loads do not depend on U):
+// mode 0 stores b; mode 1 finishes the residual F = K U - b over an output
+// that already holds K U (the FP64 apply), and writes the Dirichlet rows'
+// ResidualApplyDirichletBC values from U (read there only).
+struct ULoadsOp {
+  static constexpr int NF = 3, NC = 3;  // v, dth, p
+  const double *U, *v, *dth, *p, *BCU;
+  double *out;
+  const double *E_elem;
+  Grid G;
+  BcMask bc;
+  uint8_t ftype[3][6];
+  uint8_t vtype[3][8];
+  double E, lam1, mu1, eta, vol, alpha, beta;
+  int crack, mode;
+  __device__ double load(int f, int64_t n) const { return f == 0 ? v[n] : f == 1 ? dth[n] : p[n]; }
+  __device__ double aux(int ex, int ey, int ek) const { return E_elem ? E_elem[cell_id(G, ex, ey, ek)] : E; }
+  __device__ void element(int, int, int, const double (&X)[2][4][3], double Ee, double (&r)[8][3]) const {
+    double fld[3][8];
+#pragma unroll
+    for (int a = 0; a < 8; a++)
+#pragma unroll
+      for (int f = 0; f < 3; f++) fld[f][a] = X[a >> 2][a & 3][f];
+    const double coefalpha = (3. * lam1 + 2. * mu1) * Ee * alpha;
+    u_loads_elem(fld, coefalpha, beta, eta, vol, crack != 0, 1. / G.hx, 1. / G.hy, 1. / G.hz, mode == 1 ? -1. : 1., r);
+  }
+  __device__ void store(int i, int j, int k, const double (&acc)[3], const double (&)[3]) const {
+    const int64_t n = node_id(G, i, j, k);
+    for (int c = 0; c < 3; c++) {
+      double val = acc[c];
+      if (mode == 1) {
+        if (on_dirichlet(G, bc.face[c], bc.vert[c], i, j, k)) {
+          int t = 0;
+          const bool onf[6] = {i == 0, i == G.nx - 1, j == 0, j == G.ny - 1, k == 0, k == G.nz - 1};
+          for (int f = 0; f < 6; f++)
+            if (ftype[c][f] && onf[f]) t = ftype[c][f];
+          int ix = (i == 0) ? 0 : (i == G.nx - 1) ? 1 : -1;
+          int iy = (j == 0) ? 0 : (j == G.ny - 1) ? 1 : -1;
+          int iz = (k == 0) ? 0 : (k == G.nz - 1) ? 1 : -1;
+          if (ix >= 0 && iy >= 0 && iz >= 0 && vtype[c][ix + 2 * iy + 4 * iz]) t = vtype[c][ix + 2 * iy + 4 * iz];
+          const double u = U[3 * n + c];
+          val = t == 3 ? u - BCU[3 * n + c] : t == 2 ? u - 1. : u;
+        } else {
+          val = out[3 * n + c] + acc[c];  // K U - b
+        }
+      }
+      out[3 * n + c] = val;
+    }
+  }
+};
+__global__ __launch_bounds__(256) void k_u_loads(ULoadsOp op, int OZ) {
+  march<double, 16, 16>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
 template <bool UNI>
 __global__ __launch_bounds__(256) void k_u_rhs(URhsOp<UNI> op, int OZ) {
   march<double, 16, 16>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z);
@@ -684,6 +1002,16 @@ static void launch_u_apply_u(const DevProblem &P, const T *x, T *y, const T *v, 
   op.ihx = T(1. / P.G.hx); op.ihy = T(1. / P.G.hy); op.ihz = T(1. / P.G.hz);
   int tiles = ((P.G.nx + 14) / 15) * ((P.G.ny + 14) / 15);
   if constexpr (sizeof(T) == 8 && !UNI) {
+    if (P.uq) {
+      UApplyLdsqOp<false> q;
+      q.x = (const double *)x; q.y = (double *)y; q.part = nullptr; q.uq = P.uq;
+      q.G = P.G; q.bc = P.bcU; q.lam1 = P.lam1; q.mu1 = P.mu1;
+      q.ihx = 1. / P.G.hx; q.ihy = 1. / P.G.hy; q.ihz = 1. / P.G.hz;
+      static int lqslots = device_slots(k_u_apply_ldsq, 256);
+      const int OZ = choose_oz(P.G, tiles, lqslots);
+      hipLaunchKernelGGL(k_u_apply_ldsq, march_grid(P.G, 16, 16, OZ), dim3(256), 0, s, q, OZ);
+      return;
+    }
     static int slots = device_slots(k_u_apply_lds, 256);
     int OZ = choose_oz(P.G, tiles, slots);
     hipLaunchKernelGGL(k_u_apply_lds, march_grid(P.G, 16, 16, OZ), dim3(256), 0, s, op, OZ);
@@ -870,6 +1198,18 @@ int launch_u_apply_dot(const DevProblem &P, const double *x, double *y, double *
   op.vol = P.G.hx * P.G.hy * P.G.hz;
   op.ihx = 1. / P.G.hx; op.ihy = 1. / P.G.hy; op.ihz = 1. / P.G.hz;
   op.part = part;
+  if (P.uq) {
+    UApplyLdsqDotOp q;
+    q.x = x; q.y = y; q.part = part; q.uq = P.uq;
+    q.G = P.G; q.bc = P.bcU; q.lam1 = P.lam1; q.mu1 = P.mu1;
+    q.ihx = 1. / P.G.hx; q.ihy = 1. / P.G.hy; q.ihz = 1. / P.G.hz;
+    static int lqslots = device_slots(k_u_apply_ldsq_dot, 256);
+    const int tiles = ((P.G.nx + 14) / 15) * ((P.G.ny + 14) / 15);
+    const int OZ = choose_oz(P.G, tiles, lqslots);
+    const dim3 g = march_grid(P.G, 16, 16, OZ);
+    hipLaunchKernelGGL(k_u_apply_ldsq_dot, g, dim3(256), 0, s, q, OZ);
+    return (int)(g.x * g.y * g.z);
+  }
   static int slots = device_slots(k_u_apply_lds_dot, 256);
   const int tiles = ((P.G.nx + 14) / 15) * ((P.G.ny + 14) / 15);
   const int OZ = choose_oz(P.G, tiles, slots);
@@ -879,6 +1219,37 @@ int launch_u_apply_dot(const DevProblem &P, const double *x, double *y, double *
 }
 void launch_u_apply_f32(const DevProblem &P, const float *x, float *y, hipStream_t s) {
   launch_u_apply_t<float>(P, x, y, P.V32, s);
+}
+
+template <int MODE>
+static void launch_u_smooth_t(const DevProblem &P, const float *b, const float *dinv, float c2, const float *x,
+                              float *xo, float *ro, hipStream_t s) {
+  UApplySmoothOp<MODE> op;
+  op.x = x; op.y = nullptr; op.v = P.V32;
+  op.E_elem = P.E_elem;
+  op.cmask = nullptr;
+  op.G = P.G; op.bc = P.bcU;
+  op.E = float(P.E); op.lam1 = float(P.lam1); op.mu1 = float(P.mu1); op.eta = float(P.eta);
+  op.vol = float(P.G.hx * P.G.hy * P.G.hz);
+  op.ihx = float(1. / P.G.hx); op.ihy = float(1. / P.G.hy); op.ihz = float(1. / P.G.hz);
+  op.b = b; op.dinv = dinv; op.xo = xo; op.ro = ro; op.c2 = c2;
+  if (P.G.nx >= SMALL_NX_LO && P.G.nx <= SMALL_NX_HI) {
+    static int slots = device_slots(k_u_apply_smooth<MODE, 8, 8>, 64);
+    const int OZ = choose_oz(P.G, ((P.G.nx + 6) / 7) * ((P.G.ny + 6) / 7), slots);
+    hipLaunchKernelGGL((k_u_apply_smooth<MODE, 8, 8>), march_grid(P.G, 8, 8, OZ), dim3(64), 0, s, op, OZ);
+    return;
+  }
+  static int slots = device_slots(k_u_apply_smooth<MODE, 16, 16>, 256);
+  const int OZ = choose_oz(P.G, ((P.G.nx + 14) / 15) * ((P.G.ny + 14) / 15), slots);
+  hipLaunchKernelGGL((k_u_apply_smooth<MODE, 16, 16>), march_grid(P.G, 16, 16, OZ), dim3(256), 0, s, op, OZ);
+}
+void launch_mg_presmooth(const DevProblem &P, const float *b, const float *dinv, float c2, float *x, float *r,
+                         hipStream_t s) {
+  launch_u_smooth_t<MG_PRE>(P, b, dinv, c2, nullptr, x, r, s);
+}
+void launch_mg_postsmooth(const DevProblem &P, const float *b, const float *dinv, float c2, const float *x, float *xo,
+                          hipStream_t s) {
+  launch_u_smooth_t<MG_POST>(P, b, dinv, c2, x, xo, nullptr, s);
 }
 
 void launch_u_diag(const DevProblem &P, double *d, hipStream_t s) {
@@ -914,8 +1285,26 @@ static void launch_u_rhs_t(const DevProblem &P, const double *U, const double *B
 
 void launch_u_rhs(const DevProblem &P, const double *U, const double *BCU, double *out, int mode,
                   hipStream_t s) {
-  if (P.unilateral) launch_u_rhs_t<true>(P, U, BCU, out, mode, s);
-  else launch_u_rhs_t<false>(P, U, BCU, out, mode, s);
+  if (P.unilateral) {
+    launch_u_rhs_t<true>(P, U, BCU, out, mode, s);
+    return;
+  }
+  // UNILATERAL_NONE: K U by the FP64 apply, then the loads
+  if (mode == 1) launch_u_apply(P, U ? U : P.zero3, out, s);
+  ULoadsOp op;
+  op.U = U ? U : P.zero3; op.v = P.V; op.dth = P.dth; op.p = P.p; op.BCU = BCU;
+  op.out = out; op.E_elem = P.E_elem; op.G = P.G; op.bc = P.bcU;
+  for (int c = 0; c < 3; c++) {
+    for (int f = 0; f < 6; f++) op.ftype[c][f] = (uint8_t)P.hbcU[c].face[f];
+    for (int v = 0; v < 8; v++) op.vtype[c][v] = (uint8_t)P.hbcU[c].vertex[v];
+  }
+  op.E = P.E; op.lam1 = P.lam1; op.mu1 = P.mu1; op.eta = P.eta;
+  op.vol = P.G.hx * P.G.hy * P.G.hz; op.alpha = P.alpha; op.beta = P.beta;
+  op.crack = P.hasCrackPressure; op.mode = mode;
+  const int tiles = ((P.G.nx + 14) / 15) * ((P.G.ny + 14) / 15);
+  static int slots = device_slots(k_u_loads, 256);
+  const int OZ = choose_oz(P.G, tiles, slots);
+  hipLaunchKernelGGL(k_u_loads, march_grid(P.G, 16, 16, OZ), dim3(256), 0, s, op, OZ);
 }
 
 }  // namespace vfg
@@ -927,11 +1316,12 @@ namespace vfg {
 // Persistent walk over a work list.  dcount (optional) holds the item count on
 // the device, so a captured graph follows a list that changes between replays;
 // flags (optional) skip items of members whose flags[p * fstride] is 0.
-template <typename T>
-__global__ __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) void k_batch_u_apply(const BApply<T> *__restrict__ probs,
-                                                       const BItem *__restrict__ items, int nitems,
-                                                       const int *__restrict__ dcount,
-                                                       const double *__restrict__ flags, int fstride) {
+// Schedules (the autotuner's choice): TX x TY element tiles of (TX-1) x (TY-1)
+// owned node columns, on the register march or the LDS-ring march.
+template <typename T, int TX, int TY, bool LDS>
+__global__ __launch_bounds__(TX *TY, (sizeof(T) == 4 || LDS) ? 2 : 1) void k_batch_u_apply(
+    const BApply<T> *__restrict__ probs, const BItem *__restrict__ items, int nitems, const int *__restrict__ dcount,
+    const double *__restrict__ flags, int fstride) {
   const int n = dcount ? *dcount : nitems;
   for (int it = blockIdx.x; it < n; it += gridDim.x) {
     const BItem w = items[it];
@@ -939,19 +1329,32 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) void k_batch_u_apply(c
     const UApplyOp<T> op = probs[w.p].op;
     const int OZ = probs[w.p].OZ;
     __syncthreads();  // LDS of the previous item is free
-    march<T, 16, 16>(op.G, op, OZ, w.bx, w.by, w.bz);
+    if constexpr (LDS) march_lds<T, TX, TY, false>(op.G, op, OZ, w.bx, w.by, w.bz);
+    else march<T, TX, TY>(op.G, op, OZ, w.bx, w.by, w.bz);
   }
+}
+
+// schedule -> tile width and threads: 0: 16 x 16 register march, 1: 8 x 8
+// register march, 2: 16 x 16 LDS-ring march
+int batch_tile(int sched) { return sched == 1 ? 8 : 16; }
+template <typename T>
+static int batch_slots_t(int sched) {
+  if (sched == 1) return device_slots(k_batch_u_apply<T, 8, 8, false>, 64);
+  if (sched == 2) return device_slots(k_batch_u_apply<T, 16, 16, true>, 256);
+  return device_slots(k_batch_u_apply<T, 16, 16, false>, 256);
+}
+int batch_slots_s(int fp32, int sched) {
+  static int cache[2][BATCH_SCHEDS] = {{0}};
+  int &c = cache[fp32 ? 1 : 0][sched];
+  if (!c) c = fp32 ? batch_slots_t<float>(sched) : batch_slots_t<double>(sched);
+  return c;
 }
 
 size_t batch_op_bytes(int fp32) { return fp32 ? sizeof(BApply<float>) : sizeof(BApply<double>); }
 
 int choose_oz_pub(const Grid &G, int tiles_xy, int slots) { return choose_oz(G, tiles_xy, slots); }
 
-int batch_slots() {
-  static int s = 0;
-  if (!s) s = device_slots(k_batch_u_apply<double>, 256);
-  return s;
-}
+int batch_slots() { return batch_slots_s(0, 0); }
 
 template <typename T>
 static void fill_ops_t(const std::vector<vfg_handle *> &m, const void *const *x, void *const *y,
@@ -981,24 +1384,30 @@ void fill_ops(const std::vector<vfg_handle *> &m, const void *const *x, void *co
   else fill_ops_t<double>(m, x, y, oz, d_ops, s);
 }
 
-void launch_batch_apply(int fp32, const void *d_ops, const BItem *items, int nitems, int grid, hipStream_t s) {
-  int g = grid > 0 && grid < nitems ? grid : nitems;
-  if (fp32)
-    hipLaunchKernelGGL(k_batch_u_apply<float>, dim3(g), dim3(256), 0, s, (const BApply<float> *)d_ops, items, nitems,
-                       (const int *)nullptr, (const double *)nullptr, 0);
-  else
-    hipLaunchKernelGGL(k_batch_u_apply<double>, dim3(g), dim3(256), 0, s, (const BApply<double> *)d_ops, items,
-                       nitems, (const int *)nullptr, (const double *)nullptr, 0);
+void launch_batch_apply(int fp32, int sched, const void *d_ops, const BItem *items, int nitems, int grid,
+                        hipStream_t s) {
+  const int g = grid > 0 && grid < nitems ? grid : nitems;
+  launch_engine_u_apply(fp32, sched, d_ops, items, nitems, nullptr, g, nullptr, 0, s);
 }
 
-void launch_engine_u_apply(int fp32, const void *d_ops, const BItem *items, const int *dcount, int grid,
+template <typename T>
+static void launch_batch_t(int sched, const void *d_ops, const BItem *items, int nitems, const int *dcount, int grid,
                            const double *flags, int fstride, hipStream_t s) {
-  if (fp32)
-    hipLaunchKernelGGL(k_batch_u_apply<float>, dim3(grid), dim3(256), 0, s, (const BApply<float> *)d_ops, items, 0,
-                       dcount, flags, fstride);
+  const BApply<T> *ops = (const BApply<T> *)d_ops;
+  if (sched == 1)
+    hipLaunchKernelGGL((k_batch_u_apply<T, 8, 8, false>), dim3(grid), dim3(64), 0, s, ops, items, nitems, dcount,
+                       flags, fstride);
+  else if (sched == 2)
+    hipLaunchKernelGGL((k_batch_u_apply<T, 16, 16, true>), dim3(grid), dim3(256), 0, s, ops, items, nitems, dcount,
+                       flags, fstride);
   else
-    hipLaunchKernelGGL(k_batch_u_apply<double>, dim3(grid), dim3(256), 0, s, (const BApply<double> *)d_ops, items, 0,
-                       dcount, flags, fstride);
+    hipLaunchKernelGGL((k_batch_u_apply<T, 16, 16, false>), dim3(grid), dim3(256), 0, s, ops, items, nitems, dcount,
+                       flags, fstride);
+}
+void launch_engine_u_apply(int fp32, int sched, const void *d_ops, const BItem *items, int nitems,
+                           const int *dcount, int grid, const double *flags, int fstride, hipStream_t s) {
+  if (fp32) launch_batch_t<float>(sched, d_ops, items, nitems, dcount, grid, flags, fstride, s);
+  else launch_batch_t<double>(sched, d_ops, items, nitems, dcount, grid, flags, fstride, s);
 }
 
 // one member's operator entry for a persistent list (host memory `out`)

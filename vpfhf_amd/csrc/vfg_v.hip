@@ -54,35 +54,35 @@ static VConst make_vconst(const DevProblem &P) {
 }
 
 // -------------------------------------------------------------- setup: Vq
+// One block per (tile, element layer), one thread per tile slot: element
+// (15 bx - 1 + tx, 15 by - 1 + ty) of the V operator's tile-blocked layout
+// (halo elements are computed by every tile that holds them), so each
+// coefficient is written as one aligned, contiguous 512-byte segment per
+// wavefront.  Gauss-point values by sum factorisation (gauss_walk).
 __global__ __launch_bounds__(256) void k_v_setup(DevProblem P, const double *__restrict__ U) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= P.ne) return;
   const Grid &G = P.G;
-  const int ei = (int)(e % G.nex), ej = (int)((e / G.nex) % G.ney), ek = (int)(e / ((int64_t)G.nex * G.ney));
-  double u[8][3], th[8];
+  const int tid = threadIdx.x;
+  const int ei = blockIdx.x * 15 - 1 + tid % 16, ej = blockIdx.y * 15 - 1 + tid / 16, ek = blockIdx.z;
+  const int tile = blockIdx.x + gridDim.x * blockIdx.y;
+  double *out = P.Vq + tiled_index(tile, ek, G.nez, 27, tid, 0, 256);
+  if (ei < 0 || ei >= G.nex || ej < 0 || ej >= G.ney) {
+    for (int g = 0; g < 27; g++) out[g * 256] = 0.;
+    return;
+  }
+  const int64_t e = cell_id(G, ei, ej, ek);
+  double fld[4][8];  // u_c (values and gradients), theta - thetaRef (values)
   for (int a = 0; a < 8; a++) {
     int64_t n = node_id(G, ei + (a & 1), ej + ((a >> 1) & 1), ek + (a >> 2));
-    for (int c = 0; c < 3; c++) u[a][c] = U[3 * n + c];
-    th[a] = P.dth[n];
+    for (int c = 0; c < 3; c++) fld[c][a] = U[3 * n + c];
+    fld[3][a] = P.dth[n];
   }
-  double Ee = P.E_elem ? P.E_elem[e] : P.E;
-  double lam = P.lam1 * Ee, mu = P.mu1 * Ee;
-  const double ih[3] = {1. / G.hx, 1. / G.hy, 1. / G.hz};
+  const double Ee = P.E_elem ? P.E_elem[e] : P.E;
+  const double lam = P.lam1 * Ee, mu = P.mu1 * Ee;
   const double vol = G.hx * G.hy * G.hz;
   const double cm = P.atnum == 2 ? P.Gc / P.atcv * .5 / P.eps : 0.;
-  double q[27];
-  for (int g = 0; g < 27; g++) {
-    int gx = g % 3, gy = (g / 3) % 3, gz = g / 9;
-    double H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, t = 0.;
-    for (int a = 0; a < 8; a++) {
-      int i = a & 1, j = (a >> 1) & 1, k = a >> 2;
-      double px = i ? gP1(gx) : gP0(gx), py = j ? gP1(gy) : gP0(gy), pz = k ? gP1(gz) : gP0(gz);
-      double dg[3] = {(i ? ih[0] : -ih[0]) * py * pz, (j ? ih[1] : -ih[1]) * px * pz,
-                      (k ? ih[2] : -ih[2]) * px * py};
-      for (int c = 0; c < 3; c++)
-        for (int d = 0; d < 3; d++) H[c][d] += dg[d] * u[a][c];
-      t += px * py * pz * th[a];
-    }
+  gauss_walk<3, 1>(fld, 1. / G.hx, 1. / G.hy, 1. / G.hz,
+                   [&](int gx, int gy, int gz, const double (&val)[4], const double (&H)[3][3]) {
+    const double t = val[3];
     double e11 = H[0][0] - P.alpha * t, e22 = H[1][1] - P.alpha * t, e33 = H[2][2] - P.alpha * t;
     double e12 = (H[1][0] + H[0][1]) * .5, e23 = (H[2][1] + H[1][2]) * .5, e13 = (H[2][0] + H[0][2]) * .5;
     double s11 = (lam + 2. * mu) * e11 + lam * e22 + lam * e33;
@@ -99,28 +99,9 @@ __global__ __launch_bounds__(256) void k_v_setup(DevProblem P, const double *__r
     } else {
       W = (s11 * e11 + s22 * e22 + s33 * e33) * .5 + 2. * mu * (e12 * e12 + e23 * e23 + e13 * e13);
     }
-    double Wg = vol * gW(gx) * gW(gy) * gW(gz);
-    q[g] = Wg * (cm + 2. * W);
-  }
-  // every workgroup tile whose element block contains this element: tile b
-  // covers elements 15 b - 1 ... 15 b + 14 in each of x and y
-  const int tiles_x = (G.nx + 14) / 15, tiles_y = (G.ny + 14) / 15;
-  int bxs[2], lxs[2], nbx = 0, bys[2], lys[2], nby = 0;
-  {
-    const int b = (ei + 1) / 15, l = (ei + 1) % 15;
-    if (b < tiles_x) { bxs[nbx] = b; lxs[nbx++] = l; }
-    if (l == 0 && b >= 1) { bxs[nbx] = b - 1; lxs[nbx++] = 15; }
-  }
-  {
-    const int b = (ej + 1) / 15, l = (ej + 1) % 15;
-    if (b < tiles_y) { bys[nby] = b; lys[nby++] = l; }
-    if (l == 0 && b >= 1) { bys[nby] = b - 1; lys[nby++] = 15; }
-  }
-  for (int a = 0; a < nbx; a++)
-    for (int c = 0; c < nby; c++) {
-      const int tile = bxs[a] + tiles_x * bys[c], slot = lys[c] * 16 + lxs[a];
-      for (int g = 0; g < 27; g++) P.Vq[tiled_index(tile, ek, G.nez, 27, slot, g, 256)] = q[g];
-    }
+    const double Wg = vol * gW(gx) * gW(gy) * gW(gz);
+    out[((gz * 3 + gy) * 3 + gx) * 256] = Wg * (cm + 2. * W);
+  });
 }
 
 int64_t vq_doubles(const Grid &G) {
@@ -496,31 +477,73 @@ struct VRhsOp {
   double crhs;
   int crack;
   __device__ double load(int f, int64_t n) const { return f < 3 ? U[3 * n + f] : p[n]; }
+  // AT part: c_rhs sum_g W_g phi_a(g) = c_rhs vol / 8 for every corner (the
+  // 3-point rule integrates phi_a exactly).  Crack-pressure part: the flux
+  // F(g) = W_g p_g u_g projected on grad phi_a, sum-factorised: the x-flux
+  // summed over gx per (gy, gz), the y-flux over gy per (gx, gz), the z-flux
+  // over gz per (gx, gy), each then projected on the two other axes.
   __device__ void element(int, int, int, const double (&X)[2][4][4], double (&r)[8][1]) const {
-    const double ih[3] = {1. / G.hx, 1. / G.hy, 1. / G.hz};
     const double vol = G.hx * G.hy * G.hz;
-    for (int g = 0; g < 27; g++) {
-      int gx = g % 3, gy = (g / 3) % 3, gz = g / 9;
-      double W = vol * gW(gx) * gW(gy) * gW(gz);
-      double ug[3] = {0., 0., 0.}, pg = 0.;
-      for (int a = 0; a < 8; a++) {
-        int i = a & 1, j = (a >> 1) & 1, k = a >> 2;
-        double ph = (i ? gP1(gx) : gP0(gx)) * (j ? gP1(gy) : gP0(gy)) * (k ? gP1(gz) : gP0(gz));
-        const double *xa = X[k][j * 2 + i];
-        for (int c = 0; c < 3; c++) ug[c] += ph * xa[c];
-        pg += ph * xa[3];
+    const double at = crhs * vol * .125;
+    if (!crack) {
+#pragma unroll
+      for (int a = 0; a < 8; a++) r[a][0] += at;
+      return;
+    }
+    double fld[4][8];
+#pragma unroll
+    for (int a = 0; a < 8; a++)
+#pragma unroll
+      for (int f = 0; f < 4; f++) fld[f][a] = X[a >> 2][a & 3][f];
+    double Rx[2][2] = {{0., 0.}, {0., 0.}}, Ry[2][2] = {{0., 0.}, {0., 0.}}, Sz[3][3], Sy[3], Sx = 0.;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      Sy[q] = 0.;
+#pragma unroll
+      for (int w = 0; w < 3; w++) Sz[q][w] = 0.;
+    }
+    gauss_walk<0, 4>(fld, 0., 0., 0., [&](int gx, int gy, int gz, const double (&val)[4], const double (&)[1][3]) {
+      const double wp = vol * gW(gx) * gW(gy) * gW(gz) * val[3];
+      Sx += wp * val[0];
+      Sy[gx] += wp * val[1];
+      Sz[gy][gx] += wp * val[2];
+      if (gx == 2) {  // end of a row: the x-flux of (gy, gz) onto phi_j(y) phi_k(z)
+#pragma unroll
+        for (int k = 0; k < 2; k++)
+#pragma unroll
+          for (int j = 0; j < 2; j++) Rx[k][j] += (k ? gP1(gz) : gP0(gz)) * (j ? gP1(gy) : gP0(gy)) * Sx;
+        Sx = 0.;
       }
-      for (int a = 0; a < 8; a++) {
-        int i = a & 1, j = (a >> 1) & 1, k = a >> 2;
-        double px = i ? gP1(gx) : gP0(gx), py = j ? gP1(gy) : gP0(gy), pz = k ? gP1(gz) : gP0(gz);
-        double val = W * px * py * pz * crhs;
-        if (crack) {
-          double dg[3] = {(i ? ih[0] : -ih[0]) * py * pz, (j ? ih[1] : -ih[1]) * px * pz,
-                          (k ? ih[2] : -ih[2]) * px * py};
-          val += W * pg * (ug[0] * dg[0] + ug[1] * dg[1] + ug[2] * dg[2]);
+      if (gx == 2 && gy == 2) {  // end of a layer: the y-flux of (gx, gz) onto phi_i(x) phi_k(z)
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+#pragma unroll
+          for (int k = 0; k < 2; k++)
+#pragma unroll
+            for (int i = 0; i < 2; i++) Ry[k][i] += (k ? gP1(gz) : gP0(gz)) * (i ? gP1(q) : gP0(q)) * Sy[q];
+          Sy[q] = 0.;
         }
-        r[a][0] += val;
       }
+    });
+    double Rz[2][2];  // [j][i]
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int i = 0; i < 2; i++) {
+        double acc = 0.;
+#pragma unroll
+        for (int gy = 0; gy < 3; gy++)
+#pragma unroll
+          for (int gx = 0; gx < 3; gx++) acc += (j ? gP1(gy) : gP0(gy)) * (i ? gP1(gx) : gP0(gx)) * Sz[gy][gx];
+        Rz[j][i] = acc;
+      }
+    const double ihx = 1. / G.hx, ihy = 1. / G.hy, ihz = 1. / G.hz;
+#pragma unroll
+    for (int a = 0; a < 8; a++) {
+      const int i = a & 1, j = (a >> 1) & 1, k = a >> 2;
+      const double vx = (i ? ihx : -ihx) * Rx[k][j], vy = (j ? ihy : -ihy) * Ry[k][i],
+                   vz = (k ? ihz : -ihz) * Rz[j][i];
+      r[a][0] += at + (vx + vy + vz);
     }
   }
   __device__ void store(int i, int j, int k, const double (&acc)[1], const double (&)[4]) const {
@@ -540,7 +563,7 @@ int choose_oz(const Grid &G, int tiles_xy, int slots);
 static int oz_for(const Grid &G) { return choose_oz(G, ((G.nx + 14) / 15) * ((G.ny + 14) / 15), 768); }
 
 void launch_v_setup(const DevProblem &P, const double *U, hipStream_t s) {
-  hipLaunchKernelGGL(k_v_setup, dim3((unsigned)((P.ne + 255) / 256)), dim3(256), 0, s, P, U);
+  hipLaunchKernelGGL(k_v_setup, dim3((P.G.nx + 14) / 15, (P.G.ny + 14) / 15, P.G.nez), dim3(256), 0, s, P, U);
   VRhsOp op;
   op.U = U; op.p = P.p; op.b = P.bV; op.G = P.G;
   op.crhs = make_vconst(P).crhs;

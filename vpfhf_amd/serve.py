@@ -21,8 +21,11 @@ from . import _lib, inputs, workloads
 from .host import Request, RequestResult, ServeOpts, ServeStats, load as load_host
 
 
-def spec_request(i, seed=0, lo=24, hi=96, am_range=(1, 20), max_cracks=3):
-    """Spec and iteration count of request i of stream `seed`."""
+def spec_request(i, seed=0, lo=24, hi=96, am_range=(1, 20), max_cracks=3, altmintol=None):
+    """Spec and iteration count of request i of stream `seed`.  altmintol
+    (the reference's -altmintol, default 1e-4) ends a request's loop early
+    once ||V - Vold||_inf falls below it; 0 runs every drawn iteration (the
+    do-while of VFCommon.c:1192-1231 then stops at its iteration cap)."""
     u = inputs.uniform(seed, 5000 + i, 5)
     n = int(round(math.exp(math.log(lo) + (math.log(hi) - math.log(lo)) * u[0])))
     sigma = 0.2 + 0.6 * u[1]
@@ -33,6 +36,8 @@ def spec_request(i, seed=0, lo=24, hi=96, am_range=(1, 20), max_cracks=3):
     s["pressure"] = ("const", p)
     s["beta"] = 0.0  # injection pressure on the crack faces (a uniform Biot load would cancel it)
     s["name"] = "request%d" % i
+    if altmintol is not None:
+        s["altmintol"] = float(altmintol)
     return s, am
 
 

@@ -138,6 +138,8 @@ def options_text(spec, u_rtol=1e-10, extra=""):
         lines.append("-pc%d_center %r,%r,%r -pc%d_r %r -pc%d_theta %r -pc%d_phi %r -pc%d_thickness %r" % (
             i, c["center"][0], c["center"][1], c["center"][2], i, c["r"], i, c["theta"], i, c["phi"], i,
             c["thickness"]))
+    if spec.get("altmintol") is not None:
+        lines.append("-altmintol %r" % spec["altmintol"])
     lines.append(CLAMPED_OPTIONS)
     lines.append(extra)
     return "\n".join(lines)
