@@ -299,7 +299,9 @@ def ensemble_bench(L, n_members=64, seed=0):
         reset(); batched(); reset()
         t0 = time.time(); batched(); tb = time.time() - t0
         Ub = download()
-        res[key] = (tb, max(_relerr(a, b) for a, b in zip(Ub, U64)))
+        res[key] = (tb, max(_relerr(a, b) for a, b in zip(Ub, U64)),
+                    {"inner_iterations_max": max(infos[i].iterations for i in range(n_members)),
+                     "outer_max": max(infos[i].outer for i in range(n_members))})
     bt.close()
     for H in hosts:
         H.close()
@@ -317,6 +319,7 @@ def ensemble_bench(L, n_members=64, seed=0):
             "speedup_mixed_multigrid_batched_vs_multigrid_single": round(tmg / res["mixed_mg"][0], 2),
             "max_rel_err_mixed_vs_fp64": res["mixed"][1], "max_rel_err_batched_fp64_vs_single": res["fp64"][1],
             "max_rel_err_mixed_multigrid_vs_fp64": res["mixed_mg"][1],
+            "iterations": {k: v[2] for k, v in res.items()},
             "persistent_fp32_apply_ms": round(apply_ms, 4), "autotune": tuned}
 
 
@@ -354,12 +357,14 @@ def precision_bench(n=256, seed=0):
     return res
 
 
-def serving_bench(rates=(50., 100., 200., 500.), n_req=200, seed=0, n_check=32):
+def serving_bench(rates=(10., 20., 50., 100., 200., 500.), n_req=200, seed=0, n_check=32, max_slots=64):
     """Serving form: a Poisson stream of n_req alternate-minimisation requests
     (sides 24-96 elements, 1-3 cracks, 1-20 iterations each, every drawn
-    iteration run: altmintol 0) at each rate, 64 slots, 200 GB HBM budget,
-    FP32 u-CG inside FP64 refinement.  The first n_check requests re-run on the
-    FP64 engine give the error of the mixed-precision results."""
+    iteration run: altmintol 0) at each rate, max_slots slots, 200 GB HBM
+    budget, FP32 u-CG inside FP64 refinement.  The rates of the workload are
+    50-500 requests/s; 10 and 20 are below the device's throughput for this
+    mix and show latency without a queue.  The first n_check requests re-run
+    on the FP64 engine give the error of the mixed-precision results."""
     from vpfhf_amd import serve
     R = serve.make_requests(n_req, rate=None, seed=seed, outputs=True, altmintol=0.0)
     base_t = serve.arrival_times(n_req, 1.0, seed)  # unit-rate arrivals, scaled per rate
@@ -368,7 +373,7 @@ def serving_bench(rates=(50., 100., 200., 500.), n_req=200, seed=0, n_check=32):
         for i in range(n_req):
             R.arr[i].arrival = float(base_t[i] / rate)
         t0 = time.time()
-        res, st = serve.serve(R, max_slots=64, hbm_budget=200e9, realtime=True)
+        res, st = serve.serve(R, max_slots=max_slots, hbm_budget=200e9, realtime=True)
         if k == 0:
             Um = [u.copy() for u in R.U[:n_check]]
             Vm = [v.copy() for v in R.V[:n_check]]
@@ -390,7 +395,7 @@ def serving_bench(rates=(50., 100., 200., 500.), n_req=200, seed=0, n_check=32):
     err_u = max(_relerr(a, b) for a, b in zip(Um, C.U))
     err_v = max(_relerr(a, b) for a, b in zip(Vm, C.V))
     sides = [s["n"] for s in R.specs]
-    return {"requests": n_req, "sides": {"min": min(sides), "max": max(sides), "mean": round(float(np.mean(sides)), 1)},
+    return {"requests": n_req, "max_slots": max_slots, "sides": {"min": min(sides), "max": max(sides), "mean": round(float(np.mean(sides)), 1)},
             "am_iterations": {"min": min(ams), "max": max(ams), "mean": round(float(np.mean(ams)), 2),
                               "total": int(sum(ams)),
                               "histogram": [int(sum(1 for a in ams if a == q)) for q in range(1, 21)]},
@@ -413,6 +418,7 @@ def main():
     ap.add_argument("--no-forms", action="store_true", help="skip the ensemble, precision and serving forms")
     ap.add_argument("--forms", default="ensemble,precision,serving", help="comma-separated subset of the forms")
     ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--serve-slots", type=int, default=64, help="resident requests of the serving form")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this path runs on one MI355X and has no multi-GPU decomposition "
@@ -519,7 +525,7 @@ def main():
     H.close()
     if not (a.no_solves or a.no_forms):
         for key, fn in (("ensemble", lambda: ensemble_bench(L)), ("precision", precision_bench),
-                        ("serving", lambda: serving_bench(n_req=a.serve_requests))):
+                        ("serving", lambda: serving_bench(n_req=a.serve_requests, max_slots=a.serve_slots))):
             if key not in a.forms.split(","):
                 continue
             t0 = time.time()

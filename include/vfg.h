@@ -169,8 +169,10 @@ int vfg_energies(vfg_handle *h, const double *U, const double *V, double out[4],
  * reference: it is for callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
  * U and V are device pointers, out is a host array; the reduction partials
- * live in a static 8 MiB device array per GPU, so grids are limited to
- * 67,108,864 cells (VFG_E_ARG beyond).  Calls that overlap on one GPU share
+ * live in a static 8 MiB device array per GPU, one group per workgroup of
+ * 16 x 16 element columns by 8 layers (2 on grids of fewer than 64 such
+ * columns), so grids are limited to 262,144 such blocks, 536,870,912 cells
+ * (VFG_E_ARG beyond).  Calls that overlap on one GPU share
  * that array and race; the call synchronizes the handle's stream.  Results
  * are bitwise those of vfg_energies. */
 int vfg_energies_static(vfg_handle *h, const double *U, const double *V, double out[4]);

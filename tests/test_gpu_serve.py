@@ -54,6 +54,28 @@ def test_serve_matches_single_problem_path(mixed):
         assert abs(e[4] - en["crack_volume"]) <= tol * abs(en["crack_volume"]) + 1e-18
 
 
+def test_serve_runs_every_iteration_at_altmintol_zero():
+    """With -altmintol 0 a request runs every drawn iteration (the do-while of
+    VFCommon.c:1192-1231 stops only at its cap).  The later iterations start
+    u-solves next to their solution, whose rtol * |r0| lies below the FP64
+    residual's rounding floor: the refinement stops there (an outer step that
+    no longer halves |r|) instead of failing at its step cap.  Every request
+    finishes with status 0 after its drawn count, and the mixed engine still
+    agrees with the single-problem FP64 path run for the same count."""
+    from vpfhf_amd import serve, workloads
+    R = serve.make_requests(4, seed=17, outputs=True, lo=8, hi=14, am_range=(6, 9), altmintol=0.0)
+    res, _ = serve.serve(R, max_slots=4, realtime=False)
+    for i, s in enumerate(R.specs):
+        assert res[i]["status"] == 0, (i, res[i])
+        assert res[i]["am_iterations"] == R.ams[i], (i, res[i]["am_iterations"], R.ams[i])
+    H, _ = workloads.make_host(R.specs[0], u_rtol=1e-10)
+    info = H.altmin(maxit=R.ams[0], stop_on_tol=True)
+    U, V, _ = H.state()
+    H.close()
+    assert info.status == 0 and info.iterations == R.ams[0]
+    assert relerr(R.U[0], U) < 1e-6 and relerr(R.V[0], V) < 1e-6
+
+
 def test_serve_graph_and_plain_launches_agree():
     from vpfhf_amd import serve
     R1 = _small(3, seed=9)

@@ -618,6 +618,14 @@ extern "C" int vfg_batch_u_solve(vfg_batch *b, double *const *U, const double *c
   const BVec *vt = b->d_vt;
   const BChunk *ch = b->d_ch;
   auto V64 = [&](int p, int slot) { return (double *)b->vt[p].v[slot]; };
+  std::vector<const void *> xs(np), xs32(np);
+  std::vector<void *> ys(np), ys32(np);
+  for (int p = 0; p < np; p++) {
+    xs[p] = b->vt[p].v[VP];
+    ys[p] = b->vt[p].v[VQ];
+    xs32[p] = b->vt[p].v[VP32];
+    ys32[p] = b->vt[p].v[VQ32];
+  }
   // per member: x = U with Dirichlet values, r = F(x), dinv = diag
   hipMemcpyKind kin = loc == VFG_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
   for (int p = 0; p < np; p++) {
@@ -630,14 +638,6 @@ extern "C" int vfg_batch_u_solve(vfg_batch *b, double *const *U, const double *c
     launch_u_diag(h->P, V64(p, VD), s);
   }
   hipLaunchKernelGGL(k_bprep, gch, blk, 0, s, vt, ch);  // r = -F, dinv = 1/diag (+ FP32 copy)
-  std::vector<const void *> xs(np), xs32(np);
-  std::vector<void *> ys(np), ys32(np);
-  for (int p = 0; p < np; p++) {
-    xs[p] = b->vt[p].v[VP];
-    ys[p] = b->vt[p].v[VQ];
-    xs32[p] = b->vt[p].v[VP32];
-    ys32[p] = b->vt[p].v[VQ32];
-  }
   std::vector<int> status(np, 0), iters(np, 0), outer(np, 0);
   const bool mg = o->pc == VFG_PC_MG && o->precision == VFG_PREC_MIXED;
   std::vector<double> relres(np, 0.), S;
@@ -670,7 +670,7 @@ extern "C" int vfg_batch_u_solve(vfg_batch *b, double *const *U, const double *c
       for (int p = 0; p < np; p++) dinv[p] = (const double *)b->vt[p].v[VD];
       if ((rc = bmg_setup(b->bmg, dinv.data(), s))) return bfail(b, rc, "batched multigrid set-up failed");
     }
-    std::vector<double> rn0(np, -1.), scale(np), iscale(np), nrm(np);
+    std::vector<double> rn0(np, -1.), rprev(np, -1.), scale(np), iscale(np), nrm(np);
     std::vector<int> done(np, 0);
     const double inner_rtol = 1e-4;
     const int max_outer = 60;
@@ -687,7 +687,11 @@ extern "C" int vfg_batch_u_solve(vfg_batch *b, double *const *U, const double *c
         if (rn0[p] < 0.) rn0[p] = rn;
         if (!done[p]) relres[p] = rn0[p] > 0. ? rn / rn0[p] : 0.;
         if (!done[p] && !isfinite(rn)) { status[p] = VFG_E_NAN; done[p] = 1; }
-        if (!done[p] && (rn <= fmax(o->rtol * rn0[p], o->atol) || rn == 0.)) done[p] = 1;
+        // converged at the tolerance, or at the FP64 residual's rounding floor
+        // (an outer step no longer halves it: the rule of the single-problem solve)
+        if (!done[p] && (rn <= fmax(o->rtol * rn0[p], o->atol) || rn == 0. || (rprev[p] > 0. && rn > .5 * rprev[p])))
+          done[p] = 1;
+        rprev[p] = rn;
         if (!done[p]) { outer[p] = it + 1; any = true; }
         scale[p] = done[p] ? 0. : 1. / rn;
         iscale[p] = done[p] ? 0. : rn;

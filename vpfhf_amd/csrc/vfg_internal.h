@@ -134,6 +134,24 @@ __host__ __device__ inline int64_t tiled_index(int tile, int ek, int nez, int nc
   return (((int64_t)tile * nez + ek) * ncoef + coef) * nslot + slot;
 }
 
+// Ops whose store also reads data of the owned node that is not staged define
+//   __device__ O own_load(int i, int j, int k)      (O trivially copyable)
+//   store(i, j, k, acc, own, O o)
+// the register march loads it one node plane ahead, beside the staged plane,
+// so its latency is not exposed at the store.
+template <class Op, class = void>
+struct has_own : std::false_type {};
+template <class Op>
+struct has_own<Op, std::void_t<decltype(&Op::own_load)>> : std::true_type {};
+template <class Op, class = void>
+struct own_of {
+  using type = int;
+};
+template <class Op>
+struct own_of<Op, std::void_t<decltype(&Op::own_load)>> {
+  using type = decltype(std::declval<const Op &>().own_load(0, 0, 0));
+};
+
 // Ops that also reduce over their owned nodes define
 //   __device__ double store(...)      (returns the node's contribution)
 //   __device__ void partial(int slot, double sum)
@@ -225,14 +243,23 @@ __device__ __forceinline__ void march(const Grid &G, const Op &op, int OZ, int b
 #pragma unroll
   for (int c = 0; c < NC; c++) carry[c] = T(0);
   double racc = 0.;  // owned-node contributions (reducing ops)
+  using OwnT = typename own_of<Op>::type;
+  auto own_at = [&](int k) -> OwnT {
+    OwnT o{};
+    if constexpr (has_own<Op>::value)
+      if (owner && k >= oz0 && k >= 0 && k < G.nz) o = op.own_load(ox0 + tx, oy0 + ty, k);
+    return o;
+  };
 
   // one element layer: bottom plane xb (node plane ek), top plane xt (ek + 1)
-  auto layer = [&](int ek, const T (&xb)[4][NF], T (&xt)[4][NF], AuxT &aux_next) {
+  auto layer = [&](int ek, const T (&xb)[4][NF], T (&xt)[4][NF], AuxT &aux_next, OwnT &own_next) {
     const AuxT aux_cur = aux_next;
+    const OwnT own_cur = own_next;
     __syncthreads();
     commit();       // plane ek + 1
     fetch(ek + 2);  // in flight during this layer
     aux_next = aux_at(ek + 1);
+    own_next = own_at(ek + 1);
     __syncthreads();
     read_plane(xt);
     T r[8][NC];
@@ -280,8 +307,13 @@ __device__ __forceinline__ void march(const Grid &G, const Op &op, int OZ, int b
         carry[c] = cx[1][1][c] + s_red[NC + c][tid + TX];
       }
       if (ek >= oz0 && ek >= 0) {
-        if constexpr (RED) racc += op.store(ox0 + tx, oy0 + ty, ek, bot, xb[3]);
-        else op.store(ox0 + tx, oy0 + ty, ek, bot, xb[3]);
+        if constexpr (has_own<Op>::value) {
+          if constexpr (RED) racc += op.store(ox0 + tx, oy0 + ty, ek, bot, xb[3], own_cur);
+          else op.store(ox0 + tx, oy0 + ty, ek, bot, xb[3], own_cur);
+        } else {
+          if constexpr (RED) racc += op.store(ox0 + tx, oy0 + ty, ek, bot, xb[3]);
+          else op.store(ox0 + tx, oy0 + ty, ek, bot, xb[3]);
+        }
       }
     }
   };
@@ -291,13 +323,14 @@ __device__ __forceinline__ void march(const Grid &G, const Op &op, int OZ, int b
   commit();
   fetch(oz0);
   AuxT aux_next = aux_at(oz0 - 1);
+  OwnT own_next = own_at(oz0 - 1);
   __syncthreads();
   read_plane(xa);
   const int ek_end = min(oz0 + OZ - 1, G.nz - 1);
   for (int ek = oz0 - 1; ek <= ek_end; ek += 2) {
-    layer(ek, xa, xb2, aux_next);
+    layer(ek, xa, xb2, aux_next, own_next);
     if (ek + 1 > ek_end) break;
-    layer(ek + 1, xb2, xa, aux_next);
+    layer(ek + 1, xb2, xa, aux_next, own_next);
   }
   if constexpr (RED) {
     __shared__ double s_rs[NT / 64];
@@ -460,24 +493,33 @@ __device__ __forceinline__ void march_lds(const Grid &G, const Op &op, int OZ, i
 // f(gx, gy, gz, val[NG+NV], grad[NG][3]) is called for every point in the
 // order gz, gy, gx.  About 11 operations per point and gradient field and 3
 // per value-only field, against 32 and 8 for the direct 8-corner sums.
-template <int NG, int NV, typename F>
-__device__ __forceinline__ void gauss_walk(const double (&fld)[NG + NV][8], double ihx, double ihy, double ihz,
-                                           const F &f) {
+// The corner values come from cv(q, a) (field q, corner a), read where they
+// are used: from LDS, the array need not stay in registers for the walk.
+// UZ, UY: unroll factors of the Gauss-layer and Gauss-row loops (1 keeps one
+// layer's or row's values live, for walks with many fields).  With UZ = 1 the
+// corners are re-read in every layer (a compiler barrier keeps the reads from
+// being hoisted out of the loop, which would hold them all in registers).
+template <int NG, int NV, int UZ = 3, int UY = 3, typename CV, typename F>
+__device__ __forceinline__ void gauss_walk_cv(const CV &cv, double ihx, double ihy, double ihz, const F &f) {
   constexpr int NF = NG + NV;
   double dz[NG > 0 ? NG : 1][4];  // d/dz at the 4 (x, y) corners
 #pragma unroll
   for (int q = 0; q < NG; q++)
 #pragma unroll
-    for (int c = 0; c < 4; c++) dz[q][c] = (fld[q][c + 4] - fld[q][c]) * ihz;
-#pragma unroll
+    for (int c = 0; c < 4; c++) dz[q][c] = (cv(q, c + 4) - cv(q, c)) * ihz;
+#pragma unroll UZ
   for (int gz = 0; gz < 3; gz++) {
+    if constexpr (UZ == 1) __asm__ __volatile__("" ::: "memory");
     const double pz = gP1(gz);
     double z[NF][4];
 #pragma unroll
     for (int q = 0; q < NF; q++)
 #pragma unroll
-      for (int c = 0; c < 4; c++) z[q][c] = fld[q][c] + pz * (fld[q][c + 4] - fld[q][c]);
-#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        const double f0 = cv(q, c);
+        z[q][c] = f0 + pz * (cv(q, c + 4) - f0);
+      }
+#pragma unroll UY
     for (int gy = 0; gy < 3; gy++) {
       const double py = gP1(gy);
       double y[NF][2], dyv[NG > 0 ? NG : 1][2], dzy[NG > 0 ? NG : 1][2];
@@ -510,6 +552,11 @@ __device__ __forceinline__ void gauss_walk(const double (&fld)[NG + NV][8], doub
       }
     }
   }
+}
+template <int NG, int NV, typename F>
+__device__ __forceinline__ void gauss_walk(const double (&fld)[NG + NV][8], double ihx, double ihy, double ihz,
+                                           const F &f) {
+  gauss_walk_cv<NG, NV>([&](int q, int a) { return fld[q][a]; }, ihx, ihy, ihz, f);
 }
 
 template <typename T>

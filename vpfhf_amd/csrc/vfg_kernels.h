@@ -41,13 +41,15 @@ int64_t uq_doubles(const Grid &G);
 void launch_u_qsetup(const DevProblem &P, double *uq, hipStream_t s);
 void launch_u_apply(const DevProblem &P, const double *x, double *y, hipStream_t s);
 void launch_u_apply_f32(const DevProblem &P, const float *x, float *y, hipStream_t s);
-// FP32 multigrid level: one Chebyshev-Jacobi step from x = 0 fused with the
-// residual (x = c2 D^-1 b, r = b - A x), and one step from x fused with its
-// apply (xo = x + c2 D^-1 (b - A x)); xo must not alias x
-void launch_mg_presmooth(const DevProblem &P, const float *b, const float *dinv, float c2, float *x, float *r,
+// FP32 multigrid level: one Chebyshev-Jacobi step from x = 0 fused with its
+// apply (x = c2 D^-1 b, y = A x: the restriction forms b - y on the free
+// rows), and one step from x fused with its apply (xo = x + c2 D^-1 (b - A x));
+// xo must not alias x.  With r64 the post-smoothing also writes one partial of
+// r64 . xo per workgroup to part and returns their number (else 0 partials).
+void launch_mg_presmooth(const DevProblem &P, const float *b, const float *dinv, float c2, float *x, float *y,
                          hipStream_t s);
-void launch_mg_postsmooth(const DevProblem &P, const float *b, const float *dinv, float c2, const float *x, float *xo,
-                          hipStream_t s);
+int launch_mg_postsmooth(const DevProblem &P, const float *b, const float *dinv, float c2, const float *x, float *xo,
+                         const double *r64, double *part, hipStream_t s);
 // FP64 y = A x (no NOCOMPRESSION) with one x.y partial per workgroup in part;
 // returns the number of partials (0, and nothing launched, for NOCOMPRESSION)
 int launch_u_apply_dot(const DevProblem &P, const double *x, double *y, double *part, hipStream_t s);
@@ -64,7 +66,10 @@ void launch_mg_coarse_cheb(const DevProblem &P, const float *b, const float *din
 void launch_u_diag(const DevProblem &P, double *d, hipStream_t s);
 // cmask from the state U (UNILATERAL_NOCOMPRESSION)
 void launch_u_cmask(const DevProblem &P, const double *U, hipStream_t s);
-// mode 0: out = element load vector b; mode 1: out = VF_UResidual(U)
+// mode 0: out = element load vector b; mode 1: out = VF_UResidual(U);
+// mode 2 (UNILATERAL_NONE only): b on the free rows and, on the Dirichlet
+// rows, the value ResidualApplyDirichletBC subtracts from U there -- so that
+// VF_UResidual(U) = A U - out with A the MatShell operator (identity rows)
 void launch_u_rhs(const DevProblem &P, const double *U, const double *BCU, double *out, int mode,
                   hipStream_t s);
 
@@ -116,8 +121,11 @@ void launch_energies(const DevProblem &P, const double *U, const double *V, doub
 // geometric multigrid pieces (vfg_mg.hip)
 void launch_mg_elem_coef(const DevProblem &P, double *out, hipStream_t s);
 void launch_mg_coarsen(const Grid &Gf, const double *cf, const Grid &Gc, double *cc, hipStream_t s);
+// tmp (optional, mg_restrict_tmp_floats entries): work space of the
+// separable restriction between nested large levels
 void launch_mg_restrict(const Grid &Gf, const Grid &Gc, const BcMask &bc, const float *b, const float *y,
-                        float *bcoarse, hipStream_t s);
+                        float *bcoarse, hipStream_t s, float *tmp = nullptr);
+int64_t mg_restrict_tmp_floats(const Grid &Gf, const Grid &Gc);
 void launch_mg_prolong(const Grid &Gf, const Grid &Gc, const BcMask &bc, const float *xc, float *xf,
                        hipStream_t s);
 void launch_mg_cheb(const float *b, const float *y, const float *dinv, float *d, float *x, float c1, float c2,

@@ -115,11 +115,13 @@ __global__ __launch_bounds__(256) void k_mg_restrict4(Grid Gf, Grid Gc, BcMask b
 // and one thread per node doing all three components (the general kernels
 // form the weights from the map in FP64 per entry).
 __device__ __forceinline__ float mg_w2(int d) { return d == 0 ? 1.f : .5f; }
+// grid (ceil(nx * ny / 256), nz): node (i, j) of plane blockIdx.y from a
+// 32-bit index (no 64-bit division per thread)
 __global__ __launch_bounds__(256) void k_mg_restrict2(Grid Gf, Grid Gc, BcMask bc, const float *__restrict__ b,
                                                       const float *__restrict__ y, float *__restrict__ bcoarse) {
-  const int64_t N = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (N >= (int64_t)Gc.nx * Gc.ny * Gc.nz) return;
-  const int I = (int)(N % Gc.nx), J = (int)((N / Gc.nx) % Gc.ny), K = (int)(N / ((int64_t)Gc.nx * Gc.ny));
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= Gc.nx * Gc.ny) return;
+  const int J = t / Gc.nx, I = t - J * Gc.nx, K = blockIdx.y;
   float acc[3] = {0.f, 0.f, 0.f};
   for (int dk = -1; dk <= 1; dk++) {
     const int k = 2 * K + dk;
@@ -132,21 +134,25 @@ __global__ __launch_bounds__(256) void k_mg_restrict2(Grid Gf, Grid Gc, BcMask b
         const int i = 2 * I + di;
         if (i < 0 || i >= Gf.nx) continue;
         const int64_t n = 3 * node_id(Gf, i, j, k);
+#pragma unroll
         for (int c = 0; c < 3; c++)
           if (!on_dirichlet(Gf, bc.face[c], bc.vert[c], i, j, k))
             row[c] += mg_w2(di) * (y ? b[n + c] - y[n + c] : b[n + c]);
       }
       const float w = mg_w2(dj) * mg_w2(dk);
+#pragma unroll
       for (int c = 0; c < 3; c++) acc[c] += w * row[c];
     }
   }
+  const int64_t N = node_id(Gc, I, J, K);
+#pragma unroll
   for (int c = 0; c < 3; c++) bcoarse[3 * N + c] = on_dirichlet(Gc, bc.face[c], bc.vert[c], I, J, K) ? 0.f : acc[c];
 }
 __global__ __launch_bounds__(256) void k_mg_prolong2(Grid Gf, Grid Gc, BcMask bc, const float *__restrict__ xc,
                                                      float *__restrict__ xf) {
-  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= (int64_t)Gf.nx * Gf.ny * Gf.nz) return;
-  const int i = (int)(n % Gf.nx), j = (int)((n / Gf.nx) % Gf.ny), k = (int)(n / ((int64_t)Gf.nx * Gf.ny));
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= Gf.nx * Gf.ny) return;
+  const int j = t / Gf.nx, i = t - j * Gf.nx, k = blockIdx.y;
   const int I0 = i >> 1, J0 = j >> 1, K0 = k >> 1;
   const int mi = i & 1, mj = j & 1, mk = k & 1;  // odd: between I0 and I0 + 1
   const float wx = mi ? .5f : 1.f, wy = mj ? .5f : 1.f, wz = mk ? .5f : 1.f;
@@ -156,10 +162,71 @@ __global__ __launch_bounds__(256) void k_mg_prolong2(Grid Gf, Grid Gc, BcMask bc
       for (int c1 = 0; c1 <= mi; c1++) {
         const float w = wx * wy * wz;
         const int64_t N = 3 * node_id(Gc, I0 + c1, J0 + c2, K0 + c3);
+#pragma unroll
         for (int c = 0; c < 3; c++) acc[c] += w * xc[N + c];
       }
+  const int64_t n = node_id(Gf, i, j, k);
+#pragma unroll
   for (int c = 0; c < 3; c++)
     if (!on_dirichlet(Gf, bc.face[c], bc.vert[c], i, j, k)) xf[3 * n + c] += acc[c];
+}
+// Nested restriction in three separable passes (large levels): x, then y,
+// then z, each a contiguous sweep.  t1 holds (Gc.nx, Gf.ny, Gf.nz) nodes, t2
+// (Gc.nx, Gc.ny, Gf.nz); the fine Dirichlet rows are zeroed in the x pass,
+// the coarse ones in the z pass.
+__global__ __launch_bounds__(256) void k_mg_rsx(Grid Gf, Grid Gc, BcMask bc, const float *__restrict__ b,
+                                                const float *__restrict__ y, float *__restrict__ t1) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= Gc.nx * Gf.ny) return;
+  const int j = t / Gc.nx, I = t - j * Gc.nx, k = blockIdx.y;
+  float acc[3] = {0.f, 0.f, 0.f};
+  for (int di = -1; di <= 1; di++) {
+    const int i = 2 * I + di;
+    if (i < 0 || i >= Gf.nx) continue;
+    const int64_t n = 3 * node_id(Gf, i, j, k);
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      if (!on_dirichlet(Gf, bc.face[c], bc.vert[c], i, j, k)) acc[c] += mg_w2(di) * (y ? b[n + c] - y[n + c] : b[n + c]);
+  }
+  const int64_t o = 3 * ((int64_t)I + (int64_t)Gc.nx * (j + (int64_t)Gf.ny * k));
+#pragma unroll
+  for (int c = 0; c < 3; c++) t1[o + c] = acc[c];
+}
+__global__ __launch_bounds__(256) void k_mg_rsy(Grid Gf, Grid Gc, const float *__restrict__ t1, float *__restrict__ t2) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= Gc.nx * Gc.ny) return;
+  const int J = t / Gc.nx, I = t - J * Gc.nx, k = blockIdx.y;
+  float acc[3] = {0.f, 0.f, 0.f};
+  for (int dj = -1; dj <= 1; dj++) {
+    const int j = 2 * J + dj;
+    if (j < 0 || j >= Gf.ny) continue;
+    const int64_t n = 3 * ((int64_t)I + (int64_t)Gc.nx * (j + (int64_t)Gf.ny * k));
+#pragma unroll
+    for (int c = 0; c < 3; c++) acc[c] += mg_w2(dj) * t1[n + c];
+  }
+  const int64_t o = 3 * ((int64_t)I + (int64_t)Gc.nx * (J + (int64_t)Gc.ny * k));
+#pragma unroll
+  for (int c = 0; c < 3; c++) t2[o + c] = acc[c];
+}
+__global__ __launch_bounds__(256) void k_mg_rsz(Grid Gf, Grid Gc, BcMask bc, const float *__restrict__ t2,
+                                                float *__restrict__ bcoarse) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= Gc.nx * Gc.ny) return;
+  const int J = t / Gc.nx, I = t - J * Gc.nx, K = blockIdx.y;
+  float acc[3] = {0.f, 0.f, 0.f};
+  for (int dk = -1; dk <= 1; dk++) {
+    const int k = 2 * K + dk;
+    if (k < 0 || k >= Gf.nz) continue;
+    const int64_t n = 3 * ((int64_t)I + (int64_t)Gc.nx * (J + (int64_t)Gc.ny * k));
+#pragma unroll
+    for (int c = 0; c < 3; c++) acc[c] += mg_w2(dk) * t2[n + c];
+  }
+  const int64_t N = node_id(Gc, I, J, K);
+#pragma unroll
+  for (int c = 0; c < 3; c++) bcoarse[3 * N + c] = on_dirichlet(Gc, bc.face[c], bc.vert[c], I, J, K) ? 0.f : acc[c];
+}
+int64_t mg_restrict_tmp_floats(const Grid &Gf, const Grid &Gc) {
+  return 3 * ((int64_t)Gc.nx * Gf.ny * Gf.nz + (int64_t)Gc.nx * Gc.ny * Gf.nz);
 }
 static bool mg_nested(const Grid &Gf, const Grid &Gc) {
   return Gf.nex == 2 * Gc.nex && Gf.ney == 2 * Gc.ney && Gf.nez == 2 * Gc.nez;
@@ -213,18 +280,26 @@ void launch_mg_coarsen(const Grid &Gf, const double *cf, const Grid &Gc, double 
   hipLaunchKernelGGL(k_mg_coarsen, dim3(nbm(ne)), dim3(256), 0, s, Gf, cf, Gc, cc);
 }
 void launch_mg_restrict(const Grid &Gf, const Grid &Gc, const BcMask &bc, const float *b, const float *y,
-                        float *bcoarse, hipStream_t s) {
+                        float *bcoarse, hipStream_t s, float *tmp) {
   const int64_t n3 = 3 * (int64_t)Gc.nx * Gc.ny * Gc.nz;
   // few coarse entries: latency-bound, split each over its fine planes
   if (n3 <= (1 << 16)) hipLaunchKernelGGL(k_mg_restrict4, dim3(nbm(4 * n3)), dim3(256), 0, s, Gf, Gc, bc, b, y, bcoarse);
-  else if (mg_nested(Gf, Gc))
-    hipLaunchKernelGGL(k_mg_restrict2, dim3(nbm(n3 / 3)), dim3(256), 0, s, Gf, Gc, bc, b, y, bcoarse);
+  else if (mg_nested(Gf, Gc) && tmp) {
+    float *t1 = tmp, *t2 = tmp + 3 * (int64_t)Gc.nx * Gf.ny * Gf.nz;
+    hipLaunchKernelGGL(k_mg_rsx, dim3(nbm((int64_t)Gc.nx * Gf.ny), Gf.nz), dim3(256), 0, s, Gf, Gc, bc, b, y, t1);
+    hipLaunchKernelGGL(k_mg_rsy, dim3(nbm((int64_t)Gc.nx * Gc.ny), Gf.nz), dim3(256), 0, s, Gf, Gc, (const float *)t1, t2);
+    hipLaunchKernelGGL(k_mg_rsz, dim3(nbm((int64_t)Gc.nx * Gc.ny), Gc.nz), dim3(256), 0, s, Gf, Gc, bc, (const float *)t2,
+                       bcoarse);
+  } else if (mg_nested(Gf, Gc))
+    hipLaunchKernelGGL(k_mg_restrict2, dim3(nbm((int64_t)Gc.nx * Gc.ny), Gc.nz), dim3(256), 0, s, Gf, Gc, bc, b, y,
+                       bcoarse);
   else hipLaunchKernelGGL(k_mg_restrict, dim3(nbm(n3)), dim3(256), 0, s, Gf, Gc, bc, b, y, bcoarse);
 }
 void launch_mg_prolong(const Grid &Gf, const Grid &Gc, const BcMask &bc, const float *xc, float *xf,
                        hipStream_t s) {
   const int64_t nn = (int64_t)Gf.nx * Gf.ny * Gf.nz;
-  if (mg_nested(Gf, Gc)) hipLaunchKernelGGL(k_mg_prolong2, dim3(nbm(nn)), dim3(256), 0, s, Gf, Gc, bc, xc, xf);
+  if (mg_nested(Gf, Gc))
+    hipLaunchKernelGGL(k_mg_prolong2, dim3(nbm((int64_t)Gf.nx * Gf.ny), Gf.nz), dim3(256), 0, s, Gf, Gc, bc, xc, xf);
   else hipLaunchKernelGGL(k_mg_prolong, dim3(nbm(nn)), dim3(256), 0, s, Gf, Gc, bc, xc, xf);
 }
 void launch_mg_cheb(const float *b, const float *y, const float *dinv, float *d, float *x, float c1, float c2,

@@ -90,67 +90,107 @@ void launch_vec_apply_dirichlet(const DevProblem &P, int which, double *vec, con
 //   pressure work = sum_g W p (u . grad v)
 //   surface = Gc/(4 atCv) sum_g W [(1-v)^k/eps + eps |grad v|^2], k = 1 (AT1), 2 (AT2)
 //   crack volume = sum_g W u . grad v
-constexpr int EB = 256;
-int energy_blocks(const DevProblem &P) { return (int)((P.ne + EB - 1) / EB); }
+// A workgroup takes a 16 x 16 tile of element columns over EZ element layers:
+// the node planes (u, v, p, theta - thetaRef) are staged in LDS two at a time
+// and each thread walks the Gauss points of its element reading the corners
+// there, so the 48 corner values are not held in registers (two waves per
+// SIMD).  One group of 4 partials per workgroup, in a fixed order.
+constexpr int EB = 256, ET = 16, EP = ET + 1;
+static int energy_ez(const Grid &G) {
+  const int tiles = ((G.nex + ET - 1) / ET) * ((G.ney + ET - 1) / ET);
+  return tiles >= 64 ? 8 : 2;
+}
+static dim3 energy_grid(const Grid &G) {
+  const int ez = energy_ez(G);
+  return dim3((G.nex + ET - 1) / ET, (G.ney + ET - 1) / ET, (G.nez + ez - 1) / ez);
+}
+int energy_blocks(const DevProblem &P) {
+  const dim3 g = energy_grid(P.G);
+  return (int)(g.x * g.y * g.z);
+}
 
-__global__ __launch_bounds__(EB) void k_energies(DevProblem P, const double *__restrict__ U,
-                                                 const double *__restrict__ V, double *partials,
-                                                 double *cellfield, double *efield) {
+__global__ __launch_bounds__(EB, 2) void k_energies(DevProblem P, const double *__restrict__ U,
+                                                    const double *__restrict__ V, double *partials,
+                                                    double *cellfield, double *efield, int EZ) {
+  __shared__ double sp[2][6][EP * EP];
   __shared__ double sh[EB / 64];
-  const int64_t e = (int64_t)blockIdx.x * EB + threadIdx.x;
+  const Grid &G = P.G;
+  const int tid = threadIdx.x, tx = tid % ET, ty = tid / ET;
+  const int ex0 = blockIdx.x * ET, ey0 = blockIdx.y * ET, ek0 = blockIdx.z * EZ;
+  const int ek1 = min(ek0 + EZ, G.nez);
+  const int ex = ex0 + tx, ey = ey0 + ty;
+  const bool ok = ex < G.nex && ey < G.ney;
+  // fields 0-2: u_c, 3: v (values and gradients); 4: p, 5: theta - thetaRef
+  auto stage = [&](int b, int k) {
+    for (int t = tid; t < EP * EP; t += EB) {
+      const int i = ex0 + t % EP, j = ey0 + t / EP;
+      double f[6] = {0., 0., 0., 0., 0., 0.};
+      if (i < G.nx && j < G.ny) {
+        const int64_t n = node_id(G, i, j, k);
+        f[0] = U[3 * n]; f[1] = U[3 * n + 1]; f[2] = U[3 * n + 2];
+        f[3] = V[n]; f[4] = P.p[n]; f[5] = P.dth[n];
+      }
+#pragma unroll
+      for (int q = 0; q < 6; q++) sp[b][q][t] = f[q];
+    }
+  };
+  const double vol = G.hx * G.hy * G.hz;
+  const double csurf = P.Gc / P.atcv * .25, ieps = 1. / P.eps, third = 1. / 3.;
   double el = 0., pw = 0., se = 0., cv = 0.;
-  if (e < P.ne) {
-    const Grid &G = P.G;
-    const int ei = (int)(e % G.nex), ej = (int)((e / G.nex) % G.ney), ek = (int)(e / ((int64_t)G.nex * G.ney));
-    // fields 0-2: u_c, 3: v (values and gradients); 4: p, 5: theta - thetaRef
-    double fld[6][8];
-    for (int a = 0; a < 8; a++) {
-      int64_t n = node_id(G, ei + (a & 1), ej + ((a >> 1) & 1), ek + (a >> 2));
-      for (int c = 0; c < 3; c++) fld[c][a] = U[3 * n + c];
-      fld[3][a] = V[n]; fld[4][a] = P.p[n]; fld[5][a] = P.dth[n];
+  stage(0, ek0);
+  for (int ek = ek0; ek < ek1; ek++) {
+    const int b = (ek - ek0) & 1;
+    stage(b ^ 1, ek + 1);
+    __syncthreads();
+    if (ok) {
+      const int64_t e = cell_id(G, ex, ey, ek);
+      const double Ee = P.E_elem ? P.E_elem[e] : P.E;
+      const double lam = P.lam1 * Ee, mu = P.mu1 * Ee;
+      const double tk = 3. * lam + 2. * mu, cb = P.beta / tk;
+      const int c0 = ty * EP + tx;
+      auto corner = [&](int q, int a) {
+        return sp[(a >> 2) ? b ^ 1 : b][q][c0 + ((a >> 1) & 1) * EP + (a & 1)];
+      };
+      double cell = 0., cel = 0., cpw = 0., cse = 0.;
+      gauss_walk_cv<4, 2, 1, 1>(corner, 1. / G.hx, 1. / G.hy, 1. / G.hz,
+                          [&](int gx, int gy, int gz, const double (&val)[6], const double (&H)[4][3]) {
+        const double vg = val[3], pg = val[4], tg = val[5];
+        const double W = vol * gW(gx) * gW(gy) * gW(gz);
+        double tr = H[0][0] + H[1][1] + H[2][2];
+        const double tr3 = tr * third;
+        double d11 = H[0][0] - tr3, d22 = H[1][1] - tr3, d33 = H[2][2] - tr3;
+        double d12 = (H[0][1] + H[1][0]) * .5, d23 = (H[1][2] + H[2][1]) * .5, d13 = (H[0][2] + H[2][0]) * .5;
+        cel += (d11 * d11 + d22 * d22 + d33 * d33 + (d12 * d12 + d23 * d23 + d13 * d13) * 2.) * vg * vg *
+               (1. + P.eta) * mu * W;
+        // UNILATERAL_NOCOMPRESSION: the spherical term is not degraded where
+        // tr(eps) < 0 (VF_ElasticEnergyNoCompression3D_local, VFMech.c:871-903)
+        double WD = (tr3 - P.alpha * tg) * (P.unilateral && tr < 0. ? 1. : vg) - cb * pg;
+        cel += 3. * tk / 2. * WD * WD * W;
+        cel += tk / 6. * tr * tr * P.eta * W;
+        const double *gv = H[3];
+        double ugv = val[0] * gv[0] + val[1] * gv[1] + val[2] * gv[2];
+        if (P.hasCrackPressure) cpw += W * pg * ugv;
+        double g2 = gv[0] * gv[0] + gv[1] * gv[1] + gv[2] * gv[2];
+        double omv = 1. - vg;
+        cse += W * ((P.atnum == 2 ? omv * omv : omv) * ieps + g2 * P.eps) * csurf;
+        cell += ugv * W;
+      });
+      el += cel; pw += cpw; se += cse; cv += cell;
+      if (cellfield) cellfield[e] = cell / vol;
+      if (efield) {
+        efield[3 * e] = cel;
+        efield[3 * e + 1] = cpw;
+        efield[3 * e + 2] = cse;
+      }
     }
-    const double Ee = P.E_elem ? P.E_elem[e] : P.E;
-    const double lam = P.lam1 * Ee, mu = P.mu1 * Ee;
-    const double tk = 3. * lam + 2. * mu, cb = P.beta / tk;
-    const double vol = G.hx * G.hy * G.hz;
-    const double csurf = P.Gc / P.atcv * .25, ieps = 1. / P.eps, third = 1. / 3.;
-    double cell = 0.;
-    gauss_walk<4, 2>(fld, 1. / G.hx, 1. / G.hy, 1. / G.hz,
-                     [&](int gx, int gy, int gz, const double (&val)[6], const double (&H)[4][3]) {
-      const double vg = val[3], pg = val[4], tg = val[5];
-      const double W = vol * gW(gx) * gW(gy) * gW(gz);
-      double tr = H[0][0] + H[1][1] + H[2][2];
-      const double tr3 = tr * third;
-      double d11 = H[0][0] - tr3, d22 = H[1][1] - tr3, d33 = H[2][2] - tr3;
-      double d12 = (H[0][1] + H[1][0]) * .5, d23 = (H[1][2] + H[2][1]) * .5, d13 = (H[0][2] + H[2][0]) * .5;
-      el += (d11 * d11 + d22 * d22 + d33 * d33 + (d12 * d12 + d23 * d23 + d13 * d13) * 2.) * vg * vg *
-            (1. + P.eta) * mu * W;
-      // UNILATERAL_NOCOMPRESSION: the spherical term is not degraded where
-      // tr(eps) < 0 (VF_ElasticEnergyNoCompression3D_local, VFMech.c:871-903)
-      double WD = (tr3 - P.alpha * tg) * (P.unilateral && tr < 0. ? 1. : vg) - cb * pg;
-      el += 3. * tk / 2. * WD * WD * W;
-      el += tk / 6. * tr * tr * P.eta * W;
-      const double *gv = H[3];
-      double ugv = val[0] * gv[0] + val[1] * gv[1] + val[2] * gv[2];
-      if (P.hasCrackPressure) pw += W * pg * ugv;
-      double g2 = gv[0] * gv[0] + gv[1] * gv[1] + gv[2] * gv[2];
-      double omv = 1. - vg;
-      se += W * ((P.atnum == 2 ? omv * omv : omv) * ieps + g2 * P.eps) * csurf;
-      cv += ugv * W;
-      cell += ugv * W;
-    });
-    if (cellfield) cellfield[e] = cell / vol;
-    if (efield) {
-      efield[3 * e] = el;
-      efield[3 * e + 1] = pw;
-      efield[3 * e + 2] = se;
-    }
+    __syncthreads();  // plane b is restaged for layer ek + 2
   }
   double r0 = block_sum<EB>(el, sh), r1 = block_sum<EB>(pw, sh), r2 = block_sum<EB>(se, sh),
          r3 = block_sum<EB>(cv, sh);
-  if (threadIdx.x == 0) {
-    partials[blockIdx.x * 4 + 0] = r0; partials[blockIdx.x * 4 + 1] = r1;
-    partials[blockIdx.x * 4 + 2] = r2; partials[blockIdx.x * 4 + 3] = r3;
+  if (tid == 0) {
+    const int64_t blk = blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z);
+    partials[blk * 4 + 0] = r0; partials[blk * 4 + 1] = r1;
+    partials[blk * 4 + 2] = r2; partials[blk * 4 + 3] = r3;
   }
 }
 
@@ -167,8 +207,9 @@ __global__ __launch_bounds__(256) void k_reduce_partials(const double *partials,
 
 void launch_energies(const DevProblem &P, const double *U, const double *V, double *partials,
                      double *cellfield, double *efield, double *result4, hipStream_t s) {
-  int nb = energy_blocks(P);
-  hipLaunchKernelGGL(k_energies, dim3(nb), dim3(EB), 0, s, P, U, V, partials, cellfield, efield);
+  const int nb = energy_blocks(P);
+  hipLaunchKernelGGL(k_energies, energy_grid(P.G), dim3(EB), 0, s, P, U, V, partials, cellfield, efield,
+                     energy_ez(P.G));
   hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, s, (const double *)partials, nb, 4, result4);
 }
 
@@ -352,7 +393,7 @@ void launch_maxabs_diff(const double *a, const double *b, int64_t n, double *par
 #include <mutex>
 #include "vfg_handle.h"
 
-static constexpr int ENERGY_STATIC_BLOCKS = 1 << 18;  // grids of up to 2^18 * 256 = 67,108,864 cells
+static constexpr int ENERGY_STATIC_BLOCKS = 1 << 18;  // workgroups of 16 x 16 x 8 cells: 536,870,912 cells
 static __device__ __align__(256) double scratch[4 * ENERGY_STATIC_BLOCKS + 4];
 
 namespace {
@@ -385,9 +426,9 @@ extern "C" int vfg_energies_static(vfg_handle *h, const double *U, const double 
   const int nb = vfg::energy_blocks(h->P);
   if (nb > ENERGY_STATIC_BLOCKS)
     return vfg_fail(h, VFG_E_ARG,
-                    "vfg_energies_static: %lld cells exceed the static scratch limit of %lld cells "
-                    "(use vfg_energies)",
-                    (long long)h->P.ne, (long long)ENERGY_STATIC_BLOCKS * vfg::EB);
+                    "vfg_energies_static: %lld cells need %d partial groups, more than the %d of the "
+                    "static scratch array (use vfg_energies)",
+                    (long long)h->P.ne, nb, ENERGY_STATIC_BLOCKS);
   double *part = nullptr;
   int rc = scratch_address(h, &part);
   if (rc) return rc;

@@ -441,7 +441,7 @@ struct Cheb {
   double lmin, lmax;
 };
 
-static void mg_apply(vfg_handle *h, const float *r, float *z);
+static int mg_apply(vfg_handle *h, const float *r, float *z, const double *r64 = nullptr, double *part = nullptr);
 
 // z = M^-1 r  (Jacobi, Chebyshev(degree) polynomial in D^-1 A, or, with
 // degree < 0, one multigrid V-cycle)
@@ -736,7 +736,9 @@ static void mg_smooth(vfg_handle *h, MgLevel &L, int k, double lo, double hi, bo
   }
 }
 
-static void mg_vcycle(vfg_handle *h, int l) {
+// r64/part (level 0 only): the post-smoothing also reduces r64 . x into part;
+// returns the number of partials (0: not fused, the caller reduces)
+static int mg_vcycle(vfg_handle *h, int l, const double *r64 = nullptr, double *part = nullptr) {
   MgHier &m = *h->mg;
   MgLevel &L = m.lv[l];
   hipStream_t s = h->stream;
@@ -748,45 +750,72 @@ static void mg_vcycle(vfg_handle *h, int l) {
       cc.k = MG_COARSE_DEG;
       cheb_step_coefs(MG_COARSE_DEG, L.lmax / 30., L.lmax, cc.c1, cc.c2);
       launch_mg_coarse_cheb(L.P, L.b, L.dinv, L.x, cc, s);
-      return;
+      return 0;
     }
     mg_smooth(h, L, MG_COARSE_DEG, L.lmax / 30., L.lmax, true);
-    return;
+    return 0;
   }
   MgLevel &C = m.lv[l + 1];
   static_assert(MG_SMOOTH == 1, "the fused smoothing kernels do one Chebyshev step");
-  // one Chebyshev-Jacobi step from x = 0 fused with the residual, restriction,
-  // coarse correction, and one step fused with its apply: the values of
-  // mg_smooth(1) + apply + restrict ... + mg_smooth(1) in three passes less.
-  // The level iterates in d (r in y) and leaves its result in x.
+  // one Chebyshev-Jacobi step from x = 0 fused with its apply, restriction of
+  // b - A x, coarse correction, and one step fused with its apply: the values
+  // of mg_smooth(1) + apply + restrict ... + mg_smooth(1) in three passes less.
+  // The level iterates in d (A d in y) and leaves its result in x.
   float c1[1], c2[1];
   cheb_step_coefs(1, L.lmax / 11., L.lmax, c1, c2);
   launch_mg_presmooth(L.P, L.b, L.dinv, c2[0], L.d, L.y, s);
-  launch_mg_restrict(L.P.G, C.P.G, L.P.bcU, L.y, nullptr, C.b, s);
+  // L.x is written only by the post-smoothing: it is the restriction's work space
+  float *tmp = mg_restrict_tmp_floats(L.P.G, C.P.G) <= L.n ? L.x : nullptr;
+  launch_mg_restrict(L.P.G, C.P.G, L.P.bcU, L.b, L.y, C.b, s, tmp);
   mg_vcycle(h, l + 1);
   launch_mg_prolong(L.P.G, C.P.G, L.P.bcU, C.x, L.d, s);
-  launch_mg_postsmooth(L.P, L.b, L.dinv, c2[0], L.d, L.x, s);
+  return launch_mg_postsmooth(L.P, L.b, L.dinv, c2[0], L.d, L.x, r64, part, s);
 }
 
-// z = V-cycle(r) on the problem's grid (r: FP32, zero on Dirichlet rows)
-static void mg_apply(vfg_handle *h, const float *r, float *z) {
+// z = V-cycle(r) on the problem's grid (r: FP32, zero on Dirichlet rows);
+// with r64, also the partials of r64 . z in part (returns their number, 0:
+// none written)
+static int mg_apply(vfg_handle *h, const float *r, float *z, const double *r64, double *part) {
   MgLevel &L = h->mg->lv[0];
   L.b = const_cast<float *>(r);
   L.x = z;
-  mg_vcycle(h, 0);
+  return mg_vcycle(h, 0, r64, part);
 }
 
-// x += alpha p ; r -= alpha q ; r32 = r
+// x += alpha p ; r -= alpha q ; r32 = r; with part, one partial of r.r per block
 __global__ __launch_bounds__(RB) void k_mg_xr(double *__restrict__ x, double *__restrict__ r,
                                               const double *__restrict__ p, const double *__restrict__ q,
-                                              float *__restrict__ r32, int64_t n, const double *sc) {
+                                              float *__restrict__ r32, int64_t n, const double *sc, double *part) {
   const double al = sc[S_ALPHA];
+  double rr = 0.;
   for (int64_t i = (int64_t)blockIdx.x * RB + threadIdx.x; i < n; i += (int64_t)gridDim.x * RB) {
     x[i] += al * p[i];
     const double ri = r[i] - al * q[i];
     r[i] = ri;
     r32[i] = (float)ri;
+    rr += ri * ri;
   }
+  if (part) {
+    rr = bsum(rr);
+    if (threadIdx.x == 0) part[blockIdx.x] = rr;
+  }
+}
+// op 2 of k_pcg_scalar (beta, rz, rr) from separate partials: rz from prz[0, nrz), rr from prr[0, nrr)
+__global__ __launch_bounds__(RB) void k_pcg_beta_split(const double *prz, int nrz, const double *prr, int nrr,
+                                                       double *sc) {
+  double a = 0., b = 0.;
+  for (int i = threadIdx.x; i < nrz; i += RB) a += prz[i];
+  for (int i = threadIdx.x; i < nrr; i += RB) b += prr[i];
+  a = bsum(a);
+  b = bsum(b);
+  if (threadIdx.x != 0 || sc[S_DONE] != 0.) return;
+  const double rz = sc[S_RZ];
+  sc[S_BETA] = rz != 0. ? a / rz : 0.;
+  sc[S_RZ] = a;
+  sc[S_RR] = b;
+  sc[S_ITS] += 1.;
+  if (!isfinite(a) || !isfinite(b)) sc[S_FLAG] = 2.;
+  else if (sc[S_TOL2] > 0. && b <= sc[S_TOL2]) sc[S_DONE] = 1.;
 }
 // partial sums of r.z and r.r with z = z32 (and p = z when p != nullptr)
 __global__ __launch_bounds__(RB) void k_mg_rz(const double *__restrict__ r, const float *__restrict__ z32,
@@ -840,12 +869,20 @@ static int pcg_mg(vfg_handle *h, int64_t n, const ApplyFn &A, double *x, double 
         na = nb;
       }
       hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(RB), 0, s, (const double *)part, na, 1, sc);
+      // r.r partials from the update, r.z partials from the V-cycle's last
+      // kernel (in two regions of part past the apply's partials)
+      double *prr = part + 65536, *prz = part + 2 * 65536;
       hipLaunchKernelGGL(k_mg_xr, dim3(nb), dim3(RB), 0, s, x, r, (const double *)p, (const double *)q, r32, n,
-                         (const double *)sc);
-      mg_apply(h, r32, z32);
-      hipLaunchKernelGGL(k_mg_rz, dim3(nb), dim3(RB), 0, s, (const double *)r, (const float *)z32,
-                         (double *)nullptr, n, part);
-      hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(RB), 0, s, (const double *)part, nb, 2, sc);
+                         (const double *)sc, prr);
+      const int nrz = mg_apply(h, r32, z32, r, prz);  // at most 2 x 65536 partials
+      if (nrz > 0) {
+        hipLaunchKernelGGL(k_pcg_beta_split, dim3(1), dim3(RB), 0, s, (const double *)prz, nrz, (const double *)prr,
+                           nb, sc);
+      } else {
+        hipLaunchKernelGGL(k_mg_rz, dim3(nb), dim3(RB), 0, s, (const double *)r, (const float *)z32,
+                           (double *)nullptr, n, part);
+        hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(RB), 0, s, (const double *)part, nb, 2, sc);
+      }
       hipLaunchKernelGGL(k_pcg_p32, dim3(nb), dim3(RB), 0, s, p, (const float *)z32, n, (const double *)sc);
     }
     it += todo;
@@ -939,8 +976,11 @@ static int u_solve_newton(vfg_handle *h, double *x, double *r, double *z, double
 // FP64 iterative refinement around the FP32 inner PCG (precision form):
 // r64 = b - A64 x in FP64; solve A32 d = r64/|r64| in FP32 to inner_rtol;
 // x += |r64| d in FP64; repeat until |r64| <= rtol |r64_0|.
+// FP64 refinement around the FP32 inner PCG.  The residual of each step is
+// -F(x) = bt - A x: one FP64 apply and a subtraction, with bt (the load vector
+// with the Dirichlet targets, formed once) in bt and A x in y64.
 static int u_solve_mixed(vfg_handle *h, double *x, double *r64, double *t64, const double *dinv64, const double *B,
-                         const vfg_solve_opts *o, vfg_solve_info *inf) {
+                         double *bt, double *y64, const vfg_solve_opts *o, vfg_solve_info *inf) {
   DevProblem &P = h->P;
   hipStream_t s = h->stream;
   const int64_t n = 3 * P.nn;
@@ -962,6 +1002,7 @@ static int u_solve_mixed(vfg_handle *h, double *x, double *r64, double *t64, con
     if ((rc = mg_setup(h, dinv64))) return rc;
     cb.degree = -1;
   }
+  launch_u_rhs(P, x, B, bt, 2, s);
   double rn0 = -1., rn_prev = 0.;
   for (int outer = 0; outer < 60; outer++) {
     int e2 = 0;
@@ -1001,8 +1042,8 @@ static int u_solve_mixed(vfg_handle *h, double *x, double *r64, double *t64, con
     inf->iterations += its * (cb.degree > 1 ? cb.degree : 1);  // a V-cycle step counts once
     if (st == VFG_E_HIP || st == VFG_E_NAN) return st;
     hipLaunchKernelGGL(k_axpy3264, dim3(nb_(n)), dim3(256), 0, s, x, (const float *)d, rn, n);
-    launch_u_rhs(P, x, B, r64, 1, s);  // F(x) in FP64
-    hipLaunchKernelGGL(k_neg, dim3(nb_(n)), dim3(256), 0, s, (const double *)r64, r64, n);
+    launch_u_apply(P, x, y64, s);  // -F(x) = bt - A x in FP64
+    launch_sub(bt, y64, r64, n, s);
     inf->outer = outer + 1;
   }
   return rc ? rc : VFG_E_MAXIT;
@@ -1048,7 +1089,7 @@ int vfg_u_solve(vfg_handle *h, double *U, const double *BCU, const vfg_solve_opt
   if (P.unilateral) {
     rc = u_solve_newton(h, x, r, z, p, q, dinv, B, o, &inf);
   } else if (o->precision == VFG_PREC_MIXED) {
-    rc = u_solve_mixed(h, x, r, q, dinv, B, o, &inf);
+    rc = u_solve_mixed(h, x, r, q, dinv, B, z, p, o, &inf);
   } else if (o->pc == VFG_PC_CHEBYSHEV) {
     rc = vfg_fail(h, VFG_E_ARG, "vfg_u_solve: the Chebyshev preconditioner is provided for the mixed-precision solve");
   } else if (o->pc == VFG_PC_MG) {

@@ -204,7 +204,10 @@ __device__ __forceinline__ void u_elem_lds_impl(const XS &xs, T cE, T lam1, T mu
   constexpr bool STORED = QSTRIDE > 0;
   const T lr = lam1 / mu1;
   const T mx = mu1 * ihx, my = mu1 * ihy, mz = mu1 * ihz;
-  T Rx[3][2][2], Ry[3][2][2], Sz[3][3][3];  // Rx[c][k][j], Ry[c][k][i], Sz[c][gy][gx]
+  // Rx[c][k][j], Ry[c][k][i]; Sz[c][gy][i]: the z-stress summed over the Gauss
+  // layers and projected onto x at once (18 accumulators rather than 27: the
+  // element then fits two waves per SIMD without spilling)
+  T Rx[3][2][2], Ry[3][2][2], Sz[3][3][2];
 #pragma unroll
   for (int c = 0; c < 3; c++) {
 #pragma unroll
@@ -214,7 +217,7 @@ __device__ __forceinline__ void u_elem_lds_impl(const XS &xs, T cE, T lam1, T mu
 #pragma unroll
     for (int p = 0; p < 3; p++)
 #pragma unroll
-      for (int q = 0; q < 3; q++) Sz[c][p][q] = T(0);
+      for (int q = 0; q < 2; q++) Sz[c][p][q] = T(0);
   }
 #pragma unroll
   for (int gz = 0; gz < 3; gz++) {
@@ -305,7 +308,10 @@ __device__ __forceinline__ void u_elem_lds_impl(const XS &xs, T cE, T lam1, T mu
         const T t00 = lt + T(2) * H00, t11 = lt + T(2) * H11, t22 = lt + T(2) * H22;
         Sx[0][gy] += cq * t00; Sx[1][gy] += cq * t01; Sx[2][gy] += cq * t02;
         Sy[0] += cq * t01; Sy[1] += cq * t11; Sy[2] += cq * t12;
-        Sz[0][gy][gx] += cq * t02; Sz[1][gy][gx] += cq * t12; Sz[2][gy][gx] += cq * t22;
+        const T z0 = cq * t02, z1 = cq * t12, z2 = cq * t22;
+        Sz[0][gy][0] += T(gP0(gx)) * z0; Sz[0][gy][1] += T(gP1(gx)) * z0;
+        Sz[1][gy][0] += T(gP0(gx)) * z1; Sz[1][gy][1] += T(gP1(gx)) * z1;
+        Sz[2][gy][0] += T(gP0(gx)) * z2; Sz[2][gy][1] += T(gP1(gx)) * z2;
       }
 #pragma unroll
       for (int c = 0; c < 3; c++) {
@@ -326,14 +332,7 @@ __device__ __forceinline__ void u_elem_lds_impl(const XS &xs, T cE, T lam1, T mu
   }
 #pragma unroll
   for (int c = 0; c < 3; c++) {
-    T b[3][2];  // [gy][i]
-#pragma unroll
-    for (int gy = 0; gy < 3; gy++) {
-      T a0 = T(0), a1 = T(0);
-#pragma unroll
-      for (int gx = 0; gx < 3; gx++) { a0 += T(gP0(gx)) * Sz[c][gy][gx]; a1 += T(gP1(gx)) * Sz[c][gy][gx]; }
-      b[gy][0] = a0; b[gy][1] = a1;
-    }
+    const T (&b)[3][2] = Sz[c];  // [gy][i]
     T Rz[2][2];  // [j][i]
 #pragma unroll
     for (int i = 0; i < 2; i++) {
@@ -432,40 +431,81 @@ __global__ __launch_bounds__(TX *TY, sizeof(T) == 4 ? 2 : 1) void k_u_apply(UApp
 
 // FP32 multigrid apply fused with its Chebyshev(1)-Jacobi step (mg_cheb_entry
 // with one step, vfg_solve.hip's V-cycle):
-//   PRE  (from x = 0): x = c2 D^-1 b staged on the fly, r = b - A x stored
-//        (0 on Dirichlet rows, which the restriction skips) with x;
-//   POST: xo = x + c2 D^-1 (b - A x), A x = x on Dirichlet rows, into a
-//        buffer other than x (neighbouring tiles still stage x).
-enum { MG_PRE = 0, MG_POST = 1 };
-template <int MODE>
-struct UApplySmoothOp : UApplyOp<float, false> {
+//   pre  (from x = 0): x = c2 D^-1 b staged on the fly; stores x and A x (the
+//        restriction forms b - A x on the free rows);
+//   post: xo = x + c2 D^-1 (b - A x), A x = x on Dirichlet rows, into a
+//        buffer other than x (neighbouring tiles still stage x); b and D^-1
+//        of the owned node are loaded a node plane ahead (own_load).  The
+//        rz form also reduces r . xo (r: the FP64 residual the V-cycle
+//        preconditions) over the owned rows, one partial per workgroup.
+struct UMgPreOp : UApplyOp<float, false> {
   const float *b, *dinv;
   float *xo, *ro;
   float c2;
-  __device__ float load(int f, int64_t n) const {
-    if (f == 3) return v[n];
-    if constexpr (MODE == MG_PRE) return c2 * dinv[3 * n + f] * b[3 * n + f];
-    else return x[3 * n + f];
-  }
+  __device__ float load(int f, int64_t n) const { return f == 3 ? v[n] : c2 * dinv[3 * n + f] * b[3 * n + f]; }
   __device__ void store(int i, int j, int k, const float (&acc)[3], const float (&own)[4]) const {
     const int64_t n = node_id(G, i, j, k);
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      const bool dir = on_dirichlet(G, bc.face[c], bc.vert[c], i, j, k);
-      const int64_t q = 3 * n + c;
-      if constexpr (MODE == MG_PRE) {
-        xo[q] = own[c];
-        ro[q] = dir ? 0.f : b[q] - acc[c];
-      } else {
-        const float yv = dir ? own[c] : acc[c];
-        xo[q] = own[c] + c2 * dinv[q] * (b[q] - yv);
-      }
+      xo[3 * n + c] = own[c];
+      ro[3 * n + c] = acc[c];
     }
   }
 };
-template <int MODE, int TX, int TY>
-__global__ __launch_bounds__(TX *TY, 2) void k_u_apply_smooth(UApplySmoothOp<MODE> op, int OZ) {
+struct MgOwn {
+  float b[3], di[3];
+};
+struct MgOwnR {
+  float b[3], di[3];
+  double r[3];
+};
+template <bool RZ>
+struct UMgPostOp : UApplyOp<float, false> {
+  const float *b, *dinv;
+  float *xo;
+  float c2;
+  const double *r64;  // RZ
+  double *part;       // RZ
+  __device__ auto own_load(int i, int j, int k) const {
+    const int64_t n = node_id(G, i, j, k);
+    std::conditional_t<RZ, MgOwnR, MgOwn> o;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      o.b[c] = b[3 * n + c];
+      o.di[c] = dinv[3 * n + c];
+      if constexpr (RZ) o.r[c] = r64[3 * n + c];
+    }
+    return o;
+  }
+  template <typename O>
+  __device__ auto store(int i, int j, int k, const float (&acc)[3], const float (&own)[4], const O &o) const {
+    const int64_t n = node_id(G, i, j, k);
+    double rz = 0.;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const float yv = on_dirichlet(G, bc.face[c], bc.vert[c], i, j, k) ? own[c] : acc[c];
+      const float z = own[c] + c2 * o.di[c] * (o.b[c] - yv);
+      xo[3 * n + c] = z;
+      if constexpr (RZ) rz += o.r[c] * (double)z;
+    }
+    if constexpr (RZ) return rz;
+  }
+};
+struct UMgPostRzOp : UMgPostOp<true> {
+  __device__ void partial(int slot, double v) const { part[slot] = v; }
+};
+template <int TX, int TY>
+__global__ __launch_bounds__(TX *TY, 2) void k_u_mg_pre(UMgPreOp op, int OZ) {
   march<float, TX, TY>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+template <int TX, int TY>
+__global__ __launch_bounds__(TX *TY, 2) void k_u_mg_post(UMgPostOp<false> op, int OZ) {
+  march<float, TX, TY>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+template <int TX, int TY>
+__global__ __launch_bounds__(TX *TY, 2) void k_u_mg_post_rz(UMgPostRzOp op, int OZ) {
+  march<float, TX, TY>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z,
+                       blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
 }
 
 // FP64 apply with the corners in LDS: at most 256 registers, two waves per
@@ -791,8 +831,10 @@ struct URhsOp {
   }
 };
 
-// Load vector of one element under UNILATERAL_NONE, sum-factorised over the
-// Gauss-point walk (v with its gradient, dth, p):
+// Load vector of one element under UNILATERAL_NONE, sum-factorised over two
+// Gauss-point walks (values of v, dth, p; then v with its gradient and p,
+// kept apart so that each walk's accumulators fit the registers of two waves
+// per SIMD):
 //   thermo-poro (VFMech.c:386-447): b[a][c] += sum_g W d_c phi_a T,
 //     T = (3 lam + 2 mu) alpha dth (v^2 + eta) + beta p v  -- the x-, y- and
 //     z-derivative projections of one scalar: T summed over gx per (gy, gz),
@@ -800,15 +842,59 @@ struct URhsOp {
 //   crack pressure (VFMech.c:533-589): b[a][c] += sum_g W p (d_c v) phi_a --
 //     a value projection, summed over gx, then gy, then gz.
 // Every contribution is multiplied by sgn (-1 for the residual).
-__device__ __forceinline__ void u_loads_elem(const double (&fld)[3][8], double coefalpha, double beta, double eta,
+template <typename CV>
+__device__ __forceinline__ void u_loads_elem(const CV &cv, double coefalpha, double beta, double eta,
                                              double vol, bool crack, double ihx, double ihy, double ihz, double sgn,
                                              double (&r)[8][3]) {
-  double Tx = 0., Ty[3] = {0., 0., 0.}, Tz[3][3], Rx[2][2] = {{0., 0.}, {0., 0.}}, Ry[2][2] = {{0., 0.}, {0., 0.}};
+  // thermo-poro: values of v, dth, p (fields 0-2)
+  {
+    double Tx = 0., Ty[3] = {0., 0., 0.}, Tz[3][3], Rx[2][2] = {{0., 0.}, {0., 0.}}, Ry[2][2] = {{0., 0.}, {0., 0.}};
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+#pragma unroll
+      for (int w = 0; w < 3; w++) Tz[q][w] = 0.;
+    gauss_walk_cv<0, 3>(cv, ihx, ihy, ihz, [&](int gx, int gy, int gz, const double (&val)[3], const double (&)[1][3]) {
+      const double W = vol * gW(gx) * gW(gy) * gW(gz);
+      const double v = val[0];
+      const double T = sgn * (W * (coefalpha * val[1] * (v * v + eta) + beta * val[2] * v));
+      Tx += T;
+      Ty[gx] += T;
+      Tz[gy][gx] += T;
+      if (gx == 2) {  // end of a row (gy, gz)
+#pragma unroll
+        for (int k = 0; k < 2; k++)
+#pragma unroll
+          for (int j = 0; j < 2; j++) Rx[k][j] += (k ? gP1(gz) : gP0(gz)) * (j ? gP1(gy) : gP0(gy)) * Tx;
+        Tx = 0.;
+      }
+      if (gx == 2 && gy == 2) {  // end of a layer gz
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+#pragma unroll
+          for (int k = 0; k < 2; k++)
+#pragma unroll
+            for (int i = 0; i < 2; i++) Ry[k][i] += (k ? gP1(gz) : gP0(gz)) * (i ? gP1(q) : gP0(q)) * Ty[q];
+          Ty[q] = 0.;
+        }
+      }
+    });
+#pragma unroll
+    for (int a = 0; a < 8; a++) {
+      const int i = a & 1, j = (a >> 1) & 1, k = a >> 2;
+      double rz = 0.;
+#pragma unroll
+      for (int gy = 0; gy < 3; gy++)
+#pragma unroll
+        for (int gx = 0; gx < 3; gx++) rz += (j ? gP1(gy) : gP0(gy)) * (i ? gP1(gx) : gP0(gx)) * Tz[gy][gx];
+      r[a][0] += (i ? ihx : -ihx) * Rx[k][j];
+      r[a][1] += (j ? ihy : -ihy) * Ry[k][i];
+      r[a][2] += (k ? ihz : -ihz) * rz;
+    }
+  }
+  if (!crack) return;
+  // crack pressure: v with its gradient (field 0) and p (field 2); the value
+  // projection summed over gx (Vx), gy (Vy), then gz straight into r
   double Vx[3][2], Vy[3][2][2];
-#pragma unroll
-  for (int q = 0; q < 3; q++)
-#pragma unroll
-    for (int w = 0; w < 3; w++) Tz[q][w] = 0.;
 #pragma unroll
   for (int c = 0; c < 3; c++)
 #pragma unroll
@@ -817,80 +903,46 @@ __device__ __forceinline__ void u_loads_elem(const double (&fld)[3][8], double c
 #pragma unroll
       for (int j = 0; j < 2; j++) Vy[c][j][i] = 0.;
     }
-  gauss_walk<1, 2>(fld, ihx, ihy, ihz, [&](int gx, int gy, int gz, const double (&val)[3], const double (&gv)[1][3]) {
-    const double W = vol * gW(gx) * gW(gy) * gW(gz);
-    const double v = val[0];
-    const double T = sgn * (W * (coefalpha * val[1] * (v * v + eta) + beta * val[2] * v));
-    Tx += T;
-    Ty[gx] += T;
-    Tz[gy][gx] += T;
-    if (crack) {
-      const double cp = sgn * W * val[2];
+  gauss_walk_cv<1, 1>([&](int q, int a) { return cv(q ? 2 : 0, a); }, ihx, ihy, ihz,
+                      [&](int gx, int gy, int gz, const double (&val)[2], const double (&gv)[1][3]) {
+    const double cp = sgn * (vol * gW(gx) * gW(gy) * gW(gz)) * val[1];
 #pragma unroll
-      for (int c = 0; c < 3; c++) {
-        const double Gc = cp * gv[0][c];
-        Vx[c][0] += gP0(gx) * Gc;
-        Vx[c][1] += gP1(gx) * Gc;
-      }
+    for (int c = 0; c < 3; c++) {
+      const double Gc = cp * gv[0][c];
+      Vx[c][0] += gP0(gx) * Gc;
+      Vx[c][1] += gP1(gx) * Gc;
     }
     if (gx == 2) {  // end of a row (gy, gz)
 #pragma unroll
-      for (int k = 0; k < 2; k++)
+      for (int c = 0; c < 3; c++)
 #pragma unroll
-        for (int j = 0; j < 2; j++) Rx[k][j] += (k ? gP1(gz) : gP0(gz)) * (j ? gP1(gy) : gP0(gy)) * Tx;
-      Tx = 0.;
-      if (crack) {
+        for (int i = 0; i < 2; i++) {
+          Vy[c][0][i] += gP0(gy) * Vx[c][i];
+          Vy[c][1][i] += gP1(gy) * Vx[c][i];
+          Vx[c][i] = 0.;
+        }
+    }
+    if (gx == 2 && gy == 2) {  // end of a layer gz: into the corners (j, i) of both node planes
 #pragma unroll
-        for (int c = 0; c < 3; c++)
+      for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
 #pragma unroll
           for (int i = 0; i < 2; i++) {
-            Vy[c][0][i] += gP0(gy) * Vx[c][i];
-            Vy[c][1][i] += gP1(gy) * Vx[c][i];
-            Vx[c][i] = 0.;
+            r[(0 * 2 + j) * 2 + i][c] += gP0(gz) * Vy[c][j][i];
+            r[(1 * 2 + j) * 2 + i][c] += gP1(gz) * Vy[c][j][i];
+            Vy[c][j][i] = 0.;
           }
-      }
-    }
-    if (gx == 2 && gy == 2) {  // end of a layer gz
-#pragma unroll
-      for (int q = 0; q < 3; q++) {
-#pragma unroll
-        for (int k = 0; k < 2; k++)
-#pragma unroll
-          for (int i = 0; i < 2; i++) Ry[k][i] += (k ? gP1(gz) : gP0(gz)) * (i ? gP1(q) : gP0(q)) * Ty[q];
-        Ty[q] = 0.;
-      }
-      if (crack) {
-#pragma unroll
-        for (int c = 0; c < 3; c++)
-#pragma unroll
-          for (int j = 0; j < 2; j++)
-#pragma unroll
-            for (int i = 0; i < 2; i++) {  // straight into the corners (j, i) of both node planes
-              r[(0 * 2 + j) * 2 + i][c] += gP0(gz) * Vy[c][j][i];
-              r[(1 * 2 + j) * 2 + i][c] += gP1(gz) * Vy[c][j][i];
-              Vy[c][j][i] = 0.;
-            }
-      }
     }
   });
-#pragma unroll
-  for (int a = 0; a < 8; a++) {
-    const int i = a & 1, j = (a >> 1) & 1, k = a >> 2;
-    double rz = 0.;
-#pragma unroll
-    for (int gy = 0; gy < 3; gy++)
-#pragma unroll
-      for (int gx = 0; gx < 3; gx++) rz += (j ? gP1(gy) : gP0(gy)) * (i ? gP1(gx) : gP0(gx)) * Tz[gy][gx];
-    r[a][0] += (i ? ihx : -ihx) * Rx[k][j];
-    r[a][1] += (j ? ihy : -ihy) * Ry[k][i];
-    r[a][2] += (k ? ihz : -ihz) * rz;
-  }
 }
 
 // The load vector alone (UNILATERAL_NONE: the loads do not depend on U):
 // mode 0 stores b; mode 1 finishes the residual F = K U - b over an output
 // that already holds K U (the FP64 apply), and writes the Dirichlet rows'
-// ResidualApplyDirichletBC values from U (read there only).
+// ResidualApplyDirichletBC values from U (read there only); mode 2 stores b
+// with the Dirichlet rows' targets instead, so that F(U) = A U - b for the
+// MatShell operator A.
 struct ULoadsOp {
   static constexpr int NF = 3, NC = 3;  // v, dth, p
   const double *U, *v, *dth, *p, *BCU;
@@ -911,34 +963,43 @@ struct ULoadsOp {
 #pragma unroll
       for (int f = 0; f < 3; f++) fld[f][a] = X[a >> 2][a & 3][f];
     const double coefalpha = (3. * lam1 + 2. * mu1) * Ee * alpha;
-    u_loads_elem(fld, coefalpha, beta, eta, vol, crack != 0, 1. / G.hx, 1. / G.hy, 1. / G.hz, mode == 1 ? -1. : 1., r);
+    u_loads_elem([&](int q, int a) { return fld[q][a]; }, coefalpha, beta, eta, vol, crack != 0, 1. / G.hx,
+                 1. / G.hy, 1. / G.hz, mode == 1 ? -1. : 1., r);
+  }
+  template <typename XS>
+  __device__ void element_lds(int, int, int, const XS &xs, double Ee, double (&r)[8][3]) const {
+    const double coefalpha = (3. * lam1 + 2. * mu1) * Ee * alpha;
+    u_loads_elem([&](int q, int a) { return xs(a >> 2, a & 3, q); }, coefalpha, beta, eta, vol, crack != 0, 1. / G.hx, 1. / G.hy, 1. / G.hz, mode == 1 ? -1. : 1., r);
   }
   __device__ void store(int i, int j, int k, const double (&acc)[3], const double (&)[3]) const {
     const int64_t n = node_id(G, i, j, k);
     for (int c = 0; c < 3; c++) {
       double val = acc[c];
-      if (mode == 1) {
-        if (on_dirichlet(G, bc.face[c], bc.vert[c], i, j, k)) {
-          int t = 0;
-          const bool onf[6] = {i == 0, i == G.nx - 1, j == 0, j == G.ny - 1, k == 0, k == G.nz - 1};
-          for (int f = 0; f < 6; f++)
-            if (ftype[c][f] && onf[f]) t = ftype[c][f];
-          int ix = (i == 0) ? 0 : (i == G.nx - 1) ? 1 : -1;
-          int iy = (j == 0) ? 0 : (j == G.ny - 1) ? 1 : -1;
-          int iz = (k == 0) ? 0 : (k == G.nz - 1) ? 1 : -1;
-          if (ix >= 0 && iy >= 0 && iz >= 0 && vtype[c][ix + 2 * iy + 4 * iz]) t = vtype[c][ix + 2 * iy + 4 * iz];
-          const double u = U[3 * n + c];
-          val = t == 3 ? u - BCU[3 * n + c] : t == 2 ? u - 1. : u;
-        } else {
-          val = out[3 * n + c] + acc[c];  // K U - b
-        }
+      if (mode == 2 && on_dirichlet(G, bc.face[c], bc.vert[c], i, j, k)) {
+        // the value the residual's Dirichlet row subtracts from U
+        val = dirichlet_value(i, j, k, c, n);
+      } else if (mode == 1) {
+        if (on_dirichlet(G, bc.face[c], bc.vert[c], i, j, k)) val = U[3 * n + c] - dirichlet_value(i, j, k, c, n);
+        else val = out[3 * n + c] + acc[c];  // K U - b
       }
       out[3 * n + c] = val;
     }
   }
+  // ResidualApplyDirichletBC's target on a Dirichlet row: BCU (type 3), 1 (type 2), else 0
+  __device__ double dirichlet_value(int i, int j, int k, int c, int64_t n) const {
+    int t = 0;
+    const bool onf[6] = {i == 0, i == G.nx - 1, j == 0, j == G.ny - 1, k == 0, k == G.nz - 1};
+    for (int f = 0; f < 6; f++)
+      if (ftype[c][f] && onf[f]) t = ftype[c][f];
+    const int ix = (i == 0) ? 0 : (i == G.nx - 1) ? 1 : -1;
+    const int iy = (j == 0) ? 0 : (j == G.ny - 1) ? 1 : -1;
+    const int iz = (k == 0) ? 0 : (k == G.nz - 1) ? 1 : -1;
+    if (ix >= 0 && iy >= 0 && iz >= 0 && vtype[c][ix + 2 * iy + 4 * iz]) t = vtype[c][ix + 2 * iy + 4 * iz];
+    return t == 3 ? BCU[3 * n + c] : t == 2 ? 1. : 0.;
+  }
 };
-__global__ __launch_bounds__(256) void k_u_loads(ULoadsOp op, int OZ) {
-  march<double, 16, 16>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z);
+__global__ __launch_bounds__(256, 2) void k_u_loads(ULoadsOp op, int OZ) {
+  march_lds<double, 16, 16, false>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 template <bool UNI>
@@ -1221,35 +1282,57 @@ void launch_u_apply_f32(const DevProblem &P, const float *x, float *y, hipStream
   launch_u_apply_t<float>(P, x, y, P.V32, s);
 }
 
-template <int MODE>
-static void launch_u_smooth_t(const DevProblem &P, const float *b, const float *dinv, float c2, const float *x,
-                              float *xo, float *ro, hipStream_t s) {
-  UApplySmoothOp<MODE> op;
-  op.x = x; op.y = nullptr; op.v = P.V32;
+template <class Op>
+static void mg_op_fill(Op &op, const DevProblem &P, const float *b, const float *dinv, float c2) {
+  op.y = nullptr; op.v = P.V32;
   op.E_elem = P.E_elem;
   op.cmask = nullptr;
   op.G = P.G; op.bc = P.bcU;
   op.E = float(P.E); op.lam1 = float(P.lam1); op.mu1 = float(P.mu1); op.eta = float(P.eta);
   op.vol = float(P.G.hx * P.G.hy * P.G.hz);
   op.ihx = float(1. / P.G.hx); op.ihy = float(1. / P.G.hy); op.ihz = float(1. / P.G.hz);
-  op.b = b; op.dinv = dinv; op.xo = xo; op.ro = ro; op.c2 = c2;
-  if (P.G.nx >= SMALL_NX_LO && P.G.nx <= SMALL_NX_HI) {
-    static int slots = device_slots(k_u_apply_smooth<MODE, 8, 8>, 64);
-    const int OZ = choose_oz(P.G, ((P.G.nx + 6) / 7) * ((P.G.ny + 6) / 7), slots);
-    hipLaunchKernelGGL((k_u_apply_smooth<MODE, 8, 8>), march_grid(P.G, 8, 8, OZ), dim3(64), 0, s, op, OZ);
-    return;
-  }
-  static int slots = device_slots(k_u_apply_smooth<MODE, 16, 16>, 256);
-  const int OZ = choose_oz(P.G, ((P.G.nx + 14) / 15) * ((P.G.ny + 14) / 15), slots);
-  hipLaunchKernelGGL((k_u_apply_smooth<MODE, 16, 16>), march_grid(P.G, 16, 16, OZ), dim3(256), 0, s, op, OZ);
+  op.b = b; op.dinv = dinv; op.c2 = c2;
+}
+// launches kernel 8x8 or 16x16 by the grid's x extent; returns the workgroup
+// count, or 0 without launching when it would exceed max_wg
+template <class Op, class K8, class K16>
+static int mg_launch(const DevProblem &P, const Op &op, K8 k8, K16 k16, int &slots8, int &slots16, hipStream_t s,
+                     int64_t max_wg = INT32_MAX) {
+  const bool small = P.G.nx >= SMALL_NX_LO && P.G.nx <= SMALL_NX_HI;
+  if (small && !slots8) slots8 = device_slots(k8, 64);
+  if (!small && !slots16) slots16 = device_slots(k16, 256);
+  const int T = small ? 8 : 16;
+  const int OZ = choose_oz(P.G, ((P.G.nx + T - 2) / (T - 1)) * ((P.G.ny + T - 2) / (T - 1)), small ? slots8 : slots16);
+  const dim3 g = march_grid(P.G, T, T, OZ);
+  const int64_t nwg = (int64_t)g.x * g.y * g.z;
+  if (nwg > max_wg) return 0;
+  if (small) hipLaunchKernelGGL(k8, g, dim3(64), 0, s, op, OZ);
+  else hipLaunchKernelGGL(k16, g, dim3(256), 0, s, op, OZ);
+  return (int)nwg;
 }
 void launch_mg_presmooth(const DevProblem &P, const float *b, const float *dinv, float c2, float *x, float *r,
                          hipStream_t s) {
-  launch_u_smooth_t<MG_PRE>(P, b, dinv, c2, nullptr, x, r, s);
+  UMgPreOp op;
+  mg_op_fill(op, P, b, dinv, c2);
+  op.x = nullptr; op.xo = x; op.ro = r;
+  static int s8 = 0, s16 = 0;
+  mg_launch(P, op, k_u_mg_pre<8, 8>, k_u_mg_pre<16, 16>, s8, s16, s);
 }
-void launch_mg_postsmooth(const DevProblem &P, const float *b, const float *dinv, float c2, const float *x, float *xo,
-                          hipStream_t s) {
-  launch_u_smooth_t<MG_POST>(P, b, dinv, c2, x, xo, nullptr, s);
+int launch_mg_postsmooth(const DevProblem &P, const float *b, const float *dinv, float c2, const float *x, float *xo,
+                         const double *r64, double *part, hipStream_t s) {
+  if (r64) {
+    UMgPostRzOp op;
+    mg_op_fill(op, P, b, dinv, c2);
+    op.x = x; op.xo = xo; op.r64 = r64; op.part = part;
+    static int s8 = 0, s16 = 0;
+    const int n = mg_launch(P, op, k_u_mg_post_rz<8, 8>, k_u_mg_post_rz<16, 16>, s8, s16, s, 2 * 65536);
+    if (n) return n;  // else too many workgroups for the partials: the plain form
+  }
+  UMgPostOp<false> op;
+  mg_op_fill(op, P, b, dinv, c2);
+  op.x = x; op.xo = xo; op.r64 = nullptr; op.part = nullptr;
+  static int s8 = 0, s16 = 0;
+  return mg_launch(P, op, k_u_mg_post<8, 8>, k_u_mg_post<16, 16>, s8, s16, s);
 }
 
 void launch_u_diag(const DevProblem &P, double *d, hipStream_t s) {
@@ -1285,8 +1368,14 @@ static void launch_u_rhs_t(const DevProblem &P, const double *U, const double *B
 
 void launch_u_rhs(const DevProblem &P, const double *U, const double *BCU, double *out, int mode,
                   hipStream_t s) {
-  if (P.unilateral) {
+  if (P.unilateral) {  // (mode 2 is not used with NOCOMPRESSION: its load depends on U)
     launch_u_rhs_t<true>(P, U, BCU, out, mode, s);
+    return;
+  }
+  // small grids (ensemble members, served requests): the apply and the loads
+  // in one launch, which is latency-bound there
+  if (mode != 2 && P.nn < (1 << 19)) {
+    launch_u_rhs_t<false>(P, U, BCU, out, mode, s);
     return;
   }
   // UNILATERAL_NONE: K U by the FP64 apply, then the loads
