@@ -237,9 +237,12 @@ def ensemble_bench(L, n_members=64, seed=0):
     """Batching form: 64 independent problems, sides log-uniform in [16, 64]
     elements, own lognormal E, 1-4 cracks, injection pressure.  The plain FP64
     GPU drop-in solves them one after another (vfg_u_solve, Jacobi-PCG to
-    1e-10; also timed with the multigrid preconditioner); the batched form solves all at once (FP32 persistent apply inside
-    FP64 refinement, graph-captured CG, autotuned tiles).  Each path runs once
-    untimed, then timed from the same initial state."""
+    1e-10; also timed with the multigrid preconditioner); the batched forms
+    solve all at once (persistent applies, graph-captured CG, autotuned
+    tiles): FP64 CG with Jacobi, FP32 CG inside FP64 refinement with Jacobi or
+    with the members' V-cycles batched, and FP64 CG under the batched FP32
+    V-cycles.  Each path runs once untimed, then timed from the same initial
+    state."""
     import ctypes as C
     from vpfhf_amd import _lib, workloads
     from vpfhf_amd.batch import Batch
@@ -290,7 +293,7 @@ def ensemble_bench(L, n_members=64, seed=0):
     infos = (_lib.SolveInfo * n_members)()
     res = {}
     for key, mixed, pc in (("fp64", False, _lib.PC_JACOBI), ("mixed", True, _lib.PC_JACOBI),
-                           ("mixed_mg", True, _lib.PC_MG)):
+                           ("mixed_mg", True, _lib.PC_MG), ("fp64_mg", False, _lib.PC_MG)):
         om = _lib.SolveOpts(1e-10, 0., 100000, pc, _lib.PREC_MIXED if mixed else _lib.PREC_FP64, 4)
 
         def batched():
@@ -317,6 +320,10 @@ def ensemble_bench(L, n_members=64, seed=0):
             "speedup_mixed_batched_vs_fp64_single": round(t64 / res["mixed"][0], 2),
             "speedup_mixed_multigrid_batched_vs_fp64_single": round(t64 / res["mixed_mg"][0], 2),
             "speedup_mixed_multigrid_batched_vs_multigrid_single": round(tmg / res["mixed_mg"][0], 2),
+            "batched_fp64_multigrid_solves_per_s": round(n_members / res["fp64_mg"][0], 2),
+            "speedup_fp64_multigrid_batched_vs_fp64_single": round(t64 / res["fp64_mg"][0], 2),
+            "speedup_fp64_multigrid_batched_vs_multigrid_single": round(tmg / res["fp64_mg"][0], 2),
+            "max_rel_err_fp64_multigrid_batched_vs_fp64": res["fp64_mg"][1],
             "max_rel_err_mixed_vs_fp64": res["mixed"][1], "max_rel_err_batched_fp64_vs_single": res["fp64"][1],
             "max_rel_err_mixed_multigrid_vs_fp64": res["mixed_mg"][1],
             "iterations": {k: v[2] for k, v in res.items()},

@@ -83,6 +83,43 @@ def test_batch_u_solve(mixed):
     B2.close()
 
 
+def test_batch_u_solve_fp64_multigrid():
+    """FP64 batched CG under the members' FP32 V-cycles, batched level by level
+    (the single-problem default solve, for the whole ensemble): within 1e-8 of
+    each member's own FP64 solve, bitwise the same alone and in the mix and
+    with graphs on or off, and a quarter of the Jacobi iterations or fewer."""
+    import vpfhf_amd as V
+    from vpfhf_amd.batch import Batch
+    from vpfhf_amd import _lib
+    hosts = _members([8, 13, 19, 25, 11, 32], seed=4)
+    Us = [h.state()[0] for h in hosts]
+    Bs = [np.zeros_like(u) for u in Us]
+    B = Batch(hosts)
+    B.set_graph(True, 8)
+    Ub, infos = B.u_solve(Us, Bs, rtol=1e-10, pc=_lib.PC_MG)
+    assert all(i.status == 0 for i in infos)
+    _, ij = B.u_solve(Us, Bs, rtol=1e-10)
+    assert sum(i.iterations for i in infos) * 4 < sum(i.iterations for i in ij)
+    L = V.load()
+    for h, U0, Ug in zip(hosts, Us, Ub):
+        Ur = np.array(U0)
+        info = _lib.SolveInfo()
+        o = _lib.SolveOpts(1e-11, 0., 100000, _lib.PC_MG, _lib.PREC_FP64, 4)
+        Z = np.zeros_like(U0)
+        V.check(L.vfg_u_solve(h.h, Ur.ctypes.data, Z.ctypes.data, o, info, V.HOST), h.h)
+        assert relerr(Ug, Ur) < 1e-8, relerr(Ug, Ur)
+    B2 = Batch([hosts[5]])
+    B2.set_graph(True, 8)
+    Ua, _ = B2.u_solve([Us[5]], [Bs[5]], rtol=1e-10, pc=_lib.PC_MG)
+    assert np.array_equal(Ua[0], Ub[5])
+    B.set_graph(False, 8)
+    Un, _ = B.u_solve(Us, Bs, rtol=1e-10, pc=_lib.PC_MG)
+    for a, b in zip(Un, Ub):
+        assert np.array_equal(a, b)
+    B.close()
+    B2.close()
+
+
 def test_batch_u_solve_multigrid():
     """Batched V-cycles (VFG_PC_MG, FP32 inner CG in FP64 refinement): within
     1e-7 of each member's single-problem FP64 solve, bitwise the same alone and

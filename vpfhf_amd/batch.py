@@ -78,7 +78,8 @@ class Batch:
         return [d.download(dt, len(x)) for d, x in zip(dy, xs)]
 
     def u_solve(self, Us, BCUs, rtol=1e-10, maxit=100000, mixed=False, pc=_lib.PC_JACOBI):
-        """Batched u-solve; pc = PC_MG (with mixed) runs the members' V-cycles batched."""
+        """Batched u-solve; pc = PC_MG runs the members' FP32 V-cycles batched (under
+        the FP64 CG, or the FP32 CG of the mixed refinement)."""
         Us = [np.array(u, np.float64, copy=True) for u in Us]
         Bs = [np.ascontiguousarray(b, np.float64) for b in BCUs]
         ua = (C.c_void_p * len(Us))(*[u.ctypes.data for u in Us])

@@ -193,6 +193,58 @@ __global__ __launch_bounds__(BRB) void k_bp(const BVec *vt, int o, const BChunk 
   for (int k = threadIdx.x; k < c.len; k += BRB) p[k] = (T)((double)z[k] + be * (double)p[k]);
 }
 
+// FP64 CG under the FP32 V-cycle (the single-problem default, batched):
+// x += alpha p, r -= alpha q in FP64, r32 = r for the V-cycle
+__global__ __launch_bounds__(BRB) void k_bxr_mg64(const BVec *vt, const BChunk *__restrict__ ch, const double *sc) {
+  const BChunk c = ch[blockIdx.x];
+  if (sc[(int64_t)c.p * B_NS + B_ACTIVE] == 0.) return;  // frozen member
+  double *x = vec<double>(vt, c, VX), *r = vec<double>(vt, c, VR);
+  const double *p = vec<double>(vt, c, VP), *q = vec<double>(vt, c, VQ);
+  float *r32 = vec<float>(vt, c, VR32);
+  const double al = sc[(int64_t)c.p * B_NS + B_ALPHA];
+  for (int k = threadIdx.x; k < c.len; k += BRB) {
+    x[k] += al * p[k];
+    const double ri = r[k] - al * q[k];
+    r[k] = ri;
+    r32[k] = (float)ri;
+  }
+}
+// r32 = r (before the first V-cycle)
+__global__ __launch_bounds__(BRB) void k_br32(const BVec *vt, const BChunk *__restrict__ ch) {
+  const BChunk c = ch[blockIdx.x];
+  const double *r = vec<double>(vt, c, VR);
+  float *r32 = vec<float>(vt, c, VR32);
+  for (int k = threadIdx.x; k < c.len; k += BRB) r32[k] = (float)r[k];
+}
+// partial sums of r.z and r.r with z the FP32 V-cycle output (and p = z at the start)
+__global__ __launch_bounds__(BRB) void k_brz64(const BVec *vt, const BChunk *__restrict__ ch, const double *sc, int setp,
+                                               double *part) {
+  const BChunk c = ch[blockIdx.x];
+  if (sc && sc[(int64_t)c.p * B_NS + B_ACTIVE] == 0.) return;
+  const double *r = vec<double>(vt, c, VR);
+  const float *z = vec<float>(vt, c, VZ32);
+  double *p = vec<double>(vt, c, VP);
+  double rz = 0., rr = 0.;
+  for (int k = threadIdx.x; k < c.len; k += BRB) {
+    const double ri = r[k], zi = (double)z[k];
+    rz += ri * zi;
+    rr += ri * ri;
+    if (setp) p[k] = zi;
+  }
+  rz = bbsum(rz);
+  rr = bbsum(rr);
+  if (threadIdx.x == 0) { part[2 * blockIdx.x] = rz; part[2 * blockIdx.x + 1] = rr; }
+}
+// p = z + beta p with z the FP32 V-cycle output
+__global__ __launch_bounds__(BRB) void k_bp64(const BVec *vt, const BChunk *__restrict__ ch, const double *sc) {
+  const BChunk c = ch[blockIdx.x];
+  if (sc[(int64_t)c.p * B_NS + B_ACTIVE] == 0.) return;  // frozen member
+  double *p = vec<double>(vt, c, VP);
+  const float *z = vec<float>(vt, c, VZ32);
+  const double be = sc[(int64_t)c.p * B_NS + B_BETA];
+  for (int k = threadIdx.x; k < c.len; k += BRB) p[k] = (double)z[k] + be * p[k];
+}
+
 // R32 = R * scale_p ; X32 = 0
 __global__ __launch_bounds__(BRB) void k_b64to32(const BVec *vt, const BChunk *__restrict__ ch, const double *scale) {
   const BChunk c = ch[blockIdx.x];
@@ -539,19 +591,30 @@ static void cg_iter(vfg_batch *b, int o, void *ops, int fp32, double rtol, int m
                      (const double *)b->d_sc, b->d_part);
   hipLaunchKernelGGL(k_bscalar, dim3(b->np), dim3(BRB), 0, s, (const double *)b->d_part, (const int *)b->d_first,
                      (const int *)b->d_nch, 1, b->d_sc, rtol, maxit);
+  constexpr bool F64 = sizeof(T) == 8;  // FP64 CG: the V-cycle works on the FP32 copies r32, z32
   if (mg) {
-    hipLaunchKernelGGL(k_bxr_mg<T>, dim3(b->nch), dim3(BRB), 0, s, (const BVec *)b->d_vt, o, ch,
-                       (const double *)b->d_sc);
+    if constexpr (F64)
+      hipLaunchKernelGGL(k_bxr_mg64, dim3(b->nch), dim3(BRB), 0, s, (const BVec *)b->d_vt, ch, (const double *)b->d_sc);
+    else
+      hipLaunchKernelGGL(k_bxr_mg<T>, dim3(b->nch), dim3(BRB), 0, s, (const BVec *)b->d_vt, o, ch,
+                         (const double *)b->d_sc);
     bmg_vcycle(b->bmg, b->d_sc + B_ACTIVE, B_NS, s);
-    hipLaunchKernelGGL(k_brz<T>, dim3(b->nch), dim3(BRB), 0, s, (const BVec *)b->d_vt, o, ch, (const double *)b->d_sc,
-                       0, b->d_part);
+    if constexpr (F64)
+      hipLaunchKernelGGL(k_brz64, dim3(b->nch), dim3(BRB), 0, s, (const BVec *)b->d_vt, ch, (const double *)b->d_sc, 0,
+                         b->d_part);
+    else
+      hipLaunchKernelGGL(k_brz<T>, dim3(b->nch), dim3(BRB), 0, s, (const BVec *)b->d_vt, o, ch, (const double *)b->d_sc,
+                         0, b->d_part);
   } else {
     hipLaunchKernelGGL(k_bxr<T>, dim3(b->nch), dim3(BRB), 0, s, (const BVec *)b->d_vt, o, ch, (const double *)b->d_sc,
                        b->d_part);
   }
   hipLaunchKernelGGL(k_bscalar, dim3(b->np), dim3(BRB), 0, s, (const double *)b->d_part, (const int *)b->d_first,
                      (const int *)b->d_nch, 2, b->d_sc, rtol, maxit);
-  hipLaunchKernelGGL(k_bp<T>, dim3(b->nch), dim3(BRB), 0, s, (const BVec *)b->d_vt, o, ch, (const double *)b->d_sc);
+  if (F64 && mg)
+    hipLaunchKernelGGL(k_bp64, dim3(b->nch), dim3(BRB), 0, s, (const BVec *)b->d_vt, ch, (const double *)b->d_sc);
+  else
+    hipLaunchKernelGGL(k_bp<T>, dim3(b->nch), dim3(BRB), 0, s, (const BVec *)b->d_vt, o, ch, (const double *)b->d_sc);
 }
 
 // runs CG on all members to rtol; the iterate state (x, r) is prepared by the caller
@@ -561,10 +624,16 @@ static int cg_run(vfg_batch *b, int o, void *ops, int fp32, double rtol, int max
   hipStream_t s = b->stream;
   if (mg) {
     // every member starts: the V-cycle on all of them (flags = 1)
+    if constexpr (sizeof(T) == 8)
+      hipLaunchKernelGGL(k_br32, dim3(b->nch), dim3(BRB), 0, s, (const BVec *)b->d_vt, (const BChunk *)b->d_ch);
     hipLaunchKernelGGL(k_bset_active, dim3(b->np), dim3(1), 0, s, b->d_sc);
     bmg_vcycle(b->bmg, b->d_sc + B_ACTIVE, B_NS, s);
-    hipLaunchKernelGGL(k_brz<T>, dim3(b->nch), dim3(BRB), 0, s, (const BVec *)b->d_vt, o, (const BChunk *)b->d_ch,
-                       (const double *)nullptr, 1, b->d_part);
+    if constexpr (sizeof(T) == 8)
+      hipLaunchKernelGGL(k_brz64, dim3(b->nch), dim3(BRB), 0, s, (const BVec *)b->d_vt, (const BChunk *)b->d_ch,
+                         (const double *)nullptr, 1, b->d_part);
+    else
+      hipLaunchKernelGGL(k_brz<T>, dim3(b->nch), dim3(BRB), 0, s, (const BVec *)b->d_vt, o, (const BChunk *)b->d_ch,
+                         (const double *)nullptr, 1, b->d_part);
   } else {
     hipLaunchKernelGGL(k_binit<T>, dim3(b->nch), dim3(BRB), 0, s, (const BVec *)b->d_vt, o, (const BChunk *)b->d_ch,
                        b->d_part);
@@ -639,12 +708,32 @@ extern "C" int vfg_batch_u_solve(vfg_batch *b, double *const *U, const double *c
   }
   hipLaunchKernelGGL(k_bprep, gch, blk, 0, s, vt, ch);  // r = -F, dinv = 1/diag (+ FP32 copy)
   std::vector<int> status(np, 0), iters(np, 0), outer(np, 0);
-  const bool mg = o->pc == VFG_PC_MG && o->precision == VFG_PREC_MIXED;
+  // multigrid: every member's FP32 V-cycle, batched level by level; under the
+  // FP64 CG (the single-problem default) or the FP32 CG of the refinement
+  const bool mg = o->pc == VFG_PC_MG;
   std::vector<double> relres(np, 0.), S;
   int rc = VFG_OK;
+  auto mg_setup_all = [&]() -> int {
+    if (!b->bmg) {
+      std::vector<float *> r32(np), z32(np);
+      for (int p = 0; p < np; p++) {
+        r32[p] = (float *)b->vt[p].v[VR32];
+        z32[p] = (float *)b->vt[p].v[VZ32];
+      }
+      BCHK(b, hipStreamSynchronize(s));
+      int e = bmg_create(b->m, r32.data(), z32.data(), &b->bmg);
+      if (e) return bfail(b, e, "batched multigrid set-up failed");
+      drop_graphs(b);
+    }
+    std::vector<const double *> dinv(np);
+    for (int p = 0; p < np; p++) dinv[p] = (const double *)b->vt[p].v[VD];
+    int e = bmg_setup(b->bmg, dinv.data(), s);
+    return e ? bfail(b, e, "batched multigrid set-up failed") : VFG_OK;
+  };
   if (o->precision != VFG_PREC_MIXED) {
     fill_ops(b->m, xs.data(), ys.data(), 0, b->oz, b->d_ops, s);
-    rc = cg_run<double>(b, 0, b->d_ops, 0, o->rtol, o->maxit, &b->gexec, S, false);
+    if (mg && (rc = mg_setup_all())) return rc;
+    rc = cg_run<double>(b, 0, b->d_ops, 0, o->rtol, o->maxit, &b->gexec, S, mg);
     for (int p = 0; p < np && !rc; p++) {
       const double *Sp = &S[(size_t)p * B_NS];
       iters[p] = (int)Sp[B_ITS];
@@ -655,24 +744,10 @@ extern "C" int vfg_batch_u_solve(vfg_batch *b, double *const *U, const double *c
   } else {
     // FP64 refinement around the FP32 inner CG
     fill_ops(b->m, xs32.data(), ys32.data(), 1, b->oz, b->d_ops32, s);
-    if (mg) {
-      if (!b->bmg) {
-        std::vector<float *> r32(np), z32(np);
-        for (int p = 0; p < np; p++) {
-          r32[p] = (float *)b->vt[p].v[VR32];
-          z32[p] = (float *)b->vt[p].v[VZ32];
-        }
-        BCHK(b, hipStreamSynchronize(s));
-        if ((rc = bmg_create(b->m, r32.data(), z32.data(), &b->bmg))) return bfail(b, rc, "batched multigrid set-up failed");
-        drop_graphs(b);
-      }
-      std::vector<const double *> dinv(np);
-      for (int p = 0; p < np; p++) dinv[p] = (const double *)b->vt[p].v[VD];
-      if ((rc = bmg_setup(b->bmg, dinv.data(), s))) return bfail(b, rc, "batched multigrid set-up failed");
-    }
+    if (mg && (rc = mg_setup_all())) return rc;
     std::vector<double> rn0(np, -1.), rprev(np, -1.), scale(np), iscale(np), nrm(np);
     std::vector<int> done(np, 0);
-    const double inner_rtol = 1e-4;
+    const double inner_rtol = mg ? 1e-5 : 1e-4;  // as the single-problem refinement
     const int max_outer = 60;
     for (int it = 0; it < max_outer && !rc; it++) {
       // per-member FP64 residual norms

@@ -27,6 +27,15 @@ void launch_batch_apply(int fp32, int sched, const void *d_ops, const BItem *ite
 void launch_engine_u_apply(int fp32, int sched, const void *d_ops, const BItem *items, int nitems,
                            const int *dcount, int grid, const double *flags, int fstride, hipStream_t s);
 void make_uop(const DevProblem &P, const void *x, void *y, int fp32, int OZ, void *out);  // host record
+// multigrid smoothing fused with the level apply, one record per member
+// (launch_mg_presmooth / launch_mg_postsmooth of every listed member at once;
+// 16 x 16 tiles, register march): post = 0: x = c2 D^-1 b into xo, A x into
+// ro; post = 1: xo = xin + c2 D^-1 (b - A xin)
+size_t mg_smooth_op_bytes(int post);
+void make_mg_smooth_op(const DevProblem &P, int post, const float *b, const float *dinv, float c2, const float *xin,
+                       float *xo, float *ro, int OZ, void *out);  // host record
+void launch_batch_mg_smooth(int post, const void *d_ops, const BItem *items, const int *dcount, int grid,
+                            const double *flags, int fstride, hipStream_t s);
 size_t vop_bytes();
 int v_oz(const Grid &G);
 void make_vop(const DevProblem &P, const double *x, double *y, const uint8_t *mask, void *out);

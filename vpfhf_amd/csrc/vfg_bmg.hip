@@ -55,7 +55,8 @@ __global__ __launch_bounds__(256) void k_bmg_restrict(const BmgVec *__restrict__
   }
 }
 
-// chunks over the fine nodes (level) of members that descend
+// chunks over the fine nodes (level) of members that descend: the coarse
+// correction is added to the level's iterate d (the post-smoothing leaves x)
 __global__ __launch_bounds__(256) void k_bmg_prolong(const BmgVec *__restrict__ lvt, int level,
                                                      const BmgChunk *__restrict__ ch, const int *__restrict__ dn,
                                                      const double *__restrict__ flags, int fstride) {
@@ -64,7 +65,7 @@ __global__ __launch_bounds__(256) void k_bmg_prolong(const BmgVec *__restrict__ 
   if (flags[(int64_t)c.p * fstride] == 0.) return;
   const BmgVec &f = lvt[c.p * BML + level];
   const BmgVec &g = lvt[c.p * BML + level + 1];
-  for (int t = threadIdx.x; t < c.len; t += 256) mg_prolong_node(f.G, g.G, f.bc, g.x, f.x, c.start + t);
+  for (int t = threadIdx.x; t < c.len; t += 256) mg_prolong_node(f.G, g.G, f.bc, g.x, f.d, c.start + t);
 }
 
 // ------------------------------------------------------------------ host side
@@ -82,6 +83,7 @@ struct BmgList {
 };
 struct BmgPlan {
   void *ops = nullptr;  // one FP32 apply record per member (level operator, x -> y)
+  void *ops_pre = nullptr, *ops_post = nullptr;  // fused smoothing records per member
   BmgList M, D, C;      // members with this level / that descend from it / whose coarsest it is
   Grow<BmgChunk> rch;   // restriction: coarse dofs of the descending members
   Grow<BmgChunk> pch;   // prolongation: fine nodes of the descending members
@@ -102,7 +104,7 @@ struct BatchMG {
   BmgCoef *d_coef = nullptr;
   std::vector<BmgVec> h_lvt;          // staging of the tables (kept: uploads are asynchronous)
   std::vector<BmgCoef> h_coef;
-  std::vector<std::vector<char>> h_ops;
+  std::vector<std::vector<char>> h_ops, h_pre, h_post;
   std::vector<void *> allocs;         // fixed allocations
   std::vector<void *> grown;          // growable buffers (freed with the object)
 };
@@ -204,6 +206,8 @@ int bmg_rebuild(BatchMG *g, hipStream_t s, bool *grew) {
     if (l >= Lmax && !P.dcnt) continue;  // nothing on this level, ever
     if (!P.dcnt && (rc = galloc(g, (void **)&P.dcnt, 8 * sizeof(int)))) break;
     if (!P.ops && (rc = galloc(g, &P.ops, batch_op_bytes(1) * g->np))) break;
+    if (!P.ops_pre && (rc = galloc(g, &P.ops_pre, mg_smooth_op_bytes(0) * g->np))) break;
+    if (!P.ops_post && (rc = galloc(g, &P.ops_post, mg_smooth_op_bytes(1) * g->np))) break;
     BmgList *ls[3] = {&P.M, &P.D, &P.C};
     for (int k = 0; k < 3 && !rc; k++)
       if ((rc = grow(g, ls[k]->items, grew)) || (rc = grow(g, ls[k]->ch, grew))) break;
@@ -247,6 +251,8 @@ static int bmg_alloc(int np, BatchMG **out) {
   g->h_lvt.assign((size_t)np * BML, BmgVec{});
   g->h_coef.assign((size_t)np * BML * 2 * BMK, BmgCoef{0.f, 0.f});
   g->h_ops.assign(BML, std::vector<char>(batch_op_bytes(1) * np, 0));
+  g->h_pre.assign(BML, std::vector<char>(mg_smooth_op_bytes(0) * np, 0));
+  g->h_post.assign(BML, std::vector<char>(mg_smooth_op_bytes(1) * np, 0));
   int rc = galloc(g, (void **)&g->d_lvt, sizeof(BmgVec) * np * BML);
   if (!rc) rc = galloc(g, (void **)&g->d_coef, sizeof(BmgCoef) * np * BML * 2 * BMK);
   if (rc) {
@@ -315,6 +321,12 @@ static void stage_member(BatchMG *g, int p) {
     cheb_coefs(V.lmax / 11., V.lmax, MG_SMOOTH, cf);           // smoother
     cheb_coefs(V.lmax / 30., V.lmax, MG_COARSE_DEG, cf + BMK);  // coarsest level
     make_uop(V.P, v.x, v.y, 1, g->oz[p][l], g->h_ops[l].data() + batch_op_bytes(1) * p);
+    // descending levels: pre-smoothing from 0 into d (A d into y), post-smoothing d -> x
+    static_assert(MG_SMOOTH == 1, "the fused smoothing records do one Chebyshev step");
+    make_mg_smooth_op(V.P, 0, v.b, v.dinv, cf[0].c2, nullptr, v.d, v.y, g->oz[p][l],
+                      g->h_pre[l].data() + mg_smooth_op_bytes(0) * p);
+    make_mg_smooth_op(V.P, 1, v.b, v.dinv, cf[0].c2, v.d, v.x, nullptr, g->oz[p][l],
+                      g->h_post[l].data() + mg_smooth_op_bytes(1) * p);
   }
 }
 static int upload_member(BatchMG *g, int p, hipStream_t s) {
@@ -324,9 +336,15 @@ static int upload_member(BatchMG *g, int p, hipStream_t s) {
       hipMemcpyAsync(g->d_coef + (size_t)p * BML * 2 * BMK, &g->h_coef[(size_t)p * BML * 2 * BMK],
                      sizeof(BmgCoef) * BML * 2 * BMK, hipMemcpyHostToDevice, s) != hipSuccess)
     return VFG_E_HIP;
+  const size_t pb = mg_smooth_op_bytes(0), qb = mg_smooth_op_bytes(1);
   for (int l = 0; l < g->L[p]; l++)
-    if (!g->lv[l].ops || hipMemcpyAsync((char *)g->lv[l].ops + ob * p, g->h_ops[l].data() + ob * p, ob,
-                                        hipMemcpyHostToDevice, s) != hipSuccess)
+    if (!g->lv[l].ops || !g->lv[l].ops_pre || !g->lv[l].ops_post ||
+        hipMemcpyAsync((char *)g->lv[l].ops + ob * p, g->h_ops[l].data() + ob * p, ob, hipMemcpyHostToDevice, s) !=
+            hipSuccess ||
+        hipMemcpyAsync((char *)g->lv[l].ops_pre + pb * p, g->h_pre[l].data() + pb * p, pb, hipMemcpyHostToDevice,
+                       s) != hipSuccess ||
+        hipMemcpyAsync((char *)g->lv[l].ops_post + qb * p, g->h_post[l].data() + qb * p, qb, hipMemcpyHostToDevice,
+                       s) != hipSuccess)
       return VFG_E_HIP;
   return VFG_OK;
 }
@@ -370,28 +388,34 @@ static void cheb(BatchMG *g, int l, const BmgList &L, int which, int step, int u
                      (const BmgCoef *)g->d_coef, step, use_y, xzero, flags, fstride);
 }
 
+static void smooth(BatchMG *g, int l, const BmgList &L, int post, const double *flags, int fstride, hipStream_t s) {
+  if (!L.items.cap) return;
+  const int grid = L.items.cap < g->slots ? L.items.cap : g->slots;
+  launch_batch_mg_smooth(post, post ? g->lv[l].ops_post : g->lv[l].ops_pre, L.items.d, g->lv[l].dcnt + 2, grid, flags,
+                         fstride, s);
+}
+
+// Descending members: one Chebyshev-Jacobi step from 0 fused with its apply
+// (d, A d), restriction of b - A d, the coarse correction added to d, one step
+// fused with its apply (d -> x) -- the single-problem V-cycle's kernels over
+// the work list.  Members whose coarsest level this is: the one-launch
+// Chebyshev solve, or Chebyshev steps with separate applies.
 void bmg_vcycle(BatchMG *g, const double *flags, int fstride, hipStream_t s) {
   for (int l = 0; l < g->Lcap; l++) {
     BmgPlan &P = g->lv[l];
-    // pre-smoothing (descending members) and the coarsest-level solve, from x = 0
+    if (P.D.ch.cap) smooth(g, l, P.D, 0, flags, fstride, s);
     if (P.cfit) {
-      cheb(g, l, P.D, 1, 0, 0, 1, flags, fstride, s);
       launch_bmg_coarse_cheb(P.ops, (const BmgVec *)g->d_lvt, l, (const BmgChunk *)P.C.ch.d,
                              (const int *)(P.dcnt + 5), P.C.ch.cap, (const BmgCoef *)g->d_coef, flags, fstride, s);
+    } else {
+      for (int i = 0; i < BMK && P.C.ch.cap; i++) {
+        if (i > 0) apply(g, l, P.C, 2, flags, fstride, s);
+        cheb(g, l, P.C, 2, i, i > 0, i == 0, flags, fstride, s);
+      }
     }
-    for (int i = 0; i < BMK && !P.cfit; i++) {
-      const int which = i < MG_SMOOTH ? 0 : 2;
-      const BmgList &L = which == 0 ? P.M : P.C;
-      if (!L.ch.cap) break;
-      if (i > 0) apply(g, l, L, which, flags, fstride, s);
-      cheb(g, l, L, which, i, i > 0, i == 0, flags, fstride, s);
-    }
-    if (P.D.ch.cap) {
-      apply(g, l, P.D, 1, flags, fstride, s);  // residual b - A x, restricted
-      if (P.rch.cap)
-        hipLaunchKernelGGL(k_bmg_restrict, dim3(P.rch.cap), dim3(256), 0, s, (const BmgVec *)g->d_lvt, l,
-                           (const BmgChunk *)P.rch.d, (const int *)(P.dcnt + 6), flags, fstride);
-    }
+    if (P.D.ch.cap && P.rch.cap)
+      hipLaunchKernelGGL(k_bmg_restrict, dim3(P.rch.cap), dim3(256), 0, s, (const BmgVec *)g->d_lvt, l,
+                         (const BmgChunk *)P.rch.d, (const int *)(P.dcnt + 6), flags, fstride);
   }
   for (int l = g->Lcap - 2; l >= 0; l--) {
     BmgPlan &P = g->lv[l];
@@ -399,10 +423,7 @@ void bmg_vcycle(BatchMG *g, const double *flags, int fstride, hipStream_t s) {
     if (P.pch.cap)
       hipLaunchKernelGGL(k_bmg_prolong, dim3(P.pch.cap), dim3(256), 0, s, (const BmgVec *)g->d_lvt, l,
                          (const BmgChunk *)P.pch.d, (const int *)(P.dcnt + 7), flags, fstride);
-    for (int i = 0; i < MG_SMOOTH; i++) {  // post-smoothing
-      apply(g, l, P.D, 1, flags, fstride, s);
-      cheb(g, l, P.D, 1, i, 1, 0, flags, fstride, s);
-    }
+    smooth(g, l, P.D, 1, flags, fstride, s);
   }
 }
 

@@ -1499,6 +1499,55 @@ void launch_engine_u_apply(int fp32, int sched, const void *d_ops, const BItem *
   else launch_batch_t<double>(sched, d_ops, items, nitems, dcount, grid, flags, fstride, s);
 }
 
+// multigrid smoothing records and their persistent launch
+template <class OpT>
+struct BSmooth {
+  OpT op;
+  int OZ;
+};
+using BMgPre = BSmooth<UMgPreOp>;
+using BMgPost = BSmooth<UMgPostOp<false>>;
+template <class Rec>
+__global__ __launch_bounds__(256, 2) void k_batch_mg_smooth(const Rec *__restrict__ recs, const BItem *__restrict__ items,
+                                                            const int *__restrict__ dcount,
+                                                            const double *__restrict__ flags, int fstride) {
+  const int n = *dcount;
+  for (int it = blockIdx.x; it < n; it += gridDim.x) {
+    const BItem w = items[it];
+    if (flags && flags[(int64_t)w.p * fstride] == 0.) continue;  // uniform per block
+    const auto op = recs[w.p].op;
+    const int OZ = recs[w.p].OZ;
+    __syncthreads();  // LDS of the previous item is free
+    march<float, 16, 16>(op.G, op, OZ, w.bx, w.by, w.bz);
+  }
+}
+size_t mg_smooth_op_bytes(int post) { return post ? sizeof(BMgPost) : sizeof(BMgPre); }
+void make_mg_smooth_op(const DevProblem &P, int post, const float *b, const float *dinv, float c2, const float *xin,
+                       float *xo, float *ro, int OZ, void *out) {
+  if (post) {
+    BMgPost r;
+    mg_op_fill(r.op, P, b, dinv, c2);
+    r.op.x = xin; r.op.xo = xo; r.op.r64 = nullptr; r.op.part = nullptr;
+    r.OZ = OZ;
+    *(decltype(r) *)out = r;
+  } else {
+    BMgPre r;
+    mg_op_fill(r.op, P, b, dinv, c2);
+    r.op.x = nullptr; r.op.xo = xo; r.op.ro = ro;
+    r.OZ = OZ;
+    *(decltype(r) *)out = r;
+  }
+}
+void launch_batch_mg_smooth(int post, const void *d_ops, const BItem *items, const int *dcount, int grid,
+                            const double *flags, int fstride, hipStream_t s) {
+  if (post)
+    hipLaunchKernelGGL(k_batch_mg_smooth<BMgPost>, dim3(grid), dim3(256), 0, s, (const BMgPost *)d_ops, items, dcount,
+                       flags, fstride);
+  else
+    hipLaunchKernelGGL(k_batch_mg_smooth<BMgPre>, dim3(grid), dim3(256), 0, s, (const BMgPre *)d_ops, items, dcount,
+                       flags, fstride);
+}
+
 // one member's operator entry for a persistent list (host memory `out`)
 void make_uop(const DevProblem &P, const void *x, void *y, int fp32, int OZ, void *out) {
   auto fill = [&](auto *b, auto one) {
