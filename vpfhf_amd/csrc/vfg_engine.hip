@@ -82,14 +82,17 @@ __global__ __launch_bounds__(ERB) void k_e_dot(const EVec *__restrict__ vt, cons
 }
 
 template <typename T>
-__device__ __forceinline__ void chunk_xr(const EVec &t, int o, const EChunk &c, double al, double &rz, double &rr) {
+__device__ __forceinline__ void chunk_xr(const EVec &t, int o, const EChunk &c, double al, double &rz, double &rr,
+                                         bool r32 = false) {
   T *x = (T *)t.v[o + QX] + c.start, *r = (T *)t.v[o + QR] + c.start, *z = (T *)t.v[o + QZ] + c.start;
   const T *p = (const T *)t.v[o + QP] + c.start, *q = (const T *)t.v[o + QQ] + c.start;
   const T *dinv = (const T *)t.v[o + QD] + c.start;
+  float *rf = (float *)t.v[QR32] + c.start;  // FP32 copy of r for the V-cycle (an FP64 multigrid CG)
   for (int k = threadIdx.x; k < c.len; k += ERB) {
     x[k] = (T)((double)x[k] + al * (double)p[k]);
     T ri = (T)((double)r[k] - al * (double)q[k]);
     r[k] = ri;
+    if (r32) rf[k] = (float)ri;
     T zi = (T)(dinv[k] * ri);
     z[k] = zi;
     rz += (double)ri * (double)zi;
@@ -110,7 +113,7 @@ __global__ __launch_bounds__(ERB) void k_e_xr(const EVec *__restrict__ vt, const
     const EVec t = vt[k.p];
     double rz = 0., rr = 0.;
     if (f) chunk_xr<float>(t, QX32, k, S[E_ALPHA], rz, rr);
-    else chunk_xr<double>(t, QX, k, S[E_ALPHA], rz, rr);
+    else chunk_xr<double>(t, QX, k, S[E_ALPHA], rz, rr, S[E_MGON] != 0.);
     rz = esum(rz);
     rr = esum(rr);
     if (threadIdx.x == 0) {
@@ -142,16 +145,33 @@ __global__ __launch_bounds__(ERB) void k_e_p(const EVec *__restrict__ vt, const 
   }
 }
 
-// part[2 pi] = r.z over the FP32 chunks of the slots whose z came from the
-// V-cycle this iteration (k_e_xr wrote the Jacobi value there)
+// part[2 pi] = r.z over the chunks of the slots whose z came from the
+// V-cycle this iteration (k_e_xr wrote the Jacobi value there): FP32 chunks
+// (the refinement's inner CG), and FP64 chunks (an FP64 CG under the FP32
+// V-cycle), where z is also copied to the FP64 z the direction update reads
 __global__ __launch_bounds__(ERB) void k_e_rz(const EVec *__restrict__ vt, const EChunk *__restrict__ c32,
-                                              const int *__restrict__ cnt, const double *__restrict__ sc,
-                                              double *part) {
-  const int n32 = cnt[C_F32];
-  for (int c = blockIdx.x; c < n32; c += gridDim.x) {
-    const EChunk k = c32[c];
+                                              const EChunk *__restrict__ c64, const int *__restrict__ cnt,
+                                              const double *__restrict__ sc, double *part) {
+  const int n32 = cnt[C_F32], n = n32 + cnt[C_F64];
+  for (int c = blockIdx.x; c < n; c += gridDim.x) {
+    const bool f = c < n32;
+    const EChunk k = f ? c32[c] : c64[c - n32];
     if (sc[(int64_t)k.p * E_NS + E_MGON] == 0.) continue;
-    double rz = chunk_dot<float>(vt[k.p], QR32, QZ32, k);
+    double rz;
+    if (f) {
+      rz = chunk_dot<float>(vt[k.p], QR32, QZ32, k);
+    } else {
+      const EVec t = vt[k.p];
+      const double *r = (const double *)t.v[QR] + k.start;
+      const float *z32 = (const float *)t.v[QZ32] + k.start;
+      double *z = (double *)t.v[QZ] + k.start;
+      rz = 0.;
+      for (int i = threadIdx.x; i < k.len; i += ERB) {
+        const double zi = (double)z32[i];
+        z[i] = zi;
+        rz += r[i] * zi;
+      }
+    }
     rz = esum(rz);
     if (threadIdx.x == 0) part[2 * k.pi] = rz;
   }
@@ -273,6 +293,11 @@ __global__ void k_e_r32(const double *R, double s, float *R32, float *X32, int64
   if (i >= n) return;
   R32[i] = (float)(R[i] * s);
   X32[i] = 0.f;
+}
+// Z = (double) Z32
+__global__ void k_e_f2d(const float *Z32, double *Z, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) Z[i] = (double)Z32[i];
 }
 // U += s * X32
 __global__ void k_e_axpy(double *U, const float *X32, double s, int64_t n) {
@@ -494,6 +519,17 @@ static int advance(vfg_engine *e, int p) {
         if (mixed) {
           launch_dot(h->w[QR], h->w[QR], n3, h->part, scal(e, p, E_RN), s);
           q.st = ST_U_CHECK;
+        } else if (e->mg) {  // FP64 CG under the slot's FP32 V-cycle: z0 = V-cycle(r0) of this slot alone
+          int rc = mg_lists(e);
+          if (rc) return rc;
+          launch_copy_d2f(h->w[QR], h->wf[QR32 - QX32], n3, s);
+          hipLaunchKernelGGL(k_e_flag, dim3(1), dim3(1), 0, s, e->d_one, p, 1.);
+          bmg_vcycle(e->bmg, e->d_one, 1, s);
+          hipLaunchKernelGGL(k_e_flag, dim3(1), dim3(1), 0, s, e->d_one, p, 0.);
+          hipLaunchKernelGGL(k_e_f2d, dim3(enb(n3)), dim3(256), 0, s, (const float *)h->wf[QZ32 - QX32], h->w[QZ], n3);
+          cg_init<double>(e, p, h->w[QR], (const double *)nullptr, h->w[QZ], h->w[QP], n3, e->o.u_rtol, e->o.maxit);
+          enter_cg(e, p, K_U64);
+          q.st = ST_U_CG;
         } else {
           cg_init<double>(e, p, h->w[QR], h->w[QD], h->w[QZ], h->w[QP], n3, e->o.u_rtol, e->o.maxit);
           enter_cg(e, p, K_U64);
@@ -741,7 +777,7 @@ static void e_iter(vfg_engine *e, bool with_mg) {
   if (with_mg) {  // z = V-cycle(r) of the multigrid slots, then their r.z
     bmg_vcycle(e->bmg, e->d_sc + E_MGON, E_NS, s);
     hipLaunchKernelGGL(k_e_rz, dim3(e->grid_vec), dim3(ERB), 0, s, (const EVec *)e->d_vt, (const EChunk *)e->d_ch[0],
-                       (const int *)e->d_cnt, (const double *)e->d_sc, e->d_part);
+                       (const EChunk *)e->d_ch[1], (const int *)e->d_cnt, (const double *)e->d_sc, e->d_part);
   }
   hipLaunchKernelGGL(k_e_scalar, dim3(e->ns), dim3(ERB), 0, s, (const double *)e->d_part, (const int *)e->d_first,
                      (const int *)e->d_nch, 2, e->d_sc);
@@ -753,7 +789,7 @@ static int replay(vfg_engine *e) {
   const int K = e->o.graph_chunk;
   bool with_mg = false;
   if (e->mg)
-    for (const ESlot &q : e->slot) with_mg |= q.st != ST_FREE && q.kind == K_U32;
+    for (const ESlot &q : e->slot) with_mg |= q.st != ST_FREE && (q.kind == K_U32 || q.kind == K_U64);
   if (!e->o.use_graph) {
     for (int k = 0; k < K; k++) e_iter(e, with_mg);
     ECHK(e, hipGetLastError());
@@ -848,7 +884,7 @@ int vfg_engine_create(const vfg_engine_opts *o, vfg_engine **out) {
             hipMalloc((void **)&e->d_first, sizeof(int) * e->ns) == hipSuccess &&
             hipMalloc((void **)&e->d_nch, sizeof(int) * e->ns) == hipSuccess &&
             hipMalloc((void **)&e->d_cnt, sizeof(int) * C_N) == hipSuccess;
-  e->mg = o->precision == VFG_PREC_MIXED && o->pc == VFG_PC_MG;
+  e->mg = o->pc == VFG_PC_MG;
   if (ok && e->mg)
     ok = bmg_create_slots(e->ns, &e->bmg) == VFG_OK && hipMalloc((void **)&e->d_one, 8 * (size_t)e->ns) == hipSuccess &&
          hipMemset(e->d_one, 0, 8 * (size_t)e->ns) == hipSuccess;
