@@ -126,6 +126,13 @@ int vfg_u_apply_f32(vfg_handle *h, const float *x, float *y, int loc);
  * operator (to rounding).  Default: VFG_U_FORM_RECOMPUTE. */
 enum { VFG_U_FORM_RECOMPUTE = 0, VFG_U_FORM_STORED = 1 };
 int vfg_u_set_form(vfg_handle *h, int form);
+/* PCApply of a PCSHELL around the u-MatShell: z = M r with M one FP32
+ * geometric-multigrid V-cycle (the VFG_PC_MG preconditioner of vfg_u_solve;
+ * FP64 in and out) on the free rows and z = r on the Dirichlet rows, the
+ * matrix-free stand-in for the PCHYPRE/GAMG preconditioner the reference sets
+ * on the u-KSP (VFCommon.c:1041-1047).  Each call refreshes the hierarchy's
+ * coefficients from the current fields. */
+int vfg_u_pc_apply(vfg_handle *h, const double *r, double *z, int loc);
 /* MATOP_GET_DIAGONAL of the same matrix (1 on Dirichlet rows) */
 int vfg_u_diag(vfg_handle *h, double *d, int loc);
 /* element load vector sum_e b_e of VF_GradientUThermoPoro3D_local

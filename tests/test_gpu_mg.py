@@ -110,3 +110,39 @@ def test_mg_altmin_matches_jacobi():
     assert relerr(Um, Uj) < 1e-7 and np.max(np.abs(Vm - Vj)) < 1e-7
     assert abs(em["elastic"] - ej["elastic"]) <= 1e-7 * abs(ej["elastic"])
     assert im.u_its * 5 < ij.u_its, (im.u_its, ij.u_its)
+
+
+@pytest.mark.parametrize("n,bc", [((17, 17, 17), "clamped"), ((26, 26, 26), "clamped"), ((65, 65, 33), "test1")],
+                         ids=["17-nested-small", "26-nonnested", "65x65x33-nested-separable"])
+def test_mg_vcycle_is_symmetric(n, bc):
+    """The V-cycle as an operator (vfg_u_pc_apply, the PCSHELL apply): CG needs
+    M symmetric positive definite, which holds when every restriction is the
+    transpose of its prolongation and the pre- and post-smoothing are adjoint.
+    r2.M r1 = r1.M r2 to FP32 rounding on grids that take each restriction
+    kernel (four lanes per entry onto small grids, the general non-nested
+    gather, the separable three-pass form between nested large levels);
+    r.M r > 0.  A transfer that is not the other's transpose breaks the first
+    by orders of magnitude.  On the Dirichlet rows M is the identity.""" 
+    s = S.make(n=n, seed=3, bcU=BCS[bc])
+    P = S.to_gpu(s)
+    free = P.bc_mask("U") == 0
+    rng = np.random.default_rng(9)
+    r1 = np.where(free, rng.standard_normal(free.size), 0.)
+    r2 = np.where(free, rng.standard_normal(free.size), 0.)
+    z1, z2 = P.u_pc_apply(r1), P.u_pc_apply(r2)
+    a, b = r2 @ z1, r1 @ z2
+    scale = np.linalg.norm(r1) * np.linalg.norm(z2)
+    assert abs(a - b) <= 1e-5 * scale, (a, b, scale)
+    assert r1 @ z1 > 0 and r2 @ z2 > 0
+    # the apply is deterministic and linear
+    assert np.array_equal(P.u_pc_apply(r1), z1)
+    z12 = P.u_pc_apply(r1 + 2. * r2)
+    assert np.linalg.norm(z12 - (z1 + 2. * z2)) <= 1e-5 * np.linalg.norm(z12)
+    # Dirichlet rows: identity, and they do not reach the free rows
+    rd = rng.standard_normal(free.size)
+    zd = P.u_pc_apply(r1 + np.where(free, 0., rd))
+    assert np.array_equal(zd[~free], rd[~free])
+    assert np.array_equal(zd[free], z1[free])
+    assert not free.all() and np.all(z1[~free] == 0.)
+    P.close()
+

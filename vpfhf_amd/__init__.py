@@ -165,6 +165,13 @@ class Problem:
         self._c(self.L.vfg_u_diag(self.h, _ptr(d), HOST))
         return d
 
+    def u_pc_apply(self, r):
+        """z = M r, M the multigrid V-cycle of the u-solve (PCApply of a PCSHELL)."""
+        r = np.ascontiguousarray(r, np.float64)
+        z = np.empty_like(r)
+        self._c(self.L.vfg_u_pc_apply(self.h, _ptr(r), _ptr(z), HOST))
+        return z
+
     def u_rhs(self):
         b = np.empty(3 * self.nn)
         self._c(self.L.vfg_u_rhs(self.h, _ptr(b), HOST))

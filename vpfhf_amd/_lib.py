@@ -88,6 +88,7 @@ PROTOS = {
     "vfg_u_apply_f32": (C.c_int, [H, vp, vp, C.c_int]),
     "vfg_u_set_form": (C.c_int, [H, C.c_int]),
     "vfg_u_diag": (C.c_int, [H, vp, C.c_int]),
+    "vfg_u_pc_apply": (C.c_int, [H, vp, vp, C.c_int]),
     "vfg_u_rhs": (C.c_int, [H, vp, C.c_int]),
     "vfg_u_residual": (C.c_int, [H, vp, vp, vp, C.c_int]),
     "vfg_v_set_u": (C.c_int, [H, vp, C.c_int]),

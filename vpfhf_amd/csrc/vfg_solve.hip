@@ -173,10 +173,14 @@ static int pcg(vfg_handle *h, int64_t n, const ApplyFn &A, const double *dinv, d
   double rate = -1.;  // observed residual reduction per iteration
   while (status == VFG_OK && rn > tol && it < maxit) {
     int todo = check < maxit - it ? check : maxit - it;
-    // run only the iterations the observed rate says are left (at most check)
+    // run only the iterations the observed rate says are left (at most
+    // check, at least check / 4: the Jacobi-PCG residual is not monotone, and
+    // one lucky drop must not shrink the batches to single iterations, each
+    // ending in a host synchronisation)
     if (rate > 0. && rate < 1.) {
       const double need = ceil(log(tol / rn) / log(rate));
-      if (need >= 1. && need < todo) todo = (int)need;
+      const int floor_ = check / 4 > 1 ? check / 4 : 1;
+      if (need >= 1. && need < todo) todo = (int)need > floor_ ? (int)need : (floor_ < todo ? floor_ : todo);
     }
     const double rn_before = rn;
     for (int k = 0; k < todo; k++) {
@@ -564,10 +568,12 @@ static int pcg32(vfg_handle *h, int64_t n, const std::function<void(const float 
   double rate = -1.;  // observed residual reduction per iteration
   while (rn > rtol * rn0 && it < maxit) {
     int todo = check < maxit - it ? check : maxit - it;
-    // run only the iterations the observed rate says are left (at most check)
+    // run only the iterations the observed rate says are left (at most check,
+    // at least check / 4, as in pcg)
     if (rate > 0. && rate < 1.) {
       const double need = ceil(log(rtol * rn0 / rn) / log(rate));
-      if (need >= 1. && need < todo) todo = (int)need;
+      const int floor_ = check / 4 > 1 ? check / 4 : 1;
+      if (need >= 1. && need < todo) todo = (int)need > floor_ ? (int)need : (floor_ < todo ? floor_ : todo);
     }
     const double rn_before = rn;
     for (int k = 0; k < todo; k++) {
@@ -1066,6 +1072,46 @@ void vfg_solve_opts_default(vfg_solve_opts *o) {
   o->pc = VFG_PC_MG;  // the reference preconditions its u-CG with multigrid (PCHYPRE)
   o->precision = VFG_PREC_FP64;
   o->cheb_degree = 4;
+}
+
+// on the u-Dirichlet rows: dst = 0 (src null) or dst = src
+__global__ void k_u_dirichlet_rows(Grid G, BcMask bc, const double *src, double *dst) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= (int64_t)G.nx * G.ny * G.nz) return;
+  const int i = (int)(n % G.nx), j = (int)((n / G.nx) % G.ny), k = (int)(n / ((int64_t)G.nx * G.ny));
+  for (int c = 0; c < 3; c++)
+    if (on_dirichlet(G, bc.face[c], bc.vert[c], i, j, k)) dst[3 * n + c] = src ? src[3 * n + c] : 0.;
+}
+
+// z = M r with M the V-cycle of VFG_PC_MG for the current coefficients (the
+// hierarchy's coefficients, diagonals and smoother bounds are refreshed
+// first) on the free rows, and z = r on the Dirichlet rows (the inverse of the
+// operator's identity rows): M = diag(I, M_free) is symmetric positive
+// definite when the V-cycle is
+int vfg_u_pc_apply(vfg_handle *h, const double *r, double *z, int loc) {
+  if (!h) return vfg_fail(nullptr, VFG_E_ARG, "NULL handle");
+  if (!r || !z) return vfg_fail(h, VFG_E_ARG, "vfg_u_pc_apply: NULL array");
+  const int64_t n = 3 * h->P.nn;
+  hipStream_t s = h->stream;
+  int rc;
+  if ((rc = vfg_work(h, 7, 2))) return rc;
+  double *rd = h->w[1], *rin = h->w[2], *q = h->w[4], *dinv = h->w[5];
+  if ((rc = vfg_in(h, rin, r, 8 * n, loc))) return rc;
+  launch_u_diag(h->P, q, s);
+  hipLaunchKernelGGL(k_recip, dim3(nb_(n)), dim3(256), 0, s, (const double *)q, dinv, n);
+  if ((rc = mg_setup(h, dinv))) return rc;
+  if (hipMemcpyAsync(rd, rin, 8 * n, hipMemcpyDeviceToDevice, s) != hipSuccess)
+    return vfg_fail(h, VFG_E_HIP, "vfg_u_pc_apply: copy failed");
+  const unsigned nbn = (unsigned)((h->P.nn + 255) / 256);
+  hipLaunchKernelGGL(k_u_dirichlet_rows, dim3(nbn), dim3(256), 0, s, h->P.G, h->P.bcU, (const double *)nullptr, rd);
+  launch_copy_d2f(rd, h->wf[0], n, s);
+  mg_apply(h, h->wf[0], h->wf[1]);
+  launch_copy_f2d(h->wf[1], rd, n, s);
+  hipLaunchKernelGGL(k_u_dirichlet_rows, dim3(nbn), dim3(256), 0, s, h->P.G, h->P.bcU, (const double *)rin, rd);
+  hipError_t e = hipMemcpyAsync(z, rd, 8 * n, loc == VFG_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, s);
+  if (e != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+    return vfg_fail(h, VFG_E_HIP, "vfg_u_pc_apply: kernel execution failed");
+  return VFG_OK;
 }
 
 int vfg_u_solve(vfg_handle *h, double *U, const double *BCU, const vfg_solve_opts *o, vfg_solve_info *info, int loc) {

@@ -630,33 +630,77 @@ struct UDiagOp {
   BcMask bc;
   double E, lam1, mu1, eta, vol;
   __device__ double load(int, int64_t n) const { return v[n]; }
+  // diag_{a,c} = sum_g A_g (d_c N_a)^2 + M_g |grad N_a|^2 with M = W s mu and
+  // A = M + the spherical weight.  (d_x N_a)^2 = P_j(y)^2 P_k(z)^2 / hx^2 does
+  // not depend on the node's x-index i, so the x-term needs the sums of A and M
+  // over gx on the 3 x 3 (gy, gz) lines only, contracted with P_j^2 P_k^2; the
+  // y- and z-terms alike.
   __device__ void element(int ex, int ey, int ek, const double (&xin)[2][4][1], double (&r)[8][3]) const {
     double Ee = E_elem ? E_elem[cell_id(G, ex, ey, ek)] : E;
     double cE = Ee * vol;
     const uint32_t cm = cmask ? cmask[cell_id(G, ex, ey, ek)] : 0u;
     const double kap1 = (lam1 + 2. * mu1 / 3.) * (1. + eta);
-    const double ih[3] = {1. / G.hx, 1. / G.hy, 1. / G.hz};
+    const double i2[3] = {1. / (G.hx * G.hx), 1. / (G.hy * G.hy), 1. / (G.hz * G.hz)};
+    double vx[2][2][3];  // v along x on the (j, k) lines
+#pragma unroll
+    for (int k = 0; k < 2; k++)
+#pragma unroll
+      for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int gx = 0; gx < 3; gx++) vx[k][j][gx] = gP0(gx) * xin[k][j * 2][0] + gP1(gx) * xin[k][j * 2 + 1][0];
+    // [A, M] summed over gx ([gz][gy]), over gy ([gz][gx]), over gz ([gy][gx])
+    double Sx[2][3][3] = {}, Sy[2][3][3] = {}, Sz[2][3][3] = {};
+#pragma unroll
     for (int gz = 0; gz < 3; gz++)
+#pragma unroll
       for (int gy = 0; gy < 3; gy++)
+#pragma unroll
         for (int gx = 0; gx < 3; gx++) {
-          double vv = 0.;
-          for (int a = 0; a < 8; a++) {
-            int i = a & 1, j = (a >> 1) & 1, k = a >> 2;
-            vv += (i ? gP1(gx) : gP0(gx)) * (j ? gP1(gy) : gP0(gy)) * (k ? gP1(gz) : gP0(gz)) * xin[k][j * 2 + i][0];
-          }
+          const double vv = gP0(gz) * (gP0(gy) * vx[0][0][gx] + gP1(gy) * vx[0][1][gx]) +
+                            gP1(gz) * (gP0(gy) * vx[1][0][gx] + gP1(gy) * vx[1][1][gx]);
           const double w = cE * gW(gx) * gW(gy) * gW(gz), s = vv * vv + eta;
           // spherical weight: lam s, or kappa (1 + eta) - 2 mu s / 3 at a compressed point
           const double lw = ((cm >> ((gz * 3 + gy) * 3 + gx)) & 1u) ? w * (kap1 - 2. * mu1 * s / 3.) : w * s * lam1;
-          const double mw = w * s * mu1;
-          for (int a = 0; a < 8; a++) {
-            int i = a & 1, j = (a >> 1) & 1, k = a >> 2;
-            double px = i ? gP1(gx) : gP0(gx), py = j ? gP1(gy) : gP0(gy), pz = k ? gP1(gz) : gP0(gz);
-            double dg[3] = {(i ? ih[0] : -ih[0]) * py * pz, (j ? ih[1] : -ih[1]) * px * pz,
-                            (k ? ih[2] : -ih[2]) * px * py};
-            double g2 = dg[0] * dg[0] + dg[1] * dg[1] + dg[2] * dg[2];
-            for (int c = 0; c < 3; c++) r[a][c] += (lw + mw) * dg[c] * dg[c] + mw * g2;
-          }
+          const double mw = w * s * mu1, aw = lw + mw;
+          Sx[0][gz][gy] += aw;
+          Sx[1][gz][gy] += mw;
+          Sy[0][gz][gx] += aw;
+          Sy[1][gz][gx] += mw;
+          Sz[0][gy][gx] += aw;
+          Sz[1][gy][gx] += mw;
         }
+    // T[q][b][a] = sum over the two remaining directions of S[q] P_a^2 P_b^2
+    double Tx[2][2][2], Ty[2][2][2], Tz[2][2][2];
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+#pragma unroll
+      for (int b = 0; b < 2; b++)
+#pragma unroll
+        for (int a = 0; a < 2; a++) {
+          double tx = 0., ty = 0., tz = 0.;
+#pragma unroll
+          for (int u = 0; u < 3; u++) {
+            const double pb = b ? gP1(u) : gP0(u);
+#pragma unroll
+            for (int t = 0; t < 3; t++) {
+              const double pa = a ? gP1(t) : gP0(t), w2 = pa * pa * pb * pb;
+              tx += Sx[q][u][t] * w2;
+              ty += Sy[q][u][t] * w2;
+              tz += Sz[q][u][t] * w2;
+            }
+          }
+          Tx[q][b][a] = tx;  // [k][j]
+          Ty[q][b][a] = ty;  // [k][i]
+          Tz[q][b][a] = tz;  // [j][i]
+        }
+#pragma unroll
+    for (int a = 0; a < 8; a++) {
+      const int i = a & 1, j = (a >> 1) & 1, k = a >> 2;
+      const double g2 = i2[0] * Tx[1][k][j] + i2[1] * Ty[1][k][i] + i2[2] * Tz[1][j][i];
+      r[a][0] += i2[0] * Tx[0][k][j] + g2;
+      r[a][1] += i2[1] * Ty[0][k][i] + g2;
+      r[a][2] += i2[2] * Tz[0][j][i] + g2;
+    }
   }
   __device__ void store(int i, int j, int k, const double (&acc)[3], const double (&)[1]) const {
     int64_t n = node_id(G, i, j, k);
