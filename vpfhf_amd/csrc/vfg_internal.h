@@ -41,6 +41,21 @@ __device__ __forceinline__ int64_t cell_id(const Grid &G, int i, int j, int k) {
   return (int64_t)i + (int64_t)G.nex * ((int64_t)j + (int64_t)G.ney * k);
 }
 
+// Workgroups go to the eight XCDs round-robin by linear id, and each XCD has
+// its own L2.  xcd_block renumbers a 3-D grid so that XCD k runs one
+// contiguous range of blocks in (x, y, z) order: tiles that are neighbours in
+// x and y, whose halo node lines overlap, then meet in the same L2.
+struct Blk3 {
+  int x, y, z;
+};
+__device__ __forceinline__ Blk3 xcd_block() {
+  const unsigned gx = gridDim.x, gy = gridDim.y, n = gx * gy * gridDim.z;
+  const unsigned id = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const unsigned per = n / 8, rem = n % 8, x = id % 8, loc = id / 8;
+  const unsigned l = (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + loc;
+  return Blk3{(int)(l % gx), (int)((l / gx) % gy), (int)(l / (gx * gy))};
+}
+
 // tr(eps(u)) at Gauss point (gx, gy, gz) of an element from its 8 nodal
 // displacements (local node a = (k*2+j)*2+i), accumulated per component over
 // the nodes and summed as (d1u1 + d2u2) + d3u3: every kernel that branches on
@@ -562,6 +577,16 @@ __device__ __forceinline__ void gauss_walk(const double (&fld)[NG + NV][8], doub
 template <typename T>
 __device__ __forceinline__ T interp(int g, T f0, T f1) {
   return T(gP0(g)) * f0 + T(gP1(g)) * f1;
+}
+
+// the marches at a renumbered block (xcd_block)
+template <typename T, int TX, int TY, class Op>
+__device__ __forceinline__ void march(const Grid &G, const Op &op, int OZ, Blk3 b, int rslot = 0) {
+  march<T, TX, TY>(G, op, OZ, b.x, b.y, b.z, rslot);
+}
+template <typename T, int TX, int TY, bool PF, class Op>
+__device__ __forceinline__ void march_lds(const Grid &G, const Op &op, int OZ, Blk3 b, int rslot = 0) {
+  march_lds<T, TX, TY, PF>(G, op, OZ, b.x, b.y, b.z, rslot);
 }
 
 }  // namespace vfg

@@ -116,7 +116,8 @@ __global__ __launch_bounds__(EB, 2) void k_energies(DevProblem P, const double *
   __shared__ double sh[EB / 64];
   const Grid &G = P.G;
   const int tid = threadIdx.x, tx = tid % ET, ty = tid / ET;
-  const int ex0 = blockIdx.x * ET, ey0 = blockIdx.y * ET, ek0 = blockIdx.z * EZ;
+  const Blk3 bk = xcd_block();
+  const int ex0 = bk.x * ET, ey0 = bk.y * ET, ek0 = bk.z * EZ;
   const int ek1 = min(ek0 + EZ, G.nez);
   const int ex = ex0 + tx, ey = ey0 + ty;
   const bool ok = ex < G.nex && ey < G.ney;
@@ -188,7 +189,7 @@ __global__ __launch_bounds__(EB, 2) void k_energies(DevProblem P, const double *
   double r0 = block_sum<EB>(el, sh), r1 = block_sum<EB>(pw, sh), r2 = block_sum<EB>(se, sh),
          r3 = block_sum<EB>(cv, sh);
   if (tid == 0) {
-    const int64_t blk = blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z);
+    const int64_t blk = bk.x + (int64_t)gridDim.x * (bk.y + (int64_t)gridDim.y * bk.z);  // partials in block order
     partials[blk * 4 + 0] = r0; partials[blk * 4 + 1] = r1;
     partials[blk * 4 + 2] = r2; partials[blk * 4 + 3] = r3;
   }
@@ -361,11 +362,19 @@ __global__ __launch_bounds__(RB) void k_maxabs(const double *a, const double *b,
   }
   if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
-__global__ void k_maxreduce(const double *part, int nb, double *out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// max over the partials: one workgroup, strided loads, then a tree in LDS
+// (max is exact, so any order gives the same value)
+__global__ __launch_bounds__(RB) void k_maxreduce(const double *part, int nb, double *out) {
+  __shared__ double sh[RB];
   double m = 0.;
-  for (int i = 0; i < nb; i++) m = fmax(m, part[i]);
-  *out = m;
+  for (int i = threadIdx.x; i < nb; i += RB) m = fmax(m, part[i]);
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = RB / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sh[0];
 }
 void launch_vmin(const double *a, const double *b, double *o, int64_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_vmin, dim3(nblk(n)), dim3(256), 0, s, a, b, o, n);
@@ -376,7 +385,7 @@ void launch_scale(double *x, double v, int64_t n, hipStream_t s) {
 void launch_maxabs_diff(const double *a, const double *b, int64_t n, double *part, double *out, hipStream_t s) {
   int nb = red_blocks(n);
   hipLaunchKernelGGL(k_maxabs, dim3(nb), dim3(RB), 0, s, a, b, n, part);
-  hipLaunchKernelGGL(k_maxreduce, dim3(1), dim3(64), 0, s, (const double *)part, nb, out);
+  hipLaunchKernelGGL(k_maxreduce, dim3(1), dim3(RB), 0, s, (const double *)part, nb, out);
 }
 
 }  // namespace vfg

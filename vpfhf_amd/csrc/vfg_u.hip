@@ -426,7 +426,7 @@ struct UApplyOp {
 // FP32: at most 256 registers, two waves per SIMD
 template <typename T, int TX, int TY, bool UNI = false>
 __global__ __launch_bounds__(TX *TY, sizeof(T) == 4 ? 2 : 1) void k_u_apply(UApplyOp<T, UNI> op, int OZ) {
-  march<T, TX, TY>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z);
+  march<T, TX, TY>(op.G, op, OZ, xcd_block());
 }
 
 // FP32 multigrid apply fused with its Chebyshev(1)-Jacobi step (mg_cheb_entry
@@ -496,23 +496,23 @@ struct UMgPostRzOp : UMgPostOp<true> {
 };
 template <int TX, int TY>
 __global__ __launch_bounds__(TX *TY, 2) void k_u_mg_pre(UMgPreOp op, int OZ) {
-  march<float, TX, TY>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z);
+  march<float, TX, TY>(op.G, op, OZ, xcd_block());
 }
 template <int TX, int TY>
 __global__ __launch_bounds__(TX *TY, 2) void k_u_mg_post(UMgPostOp<false> op, int OZ) {
-  march<float, TX, TY>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z);
+  march<float, TX, TY>(op.G, op, OZ, xcd_block());
 }
 template <int TX, int TY>
 __global__ __launch_bounds__(TX *TY, 2) void k_u_mg_post_rz(UMgPostRzOp op, int OZ) {
-  march<float, TX, TY>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z,
-                       blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
+  march<float, TX, TY>(op.G, op, OZ, xcd_block(), blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
 }
 
 // FP64 apply with the corners in LDS: at most 256 registers, two waves per
 // SIMD; the next plane is loaded at the start of its layer (the other
 // resident workgroup covers the latency; a register prefetch measured slower)
 __global__ __launch_bounds__(256, 2) void k_u_apply_lds(UApplyOp<double, false> op, int OZ) {
-  march_lds<double, 16, 16, false>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z);
+  const Blk3 b = xcd_block();
+  march_lds<double, 16, 16, false>(op.G, op, OZ, b.x, b.y, b.z);
 }
 
 // the same apply, also reducing p.q (p = the input, q = the output, every
@@ -533,7 +533,8 @@ struct UApplyDotOp : UApplyOp<double, false> {
   __device__ void partial(int slot, double v) const { part[slot] = v; }
 };
 __global__ __launch_bounds__(256, 2) void k_u_apply_lds_dot(UApplyDotOp op, int OZ) {
-  march_lds<double, 16, 16, false>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z,
+  const Blk3 b = xcd_block();
+  march_lds<double, 16, 16, false>(op.G, op, OZ, b.x, b.y, b.z,
                                    blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
 }
 
@@ -590,8 +591,9 @@ __global__ __launch_bounds__(256) void k_u_qsetup_lds(Grid G, const double *__re
                                                       const double *__restrict__ E_elem, double E, double eta,
                                                       double *__restrict__ uq) {
   const int tid = threadIdx.x;
-  const int ex = blockIdx.x * 15 - 1 + tid % 16, ey = blockIdx.y * 15 - 1 + tid / 16, ek = blockIdx.z;
-  double *q = uq + ((int64_t)(blockIdx.x + gridDim.x * blockIdx.y) * G.nez + ek) * 27 * 256 + tid;
+  const Blk3 b = xcd_block();
+  const int ex = b.x * 15 - 1 + tid % 16, ey = b.y * 15 - 1 + tid / 16, ek = b.z;
+  double *q = uq + ((int64_t)(b.x + gridDim.x * b.y) * G.nez + ek) * 27 * 256 + tid;
   if (ex < 0 || ex >= G.nex || ey < 0 || ey >= G.ney) {
     for (int g = 0; g < 27; g++) q[g * 256] = 0.;
     return;
@@ -708,7 +710,7 @@ struct UDiagOp {
   }
 };
 
-__global__ __launch_bounds__(256) void k_u_diag(UDiagOp op, int OZ) { march<double, 16, 16>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z); }
+__global__ __launch_bounds__(256) void k_u_diag(UDiagOp op, int OZ) { march<double, 16, 16>(op.G, op, OZ, xcd_block()); }
 
 // ------------------------------------------------------- load vector / residual
 // b_{a,c} = sum_g W d_c phi_a [(3lam+2mu) alpha dth s + beta p v]          (thermo-poro)
@@ -1043,12 +1045,12 @@ struct ULoadsOp {
   }
 };
 __global__ __launch_bounds__(256, 2) void k_u_loads(ULoadsOp op, int OZ) {
-  march_lds<double, 16, 16, false>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z);
+  march_lds<double, 16, 16, false>(op.G, op, OZ, xcd_block());
 }
 
 template <bool UNI>
 __global__ __launch_bounds__(256) void k_u_rhs(URhsOp<UNI> op, int OZ) {
-  march<double, 16, 16>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z);
+  march<double, 16, 16>(op.G, op, OZ, xcd_block());
 }
 
 // ------------------------------------------------------------ host launchers

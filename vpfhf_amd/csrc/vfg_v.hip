@@ -62,8 +62,9 @@ static VConst make_vconst(const DevProblem &P) {
 __global__ __launch_bounds__(256) void k_v_setup(DevProblem P, const double *__restrict__ U) {
   const Grid &G = P.G;
   const int tid = threadIdx.x;
-  const int ei = blockIdx.x * 15 - 1 + tid % 16, ej = blockIdx.y * 15 - 1 + tid / 16, ek = blockIdx.z;
-  const int tile = blockIdx.x + gridDim.x * blockIdx.y;
+  const Blk3 b = xcd_block();
+  const int ei = b.x * 15 - 1 + tid % 16, ej = b.y * 15 - 1 + tid / 16, ek = b.z;
+  const int tile = b.x + gridDim.x * b.y;
   double *out = P.Vq + tiled_index(tile, ek, G.nez, 27, tid, 0, 256);
   if (ei < 0 || ei >= G.nex || ej < 0 || ej >= G.ney) {
     for (int g = 0; g < 27; g++) out[g * 256] = 0.;
@@ -267,7 +268,7 @@ struct VApplyCgOp : VApplyOp {
   __device__ void partial(int slot, double v) const { part[slot] = v; }
 };
 
-__global__ __launch_bounds__(256, 3) void k_v_apply(VApplyOp op, int OZ) { march<double, 16, 16>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z); }
+__global__ __launch_bounds__(256, 3) void k_v_apply(VApplyOp op, int OZ) { march<double, 16, 16>(op.G, op, OZ, xcd_block()); }
 
 // The same over a list of workgroups.  The reduced Newton systems of the
 // V-solve live on the free nodes, a thin band around the cracks: a workgroup
@@ -461,7 +462,7 @@ struct VDiagOp {
   }
 };
 
-__global__ __launch_bounds__(256) void k_v_diag(VDiagOp op, int OZ) { march<double, 16, 16>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z); }
+__global__ __launch_bounds__(256) void k_v_diag(VDiagOp op, int OZ) { march<double, 16, 16>(op.G, op, OZ, xcd_block()); }
 __global__ __launch_bounds__(256) void k_v_diag_list(VDiagOp op, const int *__restrict__ list, int cap) {
   const VChunk q = chunk_at(op.G, list, cap, blockIdx.x);
   march<double, 16, 16>(op.G, op, q.L, q.bx, q.by, q.bz);
@@ -551,7 +552,7 @@ struct VRhsOp {
   }
 };
 
-__global__ __launch_bounds__(256) void k_v_rhs(VRhsOp op, int OZ) { march<double, 16, 16>(op.G, op, OZ, blockIdx.x, blockIdx.y, blockIdx.z); }
+__global__ __launch_bounds__(256) void k_v_rhs(VRhsOp op, int OZ) { march<double, 16, 16>(op.G, op, OZ, xcd_block()); }
 
 // ------------------------------------------------------------ launchers
 static dim3 mgrid(const Grid &G, int OZ) {
