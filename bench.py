@@ -66,11 +66,19 @@ def copy_calibration(L, h, n=1 << 26, reps=4):
 
 
 def cpu_model():
-    """Model name of the host CPU (first 'model name' of /proc/cpuinfo)."""
+    """Model name of the host CPU with its family / model / stepping numbers
+    (first processor of /proc/cpuinfo; a virtual machine may report a short
+    name)."""
     try:
+        info = {}
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            if not line.strip():
+                break
+            k, _, v = line.partition(":")
+            info.setdefault(k.strip(), v.strip())
+        if "model name" in info:
+            return "%s (family %s, model %s, stepping %s)" % (info["model name"], info.get("cpu family", "?"),
+                                                            info.get("model", "?"), info.get("stepping", "?"))
     except OSError:
         pass
     return platform.processor() or "unknown"
@@ -530,7 +538,7 @@ def main():
         out["apply_%d" % a.n3] = apply_at(a.n3, a.steps, a.warmup, a.seed)
     out.update(extra)
     H.close()
-    if not (a.no_solves or a.no_forms):
+    if not a.no_forms:
         for key, fn in (("ensemble", lambda: ensemble_bench(L)), ("precision", precision_bench),
                         ("serving", lambda: serving_bench(n_req=a.serve_requests, max_slots=a.serve_slots))):
             if key not in a.forms.split(","):
