@@ -41,6 +41,8 @@ struct vfg_handle {
   double *abh = nullptr;       // CG coefficient history of an eigenvalue estimate (3 x 64)
   uint32_t *vflag = nullptr;   // V-solve: node-plane chunks holding free nodes (v_chunk_cap flags)
   int *vlist = nullptr;        //          and their list, count and chunk length (v_chunk_cap + 2)
+  uint32_t *vflag2 = nullptr;  // the same dilated by one tile and one plane,
+  int *vlist2 = nullptr;       //          and its list
   bool v_ready = false;
   // solver work space (allocated on first use)
   double *w[16] = {nullptr};

@@ -89,6 +89,11 @@ void launch_v_diag(const DevProblem &P, double *d, hipStream_t s, const uint8_t 
 // list[cap] and L in list[cap + 1] (cap = v_chunk_cap, flag holds cap words).
 int v_chunk_cap(const Grid &G);
 void launch_v_chunk_list(const DevProblem &P, const uint8_t *mask, uint32_t *flag, int *list, hipStream_t s);
+// the same list for the flags of launch_v_chunk_list dilated by one tile and
+// one node plane (flag2: v_chunk_cap words): the chunks whose nodes share an
+// element with a listed node, where K x changes when x changes on the listed
+// nodes only
+void launch_v_dilated_list(const DevProblem &P, const uint32_t *flag, uint32_t *flag2, int *list2, hipStream_t s);
 // CG-fused masked V apply (mask: fixed or active nodes) over the listed
 // chunks: p = z + beta p_old staged, q = K p, p and q written on the chunks'
 // owned nodes (p, q must be zero on the other chunks), one partial of p.q per

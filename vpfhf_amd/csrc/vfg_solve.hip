@@ -1181,7 +1181,9 @@ int vfg_v_solve(vfg_handle *h, const double *U, double *V, const double *ub, con
   if ((rc = vfg_alloc(h, (void **)&h->act, n))) return rc;
   const int vcap = v_chunk_cap(P.G);
   if ((rc = vfg_alloc(h, (void **)&h->vflag, 4 * (size_t)vcap)) ||
-      (rc = vfg_alloc(h, (void **)&h->vlist, 4 * (size_t)(vcap + 2))))
+      (rc = vfg_alloc(h, (void **)&h->vlist, 4 * (size_t)(vcap + 2))) ||
+      (rc = vfg_alloc(h, (void **)&h->vflag2, 4 * (size_t)vcap)) ||
+      (rc = vfg_alloc(h, (void **)&h->vlist2, 4 * (size_t)(vcap + 2))))
     return rc;
   double *x = h->w[0], *r = h->w[1], *z = h->w[2], *p = h->w[3], *q = h->w[4], *dinv = h->w[5];
   double *Vd = h->w[6], *g = h->w[7], *Vn = h->w[8], *D = h->w[9], *KD = h->w[10], *ubd = nullptr;
@@ -1207,8 +1209,14 @@ int vfg_v_solve(vfg_handle *h, const double *U, double *V, const double *ub, con
   int it;
   const int nb = red_blocks(n);
   int status = VFG_OK;
+  int nstep = -1;  // chunks around the last step's free nodes (-1: no step yet)
   for (it = 0; it < (o->maxit < 200 ? o->maxit : 200); it++) {
-    launch_v_apply(P, Vd, g, 1, s);  // F(V): free rows = K V - b
+    // F(V): free rows = K V - b.  A step changes V on the free nodes only, so
+    // F changes only on the nodes that share an element with one: after the
+    // first step F is re-formed on the chunks around them (the dilated list
+    // of the step's free nodes) and kept elsewhere
+    if (nstep < 0) launch_v_apply(P, Vd, g, 1, s);
+    else launch_v_apply_list(P, Vd, g, 1, h->vlist2, nstep, s);
     hipLaunchKernelGGL(k_v_active, dim3(nb), dim3(RB), 0, s, (const double *)Vd, (const double *)g,
                        (const uint8_t *)P.vfix, (const double *)ubd, act, r, n, h->part);
     launch_dot(r, r, n, h->part + 4096, h->scal + 16, s);  // ||g_I||^2 (same value, fixed order)
@@ -1216,10 +1224,12 @@ int vfg_v_solve(vfg_handle *h, const double *U, double *V, const double *ub, con
     // system lives on a band around the cracks.  The chunks of node planes
     // holding a free node are listed, and CG runs over those only.
     launch_v_chunk_list(P, act, h->vflag, h->vlist, s);
+    launch_v_dilated_list(P, h->vflag, h->vflag2, h->vlist2, s);
     double pg2 = 0.;
-    int nlist = 0;
+    int nlist = 0, ndil = 0;
     if (hipMemcpyAsync(&pg2, h->scal + 16, 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipMemcpyAsync(&nlist, h->vlist + vcap, 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(&ndil, h->vlist2 + vcap, 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
       return vfg_fail(h, VFG_E_HIP, "vfg_v_solve: reduction failed");
     double pg = sqrt(pg2);
@@ -1270,6 +1280,7 @@ int vfg_v_solve(vfg_handle *h, const double *U, double *V, const double *ub, con
       lam *= .5;
     }
     hipMemcpyAsync(Vd, Vn, 8 * n, hipMemcpyDeviceToDevice, s);
+    nstep = ndil;
   }
   inf.outer = it;
   if (status == VFG_OK && it >= (o->maxit < 200 ? o->maxit : 200)) status = VFG_E_MAXIT;

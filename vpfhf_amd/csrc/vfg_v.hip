@@ -311,6 +311,32 @@ __global__ void k_v_mark_planes(Grid G, const uint8_t *__restrict__ mask, uint32
   flag[i / 15 + gx * (j / 15) + (int64_t)txy * k] = 1u;  // same value from every writer
 }
 
+// out[tile + txy k] = 1 where a flag is set on one of the 3 x 3 tiles around
+// the tile, on planes k - 1 .. k + 1: the (tile, plane) pairs holding a node
+// that shares an element with a node of a flagged pair (a node at a tile edge
+// meets the next tile's nodes, a node plane the planes next to it)
+__global__ void k_v_dilate_planes(const uint32_t *__restrict__ in, uint32_t *__restrict__ out, int gx, int gy,
+                                  int nz) {
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t txy = (int64_t)gx * gy;
+  if (f >= txy * nz) return;
+  const int t = (int)(f % txy), k = (int)(f / txy), tx = t % gx, ty = t / gx;
+  uint32_t any = 0u;
+  for (int dk = -1; dk <= 1; dk++) {
+    const int kk = k + dk;
+    if (kk < 0 || kk >= nz) continue;
+    for (int dy = -1; dy <= 1; dy++) {
+      const int yy = ty + dy;
+      if (yy < 0 || yy >= gy) continue;
+      for (int dx = -1; dx <= 1; dx++) {
+        const int xx = tx + dx;
+        if (xx >= 0 && xx < gx) any |= in[xx + (int64_t)gx * yy + txy * kk];
+      }
+    }
+  }
+  out[f] = any ? 1u : 0u;
+}
+
 constexpr int VLT = 1024;  // threads of the chunk-list kernel
 
 __device__ __forceinline__ int block_isum(int v, int *sh) {
@@ -663,6 +689,15 @@ void launch_v_chunk_list(const DevProblem &P, const uint8_t *mask, uint32_t *fla
   hipLaunchKernelGGL(k_v_mark_planes, dim3((unsigned)((P.nn + 255) / 256)), dim3(256), 0, s, P.G, mask, flag);
   hipLaunchKernelGGL(k_v_list_chunks, dim3(1), dim3(VLT), 0, s, (const uint32_t *)flag, txy, P.G.nz, lmin, P.G.nz,
                      1024, list, cap);
+}
+
+void launch_v_dilated_list(const DevProblem &P, const uint32_t *flag, uint32_t *flag2, int *list2, hipStream_t s) {
+  const int cap = v_chunk_cap(P.G), gx = (P.G.nx + 14) / 15, gy = (P.G.ny + 14) / 15, txy = gx * gy;
+  const int lmin = (cap + 32767) / 32768;
+  hipLaunchKernelGGL(k_v_dilate_planes, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, s, flag, flag2, gx, gy,
+                     P.G.nz);
+  hipLaunchKernelGGL(k_v_list_chunks, dim3(1), dim3(VLT), 0, s, (const uint32_t *)flag2, txy, P.G.nz, lmin, P.G.nz,
+                     1024, list2, cap);
 }
 
 int launch_v_apply_cg_list(const DevProblem &P, const double *pold, double *p, const double *z, const double *beta,
