@@ -150,7 +150,7 @@ typedef std::function<int(const double *, const double *)> XrFn;
 // between two arrays.  Same convergence test and reporting as pcg().
 static int pcg_fused(vfg_handle *h, int64_t n, const FusedFn &A, const double *dinv, double *x, double *r,
                      double *z, double *p, double *p2, double *q, double rtol, double atol, int maxit, int check,
-                     vfg_solve_info *info, const XrFn *xr = nullptr, double rn0_known = -1.);
+                     vfg_solve_info *info, const XrFn *xr = nullptr, double rn0_known = -1., int first = 0);
 
 // Preconditioned CG on the device.  On entry x is the initial guess and r the
 // initial residual b - A x.  Converged when ||r|| <= max(rtol*||r0||, atol).
@@ -218,7 +218,7 @@ __global__ void k_set_scalar(double *sc, int i, double v) { sc[i] = v; }
 
 static int pcg_fused(vfg_handle *h, int64_t n, const FusedFn &A, const double *dinv, double *x, double *r,
                      double *z, double *p, double *p2, double *q, double rtol, double atol, int maxit, int check,
-                     vfg_solve_info *info, const XrFn *xr, double rn0_known) {
+                     vfg_solve_info *info, const XrFn *xr, double rn0_known, int first) {
   hipStream_t s = h->stream;
   double *sc = h->scal, *part = h->part;
   const int nb = red_blocks(n);
@@ -239,8 +239,13 @@ static int pcg_fused(vfg_handle *h, int64_t n, const FusedFn &A, const double *d
   // solve done on the device and the rest of its batch exits at once
   if (xr) hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, s, sc, (int)S_TOL2, tol * tol);
   double *pa = p, *pb = p2;  // previous / current direction
+  bool first_batch = true;
   while (status == VFG_OK && rn > tol && it < maxit) {
     int todo = check < maxit - it ? check : maxit - it;
+    // first > 0: the caller's estimate of the iterations needed (launches
+    // after the device-side stop cost a launch each, for nothing)
+    if (first_batch && first > 0 && first < todo) todo = first;
+    first_batch = false;
     for (int k = 0; k < todo; k++) {
       const int na = A(pa, pb, q);  // pb = z + beta pa, q = A pb, part = p.q per workgroup
       hipLaunchKernelGGL(k_pcg_scalar, dim3(1), dim3(RB), 0, s, (const double *)part, na, 1, sc);
@@ -1210,6 +1215,7 @@ int vfg_v_solve(vfg_handle *h, const double *U, double *V, const double *ub, con
   const int nb = red_blocks(n);
   int status = VFG_OK;
   int nstep = -1;  // chunks around the last step's free nodes (-1: no step yet)
+  int last_cg = 0;  // CG iterations of the last Newton step
   for (it = 0; it < (o->maxit < 200 ? o->maxit : 200); it++) {
     // F(V): free rows = K V - b.  A step changes V on the free nodes only, so
     // F changes only on the nodes that share an element with one: after the
@@ -1256,8 +1262,12 @@ int vfg_v_solve(vfg_handle *h, const double *U, double *V, const double *ub, con
                               h->scal + S_DONE);
     };
     const double inner_rtol = fmax(inner_floor, fmin(0.1, 0.1 * pg / pg0));
-    // r = -g on the free nodes, so ||r0|| = pg
-    int prc = pcg_fused(h, n, A, dinv, x, r, z, p, h->w[12], q, inner_rtol, 0., 20000, 20, &inf, &XR, pg);
+    // r = -g on the free nodes, so ||r0|| = pg; the first batch holds two
+    // iterations more than the previous Newton step's solve took
+    const int its0 = inf.iterations;
+    int prc = pcg_fused(h, n, A, dinv, x, r, z, p, h->w[12], q, inner_rtol, 0., 20000, 20, &inf, &XR, pg,
+                        last_cg > 0 ? last_cg + 2 : 0);
+    last_cg = inf.iterations - its0;
     if (prc == VFG_E_HIP || prc == VFG_E_NAN) { status = prc; break; }
     // projected step with Armijo backtracking on f(V) = 1/2 V.KV - b.V
     hipLaunchKernelGGL(k_zero_fixed, dim3(nb_(n)), dim3(256), 0, s, g, (const uint8_t *)P.vfix, n);
