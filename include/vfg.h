@@ -303,11 +303,12 @@ typedef struct {
   double u_rtol;      /* u-solve relative residual (1e-10) */
   double v_rtol;      /* V-solve relative projected gradient (1e-8) */
   int    maxit;       /* CG iteration cap of an inner solve (100000) */
-  int    graph_chunk; /* CG iterations per replay (4) */
+  int    graph_chunk; /* CG iterations per replay (1) */
   int    use_graph;   /* 1: captured hipGraph replays; 0: plain launches */
   int    pc;          /* u preconditioner of the mixed engine: VFG_PC_MG (default; every
                          resident request's FP32 V-cycle, batched level by level inside the
-                         replays) or VFG_PC_JACOBI; the FP64 engine uses Jacobi */
+                         replays, under the FP32 CG of the mixed engine or the FP64 CG of the
+                         FP64 engine) or VFG_PC_JACOBI */
 } vfg_engine_opts;
 typedef struct {
   int64_t tag;

@@ -94,7 +94,7 @@ def make_requests(n, rate=None, seed=0, outputs=False, fields=False, **kw):
 
 
 def serve(reqs, max_slots=64, hbm_budget=200e9, mixed=True, realtime=True, u_rtol=1e-10, v_rtol=1e-8,
-          graph_chunk=4, use_graph=True, pc=None):
+          graph_chunk=1, use_graph=True, pc=None):
     """Runs the stream; returns (per-request result dicts, stats dict)."""
     L = load_host()
     o = ServeOpts()
