@@ -146,3 +146,41 @@ def test_mg_vcycle_is_symmetric(n, bc):
     assert not free.all() and np.all(z1[~free] == 0.)
     P.close()
 
+
+
+def test_pcshell_cg_matches_the_library_solve(oracle):
+    """The PETSc binding of the u-KSP with the shells: a host CG (PETSc's
+    KSPCG, restated) over the MatShell product (vfg_u_apply) and the PCSHELL
+    (vfg_u_pc_apply) solves the Newton system J dU = -F(U0) of
+    SNESNEWTONLS's first step (VF_UResidual at U0 carrying its Dirichlet
+    values); U0 + dU matches the oracle's solve and the iteration count is
+    the library multigrid solve's."""
+    from vpfhf_amd import _lib
+    s = S.make(n=(33, 33, 33), seed=7, bcU=S.MIXED_U)
+    op, F = S.to_oracle(oracle, s)
+    P = S.to_gpu(s)
+    BCU = oracle.vec_set_from_bc(op, "U") * 0.01
+    U0 = oracle.vec_apply_dirichlet(op, np.zeros(3 * op.nn), BCU, "U")
+    Uo = U0.copy()
+    oracle.u_solve(op, F, Uo, BCU, rtol=1e-12)
+    b = -P.VF_UResidual(U0, BCU)
+    x = np.zeros_like(b)
+    r = b.copy()
+    z = P.u_pc_apply(r)
+    p = z.copy()
+    rz, r0, its = r @ z, np.linalg.norm(b), 0
+    while np.linalg.norm(r) > 1e-10 * r0 and its < 200:
+        q = P.UIJacobian_mult(p)
+        a = rz / (p @ q)
+        x += a * p
+        r -= a * q
+        z = P.u_pc_apply(r)
+        rz, rz_old = r @ z, rz
+        p = z + (rz / rz_old) * p
+        its += 1
+    U = U0 + x
+    _, im = P.VF_StepU(U0, BCU, rtol=1e-10, pc=_lib.PC_MG)
+    print("PCSHELL CG", its, "iterations; library multigrid solve", im.iterations)
+    assert relerr(U, Uo) < 1e-8, relerr(U, Uo)
+    assert abs(its - im.iterations) <= 2, (its, im.iterations)
+    P.close()
