@@ -7,8 +7,9 @@
  *
  * Built by a PETSc host program, not by this tree (PETSc is no dependency of
  * libvfg):  $(CC) -c vfg_petsc.c $(PETSC_CC_INCLUDES) -I<repo>/include,
- * linked with -lvfg; PETSc 3.18 or later, real double scalars.  One MPI rank per GPU whose DMDAs own the whole grid, so
- * a global Vec is in the natural order libvfg reads (the adapter checks it).
+ * linked with -lvfg; PETSc 3.19 or later, real double scalars.  One MPI
+ * rank per GPU whose DMDAs own the whole grid, so a global Vec is in the
+ * natural order libvfg reads (the adapter checks it).
  */
 #ifndef VFG_PETSC_H
 #define VFG_PETSC_H
