@@ -22,7 +22,7 @@
 namespace vfg {
 
 constexpr int BCH = 4096;  // entries per Chebyshev chunk
-constexpr int BTR = 256;   // entries per transfer chunk: one gather per thread
+constexpr int BTR = 256;   // threads per transfer chunk: one fine node each (prolongation), four per coarse dof (restriction)
 
 
 // Launch grids are list capacities; dn holds the live chunk count, so a
@@ -40,7 +40,10 @@ __global__ __launch_bounds__(256) void k_bmg_cheb(const BmgVec *__restrict__ lvt
     mg_cheb_entry(v.b, use_y ? v.y : nullptr, v.dinv, v.d, v.x, k.c1, k.c2, step == 0, xzero, c.start + t);
 }
 
-// chunks over the coarse dofs (level + 1) of members that descend
+// chunks of BTR / 4 coarse dofs (level + 1) of members that descend, four
+// lanes per dof: lane kk sums the fine planes kk, kk + 4, ... of the gather
+// and the four are added by shuffles in lane order (as k_mg_restrict4), so a
+// thread's serial gather is a quarter as long
 __global__ __launch_bounds__(256) void k_bmg_restrict(const BmgVec *__restrict__ lvt, int level,
                                                       const BmgChunk *__restrict__ ch, const int *__restrict__ dn,
                                                       const double *__restrict__ flags, int fstride) {
@@ -49,10 +52,12 @@ __global__ __launch_bounds__(256) void k_bmg_restrict(const BmgVec *__restrict__
   if (flags[(int64_t)c.p * fstride] == 0.) return;
   const BmgVec &f = lvt[c.p * BML + level];
   const BmgVec &g = lvt[c.p * BML + level + 1];
-  for (int t = threadIdx.x; t < c.len; t += 256) {
-    const int64_t dof = c.start + t, N = dof / 3;
-    g.b[dof] = mg_restrict_dof(f.G, g.G, f.bc, f.b, f.y, N, (int)(dof - 3 * N));
-  }
+  const int t = (int)threadIdx.x >> 2, kk = (int)threadIdx.x & 3;
+  const bool ok = t < c.len;
+  const int64_t dof = c.start + (ok ? t : 0), N = dof / 3;
+  const float v = ok ? mg_restrict_dof_plane(f.G, g.G, f.bc, f.b, f.y, N, (int)(dof - 3 * N), kk) : 0.f;
+  const float v1 = __shfl_down(v, 1, 4), v2 = __shfl_down(v, 2, 4), v3 = __shfl_down(v, 3, 4);
+  if (ok && kk == 0) g.b[dof] = ((v + v1) + v2) + v3;
 }
 
 // chunks over the fine nodes (level) of members that descend: the coarse
@@ -200,7 +205,8 @@ int bmg_rebuild(BatchMG *g, hipStream_t s, bool *grew) {
       if (g->L[p] - 1 <= l) continue;
       const Grid &Gf = level_grid(g, p, l), &Gc = level_grid(g, p, l + 1);
       const int64_t nc = 3 * (int64_t)Gc.nx * Gc.ny * Gc.nz, nf = (int64_t)Gf.nx * Gf.ny * Gf.nz;
-      for (int64_t st = 0; st < nc; st += BTR) P.rch.h.push_back({p, st, (int)(nc - st < BTR ? nc - st : BTR)});
+      constexpr int RC = BTR / 4;  // four lanes per coarse dof
+      for (int64_t st = 0; st < nc; st += RC) P.rch.h.push_back({p, st, (int)(nc - st < RC ? nc - st : RC)});
       for (int64_t st = 0; st < nf; st += BTR) P.pch.h.push_back({p, st, (int)(nf - st < BTR ? nf - st : BTR)});
     }
     if (l >= Lmax && !P.dcnt) continue;  // nothing on this level, ever
