@@ -215,6 +215,11 @@ static int pcg(vfg_handle *h, int64_t n, const ApplyFn &A, const double *dinv, d
 }
 
 __global__ void k_set_scalar(double *sc, int i, double v) { sc[i] = v; }
+// two device counts next to a scalar, so that one copy reads all three
+__global__ void k_pack_counts(const int *a, const int *b, double *out) {
+  out[0] = (double)*a;
+  out[1] = (double)*b;
+}
 
 static int pcg_fused(vfg_handle *h, int64_t n, const FusedFn &A, const double *dinv, double *x, double *r,
                      double *z, double *p, double *p2, double *q, double rtol, double atol, int maxit, int check,
@@ -1231,13 +1236,15 @@ int vfg_v_solve(vfg_handle *h, const double *U, double *V, const double *ub, con
     // holding a free node are listed, and CG runs over those only.
     launch_v_chunk_list(P, act, h->vflag, h->vlist, s);
     launch_v_dilated_list(P, h->vflag, h->vflag2, h->vlist2, s);
-    double pg2 = 0.;
-    int nlist = 0, ndil = 0;
-    if (hipMemcpyAsync(&pg2, h->scal + 16, 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipMemcpyAsync(&nlist, h->vlist + vcap, 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipMemcpyAsync(&ndil, h->vlist2 + vcap, 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+    // ||g_I||^2 and the two list lengths in one read
+    hipLaunchKernelGGL(k_pack_counts, dim3(1), dim3(1), 0, s, (const int *)(h->vlist + vcap),
+                       (const int *)(h->vlist2 + vcap), h->scal + 17);
+    double got[3];
+    if (hipMemcpyAsync(got, h->scal + 16, 24, hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
       return vfg_fail(h, VFG_E_HIP, "vfg_v_solve: reduction failed");
+    const double pg2 = got[0];
+    const int nlist = (int)got[1], ndil = (int)got[2];
     double pg = sqrt(pg2);
     if (!isfinite(pg)) { status = VFG_E_NAN; break; }
     if (pg0 < 0.) pg0 = pg > 0. ? pg : 1.;
