@@ -292,7 +292,14 @@ static PetscErrorCode VCreateSubMatrix(Mat A, IS isrow, IS iscol, MatReuse reuse
   PetscFunctionBegin;
   PetscCall(ISEqual(isrow, iscol, &same));
   PetscCheck(same, PETSC_COMM_SELF, PETSC_ERR_SUP, "vfg_petsc: V submatrices are taken on equal row and column sets");
-  if (reuse == MAT_REUSE_MATRIX) PetscCall(MatDestroy(B));
+  if (reuse == MAT_REUSE_MATRIX) { /* same sizes: the existing shell takes the new index set */
+    PetscCall(MatShellGetContext(*B, &s));
+    PetscCall(PetscObjectReference((PetscObject)isrow));
+    PetscCall(ISDestroy(&s->is));
+    s->is = isrow;
+    PetscCall(PetscObjectStateIncrease((PetscObject)*B));
+    PetscFunctionReturn(PETSC_SUCCESS);
+  }
   PetscCall(PetscNew(&s));
   s->full = A;
   PetscCall(PetscObjectReference((PetscObject)isrow));
