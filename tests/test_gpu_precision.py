@@ -3,7 +3,7 @@ preconditioned CG) inside FP64 iterative refinement, against the plain FP64
 solve of the same system.
 
 Bar: the refined solution reaches the FP64 relative residual 1e-10 and agrees
-with the FP64 solve within 1e-7 relative (north-star bar 1e-6); the FP64
+with the FP64 solve within 1e-7 relative (the precision form's bar is 1e-6); the FP64
 residual is evaluated by the FP64 operator.
 """
 import numpy as np
